@@ -1,0 +1,56 @@
+"""Builds libiesolver_amd.so (HIP, gfx950) in-tree.
+
+    python ie-solver_amd/build.py
+
+Objects go to ie-solver_amd/_build/; the shared library lands next to this
+file so that it travels with the source tree.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libiesolver_amd.so")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
+    # keep hipcc from emitting packed FP32 arithmetic
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+HIP_SOURCES = ["rs_kernels.hip", "rs_holes.hip", "capi.cpp"]
+HOST_SOURCES = ["tree.cpp"]
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if p.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError("command failed: " + " ".join(cmd))
+    return out
+
+
+def build(verbose=False):
+    os.makedirs(OUT, exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OUT, src + ".o")
+        cmd = ["hipcc"] + HIP_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        out = _run(cmd)
+        if verbose and out:
+            print(out)
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(OUT, src + ".o")
+        _run(["g++", "-O2", "-fPIC", "-std=c++17", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run(["hipcc", "--offload-arch=gfx950", "-shared", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,898 @@
+// C ABI (include/iegpu.h): context, device memory, and the level-by-level
+// launch sequence of the RS factor + solve.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/iegpu.h"
+#include "rs_device.hpp"
+#include "rs_holes.hpp"
+#include "rs_kernels.hpp"
+#include "tree.hpp"
+
+using namespace iegpu;
+
+namespace {
+
+struct HipError {
+  hipError_t e;
+  const char* what;
+};
+
+#define HIP_TRY(expr)                                   \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) throw HipError{_e, #expr};    \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() {}
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw HipError{e, "hipMalloc"};
+    }
+    n = count;
+  }
+  void upload(const T* h, size_t count, hipStream_t st) {
+    alloc(count);
+    if (count) HIP_TRY(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+  void upload(const std::vector<T>& v, hipStream_t st) { upload(v.data(), v.size(), st); }
+};
+
+// Profile table rows: the factor stages of rs_kernels.hpp, then these.
+enum : int {
+  kProfRoot = kNumFactorStages,  // root active list + dense block + LU
+  kProfSweepUp,                  // upward sweep (+ X_rr^{-1}) per level
+  kProfRootSolve,                // root / bordered solve
+  kProfSweepDown,                // downward sweep per level
+  kProfBorder,                   // bordered-system factor (holes)
+  kProfStages
+};
+constexpr int kProfLevels = 64;
+
+// Per-(stage, level) device time from HIP events recorded around each
+// launch on the launching stream; resolved when read.
+struct Profiler {
+  bool on = false;
+  struct Mark {
+    int stage, level;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> pending;
+  std::vector<hipEvent_t> spare;
+  double ms[kProfStages][kProfLevels] = {};
+  int launches[kProfStages][kProfLevels] = {};
+
+  hipEvent_t event() {
+    if (!spare.empty()) {
+      hipEvent_t e = spare.back();
+      spare.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    return e;
+  }
+  void resolve() {
+    for (const Mark& m : pending) {
+      HIP_TRY(hipEventSynchronize(m.b));
+      float t = 0;
+      HIP_TRY(hipEventElapsedTime(&t, m.a, m.b));
+      ms[m.stage][m.level] += t;
+      launches[m.stage][m.level] += 1;
+      spare.push_back(m.a);
+      spare.push_back(m.b);
+    }
+    pending.clear();
+  }
+  void reset() {
+    resolve();
+    std::memset(ms, 0, sizeof(ms));
+    std::memset(launches, 0, sizeof(launches));
+  }
+  void release() {
+    for (const Mark& m : pending) {
+      (void)hipEventDestroy(m.a);
+      (void)hipEventDestroy(m.b);
+    }
+    pending.clear();
+    for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+    spare.clear();
+  }
+  template <class F>
+  void run(int stage, int level, hipStream_t st, F&& f) {
+    if (!on) {
+      f();
+      return;
+    }
+    Mark m{stage, std::min(std::max(level, 0), kProfLevels - 1), event(), event()};
+    HIP_TRY(hipEventRecord(m.a, st));
+    f();
+    HIP_TRY(hipEventRecord(m.b, st));
+    pending.push_back(m);
+  }
+};
+
+}  // namespace
+
+struct iegpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Profiler prof;
+
+  // geometry
+  int pde = kPdeLaplace;
+  int sol_dim = 1;
+  int n_points = 0;
+  std::vector<double> h_pts, h_nrm, h_curv, h_wts;
+  DevBuf<double> d_pts, d_nrm, d_curv, d_wts;
+
+  // tree
+  bool have_tree = false;
+  HostTree tree;
+  DevBuf<int> t_level, t_leaf, t_child, t_orig_off, t_orig_cnt, t_orig_dofs,
+      t_src_off, t_src_cnt, t_src;
+  DevBuf<double> t_cx, t_cy, t_side;
+
+  // holes
+  int n_holes = 0;
+  int hole_dim = 0;
+  DevBuf<double> d_U, d_PsiT;  // ndof x hole_dim each
+  DevBuf<double> d_mU, d_mPsiT, d_DinvC, d_gram, d_partial, d_Smat, d_tail,
+      d_ybuf, d_alpha;
+  DevBuf<int> d_Spiv, d_mask;
+
+  // factor state
+  int ncap = 0;
+  int precision = IEGPU_FP64;
+  bool factored = false;
+  DevBuf<int> s_act_cnt, s_act_list, s_chpos, s_out_cnt, s_out_list,
+      s_red_list, s_perm, s_rank, s_upd_kind, s_m_rows, s_row_pool, s_err,
+      s_root_piv;
+  DevBuf<int64_t> s_id_sz, s_row_sz, s_fac_sz, s_id_off, s_row_off, s_fac_off,
+      s_fac_top;
+  DevBuf<unsigned char> s_id_pool, s_fac_pool, s_root_mat;
+  int64_t id_cap = 0, row_cap = 0, fac_cap = 0;
+  DevBuf<double> d_work;  // solve vector for host-pointer solves
+
+  double t_factor_ms = 0, t_solve_ms = 0, t_root_ms = 0;
+  double fac_bytes = 0;
+
+  int ndof() const { return n_points * sol_dim; }
+  size_t elem() const { return precision == IEGPU_FP32 ? 4 : 8; }
+
+  DevGeom geom() const {
+    DevGeom g;
+    g.pts = d_pts.p;
+    g.nrm = d_nrm.p;
+    g.curv = d_curv.p;
+    g.wts = d_wts.p;
+    g.n_points = n_points;
+    g.sol_dim = sol_dim;
+    g.pde = pde;
+    return g;
+  }
+  DevTree dtree() const {
+    DevTree t;
+    t.n_boxes = tree.n_boxes;
+    t.n_levels = tree.n_levels;
+    t.level = t_level.p;
+    t.leaf = t_leaf.p;
+    t.child = t_child.p;
+    t.cx = t_cx.p;
+    t.cy = t_cy.p;
+    t.side = t_side.p;
+    t.orig_off = t_orig_off.p;
+    t.orig_cnt = t_orig_cnt.p;
+    t.orig_dofs = t_orig_dofs.p;
+    t.src_off = t_src_off.p;
+    t.src_cnt = t_src_cnt.p;
+    t.src = t_src.p;
+    return t;
+  }
+  DevState dstate() {
+    DevState s;
+    s.ncap = ncap;
+    s.act_cnt = s_act_cnt.p;
+    s.act_list = s_act_list.p;
+    s.chpos = s_chpos.p;
+    s.out_cnt = s_out_cnt.p;
+    s.out_list = s_out_list.p;
+    s.red_list = s_red_list.p;
+    s.perm = s_perm.p;
+    s.rank = s_rank.p;
+    s.upd_kind = s_upd_kind.p;
+    s.m_rows = s_m_rows.p;
+    s.id_sz = s_id_sz.p;
+    s.row_sz = s_row_sz.p;
+    s.fac_sz = s_fac_sz.p;
+    s.id_off = s_id_off.p;
+    s.row_off = s_row_off.p;
+    s.fac_off = s_fac_off.p;
+    s.id_pool = s_id_pool.p;
+    s.row_pool = s_row_pool.p;
+    s.fac_pool = s_fac_pool.p;
+    s.id_cap = id_cap;
+    s.row_cap = row_cap;
+    s.fac_cap = fac_cap;
+    s.fac_top = s_fac_top.p;
+    s.err = s_err.p;
+    s.root_mat = s_root_mat.p;
+    s.root_piv = s_root_piv.p;
+    return s;
+  }
+};
+
+namespace {
+
+int fail(iegpu_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int from_hip(iegpu_ctx* c, const HipError& e) {
+  const bool oom = e.e == hipErrorOutOfMemory || e.e == hipErrorMemoryAllocation;
+  std::string m = std::string(e.what) + ": " + hipGetErrorString(e.e);
+  return fail(c, oom ? IEGPU_ERR_OOM : IEGPU_ERR_HIP, m);
+}
+
+// U / Psi^T on the boundary points: ie_solver/linear_solve.cpp:20-112.
+void build_border(iegpu_ctx* c, const double* centers, const double* radii,
+                  std::vector<double>* U, std::vector<double>* PsiT) {
+  const int np = c->n_points, h = c->n_holes;
+  const int64_t nd = c->ndof();
+  if (c->pde == kPdeLaplace) {
+    c->hole_dim = h;
+    U->assign(nd * h, 0.0);
+    PsiT->assign(nd * h, 0.0);
+    for (int i = 0; i < np; i++) {
+      const double x0 = c->h_pts[2 * i], x1 = c->h_pts[2 * i + 1];
+      for (int k = 0; k < h; k++) {
+        const double r0 = x0 - centers[2 * k], r1 = x1 - centers[2 * k + 1];
+        (*U)[i + nd * k] = std::log(std::sqrt(r0 * r0 + r1 * r1));
+      }
+      for (int k = 0; k < h; k++) {
+        const double r0 = x0 - centers[2 * k], r1 = x1 - centers[2 * k + 1];
+        if (std::sqrt(r0 * r0 + r1 * r1) < radii[k] + 1e-8) {
+          (*PsiT)[i + nd * k] = c->h_wts[i];
+          break;
+        }
+      }
+    }
+    return;
+  }
+  c->hole_dim = 3 * h;
+  const int hd = 3 * h;
+  U->assign(nd * hd, 0.0);
+  PsiT->assign(nd * hd, 0.0);
+  const double scale = 1.0 / (4 * M_PI);
+  for (int i = 0; i < np; i++) {
+    const double x0 = c->h_pts[2 * i], x1 = c->h_pts[2 * i + 1];
+    const int64_t r0i = 2 * i, r1i = 2 * i + 1;
+    for (int k = 0; k < h; k++) {
+      const double r0 = x0 - centers[2 * k], r1 = x1 - centers[2 * k + 1];
+      const double rn = std::sqrt(r0 * r0 + r1 * r1);
+      const double rn2 = rn * rn;
+      (*U)[r0i + nd * (3 * k)] = scale * (std::log(1 / rn) + (1.0 / rn2) * r0 * r0);
+      (*U)[r1i + nd * (3 * k)] = scale * ((1.0 / rn2) * r1 * r0);
+      (*U)[r0i + nd * (3 * k + 1)] = scale * ((1.0 / rn2) * r0 * r1);
+      (*U)[r1i + nd * (3 * k + 1)] = scale * (std::log(1 / rn) + (1.0 / rn2) * r1 * r1);
+      (*U)[r0i + nd * (3 * k + 2)] = r1 * (scale / rn2);
+      (*U)[r1i + nd * (3 * k + 2)] = -r0 * (scale / rn2);
+    }
+    for (int k = 0; k < h; k++) {
+      const double r0 = x0 - centers[2 * k], r1 = x1 - centers[2 * k + 1];
+      if (std::sqrt(r0 * r0 + r1 * r1) < radii[k] + 1e-8) {
+        const double w = c->h_wts[i];
+        (*PsiT)[r0i + nd * (3 * k)] = w;
+        (*PsiT)[r1i + nd * (3 * k + 1)] = w;
+        (*PsiT)[r0i + nd * (3 * k + 2)] = w * x1;
+        (*PsiT)[r1i + nd * (3 * k + 2)] = -w * x0;
+        break;
+      }
+    }
+  }
+}
+
+void alloc_state(iegpu_ctx* c) {
+  const int nb = c->tree.n_boxes;
+  const int64_t nl = (int64_t)nb * c->ncap;
+  hipStream_t st = c->stream;
+  (void)st;
+  c->s_act_cnt.alloc(nb);
+  c->s_act_list.alloc(nl);
+  c->s_chpos.alloc(4 * (size_t)nb);
+  c->s_out_cnt.alloc(nb);
+  c->s_out_list.alloc(nl);
+  c->s_red_list.alloc(nl);
+  c->s_perm.alloc(nl);
+  c->s_rank.alloc(nb);
+  c->s_upd_kind.alloc(nb);
+  c->s_m_rows.alloc(nb);
+  c->s_id_sz.alloc(nb);
+  c->s_row_sz.alloc(nb);
+  c->s_fac_sz.alloc(nb);
+  c->s_id_off.alloc(nb);
+  c->s_row_off.alloc(nb);
+  c->s_fac_off.alloc(nb);
+  c->s_fac_top.alloc(1);
+  c->s_err.alloc(4);
+  c->s_root_piv.alloc(c->ncap + 64);
+  c->s_root_mat.alloc((size_t)c->ncap * c->ncap * 8);
+  c->s_id_pool.alloc((size_t)c->id_cap * c->elem());
+  c->s_row_pool.alloc((size_t)c->row_cap);
+  c->s_fac_pool.alloc((size_t)c->fac_cap * c->elem());
+}
+
+template <typename R>
+void enqueue_factor(iegpu_ctx* c, double tol) {
+  DevGeom g = c->geom();
+  DevTree t = c->dtree();
+  DevState s = c->dstate();
+  hipStream_t st = c->stream;
+  launch_reset(s, st);
+  const HostTree& T = c->tree;
+  for (int lv = T.n_levels - 1; lv >= 1; lv--) {
+    const int start = T.level_start[lv];
+    const int count = T.level_start[lv + 1] - start;
+    for (int stage = 0; stage < kNumFactorStages; stage++)
+      c->prof.run(stage, lv, st, [&] {
+        launch_factor_stage<R>(stage, g, t, s, lv, start, count, tol, st);
+      });
+  }
+  c->prof.run(kProfRoot, 0, st,
+              [&] { launch_root<R>(g, t, s, c->hole_dim == 0, st); });
+}
+
+template <typename R>
+void enqueue_border_factor(iegpu_ctx* c) {
+  DevState s = c->dstate();
+  hipStream_t st = c->stream;
+  const HostTree& T = c->tree;
+  const int64_t nd = c->ndof();
+  const int h = c->hole_dim;
+  c->prof.run(kProfBorder, 0, st, [&] {
+    launch_copy(c->d_mU.p, c->d_U.p, nd * h, st);
+    launch_copy(c->d_mPsiT.p, c->d_PsiT.p, nd * h, st);
+    SolveVec vu{c->d_mU.p, nd, h};
+    SolveVec vp{c->d_mPsiT.p, nd, h};
+    for (int lv = T.n_levels - 1; lv >= 1; lv--) {
+      const int start = T.level_start[lv];
+      const int count = T.level_start[lv + 1] - start;
+      launch_sweep_up<R>(s, start, count, vu, false, false, st);
+      launch_sweep_up<R>(s, start, count, vp, true, false, st);
+    }
+    launch_copy(c->d_DinvC.p, c->d_mU.p, nd * h, st);
+    SolveVec vd{c->d_DinvC.p, nd, h};
+    for (int lv = T.n_levels - 1; lv >= 1; lv--) {
+      const int start = T.level_start[lv];
+      const int count = T.level_start[lv + 1] - start;
+      launch_diag<R>(s, start, count, vd, st);
+    }
+    launch_mark_skel(s, c->d_mask.p, nd, st);
+    launch_border_gram(c->d_mPsiT.p, c->d_DinvC.p, c->d_mask.p, nd, h,
+                       c->d_partial.p, c->d_gram.p, st);
+    launch_border_factor<R>(s, c->d_mU.p, c->d_mPsiT.p, c->d_gram.p, nd, h,
+                            c->d_Smat.p, c->d_Spiv.p, st);
+  });
+}
+
+template <typename R>
+void enqueue_solve(iegpu_ctx* c, double* x, double* alpha, hipStream_t st) {
+  DevState s = c->dstate();
+  const HostTree& T = c->tree;
+  const int64_t nd = c->ndof();
+  SolveVec v{x, nd, 1};
+  const bool holes = c->hole_dim > 0;
+  for (int lv = T.n_levels - 1; lv >= 1; lv--) {
+    const int start = T.level_start[lv];
+    const int count = T.level_start[lv + 1] - start;
+    c->prof.run(kProfSweepUp, lv, st,
+                [&] { launch_sweep_up<R>(s, start, count, v, false, true, st); });
+  }
+  c->prof.run(kProfRootSolve, 0, st, [&] {
+    if (!holes) {
+      launch_root_solve<R>(s, v, st);
+    } else {
+      const int h = c->hole_dim;
+      launch_border_gemv(c->d_mPsiT.p, x, c->d_mask.p, nd, h, c->d_partial.p,
+                         c->d_tail.p, -1.0, st);
+      launch_border_solve(s, c->d_Smat.p, c->d_Spiv.p, h, x, c->d_tail.p,
+                          alpha, c->d_ybuf.p, st);
+      launch_red_update(x, c->d_DinvC.p, c->d_mask.p, alpha, nd, h, st);
+    }
+  });
+  for (int lv = 1; lv < T.n_levels; lv++) {
+    const int start = T.level_start[lv];
+    const int count = T.level_start[lv + 1] - start;
+    c->prof.run(kProfSweepDown, lv, st,
+                [&] { launch_sweep_down<R>(s, start, count, v, st); });
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int iegpu_create(iegpu_ctx** out, int device) {
+  if (!out) return IEGPU_ERR_INVALID;
+  *out = nullptr;
+  if (device == -1) {  // host-only context: geometry and tree, no factorization
+    iegpu_ctx* c = new iegpu_ctx();
+    c->device = -1;
+    *out = c;
+    return IEGPU_OK;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return IEGPU_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return IEGPU_ERR_INVALID;
+  std::unique_ptr<iegpu_ctx> c(new iegpu_ctx());
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) return IEGPU_ERR_HIP;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return IEGPU_ERR_HIP;
+  *out = c.release();
+  return IEGPU_OK;
+}
+
+int iegpu_destroy(iegpu_ctx* c) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (c->device < 0) {
+    delete c;
+    return IEGPU_OK;
+  }
+  (void)hipSetDevice(c->device);
+  if (c->stream) {
+    (void)hipStreamSynchronize(c->stream);
+  }
+  c->prof.release();
+  hipStream_t st = c->stream;
+  delete c;
+  if (st) (void)hipStreamDestroy(st);
+  return IEGPU_OK;
+}
+
+const char* iegpu_last_error(const iegpu_ctx* c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+int iegpu_set_geometry(iegpu_ctx* c, int pde, int n_points,
+                       const double* points, const double* normals,
+                       const double* curvatures, const double* weights) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (pde != kPdeLaplace && pde != kPdeStokes)
+    return fail(c, IEGPU_ERR_INVALID, "pde must be LAPLACE (0) or STOKES (2)");
+  if (n_points <= 0 || !points || !normals || !curvatures || !weights)
+    return fail(c, IEGPU_ERR_INVALID, "empty boundary or null array");
+  for (int i = 0; i < 2 * n_points; i++)
+    if (!std::isfinite(points[i]) || !std::isfinite(normals[i]))
+      return fail(c, IEGPU_ERR_INVALID, "non-finite point or normal");
+  for (int i = 0; i < n_points; i++)
+    if (!std::isfinite(curvatures[i]) || !std::isfinite(weights[i]))
+      return fail(c, IEGPU_ERR_INVALID, "non-finite curvature or weight");
+  try {
+    c->pde = pde;
+    c->sol_dim = (pde == kPdeStokes) ? 2 : 1;
+    c->n_points = n_points;
+    c->h_pts.assign(points, points + 2 * n_points);
+    c->h_nrm.assign(normals, normals + 2 * n_points);
+    c->h_curv.assign(curvatures, curvatures + n_points);
+    c->h_wts.assign(weights, weights + n_points);
+    if (c->device >= 0) {
+      HIP_TRY(hipSetDevice(c->device));
+      c->d_pts.upload(c->h_pts, c->stream);
+      c->d_nrm.upload(c->h_nrm, c->stream);
+      c->d_curv.upload(c->h_curv, c->stream);
+      c->d_wts.upload(c->h_wts, c->stream);
+      HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    c->have_tree = false;
+    c->factored = false;
+    c->n_holes = 0;
+    c->hole_dim = 0;
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_build_tree(iegpu_ctx* c, int leaf_cap) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (c->n_points == 0) return fail(c, IEGPU_ERR_INVALID, "set_geometry first");
+  if (leaf_cap < c->sol_dim || leaf_cap > kMaxBoxDofs / 2)
+    return fail(c, IEGPU_ERR_INVALID, "leaf_cap out of range");
+  try {
+    c->tree = HostTree();
+    c->tree.build(c->h_pts.data(), c->n_points, c->sol_dim, leaf_cap);
+    c->have_tree = true;
+    c->factored = false;
+    if (c->device < 0) return IEGPU_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    const HostTree& T = c->tree;
+    hipStream_t st = c->stream;
+    c->t_level.upload(T.level, st);
+    c->t_leaf.upload(T.leaf, st);
+    c->t_child.upload(T.child, st);
+    c->t_cx.upload(T.cx, st);
+    c->t_cy.upload(T.cy, st);
+    c->t_side.upload(T.side, st);
+    c->t_orig_off.upload(T.orig_off, st);
+    c->t_orig_cnt.upload(T.orig_cnt, st);
+    c->t_orig_dofs.upload(T.orig_dofs, st);
+    c->t_src_off.upload(T.src_off, st);
+    c->t_src_cnt.upload(T.src_cnt, st);
+    if (T.src.empty()) {
+      const std::vector<int> dummy(1, 0);
+      c->t_src.upload(dummy, st);
+    } else {
+      c->t_src.upload(T.src, st);
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    c->have_tree = true;
+    c->factored = false;
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  } catch (const std::exception& e) {
+    return fail(c, IEGPU_ERR_INVALID, e.what());
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_set_holes(iegpu_ctx* c, int n_holes, const double* centers,
+                    const double* radii) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (c->n_points == 0) return fail(c, IEGPU_ERR_INVALID, "set_geometry first");
+  if (n_holes < 0 || (n_holes > 0 && (!centers || !radii)))
+    return fail(c, IEGPU_ERR_INVALID, "bad hole arrays");
+  if (n_holes > 64) return fail(c, IEGPU_ERR_INVALID, "at most 64 holes");
+  if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    c->n_holes = n_holes;
+    c->factored = false;
+    if (n_holes == 0) {
+      c->hole_dim = 0;
+      return IEGPU_OK;
+    }
+    std::vector<double> U, PsiT;
+    build_border(c, centers, radii, &U, &PsiT);
+    const int64_t nd = c->ndof();
+    const int h = c->hole_dim;
+    hipStream_t st = c->stream;
+    c->d_U.upload(U, st);
+    c->d_PsiT.upload(PsiT, st);
+    c->d_mU.alloc(nd * h);
+    c->d_mPsiT.alloc(nd * h);
+    c->d_DinvC.alloc(nd * h);
+    c->d_gram.alloc((size_t)h * h);
+    c->d_partial.alloc((size_t)border_partial_blocks(nd) * h * h);
+    c->d_tail.alloc(h);
+    c->d_alpha.alloc(h);
+    c->d_mask.alloc(nd);
+    HIP_TRY(hipStreamSynchronize(st));
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!c->have_tree) return fail(c, IEGPU_ERR_INVALID, "build_tree first");
+  if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
+  if (!(id_tol > 0) || !std::isfinite(id_tol))
+    return fail(c, IEGPU_ERR_INVALID, "id_tol must be > 0");
+  if (precision != IEGPU_FP64 && precision != IEGPU_FP32)
+    return fail(c, IEGPU_ERR_INVALID, "precision must be FP64 or FP32");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t nd = c->ndof();
+    if (c->precision != precision) {
+      c->s_id_pool.release();
+      c->s_fac_pool.release();
+    }
+    c->precision = precision;
+    if (c->ncap == 0) c->ncap = 256;
+    if (c->id_cap == 0) c->id_cap = std::max<int64_t>(nd * 640, 1 << 22);
+    if (c->row_cap == 0) c->row_cap = std::max<int64_t>(nd * 8, 1 << 16);
+    if (c->fac_cap == 0) c->fac_cap = std::max<int64_t>(nd * 160, 1 << 22);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int errbits = 0, maxn = 0;
+    for (int attempt = 0; attempt < 8; attempt++) {
+      alloc_state(c);
+      if (c->hole_dim > 0) {
+        const int ns_max = c->ncap + c->hole_dim;
+        c->d_Smat.alloc((size_t)ns_max * ns_max);
+        c->d_Spiv.alloc(ns_max);
+        c->d_ybuf.alloc(ns_max);
+      }
+      HIP_TRY(hipEventRecord(e0, c->stream));
+      if (precision == IEGPU_FP32) {
+        enqueue_factor<float>(c, id_tol);
+      } else {
+        enqueue_factor<double>(c, id_tol);
+      }
+      HIP_TRY(hipGetLastError());
+      int h_err[4] = {0, 0, 0, 0};
+      HIP_TRY(hipMemcpyAsync(h_err, c->s_err.p, sizeof(h_err), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      errbits = h_err[0];
+      maxn = h_err[1];
+      const int grow = errbits & (kErrBoxCap | kErrIdPool | kErrRowPool | kErrFacPool);
+      if (!grow) {
+        if (c->hole_dim > 0) {
+          if (precision == IEGPU_FP32) enqueue_border_factor<float>(c);
+          else enqueue_border_factor<double>(c);
+          HIP_TRY(hipGetLastError());
+          HIP_TRY(hipMemcpyAsync(h_err, c->s_err.p, sizeof(h_err), hipMemcpyDeviceToHost, c->stream));
+        }
+        HIP_TRY(hipEventRecord(e1, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        errbits = h_err[0];
+        break;
+      }
+      if (errbits & kErrBoxCap) {
+        if (c->ncap >= kMaxBoxDofs) {
+          (void)hipEventDestroy(e0);
+          (void)hipEventDestroy(e1);
+          return fail(c, IEGPU_ERR_CAPACITY,
+                      "a box's active DOF count exceeds " + std::to_string(kMaxBoxDofs));
+        }
+        c->ncap = std::min(kMaxBoxDofs, c->ncap * 2);
+      }
+      if (errbits & kErrIdPool) c->id_cap *= 2;
+      if (errbits & kErrRowPool) c->row_cap *= 2;
+      if (errbits & kErrFacPool) c->fac_cap *= 2;
+      if (attempt == 7) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return fail(c, IEGPU_ERR_CAPACITY, "workspace growth did not converge");
+      }
+    }
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    c->t_factor_ms = ms;
+    int64_t top = 0;
+    HIP_TRY(hipMemcpy(&top, c->s_fac_top.p, sizeof(top), hipMemcpyDeviceToHost));
+    c->fac_bytes = (double)top * c->elem();
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)maxn;
+    if (errbits & kErrSingular) {
+      c->factored = false;
+      return fail(c, IEGPU_ERR_SINGULAR, "exact zero pivot in an LU factorization");
+    }
+    if (c->hole_dim > 0) {
+      // skel_factorization.cpp:209-212: every DOF is a skeleton DOF.
+      int nroot = 0;
+      HIP_TRY(hipMemcpy(&nroot, c->s_act_cnt.p, sizeof(int), hipMemcpyDeviceToHost));
+      if (nroot == nd) {
+        c->factored = false;
+        return fail(c, IEGPU_ERR_NO_COMPRESSION, "No compression possible");
+      }
+    }
+    c->factored = true;
+  } catch (const HipError& e) {
+    c->factored = false;
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_solve_device(iegpu_ctx* c, const double* d_b, double* d_mu,
+                       double* d_alpha, void* stream) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!c->factored) return fail(c, IEGPU_ERR_INVALID, "factor first");
+  if (!d_b || !d_mu) return fail(c, IEGPU_ERR_INVALID, "null vector");
+  if (c->hole_dim > 0 && !d_alpha)
+    return fail(c, IEGPU_ERR_INVALID, "alpha required with holes");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    const int64_t nd = c->ndof();
+    if (d_b != d_mu) launch_copy(d_mu, d_b, nd, st);
+    if (c->precision == IEGPU_FP32) enqueue_solve<float>(c, d_mu, d_alpha, st);
+    else enqueue_solve<double>(c, d_mu, d_alpha, st);
+    HIP_TRY(hipGetLastError());
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_solve(iegpu_ctx* c, const double* b, double* mu, double* alpha) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!c->factored) return fail(c, IEGPU_ERR_INVALID, "factor first");
+  if (!b || !mu) return fail(c, IEGPU_ERR_INVALID, "null vector");
+  if (c->hole_dim > 0 && !alpha)
+    return fail(c, IEGPU_ERR_INVALID, "alpha required with holes");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t nd = c->ndof();
+    c->d_work.alloc(nd);
+    HIP_TRY(hipMemcpyAsync(c->d_work.p, b, nd * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    double* da = c->hole_dim > 0 ? c->d_alpha.p : nullptr;
+    if (c->precision == IEGPU_FP32) enqueue_solve<float>(c, c->d_work.p, da, c->stream);
+    else enqueue_solve<double>(c, c->d_work.p, da, c->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    HIP_TRY(hipMemcpyAsync(mu, c->d_work.p, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (da) HIP_TRY(hipMemcpyAsync(alpha, da, c->hole_dim * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    c->t_solve_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_tree_info(const iegpu_ctx* c, int* n_levels, int* n_boxes, int* ndof,
+                    int* hole_dim) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (n_levels) *n_levels = c->have_tree ? c->tree.n_levels : 0;
+  if (n_boxes) *n_boxes = c->have_tree ? c->tree.n_boxes : 0;
+  if (ndof) *ndof = c->ndof();
+  if (hole_dim) *hole_dim = c->hole_dim;
+  return IEGPU_OK;
+}
+
+int iegpu_box_info(iegpu_ctx* c, int* level, int* parent, int* is_leaf,
+                   int* children, int* n_active, int* rank, int* m_rows) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!c->have_tree) return fail(c, IEGPU_ERR_INVALID, "build_tree first");
+  const HostTree& T = c->tree;
+  const int nb = T.n_boxes;
+  std::vector<int> act(nb, 0), rk(nb, 0), mr(nb, 0);
+  if (c->factored && (n_active || rank || m_rows)) {
+    try {
+      HIP_TRY(hipSetDevice(c->device));
+      HIP_TRY(hipMemcpy(act.data(), c->s_act_cnt.p, nb * sizeof(int), hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(rk.data(), c->s_rank.p, nb * sizeof(int), hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(mr.data(), c->s_m_rows.p, nb * sizeof(int), hipMemcpyDeviceToHost));
+    } catch (const HipError& e) {
+      return from_hip(c, e);
+    }
+  }
+  for (int id = 0; id < nb; id++) {
+    const int s = T.slot_of_id[id];
+    if (level) level[id] = T.level[s];
+    if (parent) parent[id] = T.parent[s] >= 0 ? T.box_id[T.parent[s]] : -1;
+    if (is_leaf) is_leaf[id] = T.leaf[s];
+    if (children)
+      for (int q = 0; q < 4; q++) {
+        const int ch = T.child[4 * s + q];
+        children[4 * id + q] = ch >= 0 ? T.box_id[ch] : -1;
+      }
+    if (n_active) n_active[id] = act[s];
+    if (rank) rank[id] = rk[s];
+    if (m_rows) m_rows[id] = mr[s];
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_box_list(iegpu_ctx* c, int box, int which, int* out, int cap) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!c->have_tree) return -fail(c, IEGPU_ERR_INVALID, "build_tree first");
+  const HostTree& T = c->tree;
+  std::vector<int> v;
+  if (which == 5) {
+    if (box < 0 || box >= T.n_levels) return -fail(c, IEGPU_ERR_INVALID, "bad level");
+    for (int s = T.level_start[box]; s < T.level_start[box + 1]; s++) v.push_back(T.box_id[s]);
+  } else {
+    if (box < 0 || box >= T.n_boxes) return -fail(c, IEGPU_ERR_INVALID, "bad box");
+    const int s = T.slot_of_id[box];
+    if (which == 0) {
+      v.assign(T.orig_dofs.begin() + T.orig_off[s],
+               T.orig_dofs.begin() + T.orig_off[s] + T.orig_cnt[s]);
+    } else if (which == 1) {
+      for (int t = 0; t < T.nbr_cnt[s]; t++) v.push_back(T.box_id[T.nbr[T.nbr_off[s] + t]]);
+    } else if (which >= 2 && which <= 4) {
+      if (!c->factored) return -fail(c, IEGPU_ERR_INVALID, "factor first");
+      try {
+        HIP_TRY(hipSetDevice(c->device));
+        int n = 0, k = 0;
+        HIP_TRY(hipMemcpy(&n, c->s_act_cnt.p + s, sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&k, c->s_rank.p + s, sizeof(int), hipMemcpyDeviceToHost));
+        const int64_t base = (int64_t)s * c->ncap;
+        if (which == 2) {
+          v.resize(n);
+          if (n) HIP_TRY(hipMemcpy(v.data(), c->s_act_list.p + base, n * sizeof(int), hipMemcpyDeviceToHost));
+        } else if (which == 3) {
+          v.resize(k);
+          if (k) HIP_TRY(hipMemcpy(v.data(), c->s_out_list.p + base, k * sizeof(int), hipMemcpyDeviceToHost));
+        } else {
+          const int r = k ? n - k : 0;
+          v.resize(r);
+          if (r) HIP_TRY(hipMemcpy(v.data(), c->s_red_list.p + base, r * sizeof(int), hipMemcpyDeviceToHost));
+        }
+      } catch (const HipError& e) {
+        return -from_hip(c, e);
+      }
+    } else {
+      return -fail(c, IEGPU_ERR_INVALID, "bad list selector");
+    }
+  }
+  if (out)
+    for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = v[i];
+  return (int)v.size();
+}
+
+int iegpu_last_timings(const iegpu_ctx* c, double* t, int nt) {
+  if (!c || !t) return IEGPU_ERR_INVALID;
+  const double v[4] = {c->t_factor_ms, c->t_solve_ms, c->t_root_ms, c->fac_bytes};
+  for (int i = 0; i < nt && i < 4; i++) t[i] = v[i];
+  return IEGPU_OK;
+}
+
+int iegpu_set_profiling(iegpu_ctx* c, int on) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    c->prof.reset();
+    c->prof.on = on != 0;
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_profile(iegpu_ctx* c, int stage, int level, double* ms,
+                  int* launches) {
+  if (!c || !ms || !launches) return IEGPU_ERR_INVALID;
+  if (stage < 0 || stage >= kProfStages || level >= kProfLevels)
+    return fail(c, IEGPU_ERR_INVALID, "bad stage or level");
+  if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    c->prof.resolve();
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  double t = 0;
+  int n = 0;
+  for (int l = 0; l < kProfLevels; l++) {
+    if (level >= 0 && l != level) continue;
+    t += c->prof.ms[stage][l];
+    n += c->prof.launches[stage][l];
+  }
+  *ms = t;
+  *launches = n;
+  return IEGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,262 @@
+// Device-side data layout and helpers for the RS factor + solve kernels
+// (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace iegpu {
+
+constexpr int kNumProxy = 128;          // ie_solver/kernel/kernel.h:11
+constexpr double kRadiusRatio = 1.5;    // ie_solver/kernel/kernel.h:12
+constexpr int kMinCompress = 16;        // skel_factorization.h:11
+constexpr int kMaxBoxDofs = 1024;       // hard cap on |active| of one box
+constexpr int kWave = 64;
+
+enum : int { kPdeLaplace = 0, kPdeStokes = 2 };
+
+// Error bits raised by device code (read back once after a factorization).
+enum : int {
+  kErrBoxCap = 1,      // a box's active list exceeded the per-box capacity
+  kErrIdPool = 2,      // ID-matrix scratch pool exhausted
+  kErrFacPool = 4,     // factor pool exhausted
+  kErrSingular = 8,    // exact zero pivot in an LU
+  kErrRowPool = 16,    // row-index pool exhausted
+};
+
+struct DevGeom {
+  const double* pts;   // 2 per point
+  const double* nrm;   // 2 per point
+  const double* curv;  // 1 per point
+  const double* wts;   // 1 per point
+  int n_points;
+  int sol_dim;
+  int pde;
+};
+
+// Static tree, slot-indexed (level-major).
+struct DevTree {
+  int n_boxes;
+  int n_levels;
+  const int* level;
+  const int* leaf;
+  const int* child;  // 4 per slot
+  const double* cx;
+  const double* cy;
+  const double* side;
+  const int* orig_off;
+  const int* orig_cnt;
+  const int* orig_dofs;
+  const int* src_off;
+  const int* src_cnt;
+  const int* src;
+};
+
+// Per-box state produced by the factorization.
+struct DevState {
+  int ncap;            // per-box list capacity
+  int* act_cnt;        // |active|
+  int* act_list;       // ncap per box
+  int* chpos;          // 4 per box: position of each child's block in active
+  int* out_cnt;        // skeleton count if compressed, else |active|
+  int* out_list;       // ncap per box: skeleton (pivot order) or active
+  int* red_list;       // ncap per box: redundant DOFs
+  int* perm;           // ncap per box: pivot order (positions into active)
+  int* rank;           // k (0: not compressed)
+  int* upd_kind;       // 0: no carried update, 1: carried update stored
+  int* m_rows;         // ID matrix height
+  int64_t* id_sz;      // scratch doubles/floats needed (per box, level-local)
+  int64_t* row_sz;
+  int64_t* fac_sz;
+  int64_t* id_off;
+  int64_t* row_off;
+  int64_t* fac_off;
+  void* id_pool;
+  int* row_pool;
+  void* fac_pool;
+  int64_t id_cap;
+  int64_t row_cap;
+  int64_t fac_cap;
+  int64_t* fac_top;    // running factor-pool offset across levels
+  int* err;            // [0] error bits, [1] max |active| seen
+  void* root_mat;      // ncap*ncap
+  int* root_piv;       // ncap
+};
+
+// Offsets of the factor blocks inside one compressed box's factor slot.
+// Layout: T (k x r) | L (k x r) | U (r x k) | X_rr LU (r x r) | piv (r) |
+//         carried Schur block (k x k).  Column-major throughout.
+struct FacLayout {
+  int64_t T, L, U, X, piv, S;
+  __host__ __device__ static FacLayout make(int k, int r) {
+    FacLayout f;
+    f.T = 0;
+    f.L = (int64_t)k * r;
+    f.U = 2 * (int64_t)k * r;
+    f.X = 3 * (int64_t)k * r;
+    f.piv = f.X + (int64_t)r * r;
+    f.S = f.piv + r;
+    return f;
+  }
+};
+
+// Upper bound of a box's factor slot from |active| alone:
+// 3kr + r^2 + r + k^2 <= n^2 + n^2/4 + n for k + r = n.
+__host__ __device__ inline int64_t fac_slot_size(int n) {
+  const int64_t nn = (int64_t)n * n;
+  int64_t s = nn + nn / 4 + n + 8;
+  return (s + 7) & ~int64_t(7);
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline int wave_sum_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Laplace double layer, ie_solver/kernel/kernel.cpp:123-165.
+__device__ inline double laplace_entry(const DevGeom& g, int t, int s) {
+  const double scale = 1.0 / (2 * M_PI);
+  const double sp1 = g.pts[2 * s], sp2 = g.pts[2 * s + 1];
+  const double tp1 = g.pts[2 * t], tp2 = g.pts[2 * t + 1];
+  const double sw = g.wts[s];
+  if (tp1 == sp1 && tp2 == sp2) return 0.5 + 0.5 * g.curv[s] * sw * scale;
+  const double r0 = tp1 - sp1, r1 = tp2 - sp2;
+  return -sw * scale * (r0 * g.nrm[2 * s] + r1 * g.nrm[2 * s + 1]) /
+         (r0 * r0 + r1 * r1);
+}
+
+// Stokes velocity double layer (2 DOFs per point, DOF d -> point d/2,
+// component d%2), ie_solver/kernel/kernel.cpp:212-290.
+__device__ inline double stokes_entry(const DevGeom& g, int t, int s) {
+  const double scale = 1.0 / M_PI;
+  const int tp = t >> 1, sp = s >> 1, a = t & 1, b = s & 1;
+  const double sp1 = g.pts[2 * sp], sp2 = g.pts[2 * sp + 1];
+  const double tp1 = g.pts[2 * tp], tp2 = g.pts[2 * tp + 1];
+  const double sn1 = g.nrm[2 * sp], sn2 = g.nrm[2 * sp + 1];
+  const double tna = g.nrm[2 * tp + a];
+  const double snb = b ? sn2 : sn1;
+  const double sw = g.wts[sp];
+  if (tp1 == sp1 && tp2 == sp2) {
+    const double pot = -0.5 * g.curv[sp] * sw * scale;
+    if (a == 0 && b == 0) return -0.5 + pot * sn2 * sn2 + sw * tna * sn1;
+    if (a == 1 && b == 1) return -0.5 + pot * sn1 * sn1 + sw * tna * sn2;
+    return -pot * sn1 * sn2 + sw * g.nrm[2 * tp + 1] * sn1;
+  }
+  const double r0 = tp1 - sp1, r1 = tp2 - sp2;
+  const double d2 = r0 * r0 + r1 * r1;
+  const double pot = sw * scale * (r0 * sn1 + r1 * sn2) / (d2 * d2);
+  const double ra = a ? r1 : r0, rb = b ? r1 : r0;
+  return pot * ra * rb + sw * tna * snb;
+}
+
+__device__ inline double kernel_entry(const DevGeom& g, int t, int s) {
+  return g.pde == kPdeStokes ? stokes_entry(g, t, s) : laplace_entry(g, t, s);
+}
+
+// Proxy point p of the circle radius r about (cx, cy):
+// ie_solver/kernel/kernel.cpp:430-441.
+__device__ inline void proxy_point(double cx, double cy, double r, int p,
+                                   double* px, double* py, double* n1,
+                                   double* n2) {
+  const double ang = 2 * M_PI * p * (1.0 / kNumProxy);
+  double sn, cs;
+  sincos(ang, &sn, &cs);
+  *px = cx + r * cs;
+  *py = cy + r * sn;
+  *n1 = cs;
+  *n2 = sn;
+}
+
+// Entry of the proxy block (rows 0..2P-1) at box DOF `d`
+// (ie_solver/kernel/kernel.cpp:459-591).  P = 128 (Laplace) or 256 (Stokes).
+__device__ inline double proxy_entry(const DevGeom& g, double cx, double cy,
+                                     double r, int row, int d) {
+  if (g.pde != kPdeStokes) {
+    const double scale = 1.0 / (2 * M_PI);
+    const bool to_proxy = row < kNumProxy;
+    const int p = to_proxy ? row : row - kNumProxy;
+    double px, py, n1, n2;
+    proxy_point(cx, cy, r, p, &px, &py, &n1, &n2);
+    const double bx = g.pts[2 * d], by = g.pts[2 * d + 1];
+    if (to_proxy) {
+      const double r0 = px - bx, r1 = py - by;
+      return -g.wts[d] * scale * (r0 * g.nrm[2 * d] + r1 * g.nrm[2 * d + 1]) /
+             (r0 * r0 + r1 * r1);
+    }
+    const double pw = 2.0 * M_PI * r / kNumProxy;
+    const double r0 = bx - px, r1 = by - py;
+    return -pw * scale * (r0 * n1 + r1 * n2) / (r0 * r0 + r1 * r1);
+  }
+  const double scale = 1.0 / M_PI;
+  const int P = 2 * kNumProxy;
+  const bool to_proxy = row < P;
+  const int rr = to_proxy ? row : row - P;
+  const int p = rr >> 1, c = rr & 1;
+  double px, py, n1, n2;
+  proxy_point(cx, cy, r, p, &px, &py, &n1, &n2);
+  const int bp = d >> 1, bc = d & 1;
+  const double bx = g.pts[2 * bp], by = g.pts[2 * bp + 1];
+  const double bn1 = g.nrm[2 * bp], bn2 = g.nrm[2 * bp + 1];
+  if (to_proxy) {  // box DOF (source comp bc) -> proxy (target comp c)
+    const double sw = g.wts[bp];
+    const double r0 = px - bx, r1 = py - by;
+    const double d2 = r0 * r0 + r1 * r1;
+    const double pot = sw * scale * (r0 * bn1 + r1 * bn2) / (d2 * d2);
+    const double ra = c ? r1 : r0, rb = bc ? r1 : r0;
+    const double tn = c ? n2 : n1, sn = bc ? bn2 : bn1;
+    return pot * ra * rb + sw * tn * sn;
+  }
+  // proxy (source comp c) -> box DOF (target comp bc)
+  const double pw = 2.0 * M_PI * r / kNumProxy;
+  const double r0 = bx - px, r1 = by - py;
+  const double d2 = r0 * r0 + r1 * r1;
+  const double pot = pw * scale * (r0 * n1 + r1 * n2) / (d2 * d2);
+  const double ra = bc ? r1 : r0, rb = c ? r1 : r0;
+  const double tn = bc ? bn2 : bn1, sn = c ? n2 : n1;
+  return pot * ra * rb + pw * tn * sn;
+}
+
+// Inner-circle test of ie_solver/kernel/kernel.cpp:384-394, evaluated
+// without fused multiply-adds so that the comparison sees the same
+// rounded distance as the host.
+__device__ inline bool in_proxy_circle(const DevGeom& g, int dof, double cx,
+                                       double cy, double side) {
+#pragma clang fp contract(off)
+  const int p = dof / g.sol_dim;
+  const double dx = cx - g.pts[2 * p], dy = cy - g.pts[2 * p + 1];
+  const double d = sqrt(dx * dx + dy * dy);
+  return d < kRadiusRatio * side;
+}
+
+// DOF list a source box contributes to ID rows: a leaf's original DOFs,
+// otherwise its current active DOFs (quadtree.cpp:1014-1022).
+struct DofList {
+  const int* p;
+  int n;
+};
+__device__ inline DofList source_dofs(const DevTree& t, const DevState& s,
+                                      int box) {
+  DofList d;
+  if (t.leaf[box]) {
+    d.p = t.orig_dofs + t.orig_off[box];
+    d.n = t.orig_cnt[box];
+  } else {
+    d.p = s.act_list + (int64_t)box * s.ncap;
+    d.n = s.act_cnt[box];
+  }
+  return d;
+}
+
+}  // namespace iegpu

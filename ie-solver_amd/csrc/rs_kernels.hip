@@ -1,0 +1,1056 @@
+// RS factorization and solve kernels for gfx950 (CDNA4, wave64).
+//
+// One workgroup per box of a tree level; levels run bottom-up (factor,
+// upward sweep) or top-down (downward sweep) as stream-ordered launches,
+// so the whole factorization is a fixed launch sequence with no host
+// synchronisation (capturable in a HIP graph).
+//
+// Reference stages (paths relative to the reference root):
+//   k_active    quadtree.cpp:1028-1074 (active = children's skeletons)
+//   k_rows      kernel.cpp:293-423     (ID row count: near field + proxy)
+//   k_assemble  kernel.cpp:123-591     (ID matrix assembly)
+//   k_qrcp      ie_mat.cpp:548-588     (dgeqp3 rank cut + dtrtrs)
+//   k_decouple  skel_factorization.cpp:77-128, 323-381
+//   k_root_*    skel_factorization.cpp:172-186
+//   k_sweep_*   skel_factorization.cpp:388-426, 507-592
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "rs_device.hpp"
+#include "rs_kernels.hpp"
+
+namespace iegpu {
+
+namespace {
+
+template <typename R>
+struct Eps;
+template <>
+struct Eps<double> {
+  // LAPACK dlamch('Epsilon') = 2^-53; dlaqp2 recomputes a column norm when
+  // the downdated estimate falls below sqrt of it.
+  static constexpr double tol3z = 1.0536712127723509e-08;
+};
+template <>
+struct Eps<float> {
+  static constexpr float tol3z = 2.44140625e-04f;
+};
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// ---------------------------------------------------------------------------
+// Level bookkeeping
+// ---------------------------------------------------------------------------
+
+__global__ void k_reset(DevState s) {
+  if (threadIdx.x == 0) {
+    s.err[0] = 0;
+    s.err[1] = 0;
+    *s.fac_top = 0;
+  }
+}
+
+// active(box) = concatenation of the children's outgoing lists (skeleton if
+// compressed, active otherwise) in child order bl, tl, tr, br; a leaf's
+// active list is its original DOF list.
+__global__ __launch_bounds__(64) void k_active(DevTree t, DevState s,
+                                               int start) {
+  const int b = start + blockIdx.x;
+  const int lane = threadIdx.x;
+  int* dst = s.act_list + (int64_t)b * s.ncap;
+  int n = 0;
+  if (t.leaf[b]) {
+    n = t.orig_cnt[b];
+    if (n > s.ncap) {
+      if (lane == 0) atomicOr(s.err, kErrBoxCap);
+      n = 0;
+    }
+    const int* src = t.orig_dofs + t.orig_off[b];
+    for (int i = lane; i < n; i += 64) dst[i] = src[i];
+  } else {
+    int pos = 0;
+    for (int q = 0; q < 4; q++) {
+      const int c = t.child[4 * b + q];
+      const int cnt = s.out_cnt[c];
+      if (lane == 0) s.chpos[4 * b + q] = pos;
+      if (pos + cnt <= s.ncap) {
+        const int* src = s.out_list + (int64_t)c * s.ncap;
+        for (int i = lane; i < cnt; i += 64) dst[pos + i] = src[i];
+      }
+      pos += cnt;
+    }
+    n = pos;
+    if (n > s.ncap) {
+      if (lane == 0) atomicOr(s.err, kErrBoxCap);
+      n = 0;
+    }
+  }
+  if (lane == 0) {
+    s.act_cnt[b] = n;
+    atomicMax(s.err + 1, n);
+  }
+}
+
+// Visits the ID row DOFs of box b in reference order (source boxes in
+// order, each list in order, filtered by the proxy circle at level >= 3).
+// Called by one full wave; `emit(pos, dof)` receives the compacted index.
+template <typename F>
+__device__ int for_each_row_dof(const DevGeom& g, const DevTree& t,
+                                const DevState& s, int b, bool filter,
+                                F emit) {
+  const int lane = threadIdx.x & 63;
+  const double cx = t.cx[b], cy = t.cy[b], side = t.side[b];
+  int cnt = 0;
+  const int so = t.src_off[b], sc = t.src_cnt[b];
+  for (int u = 0; u < sc; u++) {
+    const DofList L = source_dofs(t, s, t.src[so + u]);
+    for (int base = 0; base < L.n; base += 64) {
+      const int i = base + lane;
+      int dof = -1;
+      bool keep = false;
+      if (i < L.n) {
+        dof = L.p[i];
+        keep = !filter || in_proxy_circle(g, dof, cx, cy, side);
+      }
+      const uint64_t mask = __ballot(keep);
+      const uint64_t below = (lane == 0) ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+      if (keep) emit(cnt + __popcll(below), dof);
+      cnt += __popcll(mask);
+    }
+  }
+  return cnt;
+}
+
+__device__ inline int proxy_rows(const DevGeom& g, int level) {
+  if (level < 3) return 0;
+  return (g.pde == kPdeStokes) ? 4 * kNumProxy : 2 * kNumProxy;
+}
+
+__global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
+                                             int start, int level) {
+  const int b = start + blockIdx.x;
+  const int n = s.act_cnt[b];
+  int m = 0;
+  if (n >= kMinCompress) {
+    const int cnt =
+        for_each_row_dof(g, t, s, b, level >= 3, [](int, int) {});
+    m = 2 * cnt + proxy_rows(g, level);
+  }
+  if (threadIdx.x == 0) {
+    s.m_rows[b] = m;
+    const int64_t ld = m > 2 * n ? m : 2 * n;
+    s.id_sz[b] = (m > 0) ? ld * n : 0;
+    s.row_sz[b] = (m > 0) ? (m - proxy_rows(g, level)) / 2 : 0;
+    s.fac_sz[b] = (n > 0) ? fac_slot_size(n) : 0;
+  }
+}
+
+// Exclusive scans of the per-box sizes of one level (single workgroup).
+// ID scratch and row lists are level-local; factor slots append to the
+// running pool offset.
+__global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
+                                               int count) {
+  __shared__ int64_t sh[3][1024];
+  const int tid = threadIdx.x;
+  const int chunk = (count + 1023) / 1024;
+  const int lo = min(count, tid * chunk), hi = min(count, lo + chunk);
+  int64_t a = 0, b = 0, c = 0;
+  for (int i = lo; i < hi; i++) {
+    a += s.id_sz[start + i];
+    b += s.row_sz[start + i];
+    c += s.fac_sz[start + i];
+  }
+  sh[0][tid] = a;
+  sh[1][tid] = b;
+  sh[2][tid] = c;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int64_t x0 = 0, x1 = 0, x2 = 0;
+    if (tid >= o) {
+      x0 = sh[0][tid - o];
+      x1 = sh[1][tid - o];
+      x2 = sh[2][tid - o];
+    }
+    __syncthreads();
+    sh[0][tid] += x0;
+    sh[1][tid] += x1;
+    sh[2][tid] += x2;
+    __syncthreads();
+  }
+  const int64_t fbase = *s.fac_top;
+  int64_t oa = sh[0][tid] - a, ob = sh[1][tid] - b, oc = sh[2][tid] - c;
+  for (int i = lo; i < hi; i++) {
+    s.id_off[start + i] = oa;
+    s.row_off[start + i] = ob;
+    s.fac_off[start + i] = fbase + oc;
+    oa += s.id_sz[start + i];
+    ob += s.row_sz[start + i];
+    oc += s.fac_sz[start + i];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const int64_t ta = sh[0][1023], tb = sh[1][1023], tc = sh[2][1023];
+    int e = 0;
+    if (ta > s.id_cap) e |= kErrIdPool;
+    if (tb > s.row_cap) e |= kErrRowPool;
+    if (fbase + tc > s.fac_cap) e |= kErrFacPool;
+    if (e) atomicOr(s.err, e);
+    *s.fac_top = fbase + tc;
+  }
+}
+
+__device__ inline bool pools_ok(const DevState& s) {
+  return (s.err[0] & (kErrIdPool | kErrRowPool | kErrFacPool | kErrBoxCap)) == 0;
+}
+
+// ---------------------------------------------------------------------------
+// ID matrix assembly: [K(ic, box); K(box, ic)^T; proxy] (m x n,
+// column-major, ld m), kernel.cpp:293-423.  Consecutive threads write
+// consecutive rows of a column.
+// ---------------------------------------------------------------------------
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_assemble(DevGeom g, DevTree t,
+                                                  DevState s, int start,
+                                                  int level) {
+  const int b = start + blockIdx.x;
+  const int m = s.m_rows[b];
+  if (m == 0 || !pools_ok(s)) return;
+  const int n = s.act_cnt[b];
+  const int tid = threadIdx.x;
+  const int* act = s.act_list + (int64_t)b * s.ncap;
+  int* rows = s.row_pool + s.row_off[b];
+  R* A = reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
+  const int P = proxy_rows(g, level);
+  const int cnt = (m - P) / 2;
+  if (tid < 64)
+    for_each_row_dof(g, t, s, b, level >= 3,
+                     [&](int pos, int dof) { rows[pos] = dof; });
+  __shared__ double pxy[4][kNumProxy];
+  const double cx = t.cx[b], cy = t.cy[b];
+  const double rad = t.side[b] * kRadiusRatio;
+  if (P > 0 && tid < kNumProxy) {
+    double px, py, n1, n2;
+    proxy_point(cx, cy, rad, tid, &px, &py, &n1, &n2);
+    pxy[0][tid] = px;
+    pxy[1][tid] = py;
+    pxy[2][tid] = n1;
+    pxy[3][tid] = n2;
+  }
+  __syncthreads();
+  const double pw = 2.0 * M_PI * rad / kNumProxy;
+  for (int j = 0; j < n; j++) {
+    const int sd = act[j];
+    R* col = A + (int64_t)m * j;
+    for (int i = tid; i < m; i += 256) {
+      double v;
+      if (i < cnt) {
+        v = kernel_entry(g, rows[i], sd);
+      } else if (i < 2 * cnt) {
+        v = kernel_entry(g, sd, rows[i - cnt]);
+      } else {
+        const int row = i - 2 * cnt;
+        if (g.pde != kPdeStokes) {
+          const double scale = 1.0 / (2 * M_PI);
+          const double bx = g.pts[2 * sd], by = g.pts[2 * sd + 1];
+          if (row < kNumProxy) {
+            const double r0 = pxy[0][row] - bx, r1 = pxy[1][row] - by;
+            v = -g.wts[sd] * scale * (r0 * g.nrm[2 * sd] + r1 * g.nrm[2 * sd + 1]) /
+                (r0 * r0 + r1 * r1);
+          } else {
+            const int p = row - kNumProxy;
+            const double r0 = bx - pxy[0][p], r1 = by - pxy[1][p];
+            v = -pw * scale * (r0 * pxy[2][p] + r1 * pxy[3][p]) / (r0 * r0 + r1 * r1);
+          }
+        } else {
+          const double scale = 1.0 / M_PI;
+          const int PP = 2 * kNumProxy;
+          const bool to_proxy = row < PP;
+          const int rr = to_proxy ? row : row - PP;
+          const int p = rr >> 1, c = rr & 1;
+          const double px = pxy[0][p], py = pxy[1][p], n1 = pxy[2][p], n2 = pxy[3][p];
+          const int bp = sd >> 1, bc = sd & 1;
+          const double bx = g.pts[2 * bp], by = g.pts[2 * bp + 1];
+          const double bn1 = g.nrm[2 * bp], bn2 = g.nrm[2 * bp + 1];
+          if (to_proxy) {
+            const double sw = g.wts[bp];
+            const double r0 = px - bx, r1 = py - by;
+            const double d2 = r0 * r0 + r1 * r1;
+            const double pot = sw * scale * (r0 * bn1 + r1 * bn2) / (d2 * d2);
+            v = pot * (c ? r1 : r0) * (bc ? r1 : r0) + sw * (c ? n2 : n1) * (bc ? bn2 : bn1);
+          } else {
+            const double r0 = bx - px, r1 = by - py;
+            const double d2 = r0 * r0 + r1 * r1;
+            const double pot = pw * scale * (r0 * n1 + r1 * n2) / (d2 * d2);
+            v = pot * (bc ? r1 : r0) * (c ? r1 : r0) + pw * (bc ? bn2 : bn1) * (c ? n2 : n1);
+          }
+        }
+      }
+      col[i] = static_cast<R>(v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Interpolative decomposition by column-pivoted Householder QR, stopped at
+// the first |R_jj| < tol |R_00| (ie_mat.cpp:548-588).  The pivot rule,
+// Householder form and column-norm downdating follow LAPACK dgeqp3/dlaqp2.
+// Each wave owns a strided set of (logical) columns; column swaps are
+// index swaps.  The matrix lives in LDS when it fits, else in the box's
+// scratch slot.
+// ---------------------------------------------------------------------------
+
+template <typename R, int NT>
+__global__ __launch_bounds__(NT) void k_qrcp(DevState s, int start, double tol_d) {
+  constexpr int NW = NT / 64;
+  constexpr int kLdsElems = 131072 / sizeof(R);
+  __shared__ R smat[kLdsElems];
+  __shared__ R vn1[kMaxBoxDofs];
+  __shared__ R vn2[kMaxBoxDofs];
+  __shared__ int col[kMaxBoxDofs];
+  __shared__ R sh_tau;
+  __shared__ int sh_stop;
+
+  const int b = start + blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int n = s.act_cnt[b], m = s.m_rows[b];
+  const int* act = s.act_list + (int64_t)b * s.ncap;
+  int* out = s.out_list + (int64_t)b * s.ncap;
+  if (m == 0 || n < kMinCompress || !pools_ok(s)) {
+    for (int i = tid; i < n; i += NT) out[i] = act[i];
+    if (tid == 0) {
+      s.rank[b] = 0;
+      s.out_cnt[b] = n;
+    }
+    return;
+  }
+  const R tol = static_cast<R>(tol_d);
+  R* A = reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
+  const int64_t mn = (int64_t)m * n;
+  if (mn <= kLdsElems) {
+    for (int64_t i = tid; i < mn; i += NT) smat[i] = A[i];
+    A = smat;
+  }
+  for (int c = tid; c < n; c += NT) col[c] = c;
+  __syncthreads();
+  for (int c = wv; c < n; c += NW) {
+    const R* a = A + (int64_t)m * c;
+    R ss = 0;
+    for (int i = lane; i < m; i += 64) ss += a[i] * a[i];
+    ss = wave_sum(ss);
+    if (lane == 0) {
+      vn1[c] = sqrt(ss);
+      vn2[c] = vn1[c];
+    }
+  }
+  __syncthreads();
+
+  const int kmax = min(m, n);
+  int k = 0;
+  R thresh = 0;
+  for (int j = 0; j < kmax; j++) {
+    if (wv == 0) {
+      R best = -1;
+      int bi = n;
+      for (int q = j + lane; q < n; q += 64) {
+        const R v = vn1[q];
+        if (v > best) {
+          best = v;
+          bi = q;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const R ob = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (bi >= n) bi = j;  // all-NaN column norms: keep the current column
+      const int cj = col[j], cb = col[bi];
+      const R v1 = vn1[j], v2 = vn2[j];
+      wave_sync();
+      if (lane == 0 && bi != j) {
+        col[j] = cb;
+        col[bi] = cj;
+        vn1[bi] = v1;
+        vn2[bi] = v2;
+      }
+      const int pc = (bi != j) ? cb : cj;
+      R* v = A + (int64_t)m * pc;
+      const R alpha = v[j];
+      R ss = 0;
+      for (int i = j + 1 + lane; i < m; i += 64) ss += v[i] * v[i];
+      ss = wave_sum(ss);
+      const R xnorm = sqrt(ss);
+      R beta, tau;
+      if (xnorm == 0) {
+        tau = 0;
+        beta = alpha;
+      } else {
+        beta = -copysign(hypot(alpha, xnorm), alpha);
+        tau = (beta - alpha) / beta;
+        const R sc = 1 / (alpha - beta);
+        for (int i = j + 1 + lane; i < m; i += 64) v[i] *= sc;
+      }
+      if (j == 0) thresh = fabs(tol * beta);
+      const int stop = (j >= 1 && fabs(beta) < thresh) ? 1 : 0;
+      if (lane == 0) {
+        if (!stop) v[j] = beta;
+        sh_tau = tau;
+        sh_stop = stop;
+      }
+    }
+    __syncthreads();
+    if (sh_stop) {
+      k = j;
+      break;
+    }
+    const R tau = sh_tau;
+    const R* v = A + (int64_t)m * col[j];
+    for (int q = j + 1 + wv; q < n; q += NW) {
+      R* cq = A + (int64_t)m * col[q];
+      R part = 0;
+      for (int i = j + 1 + lane; i < m; i += 64) part += v[i] * cq[i];
+      part = wave_sum(part);
+      const R w = cq[j] + part;
+      const R tw = tau * w;
+      for (int i = j + 1 + lane; i < m; i += 64) cq[i] -= tw * v[i];
+      const R rjc = cq[j] - tw;
+      wave_sync();
+      if (lane == 0) cq[j] = rjc;
+      const R v1 = vn1[q];
+      if (v1 != 0) {
+        R temp = fabs(rjc) / v1;
+        temp = 1 - temp * temp;
+        temp = temp > 0 ? temp : R(0);
+        const R rat = v1 / vn2[q];
+        const R temp2 = temp * rat * rat;
+        if (temp2 <= Eps<R>::tol3z) {
+          R ss2 = 0;
+          for (int i = j + 1 + lane; i < m; i += 64) ss2 += cq[i] * cq[i];
+          ss2 = wave_sum(ss2);
+          const R nv = (j < m - 1) ? sqrt(ss2) : R(0);
+          if (lane == 0) {
+            vn1[q] = nv;
+            vn2[q] = nv;
+          }
+        } else if (lane == 0) {
+          vn1[q] = v1 * sqrt(temp);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (k == 0) {
+    for (int i = tid; i < n; i += NT) out[i] = act[i];
+    if (tid == 0) {
+      s.rank[b] = 0;
+      s.out_cnt[b] = n;
+    }
+    return;
+  }
+  // T = R11^{-1} R12 (k x r), one thread per column, back substitution.
+  const int r = n - k;
+  R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
+  for (int c = tid; c < r; c += NT) {
+    const R* y = A + (int64_t)m * col[k + c];
+    R* tc = F + (int64_t)k * c;
+    for (int i = k - 1; i >= 0; i--) {
+      R acc = y[i];
+      for (int l = i + 1; l < k; l++) acc -= A[i + (int64_t)m * col[l]] * tc[l];
+      tc[i] = acc / A[i + (int64_t)m * col[i]];
+    }
+  }
+  int* red = s.red_list + (int64_t)b * s.ncap;
+  int* pm = s.perm + (int64_t)b * s.ncap;
+  for (int i = tid; i < n; i += NT) {
+    const int c = col[i];
+    pm[i] = c;
+    if (i < k) out[i] = act[c];
+    else red[i - k] = act[c];
+  }
+  if (tid == 0) {
+    s.rank[b] = k;
+    s.out_cnt[b] = k;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block LU with partial pivoting (LAPACK dgetrf pivot rule: first index of
+// the largest |entry|), in place, column-major with leading dimension ld.
+// ---------------------------------------------------------------------------
+
+template <typename R, int NT>
+__device__ int block_lu(R* X, int n, int ld, int* ipiv) {
+  __shared__ int sh_p;
+  const int tid = threadIdx.x, lane = tid & 63;
+  int info = 0;
+  for (int j = 0; j < n; j++) {
+    if (tid < 64) {
+      R best = -1;
+      int bi = n;
+      for (int i = j + lane; i < n; i += 64) {
+        const R v = fabs(X[i + (int64_t)ld * j]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const R ob = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (bi >= n) bi = j;  // NaN column: no interchange
+      if (lane == 0) {
+        sh_p = bi;
+        ipiv[j] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = sh_p;
+    if (p != j) {
+      for (int c = tid; c < n; c += NT) {
+        const R a = X[j + (int64_t)ld * c];
+        X[j + (int64_t)ld * c] = X[p + (int64_t)ld * c];
+        X[p + (int64_t)ld * c] = a;
+      }
+    }
+    __syncthreads();
+    const R piv = X[j + (int64_t)ld * j];
+    if (piv != 0) {
+      const R inv = 1 / piv;
+      for (int i = j + 1 + tid; i < n; i += NT) X[i + (int64_t)ld * j] *= inv;
+    } else if (info == 0) {
+      info = j + 1;
+    }
+    __syncthreads();
+    const int len = n - j - 1;
+    const int64_t tot = (int64_t)len * len;
+    for (int64_t idx = tid; idx < tot; idx += NT) {
+      const int i = j + 1 + (int)(idx % len);
+      const int c = j + 1 + (int)(idx / len);
+      X[i + (int64_t)ld * c] -= X[i + (int64_t)ld * j] * X[j + (int64_t)ld * c];
+    }
+    __syncthreads();
+  }
+  return info;
+}
+
+// Carried Schur block of child q at (local i, local j); 0 outside the
+// child's diagonal block.
+template <typename R>
+struct ChildUpdates {
+  const R* S[4];
+  int N[4];
+  const signed char* pch;
+  const short* ploc;
+  __device__ R operator()(int i, int j) const {
+    const int q = pch[i];
+    if (q < 0 || q != pch[j] || S[q] == nullptr) return R(0);
+    return S[q][ploc[i] + (int64_t)N[q] * ploc[j]];
+  }
+};
+
+template <typename R>
+__device__ const R* carried_block(const DevState& s, int c) {
+  if (!s.upd_kind[c]) return nullptr;
+  const R* F = reinterpret_cast<const R*>(s.fac_pool) + s.fac_off[c];
+  const int k = s.rank[c];
+  if (k == 0) return F;
+  return F + FacLayout::make(k, s.act_cnt[c] - k).S;
+}
+
+// ---------------------------------------------------------------------------
+// Decouple one box (skel_factorization.cpp:77-128): with s/r the skeleton
+// and redundant positions and K_BN = K(act, act) - (children's carried
+// Schur blocks),
+//   K_rs = K(r,s) - T^T K(s,s)
+//   X_rr = K(r,r) - T^T K(s,r) - K_rs T          (LU, partial pivoting)
+//   K_sr = K(s,r) - K(s,s) T
+//   L = K_sr X_rr^{-1},  U = X_rr^{-1} K_rs,  schur = L K_rs
+// and the box's carried block = (children's update on s,s) + schur.
+// Uncompressed boxes carry their children's blocks unchanged.
+// ---------------------------------------------------------------------------
+
+template <typename R, int NT>
+__global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
+                                                 DevState s, int start) {
+  constexpr int kLdsElems = 65536 / sizeof(R);
+  __shared__ R sbuf[kLdsElems];
+  __shared__ signed char pch[kMaxBoxDofs];
+  __shared__ short ploc[kMaxBoxDofs];
+  __shared__ short sperm[kMaxBoxDofs];
+  __shared__ int spiv[kMaxBoxDofs];
+  __shared__ const R* chS[4];
+  __shared__ int chN[4];
+
+  const int b = start + blockIdx.x;
+  const int tid = threadIdx.x;
+  const int n = s.act_cnt[b];
+  if (!pools_ok(s)) return;
+  if (n == 0) {
+    if (tid == 0) s.upd_kind[b] = 0;
+    return;
+  }
+  const bool leaf = t.leaf[b] != 0;
+  if (tid < 4) {
+    const int c = leaf ? -1 : t.child[4 * b + tid];
+    chS[tid] = (c >= 0) ? carried_block<R>(s, c) : nullptr;
+    chN[tid] = (c >= 0) ? s.out_cnt[c] : 0;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += NT) {
+    int q = -1, loc = 0;
+    if (!leaf) {
+      for (int u = 0; u < 4; u++) {
+        const int p0 = s.chpos[4 * b + u];
+        if (i >= p0 && i < p0 + chN[u]) {
+          q = u;
+          loc = i - p0;
+        }
+      }
+    }
+    pch[i] = (signed char)q;
+    ploc[i] = (short)loc;
+  }
+  __syncthreads();
+  ChildUpdates<R> upd;
+  bool any = false;
+  for (int u = 0; u < 4; u++) {
+    upd.S[u] = chS[u];
+    upd.N[u] = chN[u];
+    any = any || chS[u] != nullptr;
+  }
+  upd.pch = pch;
+  upd.ploc = ploc;
+  const int k = s.rank[b];
+  R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
+  if (k == 0) {
+    if (any) {
+      for (int64_t idx = tid; idx < (int64_t)n * n; idx += NT)
+        F[idx] = upd((int)(idx % n), (int)(idx / n));
+    }
+    if (tid == 0) s.upd_kind[b] = any ? 1 : 0;
+    return;
+  }
+  const int r = n - k;
+  const FacLayout fl = FacLayout::make(k, r);
+  const int* act = s.act_list + (int64_t)b * s.ncap;
+  for (int i = tid; i < n; i += NT) sperm[i] = (short)s.perm[(int64_t)b * s.ncap + i];
+  __syncthreads();
+  R* P = (2 * (int64_t)n * n <= kLdsElems)
+             ? sbuf
+             : reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
+  R* X = P + (int64_t)n * n;
+  R* Krs = X + (int64_t)r * r;
+  R* Ksr = Krs + (int64_t)r * k;
+  for (int64_t idx = tid; idx < (int64_t)n * n; idx += NT) {
+    const int ia = sperm[idx % n], ic = sperm[idx / n];
+    P[idx] = static_cast<R>(kernel_entry(g, act[ia], act[ic])) - upd(ia, ic);
+  }
+  __syncthreads();
+  const R* T = F + fl.T;
+  for (int idx = tid; idx < r * k; idx += NT) {
+    const int i = idx % r, a = idx / r;
+    R acc = P[(k + i) + (int64_t)n * a];
+    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + (int64_t)n * a];
+    Krs[idx] = acc;
+  }
+  for (int idx = tid; idx < k * r; idx += NT) {
+    const int a = idx % k, j = idx / k;
+    R acc = P[a + (int64_t)n * (k + j)];
+    for (int l = 0; l < k; l++) acc -= P[a + (int64_t)n * l] * T[l + k * j];
+    Ksr[idx] = acc;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < r * r; idx += NT) {
+    const int i = idx % r, j = idx / r;
+    R acc = P[(k + i) + (int64_t)n * (k + j)];
+    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + (int64_t)n * (k + j)];
+    for (int l = 0; l < k; l++) acc -= Krs[i + r * l] * T[l + k * j];
+    X[idx] = acc;
+  }
+  __syncthreads();
+  const int info = block_lu<R, NT>(X, r, r, spiv);
+  if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
+  R* FX = F + fl.X;
+  R* Fp = F + fl.piv;
+  for (int idx = tid; idx < r * r; idx += NT) FX[idx] = X[idx];
+  for (int i = tid; i < r; i += NT) Fp[i] = static_cast<R>(spiv[i]);
+  // U = X^{-1} K_rs, one thread per column (dgetrs 'N').
+  R* U = F + fl.U;
+  for (int a = tid; a < k; a += NT) {
+    R* u = U + (int64_t)r * a;
+    for (int i = 0; i < r; i++) u[i] = Krs[i + r * a];
+    for (int i = 0; i < r; i++) {
+      const int p = spiv[i];
+      if (p != i) {
+        const R tmp = u[i];
+        u[i] = u[p];
+        u[p] = tmp;
+      }
+    }
+    for (int l = 0; l < r; l++) {
+      const R ul = u[l];
+      for (int i = l + 1; i < r; i++) u[i] -= X[i + r * l] * ul;
+    }
+    for (int i = r - 1; i >= 0; i--) {
+      u[i] /= X[i + r * i];
+      const R ui = u[i];
+      for (int l = 0; l < i; l++) u[l] -= X[l + r * i] * ui;
+    }
+  }
+  // L = K_sr X^{-1}, one thread per row (dgetrs 'T' on the transpose).
+  R* Lm = F + fl.L;
+  for (int a = tid; a < k; a += NT) {
+    for (int i = 0; i < r; i++) {
+      R acc = Ksr[a + k * i];
+      for (int l = 0; l < i; l++) acc -= X[l + r * i] * Lm[a + k * l];
+      Lm[a + k * i] = acc / X[i + r * i];
+    }
+    for (int i = r - 1; i >= 0; i--) {
+      R acc = Lm[a + k * i];
+      for (int l = i + 1; l < r; l++) acc -= X[l + r * i] * Lm[a + k * l];
+      Lm[a + k * i] = acc;
+    }
+    for (int i = r - 1; i >= 0; i--) {
+      const int p = spiv[i];
+      if (p != i) {
+        const R tmp = Lm[a + k * i];
+        Lm[a + k * i] = Lm[a + k * p];
+        Lm[a + k * p] = tmp;
+      }
+    }
+  }
+  __syncthreads();
+  R* S = F + fl.S;
+  for (int idx = tid; idx < k * k; idx += NT) {
+    const int a = idx % k, c = idx / k;
+    R acc = 0;
+    for (int j = 0; j < r; j++) acc += Lm[a + k * j] * Krs[j + r * c];
+    S[idx] = upd(sperm[a], sperm[c]) + acc;
+  }
+  if (tid == 0) s.upd_kind[b] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Root: K(allskel, allskel) - carried blocks of the level-1 boxes, LU.
+// ---------------------------------------------------------------------------
+
+template <typename R, int NT>
+__global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
+                                             int factor_lu) {
+  __shared__ signed char pch[kMaxBoxDofs];
+  __shared__ short ploc[kMaxBoxDofs];
+  __shared__ const R* chS[4];
+  __shared__ int chN[4];
+  const int b = 0;
+  const int tid = threadIdx.x;
+  if (tid == 0) {  // the root is never compressed
+    s.rank[b] = 0;
+    s.m_rows[b] = 0;
+    s.out_cnt[b] = s.act_cnt[b];
+  }
+  if (!pools_ok(s)) return;
+  const int n = s.act_cnt[b];
+  const bool leaf = t.leaf[b] != 0;
+  if (tid < 4) {
+    const int c = leaf ? -1 : t.child[4 * b + tid];
+    chS[tid] = (c >= 0) ? carried_block<R>(s, c) : nullptr;
+    chN[tid] = (c >= 0) ? s.out_cnt[c] : 0;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += NT) {
+    int q = -1, loc = 0;
+    if (!leaf) {
+      for (int u = 0; u < 4; u++) {
+        const int p0 = s.chpos[4 * b + u];
+        if (i >= p0 && i < p0 + chN[u]) {
+          q = u;
+          loc = i - p0;
+        }
+      }
+    }
+    pch[i] = (signed char)q;
+    ploc[i] = (short)loc;
+  }
+  __syncthreads();
+  ChildUpdates<R> upd;
+  for (int u = 0; u < 4; u++) {
+    upd.S[u] = chS[u];
+    upd.N[u] = chN[u];
+  }
+  upd.pch = pch;
+  upd.ploc = ploc;
+  const int* act = s.act_list;
+  R* A = reinterpret_cast<R*>(s.root_mat);
+  for (int64_t idx = tid; idx < (int64_t)n * n; idx += NT) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    A[idx] = static_cast<R>(kernel_entry(g, act[i], act[j])) - upd(i, j);
+  }
+  __syncthreads();
+  if (factor_lu) {
+    const int info = block_lu<R, NT>(A, n, n, s.root_piv);
+    if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Solve sweeps (skel_factorization.cpp:507-592).  One wave per box; the
+// box's skeleton and redundant entries are gathered to LDS, updated and
+// scattered back (boxes of one level touch disjoint DOFs).
+// ---------------------------------------------------------------------------
+
+template <typename R>
+__device__ void wave_lu_solve(const R* X, const R* piv, int r, double* x) {
+  const int lane = threadIdx.x & 63;
+  if (lane == 0) {
+    for (int i = 0; i < r; i++) {
+      const int p = (int)piv[i];
+      if (p != i) {
+        const double tmp = x[i];
+        x[i] = x[p];
+        x[p] = tmp;
+      }
+    }
+  }
+  wave_sync();
+  for (int l = 0; l < r; l++) {
+    const double xl = x[l];
+    for (int i = l + 1 + lane; i < r; i += 64) x[i] -= (double)X[i + r * l] * xl;
+    wave_sync();
+  }
+  for (int i = r - 1; i >= 0; i--) {
+    if (lane == 0) x[i] /= (double)X[i + r * i];
+    wave_sync();
+    const double xi = x[i];
+    for (int l = lane; l < i; l += 64) x[l] -= (double)X[l + r * i] * xi;
+    wave_sync();
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_sweep_up(DevState s, int start,
+                                                 SolveVec v, int use_Ut,
+                                                 int diag) {
+  __shared__ double xs[kMaxBoxDofs];
+  __shared__ double xr[kMaxBoxDofs];
+  const int b = start + blockIdx.x;
+  const int k = s.rank[b];
+  if (k == 0) return;
+  const int lane = threadIdx.x;
+  const int r = s.act_cnt[b] - k;
+  const FacLayout fl = FacLayout::make(k, r);
+  const R* F = reinterpret_cast<const R*>(s.fac_pool) + s.fac_off[b];
+  const int* sk = s.out_list + (int64_t)b * s.ncap;
+  const int* rd = s.red_list + (int64_t)b * s.ncap;
+  for (int c = 0; c < v.nrhs; c++) {
+    double* x = v.x + v.ldx * c;
+    for (int a = lane; a < k; a += 64) xs[a] = x[sk[a]];
+    for (int i = lane; i < r; i += 64) xr[i] = x[rd[i]];
+    wave_sync();
+    for (int i = lane; i < r; i += 64) {
+      double acc = xr[i];
+      const R* tc = F + fl.T + (int64_t)k * i;
+      for (int a = 0; a < k; a++) acc -= (double)tc[a] * xs[a];
+      xr[i] = acc;
+    }
+    wave_sync();
+    for (int a = lane; a < k; a += 64) {
+      double acc = xs[a];
+      if (use_Ut) {
+        const R* uc = F + fl.U + (int64_t)r * a;
+        for (int i = 0; i < r; i++) acc -= (double)uc[i] * xr[i];
+      } else {
+        const R* L = F + fl.L;
+        for (int i = 0; i < r; i++) acc -= (double)L[a + (int64_t)k * i] * xr[i];
+      }
+      xs[a] = acc;
+    }
+    wave_sync();
+    if (diag) wave_lu_solve(F + fl.X, F + fl.piv, r, xr);
+    for (int a = lane; a < k; a += 64) x[sk[a]] = xs[a];
+    for (int i = lane; i < r; i += 64) x[rd[i]] = xr[i];
+    wave_sync();
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_diag(DevState s, int start, SolveVec v) {
+  __shared__ double xr[kMaxBoxDofs];
+  const int b = start + blockIdx.x;
+  const int k = s.rank[b];
+  if (k == 0) return;
+  const int lane = threadIdx.x;
+  const int r = s.act_cnt[b] - k;
+  const FacLayout fl = FacLayout::make(k, r);
+  const R* F = reinterpret_cast<const R*>(s.fac_pool) + s.fac_off[b];
+  const int* rd = s.red_list + (int64_t)b * s.ncap;
+  for (int c = 0; c < v.nrhs; c++) {
+    double* x = v.x + v.ldx * c;
+    for (int i = lane; i < r; i += 64) xr[i] = x[rd[i]];
+    wave_sync();
+    wave_lu_solve(F + fl.X, F + fl.piv, r, xr);
+    for (int i = lane; i < r; i += 64) x[rd[i]] = xr[i];
+    wave_sync();
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_sweep_down(DevState s, int start,
+                                                   SolveVec v) {
+  __shared__ double xs[kMaxBoxDofs];
+  __shared__ double xr[kMaxBoxDofs];
+  const int b = start + blockIdx.x;
+  const int k = s.rank[b];
+  if (k == 0) return;
+  const int lane = threadIdx.x;
+  const int r = s.act_cnt[b] - k;
+  const FacLayout fl = FacLayout::make(k, r);
+  const R* F = reinterpret_cast<const R*>(s.fac_pool) + s.fac_off[b];
+  const int* sk = s.out_list + (int64_t)b * s.ncap;
+  const int* rd = s.red_list + (int64_t)b * s.ncap;
+  for (int c = 0; c < v.nrhs; c++) {
+    double* x = v.x + v.ldx * c;
+    for (int a = lane; a < k; a += 64) xs[a] = x[sk[a]];
+    for (int i = lane; i < r; i += 64) xr[i] = x[rd[i]];
+    wave_sync();
+    for (int i = lane; i < r; i += 64) {
+      double acc = xr[i];
+      const R* U = F + fl.U;
+      for (int a = 0; a < k; a++) acc -= (double)U[i + (int64_t)r * a] * xs[a];
+      xr[i] = acc;
+    }
+    wave_sync();
+    for (int a = lane; a < k; a += 64) {
+      double acc = xs[a];
+      const R* T = F + fl.T;
+      for (int i = 0; i < r; i++) acc -= (double)T[a + (int64_t)k * i] * xr[i];
+      xs[a] = acc;
+    }
+    wave_sync();
+    for (int a = lane; a < k; a += 64) x[sk[a]] = xs[a];
+    for (int i = lane; i < r; i += 64) x[rd[i]] = xr[i];
+    wave_sync();
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_root_solve(DevState s, SolveVec v) {
+  __shared__ double xa[kMaxBoxDofs];
+  __shared__ R piv[kMaxBoxDofs];
+  const int lane = threadIdx.x;
+  const int n = s.act_cnt[0];
+  const int* act = s.act_list;
+  const R* A = reinterpret_cast<const R*>(s.root_mat);
+  for (int i = lane; i < n; i += 64) piv[i] = static_cast<R>(s.root_piv[i]);
+  for (int c = 0; c < v.nrhs; c++) {
+    double* x = v.x + v.ldx * c;
+    for (int i = lane; i < n; i += 64) xa[i] = x[act[i]];
+    wave_sync();
+    wave_lu_solve(A, piv, n, xa);
+    for (int i = lane; i < n; i += 64) x[act[i]] = xa[i];
+    wave_sync();
+  }
+}
+
+}  // namespace
+
+void launch_reset(const DevState& s, hipStream_t st) {
+  k_reset<<<1, 64, 0, st>>>(s);
+}
+
+template <typename R>
+void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
+                         const DevState& s, int level, int start, int count,
+                         double tol, hipStream_t st) {
+  if (count <= 0) return;
+  switch (stage) {
+    case kStageActive:
+      k_active<<<count, 64, 0, st>>>(t, s, start);
+      break;
+    case kStageRows:
+      k_rows<<<count, 64, 0, st>>>(g, t, s, start, level);
+      break;
+    case kStageScan:
+      k_scan<<<1, 1024, 0, st>>>(s, start, count);
+      break;
+    case kStageAssemble:
+      k_assemble<R><<<count, 256, 0, st>>>(g, t, s, start, level);
+      break;
+    case kStageQrcp:
+      k_qrcp<R, 512><<<count, 512, 0, st>>>(s, start, tol);
+      break;
+    case kStageDecouple:
+      k_decouple<R, 256><<<count, 256, 0, st>>>(g, t, s, start);
+      break;
+    default:
+      break;
+  }
+}
+
+template <typename R>
+void launch_root(const DevGeom& g, const DevTree& t, const DevState& s,
+                 bool factor_lu, hipStream_t st) {
+  k_active<<<1, 64, 0, st>>>(t, s, 0);
+  k_root<R, 1024><<<1, 1024, 0, st>>>(g, t, s, factor_lu ? 1 : 0);
+}
+
+template <typename R>
+void launch_sweep_up(const DevState& s, int start, int count, SolveVec v,
+                     bool use_Ut, bool diag, hipStream_t st) {
+  if (count <= 0) return;
+  k_sweep_up<R><<<count, 64, 0, st>>>(s, start, v, use_Ut ? 1 : 0, diag ? 1 : 0);
+}
+
+template <typename R>
+void launch_diag(const DevState& s, int start, int count, SolveVec v,
+                 hipStream_t st) {
+  if (count <= 0) return;
+  k_diag<R><<<count, 64, 0, st>>>(s, start, v);
+}
+
+template <typename R>
+void launch_sweep_down(const DevState& s, int start, int count, SolveVec v,
+                       hipStream_t st) {
+  if (count <= 0) return;
+  k_sweep_down<R><<<count, 64, 0, st>>>(s, start, v);
+}
+
+template <typename R>
+void launch_root_solve(const DevState& s, SolveVec v, hipStream_t st) {
+  k_root_solve<R><<<1, 64, 0, st>>>(s, v);
+}
+
+#define IEGPU_INSTANTIATE(R)                                                   \
+  template void launch_factor_stage<R>(int, const DevGeom&, const DevTree&,   \
+                                       const DevState&, int, int, int, double, \
+                                       hipStream_t);                           \
+  template void launch_root<R>(const DevGeom&, const DevTree&,                \
+                               const DevState&, bool, hipStream_t);            \
+  template void launch_sweep_up<R>(const DevState&, int, int, SolveVec, bool, \
+                                   bool, hipStream_t);                         \
+  template void launch_diag<R>(const DevState&, int, int, SolveVec,           \
+                               hipStream_t);                                   \
+  template void launch_sweep_down<R>(const DevState&, int, int, SolveVec,     \
+                                     hipStream_t);                             \
+  template void launch_root_solve<R>(const DevState&, SolveVec, hipStream_t);
+
+IEGPU_INSTANTIATE(double)
+IEGPU_INSTANTIATE(float)
+
+}  // namespace iegpu

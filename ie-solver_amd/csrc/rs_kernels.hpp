@@ -1,0 +1,59 @@
+// Launch wrappers for the RS factor + solve kernels (rs_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rs_device.hpp"
+
+namespace iegpu {
+
+struct SolveVec {
+  double* x;      // device, ldx x nrhs column-major
+  int64_t ldx;
+  int nrhs;
+};
+
+void launch_reset(const DevState& s, hipStream_t st);
+
+// Stages of one tree level of the factorization, in launch order.
+enum FactorStage : int {
+  kStageActive = 0,   // active lists from the children
+  kStageRows = 1,     // ID row counts and slot sizes
+  kStageScan = 2,     // slot offsets
+  kStageAssemble = 3, // ID matrices
+  kStageQrcp = 4,     // pivoted QR, rank, T
+  kStageDecouple = 5, // Schur complement, X_rr LU, L, U
+  kNumFactorStages = 6
+};
+
+template <typename R>
+void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
+                         const DevState& s, int level, int start, int count,
+                         double tol, hipStream_t st);
+
+// Root: active list, dense skeleton block, LU.
+template <typename R>
+void launch_root(const DevGeom& g, const DevTree& t, const DevState& s,
+                 bool factor_lu, hipStream_t st);
+
+// Upward sweep of one level: x_r -= T^T x_s; x_s -= L x_r (or U^T x_r when
+// `use_Ut`); then x_r = X_rr^{-1} x_r when `diag`.
+template <typename R>
+void launch_sweep_up(const DevState& s, int start, int count, SolveVec v,
+                     bool use_Ut, bool diag, hipStream_t st);
+
+// Block-diagonal X_rr^{-1} on the redundant rows of one level.
+template <typename R>
+void launch_diag(const DevState& s, int start, int count, SolveVec v,
+                 hipStream_t st);
+
+// Downward sweep of one level: x_r -= U x_s; x_s -= T x_r.
+template <typename R>
+void launch_sweep_down(const DevState& s, int start, int count, SolveVec v,
+                       hipStream_t st);
+
+// x(allskel) = A_root^{-1} x(allskel) with the root LU.
+template <typename R>
+void launch_root_solve(const DevState& s, SolveVec v, hipStream_t st);
+
+}  // namespace iegpu

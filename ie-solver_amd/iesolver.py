@@ -1,0 +1,198 @@
+"""Python host binding of the C ABI in include/iegpu.h (ctypes).
+
+Mirrors the reference's factorize-then-solve call sequence
+(ie_solver/linear_solve.cpp:156-185, bie_time_trial):
+
+    s = RSSolver()                      # iegpu_create
+    s.set_geometry(geom, pde)           # Kernel + Boundary arrays
+    s.build_tree(leaf_cap)              # QuadTree::initialize_tree
+    s.set_holes(centers, radii)         # quadtree->U / Psi (optional)
+    s.factor(id_tol)                    # SkelFactorization::skeletonize
+    mu, alpha = s.solve(f)              # linear_solve
+
+The shared library is built in-tree (ie-solver_amd/build.py).  There is no
+CPU fallback: loading fails loudly when the library is missing, and every
+non-zero status raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libiesolver_amd.so")
+
+PDE_LAPLACE = 0
+PDE_STOKES = 2
+FP64 = 0
+FP32 = 1
+
+STATUS = {
+    0: "OK", 1: "INVALID", 2: "NO_COMPRESSION", 3: "SINGULAR", 4: "OOM",
+    5: "HIP", 6: "CAPACITY", 7: "NO_DEVICE",
+}
+
+EXPORTS = [
+    "iegpu_create", "iegpu_destroy", "iegpu_last_error", "iegpu_set_geometry",
+    "iegpu_build_tree", "iegpu_set_holes", "iegpu_factor", "iegpu_solve",
+    "iegpu_solve_device", "iegpu_tree_info", "iegpu_box_info", "iegpu_box_list",
+    "iegpu_last_timings", "iegpu_set_profiling", "iegpu_profile",
+]
+
+STAGES = ["active", "rows", "scan", "assemble", "qrcp", "decouple", "root",
+          "sweep_up", "root_solve", "sweep_down", "border"]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+class IeGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with ie-solver_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.iegpu_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+    L.iegpu_destroy.argtypes = [vp]
+    L.iegpu_last_error.argtypes = [vp]
+    L.iegpu_last_error.restype = ctypes.c_char_p
+    L.iegpu_set_geometry.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.iegpu_build_tree.argtypes = [vp, ctypes.c_int]
+    L.iegpu_set_holes.argtypes = [vp, ctypes.c_int, _dp, _dp]
+    L.iegpu_factor.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+    L.iegpu_solve.argtypes = [vp, _dp, _dp, _dp]
+    L.iegpu_solve_device.argtypes = [vp, vp, vp, vp, vp]
+    L.iegpu_tree_info.argtypes = [vp, _ip, _ip, _ip, _ip]
+    L.iegpu_box_info.argtypes = [vp, _ip, _ip, _ip, _ip, _ip, _ip, _ip]
+    L.iegpu_box_list.argtypes = [vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int]
+    L.iegpu_last_timings.argtypes = [vp, _dp, ctypes.c_int]
+    L.iegpu_set_profiling.argtypes = [vp, ctypes.c_int]
+    L.iegpu_profile.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _ip]
+    _lib = L
+    return L
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _pd(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_ip)
+
+
+class RSSolver:
+    """One boundary, its quadtree and its factorization on one GPU."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        h = ctypes.c_void_p()
+        st = self.L.iegpu_create(ctypes.byref(h), device)
+        if st != 0:
+            raise IeGpuError(st, "iegpu_create failed")
+        self.h = h
+        self.hole_dim = 0
+        self.ndof = 0
+
+    def _check(self, st):
+        if st != 0:
+            raise IeGpuError(st, self.L.iegpu_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.iegpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_geometry(self, geom, pde=PDE_LAPLACE):
+        self._arrays = [_d(geom["points"]).reshape(-1), _d(geom["normals"]).reshape(-1),
+                        _d(geom["curvatures"]), _d(geom["weights"])]
+        n = geom["points"].shape[0]
+        self._check(self.L.iegpu_set_geometry(self.h, pde, n, *[_pd(a) for a in self._arrays]))
+        self.pde = pde
+        self.ndof = n * (2 if pde == PDE_STOKES else 1)
+        self.hole_dim = 0
+
+    def build_tree(self, leaf_cap=64):
+        self._check(self.L.iegpu_build_tree(self.h, leaf_cap))
+
+    def set_holes(self, centers, radii):
+        c = _d(centers).reshape(-1)
+        r = _d(radii)
+        self._check(self.L.iegpu_set_holes(self.h, len(r), _pd(c), _pd(r)))
+        self.hole_dim = self.tree_info()[3]
+
+    def factor(self, id_tol, precision=FP64):
+        self._check(self.L.iegpu_factor(self.h, id_tol, precision))
+
+    def solve(self, b):
+        b = _d(b)
+        mu = np.zeros(self.ndof)
+        if self.hole_dim:
+            alpha = np.zeros(self.hole_dim)
+            self._check(self.L.iegpu_solve(self.h, _pd(b), _pd(mu), _pd(alpha)))
+            return mu, alpha
+        self._check(self.L.iegpu_solve(self.h, _pd(b), _pd(mu), None))
+        return mu, None
+
+    def solve_device(self, b_ptr, mu_ptr, alpha_ptr=None, stream=None):
+        self._check(self.L.iegpu_solve_device(self.h, b_ptr, mu_ptr, alpha_ptr, stream))
+
+    def tree_info(self):
+        v = [ctypes.c_int(0) for _ in range(4)]
+        self._check(self.L.iegpu_tree_info(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def box_info(self):
+        nl, nb, _, _ = self.tree_info()
+        out = dict(level=np.zeros(nb, np.int32), parent=np.zeros(nb, np.int32),
+                   is_leaf=np.zeros(nb, np.int32), children=np.zeros(4 * nb, np.int32),
+                   n_active=np.zeros(nb, np.int32), rank=np.zeros(nb, np.int32),
+                   m_rows=np.zeros(nb, np.int32))
+        self._check(self.L.iegpu_box_info(self.h, *[_pi(out[k]) for k in
+                                                    ("level", "parent", "is_leaf", "children",
+                                                     "n_active", "rank", "m_rows")]))
+        out["children"] = out["children"].reshape(nb, 4)
+        return out
+
+    def box_list(self, box, which):
+        n = self.L.iegpu_box_list(self.h, box, which, None, 0)
+        if n < 0:
+            raise IeGpuError(-n, self.L.iegpu_last_error(self.h).decode())
+        buf = np.zeros(max(n, 1), np.int32)
+        self.L.iegpu_box_list(self.h, box, which, _pi(buf), n)
+        return buf[:n].copy()
+
+    def timings(self):
+        t = np.zeros(4)
+        self._check(self.L.iegpu_last_timings(self.h, _pd(t), 4))
+        return dict(factor_ms=t[0], solve_ms=t[1], root_ms=t[2], factor_bytes=t[3])
+
+    def set_profiling(self, on=True):
+        self._check(self.L.iegpu_set_profiling(self.h, 1 if on else 0))
+
+    def profile(self, stage, level=-1):
+        """(milliseconds, launches) of one stage (name or index)."""
+        if isinstance(stage, str):
+            stage = STAGES.index(stage)
+        ms = ctypes.c_double(0)
+        n = ctypes.c_int(0)
+        self._check(self.L.iegpu_profile(self.h, stage, level, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def profile_table(self):
+        return {name: self.profile(i) for i, name in enumerate(STAGES)}

@@ -1,0 +1,130 @@
+/*
+ * iegpu.h -- C ABI of the MI355X (gfx950) recursive-skeletonization (RS)
+ * factor + solve for 2-D second-kind boundary integral equations.
+ *
+ * The reference (jpryan1/ie-solver) exposes this path as a C++ library API
+ * with no FFI; each entry point below names the reference interface it
+ * replaces (paths relative to the reference root).  Plain pointers and
+ * sizes only; host arrays are borrowed for the duration of the call;
+ * device buffers are owned by the context.  One HIP stream per context; a
+ * context is not thread-safe; separate contexts are independent.
+ *
+ * Every function returns an iegpu_status (0 = success).  On failure
+ * iegpu_last_error() describes the cause.
+ */
+#ifndef IEGPU_H_
+#define IEGPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct iegpu_ctx iegpu_ctx;
+
+typedef enum {
+  IEGPU_OK = 0,
+  IEGPU_ERR_INVALID = 1,        /* bad argument or call order            */
+  IEGPU_ERR_NO_COMPRESSION = 2, /* skel_factorization.cpp:209-212         */
+  IEGPU_ERR_SINGULAR = 3,       /* exact zero pivot (LAPACK info > 0)     */
+  IEGPU_ERR_OOM = 4,            /* device allocation failed               */
+  IEGPU_ERR_HIP = 5,            /* HIP runtime error                      */
+  IEGPU_ERR_CAPACITY = 6,       /* a box exceeded the supported size      */
+  IEGPU_ERR_NO_DEVICE = 7       /* no HIP device available                */
+} iegpu_status;
+
+/* ie_solver_config::Pde (ie_solver/io/io.h:19-23). */
+enum { IEGPU_PDE_LAPLACE = 0, IEGPU_PDE_STOKES = 2 };
+
+/* Arithmetic of the stored factorization. */
+enum { IEGPU_FP64 = 0, IEGPU_FP32 = 1 };
+
+/* Create / destroy a context on HIP device `device` (device = -1: a
+ * host-only context that builds trees but cannot factor).  No reference
+ * counterpart (the reference holds state in QuadTree/SkelFactorization
+ * objects, ie_solver/quadtree/quadtree.h:95-139). */
+int iegpu_create(iegpu_ctx** ctx, int device);
+int iegpu_destroy(iegpu_ctx* ctx);
+const char* iegpu_last_error(const iegpu_ctx* ctx);
+
+/* Boundary discretization: n_points nodes, SoA arrays points[2n],
+ * normals[2n], curvatures[n], weights[n]; pde selects the kernel and the
+ * DOFs per node (1 Laplace, 2 Stokes).
+ * Replaces Kernel(sol_dim, dom_dim, pde, boundary, ...) and the Boundary
+ * data contract: ie_solver/kernel/kernel.h:28-35,
+ * ie_solver/boundaries/boundary.h:39-41. */
+int iegpu_set_geometry(iegpu_ctx* ctx, int pde, int n_points,
+                       const double* points, const double* normals,
+                       const double* curvatures, const double* weights);
+
+/* Build the quadtree (leaf boxes hold at most leaf_cap DOFs) and upload it.
+ * Replaces QuadTree::initialize_tree, ie_solver/quadtree/quadtree.h:114-117
+ * (quadtree.cpp:39-180; cap = MAX_LEAF_DOFS, quadtree.h:10). */
+int iegpu_build_tree(iegpu_ctx* ctx, int leaf_cap);
+
+/* Multiply connected domains: hole centres[2h] and radii[h]; builds the
+ * U / Psi border matrices on the boundary points.  Replaces
+ * initialize_U_mat / initialize_Psi_mat and the assignment to
+ * quadtree->U / quadtree->Psi, ie_solver/linear_solve.cpp:20-112,256-257.
+ * n_holes = 0 clears them. */
+int iegpu_set_holes(iegpu_ctx* ctx, int n_holes, const double* centers,
+                    const double* radii);
+
+/* Recursive-skeletonization factorization with ID tolerance id_tol
+ * (> 0) in IEGPU_FP64 or IEGPU_FP32.  Replaces
+ * SkelFactorization::skeletonize, skel_factorization.h:55
+ * (skel_factorization.cpp:140-310). */
+int iegpu_factor(iegpu_ctx* ctx, double id_tol, int precision);
+
+/* Solve with the stored factorization: mu (ndof) from b (ndof); with holes
+ * alpha (hole_dim) is also returned (must be non-NULL).  Host pointers.
+ * Replaces linear_solve, ie_solver/linear_solve.h:33-35 (solve /
+ * multiply_connected_solve, skel_factorization.h:92-94). */
+int iegpu_solve(iegpu_ctx* ctx, const double* b, double* mu, double* alpha);
+
+/* As iegpu_solve on device buffers (b may alias mu), ordered on `stream`
+ * (NULL: the context's stream). */
+int iegpu_solve_device(iegpu_ctx* ctx, const double* d_b, double* d_mu,
+                       double* d_alpha, void* stream);
+
+/* Sizes: tree levels, boxes, DOFs, hole border dimension. */
+int iegpu_tree_info(const iegpu_ctx* ctx, int* n_levels, int* n_boxes,
+                    int* ndof, int* hole_dim);
+
+/* Per box, indexed by creation order (QuadTreeNode::id order from the
+ * root): level, parent id, leaf flag, children ids[4] (bl, tl, tr, br;
+ * -1 none), and after a factorization |active|, rank (0: not compressed)
+ * and ID matrix height.  Any pointer may be NULL.  Mirrors the per-node
+ * fields of QuadTreeNode, ie_solver/quadtree/quadtree.h:51-82. */
+int iegpu_box_info(iegpu_ctx* ctx, int* level, int* parent, int* is_leaf,
+                   int* children, int* n_active, int* rank, int* m_rows);
+
+/* One box list (creation id `box`): which = 0 original DOFs, 1 neighbour
+ * ids, 2 active DOFs, 3 skeleton DOFs, 4 redundant DOFs; which = 5 lists
+ * the box ids of tree level `box` in the reference's level order.
+ * Returns the length (>= 0) and copies at most cap entries, or a negative
+ * status.  Mirrors InteractionLists, ie_solver/quadtree/quadtree.h:16-49. */
+int iegpu_box_list(iegpu_ctx* ctx, int box, int which, int* out, int cap);
+
+/* Device milliseconds of the last factor / solve, split by stage:
+ * t[0] factor total, t[1] solve total, t[2] root LU, t[3] bytes of
+ * stored factors (as a double). */
+int iegpu_last_timings(const iegpu_ctx* ctx, double* t, int nt);
+
+/* Per-stage device timing with HIP events recorded around each launch on
+ * the launching stream (stands in for the reference's omp_get_wtime
+ * accumulators, skel_factorization.cpp:45-56,155-166).  on != 0 enables
+ * and resets the counters.  Stages: 0 active lists, 1 ID row counts,
+ * 2 slot offsets, 3 ID-matrix assembly, 4 pivoted QR, 5 decouple, 6 root,
+ * 7 upward sweep, 8 root / bordered solve, 9 downward sweep, 10 bordered
+ * factor.  level = -1 sums all levels. */
+int iegpu_set_profiling(iegpu_ctx* ctx, int on);
+int iegpu_profile(iegpu_ctx* ctx, int stage, int level, double* ms,
+                  int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IEGPU_H_ */

@@ -1,0 +1,245 @@
+"""ctypes wrapper of the CPU oracle (liboracle_rs.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.  See
+oracle/rs_oracle.cpp for the reference functions each entry restates.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_rs.so")
+_lib = None
+
+PDE_LAPLACE = 0
+PDE_STOKES = 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build(force=False):
+    """Compile the oracle with its Makefile (CPU only)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"],
+                       check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.rso_create.restype = ctypes.c_void_p
+        L.rso_create.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                 ctypes.c_int]
+        L.rso_destroy.argtypes = [ctypes.c_void_p]
+        L.rso_record_id_mats.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.rso_set_holes.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.rso_hole_dim.argtypes = [ctypes.c_void_p]
+        L.rso_get_U_Psi.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.rso_factor.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        L.rso_solve.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int]
+        L.rso_num_levels.argtypes = [ctypes.c_void_p]
+        L.rso_num_boxes.argtypes = [ctypes.c_void_p]
+        L.rso_box_info.argtypes = [ctypes.c_void_p, _ip, _ip, _ip, _ip, _dp,
+                                   _ip, _ip, _ip]
+        L.rso_box_list.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                   _ip, ctypes.c_int]
+        L.rso_allskel.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int]
+        L.rso_get_id_mat.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp,
+                                     ctypes.c_long, _ip, _ip]
+        L.rso_kernel_block.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                       _dp, ctypes.c_int, _ip, ctypes.c_int, _ip,
+                                       _dp]
+        L.rso_proxy_block.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                      _dp, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_int, _ip, _dp]
+        L.rso_id.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
+                             _ip, _dp]
+        L.rso_eval.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                               ctypes.c_int, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _pd(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_ip)
+
+
+class OracleProblem:
+    """One boundary, its quadtree and (after factor()) its RS factorization."""
+
+    def __init__(self, geom, pde=PDE_LAPLACE, leaf_cap=64, record_id_mats=False):
+        self.geom = geom
+        self.pde = pde
+        self.sol_dim = 2 if pde == PDE_STOKES else 1
+        self.n_points = geom["points"].shape[0]
+        self._keep = [_d(geom["points"]).reshape(-1), _d(geom["normals"]).reshape(-1),
+                      _d(geom["curvatures"]), _d(geom["weights"])]
+        self.h = lib().rso_create(pde, self.n_points, *[_pd(a) for a in self._keep],
+                                  leaf_cap)
+        if not self.h:
+            raise ValueError("rso_create failed")
+        if record_id_mats:
+            lib().rso_record_id_mats(self.h, 1)
+        self.hole_dim = 0
+        holes = geom.get("holes")
+        if holes is not None and len(holes["radii"]) > 0:
+            c = _d(holes["centers"]).reshape(-1)
+            r = _d(holes["radii"])
+            lib().rso_set_holes(self.h, len(r), _pd(c), _pd(r))
+            self.hole_dim = lib().rso_hole_dim(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().rso_destroy(self.h)
+            self.h = None
+
+    @property
+    def ndof(self):
+        return self.n_points * self.sol_dim
+
+    def factor(self, id_tol, num_threads=1):
+        return lib().rso_factor(self.h, id_tol, num_threads)
+
+    def solve(self, f, num_threads=1):
+        f = _d(f)
+        mu = np.zeros(self.ndof)
+        if self.hole_dim:
+            alpha = np.zeros(self.hole_dim)
+            st = lib().rso_solve(self.h, _pd(f), _pd(mu), _pd(alpha), num_threads)
+            if st:
+                raise RuntimeError("oracle solve failed")
+            return mu, alpha
+        st = lib().rso_solve(self.h, _pd(f), _pd(mu), None, num_threads)
+        if st:
+            raise RuntimeError("oracle solve failed")
+        return mu, None
+
+    def U_Psi(self):
+        if not self.hole_dim:
+            return None, None
+        U = np.zeros((self.hole_dim, self.ndof)).T.copy(order="F")
+        Psi = np.zeros((self.hole_dim, self.ndof), order="F")
+        Ub = np.zeros(self.ndof * self.hole_dim)
+        Pb = np.zeros(self.ndof * self.hole_dim)
+        lib().rso_get_U_Psi(self.h, _pd(Ub), _pd(Pb))
+        U = Ub.reshape(self.hole_dim, self.ndof).T
+        Psi = Pb.reshape(self.ndof, self.hole_dim).T
+        return U, Psi
+
+    def num_levels(self):
+        return lib().rso_num_levels(self.h)
+
+    def num_boxes(self):
+        return lib().rso_num_boxes(self.h)
+
+    def box_info(self):
+        nb = self.num_boxes()
+        out = dict(level=np.zeros(nb, np.int32), parent=np.zeros(nb, np.int32),
+                   is_leaf=np.zeros(nb, np.int32), children=np.zeros(4 * nb, np.int32),
+                   bl_side=np.zeros(3 * nb), n_active=np.zeros(nb, np.int32),
+                   rank=np.zeros(nb, np.int32), m_rows=np.zeros(nb, np.int32))
+        lib().rso_box_info(self.h, _pi(out["level"]), _pi(out["parent"]),
+                           _pi(out["is_leaf"]), _pi(out["children"]),
+                           _pd(out["bl_side"]), _pi(out["n_active"]),
+                           _pi(out["rank"]), _pi(out["m_rows"]))
+        out["children"] = out["children"].reshape(nb, 4)
+        out["bl_side"] = out["bl_side"].reshape(nb, 3)
+        return out
+
+    def box_list(self, box, which):
+        n = lib().rso_box_list(self.h, box, which, None, 0)
+        if n < 0:
+            raise IndexError(box)
+        buf = np.zeros(max(n, 1), np.int32)
+        lib().rso_box_list(self.h, box, which, _pi(buf), n)
+        return buf[:n].copy()
+
+    def level_order(self, level):
+        return self.box_list(level, 5)
+
+    def allskel(self):
+        n = lib().rso_allskel(self.h, None, 0)
+        buf = np.zeros(max(n, 1), np.int32)
+        lib().rso_allskel(self.h, _pi(buf), n)
+        return buf[:n].copy()
+
+    def id_mat(self, box):
+        m = ctypes.c_int(0)
+        n = ctypes.c_int(0)
+        lib().rso_get_id_mat(self.h, box, None, 0, ctypes.byref(m), ctypes.byref(n))
+        buf = np.zeros(max(m.value * n.value, 1))
+        lib().rso_get_id_mat(self.h, box, _pd(buf), buf.size, ctypes.byref(m),
+                             ctypes.byref(n))
+        return buf[: m.value * n.value].reshape(n.value, m.value).T.copy()
+
+
+def kernel_block(geom, I, J, pde=PDE_LAPLACE):
+    pts = [_d(geom["points"]).reshape(-1), _d(geom["normals"]).reshape(-1),
+           _d(geom["curvatures"]), _d(geom["weights"])]
+    I = _i(I)
+    J = _i(J)
+    out = np.zeros(len(I) * len(J))
+    lib().rso_kernel_block(pde, geom["points"].shape[0], *[_pd(a) for a in pts],
+                           len(I), _pi(I), len(J), _pi(J), _pd(out))
+    return out.reshape(len(J), len(I)).T.copy()
+
+
+def proxy_block(geom, cx, cy, r, box, pde=PDE_LAPLACE):
+    pts = [_d(geom["points"]).reshape(-1), _d(geom["normals"]).reshape(-1),
+           _d(geom["curvatures"]), _d(geom["weights"])]
+    box = _i(box)
+    rows = 256 if pde == PDE_LAPLACE else 512
+    out = np.zeros(rows * len(box))
+    lib().rso_proxy_block(pde, geom["points"].shape[0], *[_pd(a) for a in pts],
+                          cx, cy, r, len(box), _pi(box), _pd(out))
+    return out.reshape(len(box), rows).T.copy()
+
+
+def interp_decomp(A, tol):
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = A.shape
+    flat = np.ascontiguousarray(A.T.reshape(-1))
+    perm = np.zeros(n, np.int32)
+    T = np.zeros(n * n)
+    k = lib().rso_id(m, n, _pd(flat), tol, _pi(perm), _pd(T))
+    if k == 0:
+        return 0, None, None
+    return k, perm.copy(), T[: k * (n - k)].reshape(n - k, k).T.copy()
+
+
+def evaluate(geom, targets, mu, pde=PDE_LAPLACE):
+    pts = _d(geom["points"]).reshape(-1)
+    nrm = _d(geom["normals"]).reshape(-1)
+    w = _d(geom["weights"])
+    tg = _d(targets).reshape(-1)
+    nt = tg.size // 2
+    mu = _d(mu)
+    u = np.zeros(nt * (2 if pde == PDE_STOKES else 1))
+    lib().rso_eval(pde, geom["points"].shape[0], _pd(pts), _pd(nrm), _pd(w), nt,
+                   _pd(tg), _pd(mu), _pd(u))
+    return u
+
+
+def dense_matrix(geom, pde=PDE_LAPLACE):
+    n = geom["points"].shape[0] * (2 if pde == PDE_STOKES else 1)
+    idx = np.arange(n, dtype=np.int32)
+    return kernel_block(geom, idx, idx, pde)

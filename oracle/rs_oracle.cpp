@@ -1,0 +1,1154 @@
+// CPU oracle for the recursive-skeletonization (RS) factor + solve path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is a from-scratch CPU restatement of
+// the reference algorithm (jpryan1/ie-solver), written to check the HIP
+// product path.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load the library built from it; the product never
+// links, loads or calls it.
+//
+// Arithmetic: FP64 throughout, dense kernels through LAPACKE/CBLAS of the
+// LP64 OpenBLAS bundled with scipy (exported with a `scipy_` prefix).  The
+// reference calls the same LAPACK routines (dgeqp3, dtrtrs, dgetrf, dgetrs,
+// dgemm) from ie_solver/ie_mat.cpp.  Like the reference, boxes of one tree
+// level are processed in an OpenMP parallel-for with BLAS threads = 1.
+//
+// What follows which reference function (paths relative to the reference
+// root):
+//   kernel entries        ie_solver/kernel/kernel.cpp:123-165 (Laplace),
+//                         :212-290 (Stokes)
+//   proxy blocks          ie_solver/kernel/kernel.cpp:426-591
+//   ID-matrix rows        ie_solver/kernel/kernel.cpp:293-423
+//   interpolative decomp. ie_solver/ie_mat.cpp:548-588
+//   tree build            ie_solver/quadtree/quadtree.cpp:39-432
+//   active / near lists   ie_solver/quadtree/quadtree.cpp:989-1074
+//   skeletonize           ie_solver/skel_factorization/skel_factorization.cpp:140-310
+//   decouple              .../skel_factorization.cpp:77-128
+//   Schur updates         .../skel_factorization.cpp:323-381
+//   solve                 .../skel_factorization.cpp:507-592
+//   multiply-conn. solve  .../skel_factorization.cpp:596-777
+//   U / Psi matrices      ie_solver/linear_solve.cpp:20-112
+//   interior evaluation   ie_solver/initialization.cpp:11-135,
+//                         ie_solver/kernel/kernel.cpp:29-77
+//
+// Parity pins for every function here live in tests/test_oracle.py
+// (closed forms, dense LAPACK solves, brute-force tree checks, scipy
+// dgeqp3).  Per-box pivot orders inherit OpenBLAS's dgeqp3 and are pinned
+// only through the ID reconstruction bound and the end-to-end solution.
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+extern "C" {
+int scipy_LAPACKE_dgeqp3(int layout, int m, int n, double* a, int lda,
+                         int* jpvt, double* tau);
+int scipy_LAPACKE_dtrtrs(int layout, char uplo, char trans, char diag, int n,
+                         int nrhs, const double* a, int lda, double* b,
+                         int ldb);
+int scipy_LAPACKE_dgetrf(int layout, int m, int n, double* a, int lda,
+                         int* ipiv);
+int scipy_LAPACKE_dgetrs(int layout, char trans, int n, int nrhs,
+                         const double* a, int lda, const int* ipiv, double* b,
+                         int ldb);
+void scipy_cblas_dgemm(int layout, int ta, int tb, int m, int n, int k,
+                       double alpha, const double* a, int lda, const double* b,
+                       int ldb, double beta, double* c, int ldc);
+void scipy_openblas_set_num_threads(int);
+}
+
+namespace {
+
+constexpr int kColMajor = 102;
+constexpr int kNoTrans = 111;
+constexpr int kPdeLaplace = 0;
+constexpr int kPdeStokes = 2;
+constexpr int kNumProxy = 128;        // kernel.h:11
+constexpr double kRadiusRatio = 1.5;  // kernel.h:12
+constexpr int kMinDofsToCompress = 16;  // skel_factorization.h:11
+
+// Column-major dense matrix.
+struct Mat {
+  int h = 0, w = 0;
+  std::vector<double> a;
+  Mat() {}
+  Mat(int h_, int w_) : h(h_), w(w_), a((size_t)h_ * w_, 0.0) {}
+  double& operator()(int i, int j) { return a[i + (size_t)h * j]; }
+  double operator()(int i, int j) const { return a[i + (size_t)h * j]; }
+  double* data() { return a.data(); }
+  const double* data() const { return a.data(); }
+};
+
+// C = A*B (ie_mat.cpp:316-323, dgemm NN).
+Mat mul(const Mat& A, const Mat& B) {
+  Mat C(A.h, B.w);
+  if (A.h == 0 || B.w == 0) return C;
+  if (A.w == 0) return C;
+  scipy_cblas_dgemm(kColMajor, kNoTrans, kNoTrans, A.h, B.w, A.w, 1.0,
+                    A.data(), A.h, B.data(), B.h, 0.0, C.data(), A.h);
+  return C;
+}
+
+Mat transpose(const Mat& A) {
+  Mat T(A.w, A.h);
+  for (int j = 0; j < A.w; j++)
+    for (int i = 0; i < A.h; i++) T(j, i) = A(i, j);
+  return T;
+}
+
+Mat sub(const Mat& A, const Mat& B) {
+  Mat C(A.h, A.w);
+  for (size_t i = 0; i < A.a.size(); i++) C.a[i] = A.a[i] - B.a[i];
+  return C;
+}
+
+Mat gather(const Mat& A, const std::vector<int>& I, const std::vector<int>& J) {
+  Mat R((int)I.size(), (int)J.size());
+  for (int j = 0; j < (int)J.size(); j++)
+    for (int i = 0; i < (int)I.size(); i++) R(i, j) = A(I[i], J[j]);
+  return R;
+}
+
+// Gather rows I of all columns.
+Mat gather_rows(const Mat& A, const std::vector<int>& I) {
+  Mat R((int)I.size(), A.w);
+  for (int j = 0; j < A.w; j++)
+    for (int i = 0; i < (int)I.size(); i++) R(i, j) = A(I[i], j);
+  return R;
+}
+
+// LU with partial pivoting (ie_mat.cpp:449-455).
+void lu_factor(const Mat& X, Mat* LU, std::vector<int>* piv, int* info) {
+  *LU = X;
+  piv->assign(X.h, 0);
+  int inf = 0;
+  if (X.h > 0)
+    inf = scipy_LAPACKE_dgetrf(kColMajor, X.h, X.w, LU->data(), X.h,
+                               piv->data());
+  if (info) *info = inf;
+}
+
+// X^{-1} K (ie_mat.cpp:490-503, dgetrs 'N').
+Mat lu_left_solve(const Mat& LU, const std::vector<int>& piv, const Mat& K) {
+  Mat U = K;
+  if (LU.h == 0 || K.w == 0) return U;
+  scipy_LAPACKE_dgetrs(kColMajor, 'N', LU.h, U.w, LU.data(), LU.h, piv.data(),
+                       U.data(), U.h);
+  return U;
+}
+
+// K X^{-1} (ie_mat.cpp:506-519: dgetrs 'T' on K^T, transposed back).
+Mat lu_right_solve(const Mat& LU, const std::vector<int>& piv, const Mat& K) {
+  Mat Kt = transpose(K);
+  if (LU.h > 0 && Kt.w > 0)
+    scipy_LAPACKE_dgetrs(kColMajor, 'T', LU.h, Kt.w, LU.data(), LU.h,
+                         piv.data(), Kt.data(), Kt.h);
+  return transpose(Kt);
+}
+
+// Interpolative decomposition by column-pivoted QR (ie_mat.cpp:548-588).
+// Returns the skeleton count k (0: no compression).  perm holds the dgeqp3
+// column order (0-based); T = R11^{-1} R12 is k x (n-k).
+int interp_decomp(const Mat& A, double tol, std::vector<int>* perm, Mat* T) {
+  Mat R = A;
+  const int m = A.h, n = A.w;
+  std::vector<int> jpvt(n, 0);
+  std::vector<double> tau(std::max(1, std::min(m, n)));
+  scipy_LAPACKE_dgeqp3(kColMajor, m, n, R.data(), m, jpvt.data(), tau.data());
+  const double thresh = std::fabs(tol * R(0, 0));
+  const int diag = std::min(m, n);
+  int k = 0;
+  for (int i = 1; i < diag; i++) {
+    if (std::fabs(R(i, i)) < thresh) {
+      k = i;
+      break;
+    }
+  }
+  if (k == 0) return 0;
+  perm->clear();
+  for (int i = 0; i < n; i++) perm->push_back(jpvt[i] - 1);
+  const int r = n - k;
+  if (r > 0)
+    scipy_LAPACKE_dtrtrs(kColMajor, 'U', 'N', 'N', k, r, R.data(), m,
+                         R.data() + (size_t)m * k, m);
+  *T = Mat(k, r);
+  for (int j = 0; j < r; j++)
+    for (int i = 0; i < k; i++) (*T)(i, j) = R(i, k + j);
+  return k;
+}
+
+struct Geometry {
+  int pde = kPdeLaplace;
+  int sol_dim = 1;
+  int n_points = 0;
+  std::vector<double> pts, nrm, curv, wts;
+};
+
+// Double-layer kernel block K(I,J) over DOF indices
+// (kernel.cpp:123-165 Laplace, :212-290 Stokes).
+Mat kernel_block(const Geometry& g, const std::vector<int>& I,
+                 const std::vector<int>& J) {
+  Mat K((int)I.size(), (int)J.size());
+  if (g.pde == kPdeLaplace) {
+    const double scale = 1.0 / (2 * M_PI);
+    for (int j = 0; j < (int)J.size(); j++) {
+      const int s = J[j];
+      const double sp1 = g.pts[2 * s], sp2 = g.pts[2 * s + 1];
+      const double sn1 = g.nrm[2 * s], sn2 = g.nrm[2 * s + 1];
+      const double sw = g.wts[s], sc = g.curv[s];
+      for (int i = 0; i < (int)I.size(); i++) {
+        const int t = I[i];
+        const double tp1 = g.pts[2 * t], tp2 = g.pts[2 * t + 1];
+        if (tp1 == sp1 && tp2 == sp2) {
+          K(i, j) = 0.5 + 0.5 * sc * sw * scale;
+        } else {
+          const double r0 = tp1 - sp1, r1 = tp2 - sp2;
+          K(i, j) = -sw * scale * (r0 * sn1 + r1 * sn2) / (r0 * r0 + r1 * r1);
+        }
+      }
+    }
+  } else {
+    const double scale = 1.0 / M_PI;
+    for (int j = 0; j < (int)J.size(); j++) {
+      const int s = J[j], sp = s / 2;
+      const double sp1 = g.pts[2 * sp], sp2 = g.pts[2 * sp + 1];
+      const double sn1 = g.nrm[2 * sp], sn2 = g.nrm[2 * sp + 1];
+      const double sw = g.wts[sp], sc = g.curv[sp];
+      for (int i = 0; i < (int)I.size(); i++) {
+        const int t = I[i], tp = t / 2;
+        const double tp1 = g.pts[2 * tp], tp2 = g.pts[2 * tp + 1];
+        const double tn1 = g.nrm[2 * tp], tn2 = g.nrm[2 * tp + 1];
+        const bool te = (t % 2 == 0), se = (s % 2 == 0);
+        double v;
+        if (tp1 == sp1 && tp2 == sp2) {
+          const double pot = -0.5 * sc * sw * scale;
+          if (te && se) v = -0.5 + pot * sn2 * sn2 + sw * tn1 * sn1;
+          else if (te) v = -pot * sn1 * sn2 + sw * tn2 * sn1;
+          else if (se) v = -pot * sn1 * sn2 + sw * tn2 * sn1;
+          else v = -0.5 + pot * sn1 * sn1 + sw * tn2 * sn2;
+        } else {
+          const double r0 = tp1 - sp1, r1 = tp2 - sp2;
+          const double d2 = r0 * r0 + r1 * r1;
+          const double pot = sw * scale * (r0 * sn1 + r1 * sn2) / (d2 * d2);
+          if (te && se) v = pot * r0 * r0 + sw * tn1 * sn1;
+          else if (te) v = pot * r0 * r1 + sw * tn1 * sn2;
+          else if (se) v = pot * r1 * r0 + sw * tn2 * sn1;
+          else v = pot * r1 * r1 + sw * tn2 * sn2;
+        }
+        K(i, j) = v;
+      }
+    }
+  }
+  return K;
+}
+
+// Proxy-circle block (kernel.cpp:426-591): 128 points on radius r around
+// (cx,cy).  Rows [0,P): proxy <- box; rows [P,2P): (box <- proxy)^T, with
+// P = 128 (Laplace) or 256 (Stokes, two components per point).
+Mat proxy_block(const Geometry& g, double cx, double cy, double r,
+                const std::vector<int>& box) {
+  const double pw = 2.0 * M_PI * r / kNumProxy;
+  double px[kNumProxy], py[kNumProxy], pn1[kNumProxy], pn2[kNumProxy];
+  for (int i = 0; i < kNumProxy; i++) {
+    const double ang = 2 * M_PI * i * (1.0 / kNumProxy);
+    px[i] = cx + r * std::cos(ang);
+    py[i] = cy + r * std::sin(ang);
+    pn1[i] = std::cos(ang);
+    pn2[i] = std::sin(ang);
+  }
+  const int nb = (int)box.size();
+  if (g.pde == kPdeLaplace) {
+    const double scale = 1.0 / (2 * M_PI);
+    Mat M(2 * kNumProxy, nb);
+    for (int j = 0; j < nb; j++) {
+      const int s = box[j];
+      const double sp1 = g.pts[2 * s], sp2 = g.pts[2 * s + 1];
+      const double sn1 = g.nrm[2 * s], sn2 = g.nrm[2 * s + 1];
+      const double sw = g.wts[s];
+      for (int p = 0; p < kNumProxy; p++) {
+        const double r0 = px[p] - sp1, r1 = py[p] - sp2;
+        M(p, j) = -sw * scale * (r0 * sn1 + r1 * sn2) / (r0 * r0 + r1 * r1);
+      }
+    }
+    for (int p = 0; p < kNumProxy; p++) {
+      for (int i = 0; i < nb; i++) {
+        const int t = box[i];
+        const double r0 = g.pts[2 * t] - px[p], r1 = g.pts[2 * t + 1] - py[p];
+        M(kNumProxy + p, i) =
+            -pw * scale * (r0 * pn1[p] + r1 * pn2[p]) / (r0 * r0 + r1 * r1);
+      }
+    }
+    return M;
+  }
+  const double scale = 1.0 / M_PI;
+  const int P = 2 * kNumProxy;
+  Mat M(2 * P, nb);
+  for (int j = 0; j < nb; j++) {  // box (source) -> proxy (target)
+    const int s = box[j], sp = s / 2;
+    const double sp1 = g.pts[2 * sp], sp2 = g.pts[2 * sp + 1];
+    const double sn1 = g.nrm[2 * sp], sn2 = g.nrm[2 * sp + 1];
+    const double sw = g.wts[sp];
+    for (int p = 0; p < kNumProxy; p++) {
+      const double tn1 = pn1[p], tn2 = pn2[p];
+      const double r0 = px[p] - sp1, r1 = py[p] - sp2;
+      const double d2 = r0 * r0 + r1 * r1;
+      const double pot = sw * scale * (r0 * sn1 + r1 * sn2) / (d2 * d2);
+      if (s % 2 == 0) {
+        M(2 * p, j) = pot * r0 * r0 + sw * tn1 * sn1;
+        M(2 * p + 1, j) = pot * r1 * r0 + sw * tn2 * sn1;
+      } else {
+        M(2 * p, j) = pot * r0 * r1 + sw * tn1 * sn2;
+        M(2 * p + 1, j) = pot * r1 * r1 + sw * tn2 * sn2;
+      }
+    }
+  }
+  for (int p = 0; p < kNumProxy; p++) {  // proxy (source) -> box (target)
+    const double sn1 = pn1[p], sn2 = pn2[p];
+    for (int i = 0; i < nb; i++) {
+      const int t = box[i], tp = t / 2;
+      const double tn1 = g.nrm[2 * tp], tn2 = g.nrm[2 * tp + 1];
+      const double r0 = g.pts[2 * tp] - px[p], r1 = g.pts[2 * tp + 1] - py[p];
+      const double d2 = r0 * r0 + r1 * r1;
+      const double pot = pw * scale * (r0 * sn1 + r1 * sn2) / (d2 * d2);
+      if (t % 2 == 0) {
+        M(P + 2 * p, i) = pot * r0 * r0 + pw * tn1 * sn1;
+        M(P + 2 * p + 1, i) = pot * r0 * r1 + pw * tn1 * sn2;
+      } else {
+        M(P + 2 * p, i) = pot * r1 * r0 + pw * tn2 * sn1;
+        M(P + 2 * p + 1, i) = pot * r1 * r1 + pw * tn2 * sn2;
+      }
+    }
+  }
+  return M;
+}
+
+struct Node {
+  int id = 0, level = 0;
+  bool is_leaf = true, compressed = false;
+  double corners[8] = {0};  // {BL, TL, TR, BR} as (x,y) pairs
+  double side = 0;
+  Node* parent = nullptr;
+  Node* children[4] = {nullptr, nullptr, nullptr, nullptr};  // bl, tl, tr, br
+  std::vector<Node*> neighbors;
+  std::vector<int> original, active, skel, redundant, near, perm;
+  int m_rows = 0;  // ID matrix height at compression time
+  Mat T, L, U, Xlu, schur, id_mat;
+  std::vector<int> piv;
+};
+
+struct Problem {
+  Geometry g;
+  int leaf_cap = 64;
+  std::vector<std::vector<Node*>> levels;
+  std::vector<Node*> nodes;  // by id (creation order)
+  Node* root = nullptr;
+  bool record_id_mats = false;
+  // factor state
+  std::vector<int> allskel;
+  Mat allskel_mat, allskel_lu;
+  std::vector<int> allskel_piv;
+  int n_holes = 0;
+  std::vector<double> hole_cx, hole_cy, hole_r;
+  Mat Umat, Psi, S_lu;
+  std::vector<int> S_piv;
+  bool factored = false;
+  int status = 0;
+
+  ~Problem() {
+    for (Node* n : nodes) delete n;
+  }
+
+  Node* new_node() {
+    Node* n = new Node();
+    n->id = (int)nodes.size();
+    nodes.push_back(n);
+    return n;
+  }
+
+  int ndof() const { return g.n_points * g.sol_dim; }
+
+  // quadtree.cpp:275-432 (node_subdivide).
+  void subdivide(Node* node) {
+    node->is_leaf = false;
+    const double midx = ((node->corners[6] - node->corners[0]) / 2.0) +
+                        node->corners[0];
+    const double midy = ((node->corners[3] - node->corners[1]) / 2.0) +
+                        node->corners[1];
+    Node* bl = new_node();
+    Node* tl = new_node();
+    Node* tr = new_node();
+    Node* br = new_node();
+    const double* c = node->corners;
+    const double cbl[8] = {c[0], c[1], c[2], midy, midx, midy, midx, c[7]};
+    const double ctl[8] = {c[0], midy, c[2], c[3], midx, c[5], midx, midy};
+    const double ctr[8] = {midx, midy, midx, c[3], c[4], c[5], c[6], midy};
+    const double cbr[8] = {midx, c[1], midx, midy, c[4], midy, c[6], c[7]};
+    std::memcpy(bl->corners, cbl, sizeof(cbl));
+    std::memcpy(tl->corners, ctl, sizeof(ctl));
+    std::memcpy(tr->corners, ctr, sizeof(ctr));
+    std::memcpy(br->corners, cbr, sizeof(cbr));
+    for (Node* ch : {bl, tl, tr, br}) {
+      ch->level = node->level + 1;
+      ch->side = node->side / 2.0;
+      ch->parent = node;
+    }
+    if ((int)levels.size() < node->level + 2) levels.emplace_back();
+    auto& lv = levels[node->level + 1];
+    lv.push_back(bl);
+    lv.push_back(br);
+    lv.push_back(tr);
+    lv.push_back(tl);
+    node->children[0] = bl;
+    node->children[1] = tl;
+    node->children[2] = tr;
+    node->children[3] = br;
+    const int sd = g.sol_dim;
+    for (size_t idx = 0; idx < node->original.size(); idx += sd) {
+      const int mi = node->original[idx];
+      const int pi = mi / sd;
+      const double x = g.pts[2 * pi], y = g.pts[2 * pi + 1];
+      Node* dst;
+      if (x < midx && y < midy) dst = bl;
+      else if (x < midx && y >= midy) dst = tl;
+      else if (x >= midx && y < midy) dst = br;
+      else dst = tr;
+      for (int i = 0; i < sd; i++) dst->original.push_back(mi + i);
+    }
+    for (Node* ch : {bl, tl, tr, br})
+      if ((int)ch->original.size() > leaf_cap) subdivide(ch);
+  }
+
+  // quadtree.cpp:230-269 (recursive_add).
+  void add_point(Node* node, double x, double y, int pi) {
+    for (int i = 0; i < g.sol_dim; i++)
+      node->original.push_back(g.sol_dim * pi + i);
+    const double midx = ((node->corners[6] - node->corners[0]) / 2.0) +
+                        node->corners[0];
+    const double midy = ((node->corners[3] - node->corners[1]) / 2.0) +
+                        node->corners[1];
+    Node* child;
+    if (x < midx && y < midy) child = node->children[0];
+    else if (x < midx && y >= midy) child = node->children[1];
+    else if (x >= midx && y < midy) child = node->children[3];
+    else child = node->children[2];
+    if (child) {
+      add_point(child, x, y, pi);
+    } else if (node->is_leaf && (int)node->original.size() > leaf_cap) {
+      subdivide(node);
+    }
+  }
+
+  // quadtree.cpp:185-227 (get_descendent_neighbors), wall test kept as
+  // written there (the side-wall branch fires only below the bottom edge).
+  void descendant_neighbors(Node* big, Node* small) {
+    const double top = big->corners[3], bottom = big->corners[1];
+    const double left = big->corners[0], right = big->corners[4];
+    for (int i = 0; i < 8; i += 2) {
+      const double x = small->corners[i], y = small->corners[i + 1];
+      bool hit = false;
+      if (x > left && x < right) {
+        if (std::fabs(y - top) < 1e-14 || std::fabs(y - bottom) < 1e-14)
+          hit = true;
+      }
+      if (!hit && y < top && y < bottom) {
+        if (std::fabs(x - left) < 1e-14 || std::fabs(x - right) < 1e-14)
+          hit = true;
+      }
+      if (hit) {
+        big->neighbors.push_back(small);
+        small->neighbors.push_back(big);
+        break;
+      }
+    }
+    for (Node* ch : small->children)
+      if (ch) descendant_neighbors(big, ch);
+  }
+
+  // quadtree.cpp:39-180 (initialize_tree).
+  void build_tree() {
+    double mn = g.pts[0], mx = g.pts[0];
+    for (double p : g.pts) {
+      if (p < mn) mn = p;
+      if (p > mx) mx = p;
+    }
+    const double tmin = mn - 0.01, tmax = mx + 0.01;
+    root = new_node();
+    root->level = 0;
+    const double rc[8] = {tmin, tmin, tmin, tmax, tmax, tmax, tmax, tmin};
+    std::memcpy(root->corners, rc, sizeof(rc));
+    root->side = tmax - tmin;
+    levels.push_back({root});
+    for (int i = 0; i < g.n_points; i++)
+      add_point(root, g.pts[2 * i], g.pts[2 * i + 1], i);
+    for (size_t lv = 0; lv < levels.size(); lv++) {
+      for (Node* a : levels[lv]) {
+        if (a->parent) {
+          for (Node* sib : a->parent->children)
+            if (sib->id != a->id) a->neighbors.push_back(sib);
+          for (Node* pn : a->parent->neighbors) {
+            for (Node* cz : pn->children) {
+              if (!cz || cz->level != a->level) continue;
+              const double dx = a->corners[0] - cz->corners[0];
+              const double dy = a->corners[1] - cz->corners[1];
+              const double dist = std::sqrt(dx * dx + dy * dy);
+              if (dist < a->side * std::sqrt(2.0) + 1e-5)
+                a->neighbors.push_back(cz);
+            }
+          }
+        }
+        if (a->is_leaf) {
+          for (size_t n = 0; n < a->neighbors.size(); n++) {
+            Node* nb = a->neighbors[n];
+            if (nb->level != a->level) continue;
+            for (Node* ch : nb->children)
+              if (ch) descendant_neighbors(a, ch);
+          }
+        }
+      }
+    }
+  }
+
+  // quadtree.cpp:1028-1074 (remove_inactive_dofs_at_box, source lists).
+  void refresh_box(Node* node) {
+    node->skel.clear();
+    node->redundant.clear();
+    node->active.clear();
+    if (!node->is_leaf) {
+      for (Node* ch : node->children) {
+        const auto& src = ch->compressed ? ch->skel : ch->active;
+        node->active.insert(node->active.end(), src.begin(), src.end());
+      }
+    } else {
+      node->active = node->original;
+    }
+  }
+
+  // quadtree.cpp:997-1025 (remove_inactive_dofs_at_level).
+  void refresh_level(int lv) {
+    for (Node* n : levels[lv])
+      if (!n->compressed) refresh_box(n);
+    for (Node* a : levels[lv]) {
+      a->near.clear();
+      for (Node* nb : a->neighbors) {
+        if (nb->level > a->level) continue;
+        const auto& src = nb->is_leaf ? nb->original : nb->active;
+        a->near.insert(a->near.end(), src.begin(), src.end());
+      }
+    }
+  }
+
+  // kernel.cpp:293-423 (get_id_mat, weak admissibility).
+  Mat id_matrix(const Node* node) const {
+    const double cx = node->corners[0] + node->side / 2.0;
+    const double cy = node->corners[1] + node->side / 2.0;
+    const std::vector<int>& box = node->active;
+    std::vector<int> rows;
+    bool use_proxy = true;
+    if (node->level == 1 || node->level == 2) {
+      use_proxy = false;
+      for (Node* o : levels[node->level])
+        if (o->id != node->id)
+          rows.insert(rows.end(), o->active.begin(), o->active.end());
+      if (node->level == 2)
+        for (Node* o : levels[1])
+          if (o->is_leaf)
+            rows.insert(rows.end(), o->original.begin(), o->original.end());
+    } else {
+      for (int mi : node->near) {
+        const int pi = mi / g.sol_dim;
+        const double x = g.pts[2 * pi], y = g.pts[2 * pi + 1];
+        const double dist = std::sqrt(std::pow(cx - x, 2) + std::pow(cy - y, 2));
+        if (dist < kRadiusRatio * node->side) rows.push_back(mi);
+      }
+    }
+    Mat pxy;
+    if (use_proxy) pxy = proxy_block(g, cx, cy, node->side * kRadiusRatio, box);
+    const int nr = (int)rows.size(), nc = (int)box.size();
+    Mat M(2 * nr + pxy.h, nc);
+    Mat A = kernel_block(g, rows, box);
+    Mat B = kernel_block(g, box, rows);
+    for (int j = 0; j < nc; j++) {
+      for (int i = 0; i < nr; i++) M(i, j) = A(i, j);
+      for (int i = 0; i < nr; i++) M(nr + i, j) = B(j, i);
+      for (int i = 0; i < pxy.h; i++) M(2 * nr + i, j) = pxy(i, j);
+    }
+    return M;
+  }
+
+  // skel_factorization.cpp:340-381: add every compressed descendant's
+  // schur_update onto the BN entries it shares (pre-order over children).
+  void add_update(Mat* upd, const std::unordered_map<int, int>& where,
+                  const Node* d) const {
+    std::vector<int> sn_idx, bn_idx;
+    for (int a = 0; a < (int)d->skel.size(); a++) {
+      auto it = where.find(d->skel[a]);
+      if (it != where.end()) {
+        sn_idx.push_back(a);
+        bn_idx.push_back(it->second);
+      }
+    }
+    for (size_t i = 0; i < bn_idx.size(); i++)
+      for (size_t j = 0; j < bn_idx.size(); j++)
+        (*upd)(bn_idx[i], bn_idx[j]) += d->schur(sn_idx[i], sn_idx[j]);
+  }
+  void descendant_updates(Mat* upd, const std::unordered_map<int, int>& where,
+                          const Node* node) const {
+    for (Node* ch : node->children) {
+      if (ch->compressed) add_update(upd, where, ch);
+      if (!ch->is_leaf) descendant_updates(upd, where, ch);
+    }
+  }
+  Mat schur_updates(const std::vector<int>& BN, const Node* node) const {
+    Mat upd((int)BN.size(), (int)BN.size());
+    std::unordered_map<int, int> where;
+    for (int i = 0; i < (int)BN.size(); i++) where[BN[i]] = i;
+    if (!node->is_leaf) descendant_updates(&upd, where, node);
+    return upd;
+  }
+
+  // skel_factorization.cpp:38-68 (id_compress).
+  int compress(Node* node, double tol) {
+    Mat M = id_matrix(node);
+    node->m_rows = M.h;
+    if (record_id_mats) node->id_mat = M;
+    if (M.h == 0) return 0;
+    std::vector<int> p;
+    Mat T;
+    const int k = interp_decomp(M, tol, &p, &T);
+    if (k == 0) return 0;
+    node->T = T;
+    node->perm = p;
+    node->skel.clear();
+    node->redundant.clear();
+    const int n = (int)node->active.size();
+    for (int i = 0; i < k; i++) node->skel.push_back(node->active[p[i]]);
+    for (int i = k; i < n; i++) node->redundant.push_back(node->active[p[i]]);
+    return k;
+  }
+
+  // skel_factorization.cpp:77-128 (decouple).
+  int decouple(Node* node) {
+    const std::vector<int>& BN = node->active;
+    Mat upd = schur_updates(BN, node);
+    Mat KBN = sub(kernel_block(g, BN, BN), upd);
+    const int k = (int)node->skel.size(), r = (int)node->redundant.size();
+    std::vector<int> s(node->perm.begin(), node->perm.begin() + k);
+    std::vector<int> rr(node->perm.begin() + k, node->perm.end());
+    const Mat& T = node->T;
+    Mat Tt = transpose(T);
+    Mat Krs = sub(gather(KBN, rr, s), mul(Tt, gather(KBN, s, s)));
+    Mat X = sub(sub(gather(KBN, rr, rr), mul(Tt, gather(KBN, s, rr))),
+                mul(Krs, T));
+    Mat Ksr = sub(gather(KBN, s, rr), mul(gather(KBN, s, s), T));
+    int info = 0;
+    lu_factor(X, &node->Xlu, &node->piv, &info);
+    node->L = lu_right_solve(node->Xlu, node->piv, Ksr);
+    node->U = lu_left_solve(node->Xlu, node->piv, Krs);
+    node->schur = mul(node->L, Krs);
+    node->compressed = true;
+    (void)r;
+    return info;
+  }
+
+  // vec(b) += M * vec(a) or M^T * vec(a) over all columns of X
+  // (skel_factorization.cpp:388-406).
+  static void sweep(const Mat& M, Mat* X, const std::vector<int>& a,
+                    const std::vector<int>& b, bool trans, double sign) {
+    if (a.empty() || b.empty()) return;
+    Mat xa = gather_rows(*X, a);
+    Mat prod = trans ? mul(transpose(M), xa) : mul(M, xa);
+    for (int c = 0; c < X->w; c++)
+      for (int i = 0; i < (int)b.size(); i++)
+        (*X)(b[i], c) += sign * prod(i, c);
+  }
+
+  // vec(rows) = LU^{-1} vec(rows) (skel_factorization.cpp:419-426).
+  static void diag_inv(const Mat& LU, const std::vector<int>& piv, Mat* X,
+                       const std::vector<int>& rows) {
+    if (rows.empty()) return;
+    Mat xr = gather_rows(*X, rows);
+    Mat y = lu_left_solve(LU, piv, xr);
+    for (int c = 0; c < X->w; c++)
+      for (int i = 0; i < (int)rows.size(); i++) (*X)(rows[i], c) = y(i, c);
+  }
+
+  std::vector<Node*> all_nodes_bottom_up() const {
+    std::vector<Node*> v;
+    for (int lv = (int)levels.size() - 1; lv >= 0; lv--)
+      for (Node* n : levels[lv]) v.push_back(n);
+    return v;
+  }
+
+  void upward_T_L(Mat* X, int nt) const {
+    for (int lv = (int)levels.size() - 1; lv >= 0; lv--) {
+      const auto& L = levels[lv];
+#pragma omp parallel for num_threads(nt) schedule(dynamic)
+      for (int i = 0; i < (int)L.size(); i++) {
+        const Node* n = L[i];
+        if (!n->compressed) continue;
+        sweep(n->T, X, n->skel, n->redundant, true, -1.0);
+        sweep(n->L, X, n->redundant, n->skel, false, -1.0);
+      }
+    }
+  }
+  void upward_T_Ut(Mat* X, int nt) const {
+    for (int lv = (int)levels.size() - 1; lv >= 0; lv--) {
+      const auto& L = levels[lv];
+#pragma omp parallel for num_threads(nt) schedule(dynamic)
+      for (int i = 0; i < (int)L.size(); i++) {
+        const Node* n = L[i];
+        if (!n->compressed) continue;
+        sweep(n->T, X, n->skel, n->redundant, true, -1.0);
+        sweep(n->U, X, n->redundant, n->skel, true, -1.0);
+      }
+    }
+  }
+  void diag_all(Mat* X, int nt) const {
+    std::vector<Node*> all = all_nodes_bottom_up();
+#pragma omp parallel for num_threads(nt) schedule(dynamic)
+    for (int i = 0; i < (int)all.size(); i++) {
+      const Node* n = all[i];
+      if (n->redundant.empty() || !n->compressed) continue;
+      diag_inv(n->Xlu, n->piv, X, n->redundant);
+    }
+  }
+  void downward_U_T(Mat* X, int nt) const {
+    for (int lv = 0; lv < (int)levels.size(); lv++) {
+      const auto& L = levels[lv];
+#pragma omp parallel for num_threads(nt) schedule(dynamic)
+      for (int i = (int)L.size() - 1; i >= 0; i--) {
+        const Node* n = L[i];
+        if (!n->compressed) continue;
+        sweep(n->U, X, n->skel, n->redundant, false, -1.0);
+        sweep(n->T, X, n->redundant, n->skel, false, -1.0);
+      }
+    }
+  }
+
+  // linear_solve.cpp:20-112 (initialize_U_mat / initialize_Psi_mat on the
+  // boundary points).
+  void build_U_Psi() {
+    const int np = g.n_points, h = n_holes;
+    if (h == 0) {
+      Umat = Mat();
+      Psi = Mat();
+      return;
+    }
+    if (g.pde == kPdeLaplace) {
+      Umat = Mat(np, h);
+      Psi = Mat(h, np);
+      for (int i = 0; i < np; i++) {
+        for (int k = 0; k < h; k++) {
+          const double r0 = g.pts[2 * i] - hole_cx[k];
+          const double r1 = g.pts[2 * i + 1] - hole_cy[k];
+          Umat(i, k) = std::log(std::sqrt(r0 * r0 + r1 * r1));
+        }
+        for (int k = 0; k < h; k++) {
+          const double r0 = g.pts[2 * i] - hole_cx[k];
+          const double r1 = g.pts[2 * i + 1] - hole_cy[k];
+          if (std::sqrt(r0 * r0 + r1 * r1) < hole_r[k] + 1e-8) {
+            Psi(k, i) = g.wts[i];
+            break;
+          }
+        }
+      }
+      return;
+    }
+    Umat = Mat(2 * np, 3 * h);
+    Psi = Mat(3 * h, 2 * np);
+    const double scale = 1.0 / (4 * M_PI);
+    for (int i = 0; i < np; i++) {
+      const double x0 = g.pts[2 * i], x1 = g.pts[2 * i + 1];
+      for (int k = 0; k < h; k++) {
+        const double r0 = x0 - hole_cx[k], r1 = x1 - hole_cy[k];
+        const double rn = std::sqrt(r0 * r0 + r1 * r1);
+        const double rn2 = std::pow(rn, 2);
+        Umat(2 * i, 3 * k) = scale * (std::log(1 / rn) + (1.0 / rn2) * r0 * r0);
+        Umat(2 * i + 1, 3 * k) = scale * ((1.0 / rn2) * r1 * r0);
+        Umat(2 * i, 3 * k + 1) = scale * ((1.0 / rn2) * r0 * r1);
+        Umat(2 * i + 1, 3 * k + 1) =
+            scale * (std::log(1 / rn) + (1.0 / rn2) * r1 * r1);
+        Umat(2 * i, 3 * k + 2) = r1 * (scale / rn2);
+        Umat(2 * i + 1, 3 * k + 2) = -r0 * (scale / rn2);
+      }
+      for (int k = 0; k < h; k++) {
+        const double r0 = x0 - hole_cx[k], r1 = x1 - hole_cy[k];
+        if (std::sqrt(r0 * r0 + r1 * r1) < hole_r[k] + 1e-8) {
+          const double w = g.wts[i];
+          Psi(3 * k, 2 * i) = w;
+          Psi(3 * k + 1, 2 * i + 1) = w;
+          Psi(3 * k + 2, 2 * i) = w * x1;
+          Psi(3 * k + 2, 2 * i + 1) = -w * x0;
+          break;
+        }
+      }
+    }
+  }
+
+  void split_skel_red(std::vector<int>* red) const {
+    std::vector<int> sorted = allskel;
+    std::sort(sorted.begin(), sorted.end());
+    size_t si = 0;
+    red->clear();
+    for (int i = 0; i < ndof(); i++) {
+      if (si < sorted.size() && i == sorted[si]) si++;
+      else red->push_back(i);
+    }
+  }
+
+  // skel_factorization.cpp:140-310 (skeletonize).
+  int factor(double tol, int nt) {
+    status = 0;
+    const int nl = (int)levels.size();
+    for (int lv = nl - 1; lv > 0; lv--) {
+      refresh_level(lv);
+      auto& L = levels[lv];
+      std::vector<int> st(L.size(), 0);
+#pragma omp parallel for num_threads(nt) schedule(dynamic)
+      for (int i = 0; i < (int)L.size(); i++) {
+        Node* n = L[i];
+        if (n->compressed || (int)n->active.size() < kMinDofsToCompress)
+          continue;
+        if (compress(n, tol) == 0) continue;
+        st[i] = decouple(n);
+      }
+      for (int s : st)
+        if (s != 0) status = 3;
+    }
+    for (int lv = nl - 1; lv >= 0; lv--) refresh_level(lv);
+    allskel = root->active;
+    if (!allskel.empty()) {
+      Mat upd = schur_updates(allskel, root);
+      allskel_mat = sub(kernel_block(g, allskel, allskel), upd);
+    }
+    if (n_holes == 0) {
+      int info = 0;
+      lu_factor(allskel_mat, &allskel_lu, &allskel_piv, &info);
+      if (info != 0) status = 3;
+      factored = true;
+      return status;
+    }
+    std::vector<int> red;
+    split_skel_red(&red);
+    if (red.empty()) return status = 2;  // skel_factorization.cpp:209-212
+    Mat mU = Umat;
+    Mat mPsiT = transpose(Psi);
+    upward_T_L(&mU, nt);
+    upward_T_Ut(&mPsiT, nt);
+    // D^{-1} on the redundant rows of mU: redundant rows of every node are
+    // disjoint, so applying it in place on the full-length matrix equals
+    // the reference's compacted Dinv_C_nonzero.
+    Mat DinvC = mU;
+    diag_all(&DinvC, nt);
+    const int nsk = (int)allskel.size(), hh = Psi.h;
+    Mat S(nsk + hh, nsk + hh);
+    for (int j = 0; j < nsk; j++)
+      for (int i = 0; i < nsk; i++) S(i, j) = allskel_mat(i, j);
+    for (int j = 0; j < nsk; j++)
+      for (int i = 0; i < hh; i++) S(nsk + i, j) = mPsiT(allskel[j], i);
+    for (int j = 0; j < hh; j++)
+      for (int i = 0; i < nsk; i++) S(i, nsk + j) = mU(allskel[i], j);
+    Mat PsiRed = transpose(gather_rows(mPsiT, red));  // hh x |red|
+    Mat CRed = gather_rows(DinvC, red);               // |red| x hh
+    Mat PC = mul(PsiRed, CRed);
+    for (int j = 0; j < hh; j++)
+      for (int i = 0; i < hh; i++)
+        S(nsk + i, nsk + j) = (i == j ? -1.0 : 0.0) - PC(i, j);
+    int info = 0;
+    lu_factor(S, &S_lu, &S_piv, &info);
+    if (info != 0) status = 3;
+    factored = true;
+    return status;
+  }
+
+  // skel_factorization.cpp:507-592 (solve).
+  void solve(const double* b, double* x, int nt) const {
+    Mat X(ndof(), 1);
+    std::memcpy(X.data(), b, sizeof(double) * ndof());
+    upward_T_L(&X, nt);
+    diag_all(&X, nt);
+    if (!allskel.empty()) diag_inv(allskel_lu, allskel_piv, &X, allskel);
+    downward_U_T(&X, nt);
+    std::memcpy(x, X.data(), sizeof(double) * ndof());
+  }
+
+  // skel_factorization.cpp:596-777 (multiply_connected_solve).
+  void solve_holes(const double* b, double* mu, double* alpha, int nt) const {
+    std::vector<int> red;
+    split_skel_red(&red);
+    Mat X(ndof(), 1);
+    std::memcpy(X.data(), b, sizeof(double) * ndof());
+    Mat mU = Umat;
+    upward_T_L(&X, nt);
+    upward_T_L(&mU, nt);
+    Mat w = gather_rows(X, red);
+    Mat z = gather_rows(X, allskel);
+    Mat mPsiT = transpose(Psi);
+    upward_T_Ut(&mPsiT, nt);
+    // Block-diagonal D^{-1} applied on the full-length vector, then
+    // restricted to the redundant rows.
+    Mat Dw = X;
+    diag_all(&Dw, nt);
+    Mat Dinv_w = gather_rows(Dw, red);
+    const int nsk = (int)allskel.size(), hh = Psi.h;
+    Mat M(nsk + hh, 1);
+    for (int i = 0; i < nsk; i++) M(i, 0) = z(i, 0);
+    Mat PsiRed = transpose(gather_rows(mPsiT, red));
+    Mat PD = mul(PsiRed, Dinv_w);
+    for (int i = 0; i < hh; i++) M(nsk + i, 0) = -PD(i, 0);
+    Mat y = lu_left_solve(S_lu, S_piv, M);
+    for (int i = 0; i < hh; i++) alpha[i] = y(nsk + i, 0);
+    Mat alph(hh, 1);
+    for (int i = 0; i < hh; i++) alph(i, 0) = alpha[i];
+    Mat N = sub(w, mul(gather_rows(mU, red), alph));
+    Mat Nfull(ndof(), 1);
+    for (size_t i = 0; i < red.size(); i++) Nfull(red[i], 0) = N(i, 0);
+    diag_all(&Nfull, nt);
+    for (size_t i = 0; i < red.size(); i++) X(red[i], 0) = Nfull(red[i], 0);
+    for (int i = 0; i < nsk; i++) X(allskel[i], 0) = y(i, 0);
+    downward_U_T(&X, nt);
+    std::memcpy(mu, X.data(), sizeof(double) * ndof());
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+void* rso_create(int pde, int n_points, const double* points,
+                 const double* normals, const double* curvatures,
+                 const double* weights, int leaf_cap) {
+  if (n_points <= 0 || (pde != kPdeLaplace && pde != kPdeStokes)) return nullptr;
+  Problem* p = new Problem();
+  p->g.pde = pde;
+  p->g.sol_dim = (pde == kPdeStokes) ? 2 : 1;
+  p->g.n_points = n_points;
+  p->g.pts.assign(points, points + 2 * n_points);
+  p->g.nrm.assign(normals, normals + 2 * n_points);
+  p->g.curv.assign(curvatures, curvatures + n_points);
+  p->g.wts.assign(weights, weights + n_points);
+  p->leaf_cap = leaf_cap;
+  p->build_tree();
+  return p;
+}
+
+void rso_destroy(void* h) { delete static_cast<Problem*>(h); }
+
+void rso_record_id_mats(void* h, int on) {
+  static_cast<Problem*>(h)->record_id_mats = on != 0;
+}
+
+int rso_set_holes(void* h, int n_holes, const double* centers,
+                  const double* radii) {
+  Problem* p = static_cast<Problem*>(h);
+  p->n_holes = n_holes;
+  p->hole_cx.clear();
+  p->hole_cy.clear();
+  p->hole_r.clear();
+  for (int k = 0; k < n_holes; k++) {
+    p->hole_cx.push_back(centers[2 * k]);
+    p->hole_cy.push_back(centers[2 * k + 1]);
+    p->hole_r.push_back(radii[k]);
+  }
+  p->build_U_Psi();
+  return 0;
+}
+
+int rso_hole_dim(void* h) {
+  Problem* p = static_cast<Problem*>(h);
+  return p->n_holes == 0 ? 0 : p->Psi.h;
+}
+
+// U (ndof x hdim) and Psi (hdim x ndof), column-major.
+int rso_get_U_Psi(void* h, double* U, double* Psi) {
+  Problem* p = static_cast<Problem*>(h);
+  if (p->n_holes == 0) return 0;
+  std::memcpy(U, p->Umat.data(), sizeof(double) * p->Umat.a.size());
+  std::memcpy(Psi, p->Psi.data(), sizeof(double) * p->Psi.a.size());
+  return 0;
+}
+
+int rso_factor(void* h, double id_tol, int num_threads) {
+  scipy_openblas_set_num_threads(1);
+  if (!(id_tol > 0)) return 1;
+  return static_cast<Problem*>(h)->factor(id_tol, num_threads);
+}
+
+int rso_solve(void* h, const double* b, double* mu, double* alpha,
+              int num_threads) {
+  Problem* p = static_cast<Problem*>(h);
+  if (!p->factored) return 1;
+  if (p->n_holes > 0) {
+    if (!alpha) return 1;
+    p->solve_holes(b, mu, alpha, num_threads);
+  } else {
+    p->solve(b, mu, num_threads);
+  }
+  return 0;
+}
+
+int rso_num_levels(void* h) { return (int)static_cast<Problem*>(h)->levels.size(); }
+int rso_num_boxes(void* h) { return (int)static_cast<Problem*>(h)->nodes.size(); }
+
+// Per box (by creation id): level, parent id (-1 root), leaf flag,
+// children ids (4, -1 if none), corners[0..1] and side, and, after a
+// factorization, |active|, rank (0 if not compressed) and ID height.
+int rso_box_info(void* h, int* level, int* parent, int* is_leaf,
+                 int* children, double* bl_side, int* n_active, int* rank,
+                 int* m_rows) {
+  Problem* p = static_cast<Problem*>(h);
+  for (size_t b = 0; b < p->nodes.size(); b++) {
+    const Node* n = p->nodes[b];
+    level[b] = n->level;
+    parent[b] = n->parent ? n->parent->id : -1;
+    is_leaf[b] = n->is_leaf ? 1 : 0;
+    for (int c = 0; c < 4; c++)
+      children[4 * b + c] = n->children[c] ? n->children[c]->id : -1;
+    bl_side[3 * b] = n->corners[0];
+    bl_side[3 * b + 1] = n->corners[1];
+    bl_side[3 * b + 2] = n->side;
+    n_active[b] = (int)n->active.size();
+    rank[b] = n->compressed ? (int)n->skel.size() : 0;
+    m_rows[b] = n->m_rows;
+  }
+  return 0;
+}
+
+// Box lists: which = 0 original, 1 neighbor ids, 2 active, 3 skel,
+// 4 near, 5 level order (box = level index).  Returns the list length
+// (copies at most cap entries).
+int rso_box_list(void* h, int box, int which, int* out, int cap) {
+  Problem* p = static_cast<Problem*>(h);
+  std::vector<int> v;
+  if (which == 5) {
+    if (box < 0 || box >= (int)p->levels.size()) return -1;
+    for (Node* n : p->levels[box]) v.push_back(n->id);
+  } else {
+    if (box < 0 || box >= (int)p->nodes.size()) return -1;
+    const Node* n = p->nodes[box];
+    switch (which) {
+      case 0: v = n->original; break;
+      case 1: for (Node* q : n->neighbors) v.push_back(q->id); break;
+      case 2: v = n->active; break;
+      case 3: v = n->skel; break;
+      case 4: v = n->near; break;
+      default: return -1;
+    }
+  }
+  for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = v[i];
+  return (int)v.size();
+}
+
+int rso_allskel(void* h, int* out, int cap) {
+  Problem* p = static_cast<Problem*>(h);
+  for (int i = 0; i < (int)p->allskel.size() && i < cap; i++)
+    out[i] = p->allskel[i];
+  return (int)p->allskel.size();
+}
+
+// ID matrix recorded at compression time (rso_record_id_mats before the
+// factorization).  Returns 0 and sets m, n; copies if cap is large enough.
+int rso_get_id_mat(void* h, int box, double* out, long cap, int* m, int* n) {
+  Problem* p = static_cast<Problem*>(h);
+  if (box < 0 || box >= (int)p->nodes.size()) return -1;
+  const Mat& M = p->nodes[box]->id_mat;
+  *m = M.h;
+  *n = M.w;
+  if ((long)M.a.size() <= cap && !M.a.empty())
+    std::memcpy(out, M.data(), sizeof(double) * M.a.size());
+  return 0;
+}
+
+// Kernel block K(I,J) on raw geometry (column-major ni x nj).
+int rso_kernel_block(int pde, int n_points, const double* points,
+                     const double* normals, const double* curvatures,
+                     const double* weights, int ni, const int* I, int nj,
+                     const int* J, double* out) {
+  Geometry g;
+  g.pde = pde;
+  g.sol_dim = (pde == kPdeStokes) ? 2 : 1;
+  g.n_points = n_points;
+  g.pts.assign(points, points + 2 * n_points);
+  g.nrm.assign(normals, normals + 2 * n_points);
+  g.curv.assign(curvatures, curvatures + n_points);
+  g.wts.assign(weights, weights + n_points);
+  Mat K = kernel_block(g, std::vector<int>(I, I + ni), std::vector<int>(J, J + nj));
+  std::memcpy(out, K.data(), sizeof(double) * K.a.size());
+  return 0;
+}
+
+// Proxy block for a box of DOFs (rows 256 Laplace / 512 Stokes).
+int rso_proxy_block(int pde, int n_points, const double* points,
+                    const double* normals, const double* curvatures,
+                    const double* weights, double cx, double cy, double r,
+                    int nb, const int* box, double* out) {
+  Geometry g;
+  g.pde = pde;
+  g.sol_dim = (pde == kPdeStokes) ? 2 : 1;
+  g.n_points = n_points;
+  g.pts.assign(points, points + 2 * n_points);
+  g.nrm.assign(normals, normals + 2 * n_points);
+  g.curv.assign(curvatures, curvatures + n_points);
+  g.wts.assign(weights, weights + n_points);
+  Mat M = proxy_block(g, cx, cy, r, std::vector<int>(box, box + nb));
+  std::memcpy(out, M.data(), sizeof(double) * M.a.size());
+  return M.h;
+}
+
+// Restated ID on a given matrix: returns k, fills perm (n) and T (k x (n-k)).
+int rso_id(int m, int n, const double* A, double tol, int* perm, double* T) {
+  scipy_openblas_set_num_threads(1);
+  Mat M(m, n);
+  std::memcpy(M.data(), A, sizeof(double) * (size_t)m * n);
+  std::vector<int> p;
+  Mat Tm;
+  const int k = interp_decomp(M, tol, &p, &Tm);
+  if (k == 0) return 0;
+  for (int i = 0; i < n; i++) perm[i] = p[i];
+  std::memcpy(T, Tm.data(), sizeof(double) * Tm.a.size());
+  return k;
+}
+
+// Interior evaluation u = K_domain * mu (initialization.cpp:11-135): the
+// Laplace kernel with an off-surface target, or the Stokes double layer
+// without the normal term.  Targets are assumed inside the domain.
+int rso_eval(int pde, int n_points, const double* points, const double* normals,
+             const double* weights, int n_targets, const double* targets,
+             const double* mu, double* u) {
+  for (int t = 0; t < n_targets; t++) {
+    const double x0 = targets[2 * t], x1 = targets[2 * t + 1];
+    if (pde == kPdeLaplace) {
+      const double scale = 1.0 / (2 * M_PI);
+      double acc = 0;
+      for (int j = 0; j < n_points; j++) {
+        const double r0 = x0 - points[2 * j], r1 = x1 - points[2 * j + 1];
+        const double v = -weights[j] * scale *
+                         (r0 * normals[2 * j] + r1 * normals[2 * j + 1]) /
+                         (r0 * r0 + r1 * r1);
+        acc += v * mu[j];
+      }
+      u[t] = acc;
+    } else {
+      const double scale = 1.0 / M_PI;
+      double a0 = 0, a1 = 0;
+      for (int j = 0; j < n_points; j++) {
+        const double r0 = x0 - points[2 * j], r1 = x1 - points[2 * j + 1];
+        const double d2 = r0 * r0 + r1 * r1;
+        const double pot = weights[j] * scale *
+                           (r0 * normals[2 * j] + r1 * normals[2 * j + 1]) /
+                           (d2 * d2);
+        a0 += pot * r0 * r0 * mu[2 * j] + pot * r0 * r1 * mu[2 * j + 1];
+        a1 += pot * r1 * r0 * mu[2 * j] + pot * r1 * r1 * mu[2 * j + 1];
+      }
+      u[2 * t] = a0;
+      u[2 * t + 1] = a1;
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"
