@@ -1,0 +1,124 @@
+"""Host-side checks of the product library (CPU only, no GPU calls)."""
+import ctypes
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import geometry as G
+import iesolver
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "iegpu.h")).read()
+    declared = set(re.findall(r"^(?:int|const char\*)\s+(iegpu_\w+)\s*\(", header, re.M))
+    out = subprocess.run(["nm", "-D", "--defined-only", iesolver.LIB_PATH],
+                         stdout=subprocess.PIPE, text=True, check=True).stdout
+    exported = set(re.findall(r" T (iegpu_\w+)", out))
+    assert declared, "no declarations parsed"
+    assert declared <= exported, declared - exported
+    assert set(iesolver.EXPORTS) <= exported
+    iesolver.load()
+
+
+@pytest.mark.parametrize("case", ["circle", "star", "wavy", "holes_stokes", "annulus_stokes"])
+def test_host_tree_matches_oracle(case):
+    if case == "circle":
+        g, pde, cap = G.circle(4096), 0, 64
+    elif case == "star":
+        g, pde, cap = G.star(20000), 0, 64
+    elif case == "wavy":
+        g, pde, cap = G.wavy(30000, seed=3), 0, 64
+    elif case == "holes_stokes":
+        g, pde, cap = G.circle_with_holes(8000, 8, 0), 2, 64
+    else:
+        g, pde, cap = G.annulus(2000), 2, 32
+    s = iesolver.RSSolver(device=-1)
+    s.set_geometry(g, pde)
+    s.build_tree(cap)
+    o = oracle.OracleProblem(g, pde=pde, leaf_cap=cap)
+    nl, nb, nd, _ = s.tree_info()
+    assert (nl, nb) == (o.num_levels(), o.num_boxes())
+    bi, oi = s.box_info(), o.box_info()
+    for k in ("level", "parent", "is_leaf", "children"):
+        assert np.array_equal(bi[k], oi[k]), k
+    for b in range(nb):
+        assert np.array_equal(s.box_list(b, 0), o.box_list(b, 0))
+        assert np.array_equal(s.box_list(b, 1), o.box_list(b, 1))
+    for lv in range(nl):
+        assert np.array_equal(s.box_list(lv, 5), o.level_order(lv))
+
+
+def test_error_codes():
+    s = iesolver.RSSolver(device=-1)
+    g = G.circle(64)
+    with pytest.raises(iesolver.IeGpuError) as e:
+        s.set_geometry(g, pde=1)
+    assert e.value.code == 1
+    bad = dict(g)
+    bad["points"] = g["points"].copy()
+    bad["points"][3, 0] = np.nan
+    with pytest.raises(iesolver.IeGpuError) as e:
+        s.set_geometry(bad)
+    assert e.value.code == 1
+    with pytest.raises(iesolver.IeGpuError) as e:
+        s.build_tree(64)  # no geometry yet
+    assert e.value.code == 1
+    s.set_geometry(g)
+    with pytest.raises(iesolver.IeGpuError) as e:
+        s.build_tree(0)
+    assert e.value.code == 1
+    s.build_tree(16)
+    with pytest.raises(iesolver.IeGpuError) as e:
+        s.factor(1e-10)
+    assert e.value.code == 7  # host-only context
+    with pytest.raises(iesolver.IeGpuError) as e:
+        s.factor(-1.0)
+    assert e.value.code in (1, 7)
+    L = iesolver.load()
+    assert L.iegpu_factor(None, 1e-10, 0) == 1
+    assert L.iegpu_destroy(None) == 1
+
+
+def test_geometry_circle_closed_form():
+    g = G.circle(1000, (0.5, 0.5), 0.5)
+    assert np.allclose(np.hypot(*(g["points"] - 0.5).T), 0.5, atol=1e-15)
+    assert np.allclose(g["normals"], (g["points"] - 0.5) / 0.5, atol=1e-14)
+    assert abs(g["weights"].sum() - math.pi) < 1e-12
+
+
+@pytest.mark.parametrize("gen", [lambda n: G.star(n), lambda n: G.wavy(n, seed=7)])
+def test_geometry_curvature_and_weights(gen):
+    n = 4000
+    g = gen(n)
+    p = g["points"]
+    # perimeter vs polygon length
+    poly = np.hypot(*(np.roll(p, -1, 0) - p).T).sum()
+    assert abs(g["weights"].sum() - poly) < 1e-5
+    # outward unit normals: orthogonal to the chord, pointing away from the centre
+    chord = np.roll(p, -1, 0) - np.roll(p, 1, 0)
+    assert np.abs(np.einsum("ij,ij->i", chord, g["normals"])).max() < 1e-3 * np.abs(chord).max()
+    assert (np.einsum("ij,ij->i", p - 0.5, g["normals"]) > 0).all()
+    # curvature: turning of the tangent per unit length
+    t = chord / np.hypot(*chord.T)[:, None]
+    ta, tb = np.roll(t, 1, 0), np.roll(t, -1, 0)
+    turn = np.arctan2(ta[:, 0] * tb[:, 1] - ta[:, 1] * tb[:, 0], np.einsum("ij,ij->i", ta, tb))
+    ds = np.hypot(*(np.roll(p, -1, 0) - np.roll(p, 1, 0)).T)
+    assert np.abs(turn / ds - g["curvatures"]).max() < 1e-2 * np.abs(g["curvatures"]).max()
+
+
+def test_holes_are_inside_and_disjoint():
+    g = G.circle_with_holes(20000, 8, seed=0)
+    c, r = g["holes"]["centers"], g["holes"]["radii"]
+    assert ((0.02 <= r) & (r <= 0.05)).all()
+    for i in range(8):
+        assert math.hypot(*(c[i] - 0.5)) + r[i] < 0.5
+        for j in range(i):
+            assert math.hypot(*(c[i] - c[j])) > r[i] + r[j]
+    assert g["points"].shape[0] == 20000
