@@ -19,7 +19,7 @@ HIP_FLAGS = [
     # keep hipcc from emitting packed FP32 arithmetic
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-HIP_SOURCES = ["rs_kernels.hip", "rs_holes.hip", "capi.cpp"]
+HIP_SOURCES = ["rs_kernels.hip", "rs_holes.hip", "rs_eval.hip", "rs_matvec.hip", "capi.cpp"]
 HOST_SOURCES = ["tree.cpp"]
 
 
@@ -49,7 +49,21 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run(["hipcc", "--offload-arch=gfx950", "-shared", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_cpp_tests()
     return LIB
+
+
+def build_cpp_tests():
+    """The C++ facade test program (tests/cpp), linked against the library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "ie_solver_gpu_test.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(root, "tests", "cpp", "ie_solver_gpu_test.bin")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"), src,
+          "-o", out, "-L", HERE, "-liesolver_amd",
+          "-Wl,-rpath,$ORIGIN/../../ie-solver_amd"])
+    return out
 
 
 if __name__ == "__main__":

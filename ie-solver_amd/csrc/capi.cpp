@@ -13,7 +13,9 @@
 #include "../../include/iegpu.h"
 #include "rs_device.hpp"
 #include "rs_holes.hpp"
+#include "rs_eval.hpp"
 #include "rs_kernels.hpp"
+#include "rs_matvec.hpp"
 #include "tree.hpp"
 
 using namespace iegpu;
@@ -165,6 +167,10 @@ struct iegpu_ctx {
   DevBuf<double> d_mU, d_mPsiT, d_DinvC, d_gram, d_partial, d_Smat, d_tail,
       d_ybuf, d_alpha;
   DevBuf<int> d_Spiv, d_mask;
+  DevBuf<double> d_hole_c;  // hole centres (2 per hole)
+  // interior evaluation buffers
+  DevBuf<double> e_tg, e_mu, e_alpha, e_partial, e_out;
+  DevBuf<int> e_mask;
 
   // factor state
   int ncap = 0;
@@ -178,6 +184,9 @@ struct iegpu_ctx {
   DevBuf<unsigned char> s_id_pool, s_fac_pool, s_root_mat;
   int64_t id_cap = 0, row_cap = 0, fac_cap = 0;
   DevBuf<double> d_work;  // solve vector for host-pointer solves
+  // iterative refinement: b, x, residual / correction (ndof), border parts,
+  // reduction partials
+  DevBuf<double> r_b, r_x, r_r, r_a, r_ra, r_part;
 
   double t_factor_ms = 0, t_solve_ms = 0, t_root_ms = 0;
   double fac_bytes = 0;
@@ -401,8 +410,11 @@ void enqueue_border_factor(iegpu_ctx* c) {
   });
 }
 
+// x (ndof) <- M^{-1} x in place; with holes alpha (hole_dim) is written and
+// g (hole_dim, device, may be NULL = 0) is the border right-hand side.
 template <typename R>
-void enqueue_solve(iegpu_ctx* c, double* x, double* alpha, hipStream_t st) {
+void enqueue_solve(iegpu_ctx* c, double* x, double* alpha, hipStream_t st,
+                   const double* g = nullptr) {
   DevState s = c->dstate();
   const HostTree& T = c->tree;
   const int64_t nd = c->ndof();
@@ -421,7 +433,7 @@ void enqueue_solve(iegpu_ctx* c, double* x, double* alpha, hipStream_t st) {
       const int h = c->hole_dim;
       launch_border_gemv(c->d_mPsiT.p, x, c->d_mask.p, nd, h, c->d_partial.p,
                          c->d_tail.p, -1.0, st);
-      launch_border_solve(s, c->d_Smat.p, c->d_Spiv.p, h, x, c->d_tail.p,
+      launch_border_solve(s, c->d_Smat.p, c->d_Spiv.p, h, x, c->d_tail.p, g,
                           alpha, c->d_ybuf.p, st);
       launch_red_update(x, c->d_DinvC.p, c->d_mask.p, alpha, nd, h, st);
     }
@@ -593,6 +605,7 @@ int iegpu_set_holes(iegpu_ctx* c, int n_holes, const double* centers,
     c->d_tail.alloc(h);
     c->d_alpha.alloc(h);
     c->d_mask.alloc(nd);
+    c->d_hole_c.upload(centers, 2 * (size_t)n_holes, st);
     HIP_TRY(hipStreamSynchronize(st));
   } catch (const HipError& e) {
     return from_hip(c, e);
@@ -855,6 +868,155 @@ int iegpu_last_timings(const iegpu_ctx* c, double* t, int nt) {
   if (!c || !t) return IEGPU_ERR_INVALID;
   const double v[4] = {c->t_factor_ms, c->t_solve_ms, c->t_root_ms, c->fac_bytes};
   for (int i = 0; i < nt && i < 4; i++) t[i] = v[i];
+  return IEGPU_OK;
+}
+
+int iegpu_solve_refined(iegpu_ctx* c, const double* b, double* x, double* alpha,
+                        double rtol, int max_iter, int* iters, double* relres) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!c->factored) return fail(c, IEGPU_ERR_INVALID, "factor first");
+  if (!b || !x || max_iter < 0 || !(rtol >= 0))
+    return fail(c, IEGPU_ERR_INVALID, "bad refinement arguments");
+  const int h = c->hole_dim;
+  if (h > 0 && !alpha) return fail(c, IEGPU_ERR_INVALID, "alpha required with holes");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int64_t nd = c->ndof();
+    const int nres = residual_blocks();
+    c->r_b.upload(b, nd, st);
+    c->r_x.alloc(nd);
+    c->r_r.alloc(nd);
+    c->r_part.alloc(std::max<int64_t>(2 * nres, matvec_partials(c->geom())));
+    if (h > 0) {
+      c->r_a.alloc(h);
+      c->r_ra.alloc(h);
+    }
+    std::vector<double> part(2 * nres);
+    auto solve = [&](double* v, double* a, const double* g) {
+      if (c->precision == IEGPU_FP32) enqueue_solve<float>(c, v, a, st, g);
+      else enqueue_solve<double>(c, v, a, st, g);
+    };
+    // x0 = M^{-1} b
+    launch_copy(c->r_x.p, c->r_b.p, nd, st);
+    solve(c->r_x.p, h ? c->r_a.p : nullptr, nullptr);
+    int it = 0;
+    double rel = 0;
+    for (;; it++) {
+      // r = b - A x  (A bordered by [U; Psi, -I] with holes)
+      launch_matvec(c->geom(), c->r_x.p, c->r_r.p, c->r_part.p, st);
+      if (h > 0)
+        launch_border_apply(c->d_U.p, c->d_PsiT.p, c->r_x.p, c->r_a.p, nd, h, c->r_r.p,
+                            c->r_ra.p, st);
+      launch_residual(c->r_b.p, c->r_r.p, nd, c->r_part.p, st);
+      HIP_TRY(hipMemcpyAsync(part.data(), c->r_part.p, sizeof(double) * 2 * nres,
+                             hipMemcpyDeviceToHost, st));
+      std::vector<double> ra(h);
+      if (h > 0) {
+        // border residual 0 - (Psi x - alpha)
+        HIP_TRY(hipMemcpyAsync(ra.data(), c->r_ra.p, sizeof(double) * h,
+                               hipMemcpyDeviceToHost, st));
+      }
+      HIP_TRY(hipStreamSynchronize(st));
+      double rr = 0, bb = 0;
+      for (int i = 0; i < nres; i++) {
+        rr += part[2 * i];
+        bb += part[2 * i + 1];
+      }
+      for (int i = 0; i < h; i++) {
+        ra[i] = -ra[i];
+        rr += ra[i] * ra[i];
+      }
+      rel = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+      if (rel <= rtol || it >= max_iter) break;
+      // x += M^{-1} r
+      if (h > 0) HIP_TRY(hipMemcpyAsync(c->r_ra.p, ra.data(), sizeof(double) * h,
+                                        hipMemcpyHostToDevice, st));
+      solve(c->r_r.p, h ? c->d_alpha.p : nullptr, h ? c->r_ra.p : nullptr);
+      launch_axpy(c->r_x.p, c->r_r.p, nd, st);
+      if (h > 0) launch_axpy(c->r_a.p, c->d_alpha.p, h, st);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(x, c->r_x.p, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    if (h > 0) HIP_TRY(hipMemcpyAsync(alpha, c->r_a.p, sizeof(double) * h, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (iters) *iters = it;
+    if (relres) *relres = rel;
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_apply(iegpu_ctx* c, const double* x, const double* alpha, double* y,
+                double* z) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (c->n_points == 0) return fail(c, IEGPU_ERR_INVALID, "set_geometry first");
+  if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
+  if (!x || !y) return fail(c, IEGPU_ERR_INVALID, "null vector");
+  const int h = c->hole_dim;
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int64_t nd = c->ndof();
+    c->r_x.upload(x, nd, st);
+    c->r_r.alloc(nd);
+    c->r_part.alloc(std::max<int64_t>(2 * residual_blocks(), matvec_partials(c->geom())));
+    launch_matvec(c->geom(), c->r_x.p, c->r_r.p, c->r_part.p, st);
+    if (h > 0 && alpha && z) {
+      c->r_a.upload(alpha, h, st);
+      c->r_ra.alloc(h);
+      launch_border_apply(c->d_U.p, c->d_PsiT.p, c->r_x.p, c->r_a.p, nd, h, c->r_r.p,
+                          c->r_ra.p, st);
+      HIP_TRY(hipMemcpyAsync(z, c->r_ra.p, sizeof(double) * h, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(y, c->r_r.p, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_evaluate(iegpu_ctx* c, int n_targets, const double* targets,
+                   const int* in_domain, const double* mu, const double* alpha,
+                   double* out) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (c->n_points == 0) return fail(c, IEGPU_ERR_INVALID, "set_geometry first");
+  if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
+  if (n_targets < 0 || (n_targets > 0 && (!targets || !mu || !out)))
+    return fail(c, IEGPU_ERR_INVALID, "bad evaluation arrays");
+  if (n_targets == 0) return IEGPU_OK;
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int sd = c->sol_dim;
+    c->e_tg.upload(targets, 2 * (size_t)n_targets, st);
+    c->e_mu.upload(mu, (size_t)c->ndof(), st);
+    const int* dmask = nullptr;
+    if (in_domain) {
+      c->e_mask.upload(in_domain, n_targets, st);
+      dmask = c->e_mask.p;
+    }
+    const double* dalpha = nullptr;
+    const bool holes = c->n_holes > 0 && alpha;
+    if (holes) {
+      c->e_alpha.upload(alpha, c->hole_dim, st);
+      dalpha = c->e_alpha.p;
+    }
+    c->e_partial.alloc((size_t)eval_chunks(c->n_points) * sd * n_targets);
+    c->e_out.alloc((size_t)sd * n_targets);
+    launch_eval(c->geom(), n_targets, c->e_tg.p, dmask, c->e_mu.p,
+                holes ? c->n_holes : 0, c->d_hole_c.p, dalpha, c->e_partial.p,
+                c->e_out.p, st);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, c->e_out.p, sizeof(double) * sd * n_targets,
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
   return IEGPU_OK;
 }
 

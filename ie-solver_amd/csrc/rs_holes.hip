@@ -167,12 +167,15 @@ __global__ __launch_bounds__(1024) void k_border_factor(DevState s,
   if (info != 0 && threadIdx.x == 0) atomicOr(s.err, kErrSingular);
 }
 
-// y = S^{-1} [x(skel); tail]; then x(skel) = y_head, alpha = y_tail.
+// y = S^{-1} [x(skel); g + tail]; then x(skel) = y_head, alpha = y_tail.
+// g is the right-hand side of the border rows (NULL: zero, as in the
+// reference's [f; 0]).
 __global__ __launch_bounds__(64) void k_border_solve(DevState s,
                                                      const double* Smat,
                                                      const int* Spiv, int h,
                                                      double* x,
                                                      const double* tail,
+                                                     const double* g,
                                                      double* alpha,
                                                      double* ybuf) {
   const int lane = threadIdx.x;
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(64) void k_border_solve(DevState s,
   const int N = ns + h;
   const int* act = s.act_list;
   for (int i = lane; i < ns; i += 64) ybuf[i] = x[act[i]];
-  for (int i = lane; i < h; i += 64) ybuf[ns + i] = tail[i];
+  for (int i = lane; i < h; i += 64) ybuf[ns + i] = tail[i] + (g ? g[i] : 0.0);
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   if (lane == 0)
@@ -268,8 +271,9 @@ void launch_border_factor(const DevState& s, const double* mU,
 
 void launch_border_solve(const DevState& s, const double* Smat,
                          const int* Spiv, int h, double* x, const double* tail,
-                         double* alpha, double* ybuf, hipStream_t st) {
-  k_border_solve<<<1, 64, 0, st>>>(s, Smat, Spiv, h, x, tail, alpha, ybuf);
+                         const double* g, double* alpha, double* ybuf,
+                         hipStream_t st) {
+  k_border_solve<<<1, 64, 0, st>>>(s, Smat, Spiv, h, x, tail, g, alpha, ybuf);
 }
 
 void launch_red_update(double* x, const double* C, const int* mask,

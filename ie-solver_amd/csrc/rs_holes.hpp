@@ -26,7 +26,8 @@ void launch_border_factor(const DevState& s, const double* mU,
                           hipStream_t st);
 void launch_border_solve(const DevState& s, const double* Smat,
                          const int* Spiv, int h, double* x, const double* tail,
-                         double* alpha, double* ybuf, hipStream_t st);
+                         const double* g, double* alpha, double* ybuf,
+                         hipStream_t st);
 void launch_red_update(double* x, const double* C, const int* mask,
                        const double* alpha, int64_t ndof, int h,
                        hipStream_t st);

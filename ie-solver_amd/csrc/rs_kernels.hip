@@ -297,15 +297,267 @@ __global__ __launch_bounds__(256) void k_assemble(DevGeom g, DevTree t,
   }
 }
 
+// Boxes the register-resident QRCP handles: m <= 64*kRegRows rows and
+// n <= kRegWaves*kRegCols columns.
+constexpr int kRegRows = 8;   // rows per lane
+constexpr int kRegCols = 4;   // columns per wave
+constexpr int kRegWaves = 16;
+__device__ inline bool qrcp_in_registers(int m, int n) {
+  return m > 0 && n >= kMinCompress && m <= 64 * kRegRows && n <= kRegWaves * kRegCols;
+}
+
+// Value of row `row` (wave-uniform) of a column held as x[reg] on lane
+// row % 64, broadcast to the wave.
+template <typename R, int RPL>
+__device__ inline R wave_row(const R (&x)[RPL], int row) {
+  R v = x[0];
+#pragma unroll
+  for (int r = 1; r < RPL; r++)
+    if (r == (row >> 6)) v = x[r];
+  return __shfl(v, row & 63, 64);
+}
+
 // ---------------------------------------------------------------------------
 // Interpolative decomposition by column-pivoted Householder QR, stopped at
 // the first |R_jj| < tol |R_00| (ie_mat.cpp:548-588).  The pivot rule,
-// Householder form and column-norm downdating follow LAPACK dgeqp3/dlaqp2.
-// Each wave owns a strided set of (logical) columns; column swaps are
-// index swaps.  The matrix lives in LDS when it fits, else in the box's
-// scratch slot.
+// Householder form and column-norm downdating follow LAPACK dgeqp3/dlaqp2;
+// logical column swaps are index swaps.
+//
+// Register-resident form: physical column c lives in VGPRs of wave
+// c % NW (slot c / NW), row i on lane i % 64 (register i / 64).  Per step
+// every wave picks the pivot redundantly from the LDS norms, the owner
+// wave forms the Householder vector and publishes it through LDS, and all
+// waves update their columns in registers.
 // ---------------------------------------------------------------------------
 
+template <typename R, int RPL, int CPW, int NW>
+__global__ __launch_bounds__(NW * 64) void k_qrcp_reg(DevState s, int start,
+                                                      double tol_d) {
+  constexpr int MAXM = RPL * 64;
+  constexpr int MAXN = CPW * NW;
+  __shared__ R vbuf[MAXM];
+  __shared__ R vn1[MAXN];
+  __shared__ R vn2[MAXN];
+  __shared__ int col[MAXN];
+  __shared__ R rbuf[MAXN * MAXN];
+  __shared__ R sh_tau, sh_thresh;
+  __shared__ int sh_stop;
+
+  const int b = start + blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int n = s.act_cnt[b], m = s.m_rows[b];
+  if (!qrcp_in_registers(m, n) || !pools_ok(s)) return;
+  const R tol = static_cast<R>(tol_d);
+  const R* A = reinterpret_cast<const R*>(s.id_pool) + s.id_off[b];
+
+  R a[CPW][RPL];
+  bool done[CPW];
+#pragma unroll
+  for (int sl = 0; sl < CPW; sl++) {
+    const int c = wv + NW * sl;
+    done[sl] = false;
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+      const int i = lane + 64 * r;
+      a[sl][r] = (c < n && i < m) ? A[i + (int64_t)m * c] : R(0);
+    }
+    R ss = 0;
+#pragma unroll
+    for (int r = 0; r < RPL; r++) ss += a[sl][r] * a[sl][r];
+    ss = wave_sum(ss);
+    if (lane == 0 && c < n) {
+      vn1[c] = sqrt(ss);
+      vn2[c] = vn1[c];
+    }
+  }
+  for (int c = tid; c < n; c += NW * 64) col[c] = c;
+  __syncthreads();
+
+  const int kmax = min(m, n);
+  int k = 0;
+  R thresh = 0;
+  for (int j = 0; j < kmax; j++) {
+    // Pivot: first logical position q >= j with the largest norm.
+    R best = -1;
+    int bi = n;
+    for (int q = j + lane; q < n; q += 64) {
+      const R v = vn1[col[q]];
+      if (v > best) {
+        best = v;
+        bi = q;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const R ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (bi >= n) bi = j;
+    const int pvt = bi;
+    const int pc = col[pvt];
+    const int owner = pc % NW, pslot = pc / NW;
+    if (wv == owner) {
+      R x[RPL];
+#pragma unroll
+      for (int r = 0; r < RPL; r++) x[r] = a[0][r];
+#pragma unroll
+      for (int sl = 1; sl < CPW; sl++)
+        if (sl == pslot)
+#pragma unroll
+          for (int r = 0; r < RPL; r++) x[r] = a[sl][r];
+      const R alpha = wave_row<R, RPL>(x, j);
+      R ss = 0;
+#pragma unroll
+      for (int r = 0; r < RPL; r++) {
+        const int i = lane + 64 * r;
+        if (i > j && i < m) ss += x[r] * x[r];
+      }
+      ss = wave_sum(ss);
+      const R xnorm = sqrt(ss);
+      R beta, tau, sc;
+      if (xnorm == 0) {
+        tau = 0;
+        beta = alpha;
+        sc = 0;
+      } else {
+        beta = -copysign(hypot(alpha, xnorm), alpha);
+        tau = (beta - alpha) / beta;
+        sc = 1 / (alpha - beta);
+      }
+#pragma unroll
+      for (int r = 0; r < RPL; r++) {
+        const int i = lane + 64 * r;
+        vbuf[i] = (i > j && i < m) ? x[r] * sc : (i == j ? R(1) : R(0));
+      }
+      const R th = (j == 0) ? fabs(tol * beta) : thresh;
+      const int stop = (j >= 1 && fabs(beta) < th) ? 1 : 0;
+      if (!stop) {
+        // R_jj of the pivot column; rows below the diagonal are not kept.
+#pragma unroll
+        for (int sl = 0; sl < CPW; sl++)
+          if (sl == pslot) {
+            done[sl] = true;
+#pragma unroll
+            for (int r = 0; r < RPL; r++)
+              if (lane + 64 * r == j) a[sl][r] = beta;
+          }
+      }
+      if (lane == 0) {
+        sh_tau = tau;
+        sh_stop = stop;
+        if (j == 0) sh_thresh = th;
+      }
+    }
+    __syncthreads();
+    if (sh_stop) {
+      k = j;
+      break;
+    }
+    if (j == 0) thresh = sh_thresh;
+    const R tau = sh_tau;
+    if (tid == 0 && pvt != j) {
+      const int t0 = col[j];
+      col[j] = col[pvt];
+      col[pvt] = t0;
+    }
+    R vr[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; r++) vr[r] = vbuf[lane + 64 * r];
+#pragma unroll
+    for (int sl = 0; sl < CPW; sl++) {
+      const int c = wv + NW * sl;
+      if (c >= n || done[sl]) continue;
+      R part = 0;
+#pragma unroll
+      for (int r = 0; r < RPL; r++) part += vr[r] * a[sl][r];
+      const R tw = tau * wave_sum(part);
+#pragma unroll
+      for (int r = 0; r < RPL; r++) a[sl][r] -= tw * vr[r];
+      const R rjc = wave_row<R, RPL>(a[sl], j);
+      const R v1 = vn1[c];
+      if (v1 != 0) {
+        R temp = fabs(rjc) / v1;
+        temp = 1 - temp * temp;
+        temp = temp > 0 ? temp : R(0);
+        const R rat = v1 / vn2[c];
+        if (temp * rat * rat <= Eps<R>::tol3z) {
+          R ss2 = 0;
+#pragma unroll
+          for (int r = 0; r < RPL; r++) {
+            const int i = lane + 64 * r;
+            if (i > j && i < m) ss2 += a[sl][r] * a[sl][r];
+          }
+          ss2 = wave_sum(ss2);
+          const R nv = (j < m - 1) ? sqrt(ss2) : R(0);
+          if (lane == 0) {
+            vn1[c] = nv;
+            vn2[c] = nv;
+          }
+        } else if (lane == 0) {
+          vn1[c] = v1 * sqrt(temp);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  const int* act = s.act_list + (int64_t)b * s.ncap;
+  int* out = s.out_list + (int64_t)b * s.ncap;
+  if (k == 0) {
+    for (int i = tid; i < n; i += NW * 64) out[i] = act[i];
+    if (tid == 0) {
+      s.rank[b] = 0;
+      s.out_cnt[b] = n;
+    }
+    return;
+  }
+  // Rows 0..k-1 of R (physical column order) to LDS.
+#pragma unroll
+  for (int sl = 0; sl < CPW; sl++) {
+    const int c = wv + NW * sl;
+    if (c < n)
+#pragma unroll
+      for (int r = 0; r < RPL; r++) {
+        const int i = lane + 64 * r;
+        if (i < k) rbuf[i + k * c] = a[sl][r];
+      }
+  }
+  __syncthreads();
+  // T = R11^{-1} R12: one wave per column, lane l holds t_l (k <= 64).
+  const int r = n - k;
+  R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
+  for (int cp = wv; cp < r; cp += NW) {
+    const int pcol = col[k + cp];
+    R t = (lane < k) ? rbuf[lane + k * pcol] : R(0);
+    for (int i = k - 1; i >= 0; i--) {
+      const int ci = col[i];
+      const R ti = __shfl(t, i, 64) / rbuf[i + k * ci];
+      if (lane == i) t = ti;
+      if (lane < i) t -= rbuf[lane + k * ci] * ti;
+    }
+    if (lane < k) F[lane + (int64_t)k * cp] = t;
+  }
+  int* red = s.red_list + (int64_t)b * s.ncap;
+  int* pm = s.perm + (int64_t)b * s.ncap;
+  for (int i = tid; i < n; i += NW * 64) {
+    const int c = col[i];
+    pm[i] = c;
+    if (i < k) out[i] = act[c];
+    else red[i - k] = act[c];
+  }
+  if (tid == 0) {
+    s.rank[b] = k;
+    s.out_cnt[b] = k;
+  }
+}
+
+// General form (any box the register form does not take): the matrix lives
+// in LDS when it fits, else in the box's scratch slot; each wave owns a
+// strided set of columns.
 template <typename R, int NT>
 __global__ __launch_bounds__(NT) void k_qrcp(DevState s, int start, double tol_d) {
   constexpr int NW = NT / 64;
@@ -320,6 +572,7 @@ __global__ __launch_bounds__(NT) void k_qrcp(DevState s, int start, double tol_d
   const int b = start + blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int n = s.act_cnt[b], m = s.m_rows[b];
+  if (pools_ok(s) && qrcp_in_registers(m, n)) return;  // k_qrcp_reg's box
   const int* act = s.act_list + (int64_t)b * s.ncap;
   int* out = s.out_list + (int64_t)b * s.ncap;
   if (m == 0 || n < kMinCompress || !pools_ok(s)) {
@@ -574,6 +827,206 @@ __device__ const R* carried_block(const DevState& s, int c) {
   return F + FacLayout::make(k, s.act_cnt[c] - k).S;
 }
 
+constexpr int kSmallBox = 64;  // decouple in LDS with lane-per-row solves
+
+// Children's carried blocks and the position map of a box's active list.
+template <typename R>
+__device__ void load_child_blocks(const DevTree& t, const DevState& s, int b,
+                                  int n, const R** chS, int* chN,
+                                  signed char* pch, short* ploc, int NT) {
+  const int tid = threadIdx.x;
+  const bool leaf = t.leaf[b] != 0;
+  if (tid < 4) {
+    const int c = leaf ? -1 : t.child[4 * b + tid];
+    chS[tid] = (c >= 0) ? carried_block<R>(s, c) : nullptr;
+    chN[tid] = (c >= 0) ? s.out_cnt[c] : 0;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += NT) {
+    int q = -1, loc = 0;
+    if (!leaf) {
+      for (int u = 0; u < 4; u++) {
+        const int p0 = s.chpos[4 * b + u];
+        if (i >= p0 && i < p0 + chN[u]) {
+          q = u;
+          loc = i - p0;
+        }
+      }
+    }
+    pch[i] = (signed char)q;
+    ploc[i] = (short)loc;
+  }
+}
+
+// Decouple of a compressed box with |active| <= 64: every operand in LDS;
+// LU by one wave, triangular solves with one matrix row (or column) per
+// lane and the pivots broadcast by shuffles.  Same operations as
+// k_decouple below.
+template <typename R>
+__global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
+                                                        DevState s, int start) {
+  constexpr int NT = 256, NW = 4;
+  constexpr int kBuf = 8704;  // >= n^2 + r^2 + 3 k r for n <= 64
+  __shared__ R buf[kBuf];
+  __shared__ signed char pch[kSmallBox];
+  __shared__ short ploc[kSmallBox];
+  __shared__ short sperm[kSmallBox];
+  __shared__ int spiv[kSmallBox], idxf[kSmallBox], idxb[kSmallBox];
+  __shared__ const R* chS[4];
+  __shared__ int chN[4];
+  __shared__ int sh_info;
+
+  const int b = start + blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int n = s.act_cnt[b], k = s.rank[b];
+  if (!pools_ok(s) || n == 0 || n > kSmallBox || k == 0) return;
+  load_child_blocks<R>(t, s, b, n, chS, chN, pch, ploc, NT);
+  for (int i = tid; i < n; i += NT) sperm[i] = (short)s.perm[(int64_t)b * s.ncap + i];
+  __syncthreads();
+  ChildUpdates<R> upd;
+  for (int u = 0; u < 4; u++) {
+    upd.S[u] = chS[u];
+    upd.N[u] = chN[u];
+  }
+  upd.pch = pch;
+  upd.ploc = ploc;
+  const int r = n - k;
+  const FacLayout fl = FacLayout::make(k, r);
+  R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
+  const int* act = s.act_list + (int64_t)b * s.ncap;
+  R* P = buf;
+  R* X = P + n * n;
+  R* Krs = X + r * r;
+  R* Ksr = Krs + r * k;
+  R* T = Ksr + k * r;
+  R* Lm = P;  // P is dead once X, K_rs and K_sr are formed
+  for (int i = tid; i < k * r; i += NT) T[i] = F[fl.T + i];
+  for (int idx = tid; idx < n * n; idx += NT) {
+    const int ia = sperm[idx % n], ic = sperm[idx / n];
+    P[idx] = static_cast<R>(kernel_entry(g, act[ia], act[ic])) - upd(ia, ic);
+  }
+  __syncthreads();
+  for (int idx = tid; idx < r * k; idx += NT) {
+    const int i = idx % r, a = idx / r;
+    R acc = P[(k + i) + n * a];
+    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + n * a];
+    Krs[idx] = acc;
+  }
+  for (int idx = tid; idx < k * r; idx += NT) {
+    const int a = idx % k, j = idx / k;
+    R acc = P[a + n * (k + j)];
+    for (int l = 0; l < k; l++) acc -= P[a + n * l] * T[l + k * j];
+    Ksr[idx] = acc;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < r * r; idx += NT) {
+    const int i = idx % r, j = idx / r;
+    R acc = P[(k + i) + n * (k + j)];
+    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + n * (k + j)];
+    for (int l = 0; l < k; l++) acc -= Krs[i + r * l] * T[l + k * j];
+    X[idx] = acc;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    // LU with partial pivoting, lane i = row i.
+    int info = 0;
+    for (int j = 0; j < r; j++) {
+      R v = (lane >= j && lane < r) ? fabs(X[lane + r * j]) : R(-1);
+      int bi = (lane >= j && lane < r) ? lane : r;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const R ov = __shfl_xor(v, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > v || (ov == v && oi < bi)) {
+          v = ov;
+          bi = oi;
+        }
+      }
+      const int p = (bi >= r) ? j : bi;
+      if (lane == 0) spiv[j] = p;
+      if (p != j && lane < r) {
+        const R tmp = X[j + r * lane];
+        X[j + r * lane] = X[p + r * lane];
+        X[p + r * lane] = tmp;
+      }
+      wave_sync();
+      const R piv = X[j + r * j];
+      if (piv == 0) {
+        if (info == 0) info = j + 1;
+      } else if (lane > j && lane < r) {
+        X[lane + r * j] *= 1 / piv;
+      }
+      wave_sync();
+      if (lane > j && lane < r) {
+        const R lij = X[lane + r * j];
+        for (int c = j + 1; c < r; c++) X[lane + r * c] -= lij * X[j + r * c];
+      }
+      wave_sync();
+    }
+    if (lane == 0) {
+      sh_info = info;
+      for (int i = 0; i < r; i++) {
+        idxf[i] = i;
+        idxb[i] = i;
+      }
+      for (int i = 0; i < r; i++) {
+        const int p = spiv[i], q = idxf[i];
+        idxf[i] = idxf[p];
+        idxf[p] = q;
+      }
+      for (int i = r - 1; i >= 0; i--) {
+        const int p = spiv[i], q = idxb[i];
+        idxb[i] = idxb[p];
+        idxb[p] = q;
+      }
+    }
+  }
+  __syncthreads();
+  if (sh_info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
+  // U = X^{-1} K_rs: wave per column, lane i = row i.
+  for (int a = wv; a < k; a += NW) {
+    R u = (lane < r) ? Krs[idxf[lane] + r * a] : R(0);
+    for (int l = 0; l < r; l++) {
+      const R ul = __shfl(u, l, 64);
+      if (lane > l && lane < r) u -= X[lane + r * l] * ul;
+    }
+    for (int i = r - 1; i >= 0; i--) {
+      const R ui = __shfl(u, i, 64) / X[i + r * i];
+      if (lane == i) u = ui;
+      if (lane < i) u -= X[lane + r * i] * ui;
+    }
+    if (lane < r) F[fl.U + lane + (int64_t)r * a] = u;
+  }
+  // L = K_sr X^{-1}: wave per row, lane j = column j (X^T z^T = row^T).
+  for (int a = wv; a < k; a += NW) {
+    R acc = (lane < r) ? Ksr[a + k * lane] : R(0);
+    for (int i = 0; i < r; i++) {
+      const R yi = __shfl(acc, i, 64) / X[i + r * i];
+      if (lane == i) acc = yi;
+      if (lane > i && lane < r) acc -= X[i + r * lane] * yi;
+    }
+    for (int i = r - 1; i >= 0; i--) {
+      const R zi = __shfl(acc, i, 64);
+      if (lane < i) acc -= X[i + r * lane] * zi;
+    }
+    const R z = __shfl(acc, lane < r ? idxb[lane] : 0, 64);
+    if (lane < r) {
+      Lm[a + k * lane] = z;
+      F[fl.L + a + (int64_t)k * lane] = z;
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < k * k; idx += NT) {
+    const int a = idx % k, c = idx / k;
+    R acc = 0;
+    for (int j = 0; j < r; j++) acc += Lm[a + k * j] * Krs[j + r * c];
+    F[fl.S + idx] = upd(sperm[a], sperm[c]) + acc;
+  }
+  for (int idx = tid; idx < r * r; idx += NT) F[fl.X + idx] = X[idx];
+  for (int i = tid; i < r; i += NT) F[fl.piv + i] = static_cast<R>(spiv[i]);
+  if (tid == 0) s.upd_kind[b] = 1;
+}
+
 // ---------------------------------------------------------------------------
 // Decouple one box (skel_factorization.cpp:77-128): with s/r the skeleton
 // and redundant positions and K_BN = K(act, act) - (children's carried
@@ -606,6 +1059,7 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
     if (tid == 0) s.upd_kind[b] = 0;
     return;
   }
+  if (n <= kSmallBox && s.rank[b] > 0) return;  // k_decouple_small's box
   const bool leaf = t.leaf[b] != 0;
   if (tid < 4) {
     const int c = leaf ? -1 : t.child[4 * b + tid];
@@ -993,9 +1447,12 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       k_assemble<R><<<count, 256, 0, st>>>(g, t, s, start, level);
       break;
     case kStageQrcp:
+      k_qrcp_reg<R, kRegRows, kRegCols, kRegWaves>
+          <<<count, kRegWaves * 64, 0, st>>>(s, start, tol);
       k_qrcp<R, 512><<<count, 512, 0, st>>>(s, start, tol);
       break;
     case kStageDecouple:
+      k_decouple_small<R><<<count, 256, 0, st>>>(g, t, s, start);
       k_decouple<R, 256><<<count, 256, 0, st>>>(g, t, s, start);
       break;
     default:

@@ -1,0 +1,210 @@
+// FP64 matrix-free apply of the second-kind boundary operator, y = K x,
+// with the entries of ie_solver/kernel/kernel.cpp:123-165 (Laplace) and
+// :212-290 (Stokes), used for the residual of iterative refinement.
+//
+// One thread per target point; sources staged through LDS in tiles of 256
+// (pre-scaled by the weights), every wave reading the same source
+// (broadcast).  The reciprocal of |r|^2 is v_rcp_f64 refined by one Newton
+// step.  For Stokes the w n_t n_s^T part of every entry sums to
+// n_t (sum_j w_j n_j . x_j), a global scalar computed first.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rs_device.hpp"
+#include "rs_matvec.hpp"
+
+namespace iegpu {
+
+namespace {
+
+constexpr int kTile = 256;
+
+__device__ inline double rcp_f64(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  const double e = fma(-d, y, 1.0);
+  return fma(y, e, y);
+}
+
+// c = sum_j w_j (n_j . x_j) over Stokes DOF pairs (block partials).
+__global__ __launch_bounds__(256) void k_wn_dot(DevGeom g, const double* x,
+                                                double* partial) {
+  __shared__ double red[256];
+  double acc = 0;
+  for (int64_t j = blockIdx.x * 256 + threadIdx.x; j < g.n_points;
+       j += (int64_t)gridDim.x * 256)
+    acc += g.wts[j] * (g.nrm[2 * j] * x[2 * j] + g.nrm[2 * j + 1] * x[2 * j + 1]);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
+                                                 const double* wn_partial,
+                                                 int wn_blocks, double* y) {
+  __shared__ double sx[kTile], sy[kTile], sa[kTile], sb[kTile], sm0[kTile],
+      sm1[kTile];
+  const int i = blockIdx.x * kTile + threadIdx.x;
+  const bool live = i < g.n_points;
+  const double xi = live ? g.pts[2 * i] : 0, yi = live ? g.pts[2 * i + 1] : 0;
+  const bool stokes = g.pde == kPdeStokes;
+  double a0 = 0, a1 = 0;
+  for (int base = 0; base < g.n_points; base += kTile) {
+    const int j = base + threadIdx.x;
+    if (j < g.n_points) {
+      const double w = g.wts[j];
+      sx[threadIdx.x] = g.pts[2 * j];
+      sy[threadIdx.x] = g.pts[2 * j + 1];
+      if (stokes) {
+        sa[threadIdx.x] = w * (1.0 / M_PI) * g.nrm[2 * j];
+        sb[threadIdx.x] = w * (1.0 / M_PI) * g.nrm[2 * j + 1];
+        sm0[threadIdx.x] = x[2 * j];
+        sm1[threadIdx.x] = x[2 * j + 1];
+      } else {
+        const double s = -w * (1.0 / (2 * M_PI)) * x[j];
+        sa[threadIdx.x] = s * g.nrm[2 * j];
+        sb[threadIdx.x] = s * g.nrm[2 * j + 1];
+      }
+    }
+    __syncthreads();
+    const int cnt = min(kTile, g.n_points - base);
+    if (stokes) {
+      for (int q = 0; q < cnt; q++) {
+        const double r0 = xi - sx[q], r1 = yi - sy[q];
+        const double d2 = fma(r0, r0, r1 * r1);
+        if (d2 == 0) continue;  // self term added below
+        const double inv = rcp_f64(d2);
+        const double pot = fma(r0, sa[q], r1 * sb[q]) * inv * inv;
+        const double proj = fma(r0, sm0[q], r1 * sm1[q]) * pot;
+        a0 = fma(r0, proj, a0);
+        a1 = fma(r1, proj, a1);
+      }
+    } else {
+      for (int q = 0; q < cnt; q++) {
+        const double r0 = xi - sx[q], r1 = yi - sy[q];
+        const double d2 = fma(r0, r0, r1 * r1);
+        if (d2 == 0) continue;
+        a0 = fma(fma(r0, sa[q], r1 * sb[q]), rcp_f64(d2), a0);
+      }
+    }
+    __syncthreads();
+  }
+  if (!live) return;
+  const double w = g.wts[i], kap = g.curv[i];
+  const double n0 = g.nrm[2 * i], n1 = g.nrm[2 * i + 1];
+  if (stokes) {
+    double c = 0;
+    for (int b = 0; b < wn_blocks; b++) c += wn_partial[b];
+    // self 2x2 block without its n n^T part (kernel.cpp:245-263)
+    const double pot = -0.5 * kap * w * (1.0 / M_PI);
+    const double t0 = -n1, t1 = n0;
+    const double x0 = x[2 * i], x1 = x[2 * i + 1];
+    const double tx = t0 * x0 + t1 * x1;
+    y[2 * i] = a0 - 0.5 * x0 + pot * t0 * tx + n0 * c;
+    y[2 * i + 1] = a1 - 0.5 * x1 + pot * t1 * tx + n1 * c;
+  } else {
+    y[i] = a0 + (0.5 + 0.5 * kap * w * (1.0 / (2 * M_PI))) * x[i];
+  }
+}
+
+// y[i] += sum_c U[i + ld c] a[c]  and  z[c] = sum_i P[i + ld c] x[i] - a[c].
+__global__ void k_border_apply_u(double* y, const double* U, const double* a,
+                                 int64_t ndof, int h) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ndof;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = y[i];
+    for (int c = 0; c < h; c++) acc += U[i + ndof * c] * a[c];
+    y[i] = acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_border_apply_psi(const double* P,
+                                                          const double* x,
+                                                          const double* a,
+                                                          int64_t ndof, int h,
+                                                          double* z) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  double acc = 0;
+  for (int64_t i = threadIdx.x; i < ndof; i += 256) acc += P[i + ndof * c] * x[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) z[c] = red[0] - a[c];
+}
+
+// r = b - y (in place on y); partial sums of r^2 and b^2 per block.
+__global__ __launch_bounds__(256) void k_residual(const double* b, double* y,
+                                                  int64_t n, double* partial) {
+  __shared__ double rr[256], bb[256];
+  double a = 0, c = 0;
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double bi = b ? b[i] : 0.0;
+    const double r = bi - y[i];
+    y[i] = r;
+    a += r * r;
+    c += bi * bi;
+  }
+  rr[threadIdx.x] = a;
+  bb[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      rr[threadIdx.x] += rr[threadIdx.x + o];
+      bb[threadIdx.x] += bb[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = rr[0];
+    partial[2 * blockIdx.x + 1] = bb[0];
+  }
+}
+
+__global__ void k_axpy(double* y, const double* x, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += x[i];
+}
+
+}  // namespace
+
+int residual_blocks() { return 256; }
+
+void launch_residual(const double* b, double* y, int64_t n, double* partial,
+                     hipStream_t st) {
+  k_residual<<<256, 256, 0, st>>>(b, y, n, partial);
+}
+
+void launch_axpy(double* y, const double* x, int64_t n, hipStream_t st) {
+  k_axpy<<<512, 256, 0, st>>>(y, x, n);
+}
+
+int matvec_partials(const DevGeom& g) {
+  (void)g;
+  return 256;
+}
+
+void launch_matvec(const DevGeom& g, const double* x, double* y, double* partial,
+                   hipStream_t st) {
+  if (g.pde == kPdeStokes) k_wn_dot<<<256, 256, 0, st>>>(g, x, partial);
+  k_apply<<<(g.n_points + kTile - 1) / kTile, kTile, 0, st>>>(
+      g, x, partial, g.pde == kPdeStokes ? 256 : 0, y);
+}
+
+void launch_border_apply(const double* U, const double* PsiT, const double* x,
+                         const double* alpha, int64_t ndof, int h, double* y,
+                         double* z, hipStream_t st) {
+  k_border_apply_u<<<512, 256, 0, st>>>(y, U, alpha, ndof, h);
+  k_border_apply_psi<<<h, 256, 0, st>>>(PsiT, x, alpha, ndof, h, z);
+}
+
+}  // namespace iegpu

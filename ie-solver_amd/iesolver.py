@@ -36,7 +36,8 @@ EXPORTS = [
     "iegpu_create", "iegpu_destroy", "iegpu_last_error", "iegpu_set_geometry",
     "iegpu_build_tree", "iegpu_set_holes", "iegpu_factor", "iegpu_solve",
     "iegpu_solve_device", "iegpu_tree_info", "iegpu_box_info", "iegpu_box_list",
-    "iegpu_last_timings", "iegpu_set_profiling", "iegpu_profile",
+    "iegpu_last_timings", "iegpu_set_profiling", "iegpu_profile", "iegpu_evaluate",
+    "iegpu_solve_refined", "iegpu_apply",
 ]
 
 STAGES = ["active", "rows", "scan", "assemble", "qrcp", "decouple", "root",
@@ -75,6 +76,10 @@ def load():
     L.iegpu_box_info.argtypes = [vp, _ip, _ip, _ip, _ip, _ip, _ip, _ip]
     L.iegpu_box_list.argtypes = [vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int]
     L.iegpu_last_timings.argtypes = [vp, _dp, ctypes.c_int]
+    L.iegpu_evaluate.argtypes = [vp, ctypes.c_int, _dp, _ip, _dp, _dp, _dp]
+    L.iegpu_solve_refined.argtypes = [vp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                      _ip, _dp]
+    L.iegpu_apply.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.iegpu_set_profiling.argtypes = [vp, ctypes.c_int]
     L.iegpu_profile.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _ip]
     _lib = L
@@ -148,6 +153,44 @@ class RSSolver:
             return mu, alpha
         self._check(self.L.iegpu_solve(self.h, _pd(b), _pd(mu), None))
         return mu, None
+
+    def solve_refined(self, b, rtol=1e-10, max_iter=10):
+        """FP64 iterative refinement preconditioned by the stored factor.
+        Returns (mu, alpha, corrections, relative residual)."""
+        b = _d(b)
+        mu = np.zeros(self.ndof)
+        alpha = np.zeros(self.hole_dim) if self.hole_dim else None
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0)
+        self._check(self.L.iegpu_solve_refined(self.h, _pd(b), _pd(mu),
+                                               None if alpha is None else _pd(alpha),
+                                               rtol, max_iter, ctypes.byref(it), ctypes.byref(rr)))
+        return mu, alpha, it.value, rr.value
+
+    def apply(self, x, alpha=None):
+        """FP64 operator: K x (+ U alpha), and Psi x - alpha with holes."""
+        x = _d(x)
+        y = np.zeros(self.ndof)
+        if self.hole_dim and alpha is not None:
+            a = _d(alpha)
+            z = np.zeros(self.hole_dim)
+            self._check(self.L.iegpu_apply(self.h, _pd(x), _pd(a), _pd(y), _pd(z)))
+            return y, z
+        self._check(self.L.iegpu_apply(self.h, _pd(x), None, _pd(y), None))
+        return y, None
+
+    def evaluate(self, targets, mu, alpha=None, in_domain=None):
+        """Interior field K_domain mu + U_forward alpha at targets (n, 2)."""
+        tg = _d(targets).reshape(-1)
+        nt = tg.size // 2
+        mu = _d(mu)
+        sd = 2 if self.pde == PDE_STOKES else 1
+        out = np.zeros(nt * sd)
+        mask = None if in_domain is None else np.ascontiguousarray(in_domain, dtype=np.int32)
+        al = None if alpha is None else _d(alpha)
+        self._check(self.L.iegpu_evaluate(self.h, nt, _pd(tg), None if mask is None else _pi(mask),
+                                          _pd(mu), None if al is None else _pd(al), _pd(out)))
+        return out
 
     def solve_device(self, b_ptr, mu_ptr, alpha_ptr=None, stream=None):
         self._check(self.L.iegpu_solve_device(self.h, b_ptr, mu_ptr, alpha_ptr, stream))

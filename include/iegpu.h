@@ -88,6 +88,35 @@ int iegpu_solve(iegpu_ctx* ctx, const double* b, double* mu, double* alpha);
 int iegpu_solve_device(iegpu_ctx* ctx, const double* d_b, double* d_mu,
                        double* d_alpha, void* stream);
 
+/* Iterative refinement in FP64 with the stored factorization (FP64 or
+ * FP32) as the preconditioner M: x0 = M^{-1} b, then x += M^{-1}(b - A x)
+ * until ||b - A x||_2 <= rtol ||b||_2 or max_iter corrections.  A is the
+ * FP64 operator applied matrix-free on the GPU (bordered by U and Psi when
+ * holes are set: [K U; Psi -I][x; alpha] = [b; 0]).  Host pointers; iters
+ * and relres (nullable) report the corrections made and the final relative
+ * residual.  No reference counterpart: the reference solves once with its
+ * FP64 factorization (ie_solver/linear_solve.cpp:115-129); this is the
+ * mixed-precision form of the same solve. */
+int iegpu_solve_refined(iegpu_ctx* ctx, const double* b, double* x,
+                        double* alpha, double rtol, int max_iter, int* iters,
+                        double* relres);
+
+/* y = K x (ndof) in FP64, matrix-free; with holes and non-NULL alpha, z,
+ * also y += U alpha and z = Psi x - alpha.  The forward operator the
+ * reference forms densely in its tests (ie_solver_test.cpp:63-93). */
+int iegpu_apply(iegpu_ctx* ctx, const double* x, const double* alpha,
+                double* y, double* z);
+
+/* Field at n_targets interior points: out (sol_dim per target) =
+ * K_domain mu + U_forward alpha, matrix-free on the GPU; targets with
+ * in_domain[t] == 0 give 0 (in_domain may be NULL: all inside); alpha may
+ * be NULL without holes.  Host pointers.  Replaces
+ * Initialization::InitializeDomainKernel and the products of schur_solve,
+ * ie_solver/initialization.cpp:11-135, ie_solver/linear_solve.cpp:140-154. */
+int iegpu_evaluate(iegpu_ctx* ctx, int n_targets, const double* targets,
+                   const int* in_domain, const double* mu, const double* alpha,
+                   double* out);
+
 /* Sizes: tree levels, boxes, DOFs, hole border dimension. */
 int iegpu_tree_info(const iegpu_ctx* ctx, int* n_levels, int* n_boxes,
                     int* ndof, int* hole_dim);

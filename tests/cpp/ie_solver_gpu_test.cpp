@@ -1,0 +1,193 @@
+// The reference's end-to-end tests (ie_solver/ie_solver_test.cpp), run
+// through the C++ facade on the GPU path.  Same cases and tolerances:
+// backward error <= 50 id_tol (:29-96) and analytic agreement <= 10 id_tol
+// (:378-562).  Laplace-Neumann and the perturbation/update tests are out of
+// scope for the GPU path.  Prints one line per case; exit status 0 iff all
+// pass.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ie_solver_gpu.hpp"
+
+using namespace ie_solver;
+
+static int g_failures = 0;
+
+static void expect_le(const std::string& name, double v, double bound) {
+  const bool ok = v <= bound;
+  std::printf("%-44s %s  value %.3e  bound %.3e\n", name.c_str(), ok ? "PASS" : "FAIL", v, bound);
+  if (!ok) g_failures++;
+}
+
+// ie_solver_test.cpp:29-96
+static double solve_err(const ie_solver_config& config, Boundary* boundary) {
+  QuadTree quadtree;
+  quadtree.initialize_tree(boundary, std::vector<double>(), config.solution_dimension,
+                           config.domain_dimension);
+  SkelFactorization sf(config.id_tol, config.is_strong_admissibility, config.num_threads);
+  Kernel kernel(config.solution_dimension, config.domain_dimension, config.pde, boundary,
+                std::vector<double>());
+  std::vector<int> all;
+  for (int i = 0; i < config.solution_dimension * (int)boundary->weights.size(); i++)
+    all.push_back(i);
+  if (!boundary->holes.empty()) {
+    ie_Mat U = initialize_U_mat(config.pde, boundary->holes, boundary->points);
+    ie_Mat Psi = initialize_Psi_mat(config.pde, boundary->holes, boundary);
+    quadtree.U = U;
+    quadtree.Psi = Psi;
+    sf.skeletonize(kernel, &quadtree);
+    ie_Mat mu, alpha;
+    linear_solve(sf, quadtree, boundary->boundary_values, &mu, &alpha);
+    ie_Mat K = kernel(all, all);
+    const int n = (int)all.size(), h = alpha.height();
+    double err = 0;
+    for (int i = 0; i < n; i++) {
+      double acc = -boundary->boundary_values.get(i, 0);
+      for (int j = 0; j < n; j++) acc += K.get(i, j) * mu.get(j, 0);
+      for (int k = 0; k < h; k++) acc += U.get(i, k) * alpha.get(k, 0);
+      err = std::fmax(err, std::fabs(acc));
+    }
+    for (int k = 0; k < h; k++) {
+      double acc = -alpha.get(k, 0);
+      for (int j = 0; j < n; j++) acc += Psi.get(k, j) * mu.get(j, 0);
+      err = std::fmax(err, std::fabs(acc));
+    }
+    return err;
+  }
+  sf.skeletonize(kernel, &quadtree);
+  ie_Mat mu;
+  linear_solve(sf, quadtree, boundary->boundary_values, &mu);
+  ie_Mat err = (kernel(all, all) * mu) - boundary->boundary_values;
+  return err.max_entry_magnitude();
+}
+
+// ie_solver_test.cpp:99-109
+static ie_Mat get_dom_sol(const ie_solver_config& config, std::vector<double>* dom,
+                          Boundary* boundary) {
+  QuadTree quadtree;
+  quadtree.initialize_tree(boundary, std::vector<double>(), config.solution_dimension,
+                           config.domain_dimension);
+  get_domain_points(config.domain_size, dom, quadtree.min, quadtree.max, quadtree.min,
+                    quadtree.max);
+  return boundary_integral_solve(config, &quadtree, *dom);
+}
+
+// ie_solver_test.cpp:378-401
+static double laplace_error(const ie_Mat& sol, const std::vector<double>& dom, Boundary* b) {
+  double dn = 0, nt = 0;
+  for (size_t i = 0; i < dom.size(); i += 2) {
+    const Vec2 x(dom[i], dom[i + 1]);
+    if (!b->is_in_domain(x)) continue;
+    const double pot = std::log(std::sqrt(std::pow(x.a[0] + 3, 2) + std::pow(x.a[1] + 2, 2))) / (2 * M_PI);
+    dn += std::pow(pot - sol.get(i / 2, 0), 2);
+    nt += pot * pot;
+  }
+  return std::sqrt(dn) / std::sqrt(nt);
+}
+
+// ie_solver_test.cpp:474-507
+static double stokes_error(const ie_Mat& sol, const std::vector<double>& dom, Boundary* b) {
+  double mx = 0;
+  for (size_t i = 0; i < dom.size(); i += 2) {
+    const Vec2 x(dom[i], dom[i + 1]);
+    if (!b->is_in_domain(x)) continue;
+    const Vec2 r = x - Vec2(0.5, 0.5);
+    Vec2 truth(-r.a[1], r.a[0]);
+    if (b->boundary_shape == Boundary::CIRCLE) {
+      truth = truth * (r.norm() / truth.norm()) * 4.;
+    } else {
+      const double p = (-1. / r.norm()) + 2 * r.norm();
+      truth = truth * (1. / truth.norm()) * p;
+    }
+    const Vec2 s(sol.get(i, 0), sol.get(i + 1, 0));
+    mx = std::fmax(mx, (truth - s).norm());
+  }
+  return mx;
+}
+
+template <class B>
+static void backward_case(const std::string& name, ie_solver_config config) {
+  B boundary;
+  boundary.initialize(config.num_boundary_points, config.boundary_condition);
+  expect_le(name, solve_err(config, &boundary), 50 * config.id_tol);
+}
+
+int main() {
+  const char* tol_env = std::getenv("IE_TEST_TOL");
+  const double tol = tol_env ? std::atof(tol_env) : DEFAULT_ID_TOL;
+  ie_solver_config lap;
+  lap.id_tol = tol;
+  ie_solver_config sto = lap;
+  sto.solution_dimension = 2;
+  sto.pde = ie_solver_config::STOKES;
+  sto.boundary_condition = TANGENT_VEC;
+
+  // ie_solver_test.cpp:112-236 (N=512) and :240-372 (N=4096)
+  for (int big = 0; big < 2; big++) {
+    const std::string p = big ? "Big" : "";
+    ie_solver_config l = lap, s = sto;
+    if (big) {
+      l.num_boundary_points = 4096;
+      s.num_boundary_points = 4096;
+    }
+    backward_case<Circle>(p + "LaplaceCircleBackwardError", l);
+    backward_case<Circle>(p + "StokesCircleBackwardError", s);
+    ie_solver_config la = l;
+    la.boundary_shape = Boundary::ANNULUS;
+    backward_case<Annulus>(p + "LaplaceAnnulusBackwardError", la);
+    ie_solver_config sa = s;
+    sa.boundary_shape = Boundary::ANNULUS;
+    backward_case<Annulus>(p + "StokesAnnulusBackwardError", sa);
+  }
+
+  {  // ie_solver_test.cpp:404-416
+    Circle b;
+    b.initialize(lap.num_boundary_points, lap.boundary_condition);
+    std::vector<double> dom;
+    ie_Mat sol = get_dom_sol(lap, &dom, &b);
+    expect_le("LaplaceCircleAnalyticAgreementElectron", laplace_error(sol, dom, &b), 10 * tol);
+  }
+  {  // ie_solver_test.cpp:419-431
+    Annulus b;
+    b.initialize(lap.num_boundary_points, lap.boundary_condition);
+    std::vector<double> dom;
+    ie_Mat sol = get_dom_sol(lap, &dom, &b);
+    expect_le("LaplaceAnnulusAnalyticAgreementElectron", laplace_error(sol, dom, &b), 10 * tol);
+  }
+  {  // ie_solver_test.cpp:528-543
+    Circle b;
+    b.initialize(sto.num_boundary_points, sto.boundary_condition);
+    std::vector<double> dom;
+    ie_Mat sol = get_dom_sol(sto, &dom, &b);
+    expect_le("StokesCircleAnalyticAgreementTangent", stokes_error(sol, dom, &b), 10 * tol);
+  }
+  {  // ie_solver_test.cpp:545-562
+    ie_solver_config c = sto;
+    c.num_boundary_points *= 4;
+    c.boundary_shape = Boundary::DONUT;
+    Donut b;
+    b.initialize(c.num_boundary_points, c.boundary_condition);
+    std::vector<double> dom;
+    ie_Mat sol = get_dom_sol(c, &dom, &b);
+    expect_le("StokesDonutAnalyticAgreementTangent", stokes_error(sol, dom, &b), 10 * tol);
+  }
+  {  // bie_time_trial (linear_solve.cpp:156-185) runs and reports times
+    ie_solver_config c = lap;
+    c.num_boundary_points = 4096;
+    Circle b;
+    b.initialize(c.num_boundary_points, c.boundary_condition);
+    QuadTree q;
+    q.initialize_tree(&b, std::vector<double>(), 1, 2);
+    double ts = 0, tv = 0;
+    bie_time_trial(c, &q, &ts, &tv);
+    std::printf("bie_time_trial N=4096: skeletonize %.3f ms, solve %.3f ms\n", ts * 1e3, tv * 1e3);
+    expect_le("TimeTrialRuns", ts > 0 && tv > 0 ? 0.0 : 1.0, 0.0);
+  }
+  std::printf("%d failure(s)\n", g_failures);
+  return g_failures == 0 ? 0 : 1;
+}
