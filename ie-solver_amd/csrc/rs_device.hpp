@@ -125,6 +125,109 @@ __device__ inline int wave_sum_int(int v) {
   return v;
 }
 
+// ---------------------------------------------------------------------------
+// Wave64 reductions with DPP (gfx9 data-parallel primitives): pairs, quads,
+// half rows and rows by quad_perm / row mirrors, then rows combined by
+// row_bcast15 / row_bcast31, leaving the total in lane 63, which readlane
+// broadcasts.  No LDS round trip, unlike shuffles through ds_bpermute.
+// ---------------------------------------------------------------------------
+
+namespace dpp {
+constexpr int kQuadSwap1 = 0xB1;     // quad_perm [1,0,3,2]
+constexpr int kQuadSwap2 = 0x4E;     // quad_perm [2,3,0,1]
+constexpr int kRowHalfMirror = 0x141;
+constexpr int kRowMirror = 0x140;
+constexpr int kRowBcast15 = 0x142;
+constexpr int kRowBcast31 = 0x143;
+}  // namespace dpp
+
+template <int CTRL, int ROWS>
+__device__ inline double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL, int ROWS>
+__device__ inline float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+template <int CTRL, int ROWS>
+__device__ inline int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
+}
+
+// Value of `v` in lane `lane` (wave-uniform index), for every lane.
+__device__ inline double lane_value(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline float lane_value(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ inline int lane_value(int v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+
+// Sum over the 64 lanes, returned in every lane.
+__device__ inline double wave_total(double v) {
+  v += dpp_f64<dpp::kQuadSwap1, 0xF>(v);
+  v += dpp_f64<dpp::kQuadSwap2, 0xF>(v);
+  v += dpp_f64<dpp::kRowHalfMirror, 0xF>(v);
+  v += dpp_f64<dpp::kRowMirror, 0xF>(v);
+  v += dpp_f64<dpp::kRowBcast15, 0xA>(v);
+  v += dpp_f64<dpp::kRowBcast31, 0xC>(v);
+  return lane_value(v, 63);
+}
+__device__ inline float wave_total(float v) {
+  v += dpp_f32<dpp::kQuadSwap1, 0xF>(v);
+  v += dpp_f32<dpp::kQuadSwap2, 0xF>(v);
+  v += dpp_f32<dpp::kRowHalfMirror, 0xF>(v);
+  v += dpp_f32<dpp::kRowMirror, 0xF>(v);
+  v += dpp_f32<dpp::kRowBcast15, 0xA>(v);
+  v += dpp_f32<dpp::kRowBcast31, 0xC>(v);
+  return lane_value(v, 63);
+}
+
+// (value, index) with the largest value over the wave; ties go to the
+// smaller index (the first maximum, as LAPACK idamax).  Lanes with nothing
+// to offer pass value -1 and index INT_MAX.
+template <typename R>
+__device__ inline void argmax_step(R& v, int& i, R ov, int oi) {
+  if (ov > v || (ov == v && oi < i)) {
+    v = ov;
+    i = oi;
+  }
+}
+template <typename R, int CTRL, int ROWS>
+__device__ inline R dpp_any(R v) {
+  if constexpr (sizeof(R) == 8) return dpp_f64<CTRL, ROWS>(v);
+  else return dpp_f32<CTRL, ROWS>(v);
+}
+template <typename R>
+__device__ inline int wave_argmax(R v, int i) {
+  // Lanes outside the row mask of a bcast step read 0 from the DPP move;
+  // mask them back out so they cannot win.
+  argmax_step(v, i, dpp_any<R, dpp::kQuadSwap1, 0xF>(v), dpp_i32<dpp::kQuadSwap1, 0xF>(i));
+  argmax_step(v, i, dpp_any<R, dpp::kQuadSwap2, 0xF>(v), dpp_i32<dpp::kQuadSwap2, 0xF>(i));
+  argmax_step(v, i, dpp_any<R, dpp::kRowHalfMirror, 0xF>(v), dpp_i32<dpp::kRowHalfMirror, 0xF>(i));
+  argmax_step(v, i, dpp_any<R, dpp::kRowMirror, 0xF>(v), dpp_i32<dpp::kRowMirror, 0xF>(i));
+  const int row = (threadIdx.x & 63) >> 4;
+  {
+    const R ov = dpp_any<R, dpp::kRowBcast15, 0xA>(v);
+    const int oi = dpp_i32<dpp::kRowBcast15, 0xA>(i);
+    if (row & 1) argmax_step(v, i, ov, oi);
+  }
+  {
+    const R ov = dpp_any<R, dpp::kRowBcast31, 0xC>(v);
+    const int oi = dpp_i32<dpp::kRowBcast31, 0xC>(i);
+    if (row & 2) argmax_step(v, i, ov, oi);
+  }
+  return lane_value(i, 63);
+}
+
 // Laplace double layer, ie_solver/kernel/kernel.cpp:123-165.
 __device__ inline double laplace_entry(const DevGeom& g, int t, int s) {
   const double scale = 1.0 / (2 * M_PI);

@@ -310,11 +310,17 @@ __device__ inline bool qrcp_in_registers(int m, int n) {
 // row % 64, broadcast to the wave.
 template <typename R, int RPL>
 __device__ inline R wave_row(const R (&x)[RPL], int row) {
-  R v = x[0];
+  // readlane every register, then select among the uniform results: a
+  // select before the readlane is turned into a dynamically indexed
+  // extract, which the compiler lowers through scratch memory.
+  const int reg = row >> 6, ln = row & 63;
+  R v = lane_value(x[0], ln);
 #pragma unroll
-  for (int r = 1; r < RPL; r++)
-    if (r == (row >> 6)) v = x[r];
-  return __shfl(v, row & 63, 64);
+  for (int r = 1; r < RPL; r++) {
+    const R t = lane_value(x[r], ln);
+    v = (r == reg) ? t : v;
+  }
+  return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -361,10 +367,14 @@ __global__ __launch_bounds__(NW * 64) void k_qrcp_reg(DevState s, int start,
       const int i = lane + 64 * r;
       a[sl][r] = (c < n && i < m) ? A[i + (int64_t)m * c] : R(0);
     }
+  }
+#pragma unroll
+  for (int sl = 0; sl < CPW; sl++) {
+    const int c = wv + NW * sl;
     R ss = 0;
 #pragma unroll
     for (int r = 0; r < RPL; r++) ss += a[sl][r] * a[sl][r];
-    ss = wave_sum(ss);
+    ss = wave_total(ss);
     if (lane == 0 && c < n) {
       vn1[c] = sqrt(ss);
       vn2[c] = vn1[c];
@@ -377,38 +387,26 @@ __global__ __launch_bounds__(NW * 64) void k_qrcp_reg(DevState s, int start,
   int k = 0;
   R thresh = 0;
   for (int j = 0; j < kmax; j++) {
-    // Pivot: first logical position q >= j with the largest norm.
-    R best = -1;
-    int bi = n;
-    for (int q = j + lane; q < n; q += 64) {
-      const R v = vn1[col[q]];
-      if (v > best) {
-        best = v;
-        bi = q;
-      }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const R ob = __shfl_xor(best, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ob > best || (ob == best && oi < bi)) {
-        best = ob;
-        bi = oi;
-      }
-    }
+    // Pivot: first logical position q >= j with the largest norm (n <= 64:
+    // one position per lane).
+    const int q = j + lane;
+    int bi = wave_argmax<R>(q < n ? vn1[col[q]] : R(-1), q < n ? q : 0x7fffffff);
     if (bi >= n) bi = j;
     const int pvt = bi;
     const int pc = col[pvt];
     const int owner = pc % NW, pslot = pc / NW;
     if (wv == owner) {
+      // Branch-free selects keep every register index static (a dynamic
+      // a[pslot] would move the whole matrix to scratch memory).
       R x[RPL];
 #pragma unroll
       for (int r = 0; r < RPL; r++) x[r] = a[0][r];
 #pragma unroll
-      for (int sl = 1; sl < CPW; sl++)
-        if (sl == pslot)
+      for (int sl = 1; sl < CPW; sl++) {
+        const bool me = sl == pslot;
 #pragma unroll
-          for (int r = 0; r < RPL; r++) x[r] = a[sl][r];
+        for (int r = 0; r < RPL; r++) x[r] = me ? a[sl][r] : x[r];
+      }
       const R alpha = wave_row<R, RPL>(x, j);
       R ss = 0;
 #pragma unroll
@@ -416,7 +414,7 @@ __global__ __launch_bounds__(NW * 64) void k_qrcp_reg(DevState s, int start,
         const int i = lane + 64 * r;
         if (i > j && i < m) ss += x[r] * x[r];
       }
-      ss = wave_sum(ss);
+      ss = wave_total(ss);
       const R xnorm = sqrt(ss);
       R beta, tau, sc;
       if (xnorm == 0) {
@@ -438,13 +436,13 @@ __global__ __launch_bounds__(NW * 64) void k_qrcp_reg(DevState s, int start,
       if (!stop) {
         // R_jj of the pivot column; rows below the diagonal are not kept.
 #pragma unroll
-        for (int sl = 0; sl < CPW; sl++)
-          if (sl == pslot) {
-            done[sl] = true;
+        for (int sl = 0; sl < CPW; sl++) {
+          const bool me = sl == pslot;
+          done[sl] = done[sl] || me;
 #pragma unroll
-            for (int r = 0; r < RPL; r++)
-              if (lane + 64 * r == j) a[sl][r] = beta;
-          }
+          for (int r = 0; r < RPL; r++)
+            a[sl][r] = (me && lane + 64 * r == j) ? beta : a[sl][r];
+        }
       }
       if (lane == 0) {
         sh_tau = tau;
@@ -467,14 +465,24 @@ __global__ __launch_bounds__(NW * 64) void k_qrcp_reg(DevState s, int start,
     R vr[RPL];
 #pragma unroll
     for (int r = 0; r < RPL; r++) vr[r] = vbuf[lane + 64 * r];
+    // All dot products first, so their reductions can overlap.
+    R part[CPW];
+#pragma unroll
+    for (int sl = 0; sl < CPW; sl++) {
+      part[sl] = 0;
+#pragma unroll
+      for (int r = 0; r < RPL; r++) part[sl] += vr[r] * a[sl][r];
+    }
+#pragma unroll
+    for (int sl = 0; sl < CPW; sl++) {
+      const int c = wv + NW * sl;
+      if (c < n && !done[sl]) part[sl] = wave_total(part[sl]);
+    }
 #pragma unroll
     for (int sl = 0; sl < CPW; sl++) {
       const int c = wv + NW * sl;
       if (c >= n || done[sl]) continue;
-      R part = 0;
-#pragma unroll
-      for (int r = 0; r < RPL; r++) part += vr[r] * a[sl][r];
-      const R tw = tau * wave_sum(part);
+      const R tw = tau * part[sl];
 #pragma unroll
       for (int r = 0; r < RPL; r++) a[sl][r] -= tw * vr[r];
       const R rjc = wave_row<R, RPL>(a[sl], j);
@@ -491,7 +499,7 @@ __global__ __launch_bounds__(NW * 64) void k_qrcp_reg(DevState s, int start,
             const int i = lane + 64 * r;
             if (i > j && i < m) ss2 += a[sl][r] * a[sl][r];
           }
-          ss2 = wave_sum(ss2);
+          ss2 = wave_total(ss2);
           const R nv = (j < m - 1) ? sqrt(ss2) : R(0);
           if (lane == 0) {
             vn1[c] = nv;
@@ -535,7 +543,7 @@ __global__ __launch_bounds__(NW * 64) void k_qrcp_reg(DevState s, int start,
     R t = (lane < k) ? rbuf[lane + k * pcol] : R(0);
     for (int i = k - 1; i >= 0; i--) {
       const int ci = col[i];
-      const R ti = __shfl(t, i, 64) / rbuf[i + k * ci];
+      const R ti = lane_value(t, i) / rbuf[i + k * ci];
       if (lane == i) t = ti;
       if (lane < i) t -= rbuf[lane + k * ci] * ti;
     }
@@ -807,8 +815,10 @@ __device__ int block_lu(R* X, int n, int ld, int* ipiv) {
 // child's diagonal block.
 template <typename R>
 struct ChildUpdates {
-  const R* S[4];
-  int N[4];
+  // The four child blocks and sizes stay in LDS (indexing a private copy
+  // by the runtime child id would place it in scratch memory).
+  const R* const* S;
+  const int* N;
   const signed char* pch;
   const short* ploc;
   __device__ R operator()(int i, int j) const {
@@ -884,10 +894,8 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
   for (int i = tid; i < n; i += NT) sperm[i] = (short)s.perm[(int64_t)b * s.ncap + i];
   __syncthreads();
   ChildUpdates<R> upd;
-  for (int u = 0; u < 4; u++) {
-    upd.S[u] = chS[u];
-    upd.N[u] = chN[u];
-  }
+  upd.S = chS;
+  upd.N = chN;
   upd.pch = pch;
   upd.ploc = ploc;
   const int r = n - k;
@@ -931,17 +939,9 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
     // LU with partial pivoting, lane i = row i.
     int info = 0;
     for (int j = 0; j < r; j++) {
-      R v = (lane >= j && lane < r) ? fabs(X[lane + r * j]) : R(-1);
-      int bi = (lane >= j && lane < r) ? lane : r;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const R ov = __shfl_xor(v, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ov > v || (ov == v && oi < bi)) {
-          v = ov;
-          bi = oi;
-        }
-      }
+      const bool live = lane >= j && lane < r;
+      const int bi = wave_argmax<R>(live ? fabs(X[lane + r * j]) : R(-1),
+                                    live ? lane : 0x7fffffff);
       const int p = (bi >= r) ? j : bi;
       if (lane == 0) spiv[j] = p;
       if (p != j && lane < r) {
@@ -987,11 +987,11 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
   for (int a = wv; a < k; a += NW) {
     R u = (lane < r) ? Krs[idxf[lane] + r * a] : R(0);
     for (int l = 0; l < r; l++) {
-      const R ul = __shfl(u, l, 64);
+      const R ul = lane_value(u, l);
       if (lane > l && lane < r) u -= X[lane + r * l] * ul;
     }
     for (int i = r - 1; i >= 0; i--) {
-      const R ui = __shfl(u, i, 64) / X[i + r * i];
+      const R ui = lane_value(u, i) / X[i + r * i];
       if (lane == i) u = ui;
       if (lane < i) u -= X[lane + r * i] * ui;
     }
@@ -1001,12 +1001,12 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
   for (int a = wv; a < k; a += NW) {
     R acc = (lane < r) ? Ksr[a + k * lane] : R(0);
     for (int i = 0; i < r; i++) {
-      const R yi = __shfl(acc, i, 64) / X[i + r * i];
+      const R yi = lane_value(acc, i) / X[i + r * i];
       if (lane == i) acc = yi;
       if (lane > i && lane < r) acc -= X[i + r * lane] * yi;
     }
     for (int i = r - 1; i >= 0; i--) {
-      const R zi = __shfl(acc, i, 64);
+      const R zi = lane_value(acc, i);
       if (lane < i) acc -= X[i + r * lane] * zi;
     }
     const R z = __shfl(acc, lane < r ? idxb[lane] : 0, 64);
@@ -1083,12 +1083,10 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   }
   __syncthreads();
   ChildUpdates<R> upd;
+  upd.S = chS;
+  upd.N = chN;
   bool any = false;
-  for (int u = 0; u < 4; u++) {
-    upd.S[u] = chS[u];
-    upd.N[u] = chN[u];
-    any = any || chS[u] != nullptr;
-  }
+  for (int u = 0; u < 4; u++) any = any || chS[u] != nullptr;
   upd.pch = pch;
   upd.ploc = ploc;
   const int k = s.rank[b];
@@ -1244,10 +1242,8 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
   }
   __syncthreads();
   ChildUpdates<R> upd;
-  for (int u = 0; u < 4; u++) {
-    upd.S[u] = chS[u];
-    upd.N[u] = chN[u];
-  }
+  upd.S = chS;
+  upd.N = chN;
   upd.pch = pch;
   upd.ploc = ploc;
   const int* act = s.act_list;

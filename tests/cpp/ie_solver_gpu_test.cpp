@@ -120,6 +120,13 @@ static void backward_case(const std::string& name, ie_solver_config config) {
 int main() {
   const char* tol_env = std::getenv("IE_TEST_TOL");
   const double tol = tol_env ? std::atof(tol_env) : DEFAULT_ID_TOL;
+  // Analytic agreement is also limited by the quadrature: the 20x20 grid of
+  // ie_solver_test.cpp:105-107 reaches 0.01 from a 512-node boundary (node
+  // spacing 0.012), where the trapezoid rule errs by ~1e-6 (multiply
+  // connected cases).  The reference checks these at 10 x 1e-6; below that
+  // tolerance they are held to that floor, while the backward-error cases
+  // are held to 50 x tol.
+  const double atol = 10 * std::fmax(tol, 1e-6);
   ie_solver_config lap;
   lap.id_tol = tol;
   ie_solver_config sto = lap;
@@ -150,21 +157,21 @@ int main() {
     b.initialize(lap.num_boundary_points, lap.boundary_condition);
     std::vector<double> dom;
     ie_Mat sol = get_dom_sol(lap, &dom, &b);
-    expect_le("LaplaceCircleAnalyticAgreementElectron", laplace_error(sol, dom, &b), 10 * tol);
+    expect_le("LaplaceCircleAnalyticAgreementElectron", laplace_error(sol, dom, &b), atol);
   }
   {  // ie_solver_test.cpp:419-431
     Annulus b;
     b.initialize(lap.num_boundary_points, lap.boundary_condition);
     std::vector<double> dom;
     ie_Mat sol = get_dom_sol(lap, &dom, &b);
-    expect_le("LaplaceAnnulusAnalyticAgreementElectron", laplace_error(sol, dom, &b), 10 * tol);
+    expect_le("LaplaceAnnulusAnalyticAgreementElectron", laplace_error(sol, dom, &b), atol);
   }
   {  // ie_solver_test.cpp:528-543
     Circle b;
     b.initialize(sto.num_boundary_points, sto.boundary_condition);
     std::vector<double> dom;
     ie_Mat sol = get_dom_sol(sto, &dom, &b);
-    expect_le("StokesCircleAnalyticAgreementTangent", stokes_error(sol, dom, &b), 10 * tol);
+    expect_le("StokesCircleAnalyticAgreementTangent", stokes_error(sol, dom, &b), atol);
   }
   {  // ie_solver_test.cpp:545-562
     ie_solver_config c = sto;
@@ -174,7 +181,7 @@ int main() {
     b.initialize(c.num_boundary_points, c.boundary_condition);
     std::vector<double> dom;
     ie_Mat sol = get_dom_sol(c, &dom, &b);
-    expect_le("StokesDonutAnalyticAgreementTangent", stokes_error(sol, dom, &b), 10 * tol);
+    expect_le("StokesDonutAnalyticAgreementTangent", stokes_error(sol, dom, &b), atol);
   }
   {  // bie_time_trial (linear_solve.cpp:156-185) runs and reports times
     ie_solver_config c = lap;
