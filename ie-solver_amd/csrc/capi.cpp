@@ -178,7 +178,7 @@ struct iegpu_ctx {
   bool factored = false;
   DevBuf<int> s_act_cnt, s_act_list, s_chpos, s_out_cnt, s_out_list,
       s_red_list, s_perm, s_rank, s_upd_kind, s_m_rows, s_row_pool, s_err,
-      s_root_piv;
+      s_root_piv, s_qkind, s_jobs, s_njobs;
   DevBuf<int64_t> s_id_sz, s_row_sz, s_fac_sz, s_id_off, s_row_off, s_fac_off,
       s_fac_top;
   DevBuf<unsigned char> s_id_pool, s_fac_pool, s_root_mat;
@@ -236,6 +236,9 @@ struct iegpu_ctx {
     s.rank = s_rank.p;
     s.upd_kind = s_upd_kind.p;
     s.m_rows = s_m_rows.p;
+    s.qkind = s_qkind.p;
+    s.jobs = s_jobs.p;
+    s.njobs = s_njobs.p;
     s.id_sz = s_id_sz.p;
     s.row_sz = s_row_sz.p;
     s.fac_sz = s_fac_sz.p;
@@ -342,6 +345,9 @@ void alloc_state(iegpu_ctx* c) {
   c->s_rank.alloc(nb);
   c->s_upd_kind.alloc(nb);
   c->s_m_rows.alloc(nb);
+  c->s_qkind.alloc(nb);
+  c->s_jobs.alloc((size_t)2 * kMaxChunks * nb);
+  c->s_njobs.alloc(kMaxLevels);
   c->s_id_sz.alloc(nb);
   c->s_row_sz.alloc(nb);
   c->s_fac_sz.alloc(nb);
@@ -540,6 +546,8 @@ int iegpu_build_tree(iegpu_ctx* c, int leaf_cap) {
   try {
     c->tree = HostTree();
     c->tree.build(c->h_pts.data(), c->n_points, c->sol_dim, leaf_cap);
+    if (c->tree.n_levels > kMaxLevels)
+      return fail(c, IEGPU_ERR_INVALID, "quadtree deeper than 64 levels");
     c->have_tree = true;
     c->factored = false;
     if (c->device < 0) return IEGPU_OK;

@@ -25,6 +25,23 @@ enum : int {
   kErrRowPool = 16,    // row-index pool exhausted
 };
 
+// How a box's interpolative decomposition runs.
+enum QrcpKind : int {
+  kQrcpNone = 0,    // not compressed (too few DOFs or no ID rows)
+  kQrcpReg = 1,     // register QRCP on the ID matrix (m <= 512, n <= 64)
+  kQrcpStack = 2,   // row-compressed in chunks first, then register QRCP
+  kQrcpLds = 3,     // general QRCP (LDS or the box's scratch slot)
+};
+constexpr int kRegMaxRows = 512;
+constexpr int kRegMaxCols = 64;
+constexpr int kMaxChunks = 8;
+constexpr int kMaxLevels = 64;  // tree depth bound (per-level job counters)
+
+// Row chunks of a tall ID matrix: ceil(m / 512), as even as possible.
+__host__ __device__ inline int id_chunks(int m) {
+  return (m + kRegMaxRows - 1) / kRegMaxRows;
+}
+
 struct DevGeom {
   const double* pts;   // 2 per point
   const double* nrm;   // 2 per point
@@ -66,6 +83,9 @@ struct DevState {
   int* rank;           // k (0: not compressed)
   int* upd_kind;       // 0: no carried update, 1: carried update stored
   int* m_rows;         // ID matrix height
+  int* qkind;          // ID route of the box (QrcpKind)
+  int* jobs;           // (box, chunk) row-compression jobs, kMaxChunks per box slot
+  int* njobs;          // per level: number of row-compression jobs
   int64_t* id_sz;      // scratch doubles/floats needed (per box, level-local)
   int64_t* row_sz;
   int64_t* fac_sz;
@@ -139,6 +159,10 @@ constexpr int kRowHalfMirror = 0x141;
 constexpr int kRowMirror = 0x140;
 constexpr int kRowBcast15 = 0x142;
 constexpr int kRowBcast31 = 0x143;
+constexpr int kQuadSwap3 = 0x1B;     // quad_perm [3,2,1,0]
+constexpr int kQuadBitrev = 0xD8;    // quad_perm [0,2,1,3]
+constexpr int kRowRor4 = 0x124;
+constexpr int kRowRor8 = 0x128;
 }  // namespace dpp
 
 template <int CTRL, int ROWS>
@@ -226,6 +250,92 @@ __device__ inline int wave_argmax(R v, int i) {
     if (row & 2) argmax_step(v, i, ov, oi);
   }
   return lane_value(i, 63);
+}
+
+// ---------------------------------------------------------------------------
+// Four-column interleave.  A wave holding four columns ("classes" 0..3) of
+// a matrix, row i on lane i % 64, keeps class sl ^ quad_class(l) in slot sl
+// of lane l.  The four column sums then need two pair exchanges followed by
+// one reduction, instead of four separate wave reductions: after the
+// exchanges lane l carries only class quad_class(l).
+// ---------------------------------------------------------------------------
+
+__device__ inline int quad_class(int lane) {
+  return ((lane & 1) << 1) | ((lane >> 1) & 1);
+}
+
+template <typename R>
+__device__ inline R shfl_xor_any(R v, int o) {
+  return __shfl_xor(v, o, 64);
+}
+
+// Per-lane partials p[sl] (slot layout above) -> the wave total of class
+// quad_class(lane), in every lane.
+template <typename R>
+__device__ inline R wave_total4(const R (&p)[4]) {
+  const R s0 = p[0] + dpp_any<R, dpp::kQuadSwap1, 0xF>(p[2]);
+  const R s1 = p[1] + dpp_any<R, dpp::kQuadSwap1, 0xF>(p[3]);
+  R t = s0 + dpp_any<R, dpp::kQuadSwap2, 0xF>(s1);
+  t += dpp_any<R, dpp::kRowRor4, 0xF>(t);
+  t += dpp_any<R, dpp::kRowRor8, 0xF>(t);
+  t += shfl_xor_any(t, 16);
+  t += shfl_xor_any(t, 32);
+  return t;
+}
+
+// Spreads class totals (lane l holding class quad_class(l)) back to the
+// slot layout: out[sl] = total of class sl ^ quad_class(lane).
+template <typename R>
+__device__ inline void spread4(R t, R (&out)[4]) {
+  out[0] = t;
+  out[1] = dpp_any<R, dpp::kQuadSwap2, 0xF>(t);
+  out[2] = dpp_any<R, dpp::kQuadSwap1, 0xF>(t);
+  out[3] = dpp_any<R, dpp::kQuadSwap3, 0xF>(t);
+}
+
+// Class totals -> lane l of every quad holds class (l & 3).
+template <typename R>
+__device__ inline R class_by_lane(R t) {
+  return dpp_any<R, dpp::kQuadBitrev, 0xF>(t);
+}
+
+// Pivot key: a column norm v >= 0 with its logical position in the low
+// seven bits, ordered as a non-negative double.  The largest key is the
+// largest norm, ties (equal to 2^-45 relative in FP64) going to the
+// smaller position, as LAPACK's first-maximum rule.  Unavailable columns
+// carry -1.
+template <typename R>
+__device__ inline double pivot_key(R v, int pos) {
+  uint64_t b;
+  if constexpr (sizeof(R) == 8)
+    b = (uint64_t)__double_as_longlong(v) & ~uint64_t(127);
+  else
+    b = (uint64_t)__float_as_uint(v) << 32;
+  return __longlong_as_double((long long)(b | (uint64_t)(127 - pos)));
+}
+__device__ inline int pivot_key_pos(double key) {
+  return 127 - (int)(__double_as_longlong(key) & 127);
+}
+
+// DPP move that leaves lanes outside the row mask with their own value.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_keep_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int l0 = (int)(b & 0xffffffffll), h0 = (int)(b >> 32);
+  const int lo = __builtin_amdgcn_update_dpp(l0, l0, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(h0, h0, CTRL, ROWS, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Maximum over the wave, in every lane.
+__device__ inline double wave_max(double v) {
+  v = fmax(v, dpp_f64<dpp::kQuadSwap1, 0xF>(v));
+  v = fmax(v, dpp_f64<dpp::kQuadSwap2, 0xF>(v));
+  v = fmax(v, dpp_f64<dpp::kRowHalfMirror, 0xF>(v));
+  v = fmax(v, dpp_f64<dpp::kRowMirror, 0xF>(v));
+  v = fmax(v, dpp_keep_f64<dpp::kRowBcast15, 0xA>(v));
+  v = fmax(v, dpp_keep_f64<dpp::kRowBcast31, 0xC>(v));
+  return lane_value(v, 63);
 }
 
 // Laplace double layer, ie_solver/kernel/kernel.cpp:123-165.

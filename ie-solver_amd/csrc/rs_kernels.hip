@@ -54,6 +54,7 @@ __global__ void k_reset(DevState s) {
     s.err[1] = 0;
     *s.fac_top = 0;
   }
+  if (threadIdx.x < kMaxLevels) s.njobs[threadIdx.x] = 0;
 }
 
 // active(box) = concatenation of the children's outgoing lists (skeleton if
@@ -142,10 +143,31 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
         for_each_row_dof(g, t, s, b, level >= 3, [](int, int) {});
     m = 2 * cnt + proxy_rows(g, level);
   }
+  int kind = kQrcpNone;
+  if (m > 0) {
+    const int nch = id_chunks(m);
+    if (n <= kRegMaxCols && m <= kRegMaxRows) kind = kQrcpReg;
+    else if (n <= kRegMaxCols && nch <= kMaxChunks && nch * n <= kRegMaxRows) kind = kQrcpStack;
+    else kind = kQrcpLds;
+  }
   if (threadIdx.x == 0) {
     s.m_rows[b] = m;
+    s.qkind[b] = kind;
     const int64_t ld = m > 2 * n ? m : 2 * n;
-    s.id_sz[b] = (m > 0) ? ld * n : 0;
+    int64_t stack = 0;
+    if (kind == kQrcpStack) {
+      // Row-compression jobs, one per chunk, and room for the stacked
+      // triangles after the ID matrix.
+      const int nch = id_chunks(m);
+      stack = (int64_t)nch * n * n;
+      const int base = atomicAdd(s.njobs + level, nch);
+      int* J = s.jobs + (int64_t)2 * kMaxChunks * start + 2 * base;
+      for (int q = 0; q < nch; q++) {
+        J[2 * q] = b;
+        J[2 * q + 1] = q;
+      }
+    }
+    s.id_sz[b] = (m > 0) ? ld * n + stack : 0;
     s.row_sz[b] = (m > 0) ? (m - proxy_rows(g, level)) / 2 : 0;
     s.fac_sz[b] = (n > 0) ? fac_slot_size(n) : 0;
   }
@@ -297,151 +319,256 @@ __global__ __launch_bounds__(256) void k_assemble(DevGeom g, DevTree t,
   }
 }
 
-// Boxes the register-resident QRCP handles: m <= 64*kRegRows rows and
-// n <= kRegWaves*kRegCols columns.
-constexpr int kRegRows = 8;   // rows per lane
-constexpr int kRegCols = 4;   // columns per wave
-constexpr int kRegWaves = 16;
-__device__ inline bool qrcp_in_registers(int m, int n) {
-  return m > 0 && n >= kMinCompress && m <= 64 * kRegRows && n <= kRegWaves * kRegCols;
-}
-
-// Value of row `row` (wave-uniform) of a column held as x[reg] on lane
-// row % 64, broadcast to the wave.
-template <typename R, int RPL>
-__device__ inline R wave_row(const R (&x)[RPL], int row) {
-  // readlane every register, then select among the uniform results: a
-  // select before the readlane is turned into a dynamically indexed
-  // extract, which the compiler lowers through scratch memory.
-  const int reg = row >> 6, ln = row & 63;
-  R v = lane_value(x[0], ln);
-#pragma unroll
-  for (int r = 1; r < RPL; r++) {
-    const R t = lane_value(x[r], ln);
-    v = (r == reg) ? t : v;
-  }
-  return v;
-}
-
 // ---------------------------------------------------------------------------
+// Register-resident QR kernels (1024 threads, 16 waves).  Wave w holds
+// columns 4w .. 4w+3 of an m x n matrix (m <= 64 RPL, n <= 64) in the
+// four-column interleave of rs_device.hpp: row i on lane i % 64, register
+// i / 64, column 4w + (sl ^ quad_class(lane)) in slot sl.  A Householder
+// step: the wave owning the pivot column forms the reflector and publishes
+// it through LDS; every wave then updates its columns in registers with
+// one four-column reduction.  Waves past the last column only take part
+// in the barriers.
+// ---------------------------------------------------------------------------
+
+constexpr int kXThreads = 1024;
+constexpr int kXRows = kRegMaxRows / 64;  // registers per lane per column
+
+// Entries of the column in slot psl (per lane) of the interleave.
+template <typename R, int RPL>
+__device__ inline void pick_slot(const R (&a)[4][RPL], int psl, R (&x)[RPL]) {
+#pragma unroll
+  for (int r = 0; r < RPL; r++) x[r] = a[0][r];
+#pragma unroll
+  for (int sl = 1; sl < 4; sl++) {
+    const bool me = psl == sl;
+#pragma unroll
+    for (int r = 0; r < RPL; r++) x[r] = me ? a[sl][r] : x[r];
+  }
+}
+
+// Householder reflector of column x at step j (j < 64), LAPACK dlarfg:
+// beta = -sign(alpha) ||x(j:)||, tau = (beta - alpha) / beta,
+// v = x(j+1:) / (alpha - beta), v_j = 1, v(0:j-1) = 0.  Writes v to vb
+// (all 64 RPL rows; rows past the matrix hold zeros in x).
+template <typename R, int RPL>
+__device__ inline void householder(const R (&x)[RPL], int j, R* vb, R* beta_out,
+                                   R* tau_out) {
+  const int lane = threadIdx.x & 63;
+  const R alpha = lane_value(x[0], j);
+  R ss = lane > j ? x[0] * x[0] : R(0);
+#pragma unroll
+  for (int r = 1; r < RPL; r++) ss += x[r] * x[r];
+  ss = wave_total(ss);
+  const R xnorm = sqrt(ss);
+  R beta, tau, sc;
+  if (xnorm == 0) {
+    tau = 0;
+    beta = alpha;
+    sc = 0;
+  } else {
+    beta = -copysign(hypot(alpha, xnorm), alpha);
+    tau = (beta - alpha) / beta;
+    sc = 1 / (alpha - beta);
+  }
+  vb[lane] = lane > j ? x[0] * sc : (lane == j ? R(1) : R(0));
+#pragma unroll
+  for (int r = 1; r < RPL; r++) vb[lane + 64 * r] = x[r] * sc;
+  *beta_out = beta;
+  *tau_out = tau;
+}
+
+// a <- (I - tau v v^T) a on this wave's columns; `live` says whether the
+// class quad_class(lane) is still to be updated (pivoted columns keep
+// their R entries).
+template <typename R, int RPL>
+__device__ inline void apply_reflector(R (&a)[4][RPL], const R* vb, R tau, bool live) {
+  const int lane = threadIdx.x & 63;
+  R vr[RPL];
+#pragma unroll
+  for (int r = 0; r < RPL; r++) vr[r] = vb[lane + 64 * r];
+  R p[4];
+#pragma unroll
+  for (int sl = 0; sl < 4; sl++) {
+    p[sl] = 0;
+#pragma unroll
+    for (int r = 0; r < RPL; r++) p[sl] += vr[r] * a[sl][r];
+  }
+  R t = wave_total4(p);
+  t = live ? tau * t : R(0);
+  R tw[4];
+  spread4(t, tw);
+#pragma unroll
+  for (int sl = 0; sl < 4; sl++)
+#pragma unroll
+    for (int r = 0; r < RPL; r++) a[sl][r] -= tw[sl] * vr[r];
+}
+
+template <typename R, int RPL>
+__device__ inline void load_columns(const R* A, int64_t ld, int row0, int rows, int n,
+                                    R (&a)[4][RPL]) {
+  const int lane = threadIdx.x & 63, c0 = 4 * (threadIdx.x >> 6);
+  const int hq = quad_class(lane);
+#pragma unroll
+  for (int sl = 0; sl < 4; sl++) {
+    const int c = c0 + (sl ^ hq);
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+      const int i = lane + 64 * r;
+      a[sl][r] = (c < n && i < rows) ? A[row0 + i + ld * c] : R(0);
+    }
+  }
+}
+
+// Row compression of a tall ID matrix: chunk q of box b (rows
+// [q ch, q ch + rows), ch = ceil(m / chunks)) is reduced by unpivoted
+// Householder QR to its n x n triangle R_q, written to rows [q n, q n + n)
+// of the box's stack (ld chunks * n, after the ID matrix in its scratch
+// slot).  The stack is Q^T A for an orthogonal Q, so its pivoted QR has
+// the same R and column order as that of A.  Grid-strided over the
+// level's jobs; one barrier per step (double-buffered reflector).
+template <typename R, int RPL>
+__global__ __launch_bounds__(kXThreads) void k_rowcomp(DevState s, int start, int level) {
+  constexpr int MAXM = 64 * RPL;
+  __shared__ R vbuf[2][MAXM];
+  __shared__ R sh_tau[2];
+  if (!pools_ok(s)) return;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int hq = quad_class(lane);
+  const int c0 = 4 * wv;
+  const int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
+  const int njobs = s.njobs[level];
+  for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
+    const int b = jobs[2 * job], q = jobs[2 * job + 1];
+    const int n = s.act_cnt[b], m = s.m_rows[b];
+    const int nch = id_chunks(m);
+    const int ch = (m + nch - 1) / nch;
+    const int r0 = q * ch, rows = min(ch, m - r0);
+    R* A = reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
+    R* S = A + (int64_t)max(m, 2 * n) * n;
+    const int64_t lds = (int64_t)nch * n;
+    const bool active = c0 < n;
+    R a[4][RPL];
+    load_columns(A, m, r0, rows, n, a);
+    const int steps = min(rows, n);
+    int dmask = 0;
+    for (int j = 0; j < steps; j++) {
+      if (wv == (j >> 2)) {
+        const int psl = (j & 3) ^ hq;
+        R x[RPL];
+        pick_slot(a, psl, x);
+        R beta, tau;
+        householder(x, j, vbuf[j & 1], &beta, &tau);
+        dmask |= 1 << (j & 3);
+        if (lane == j) {
+#pragma unroll
+          for (int sl = 0; sl < 4; sl++) a[sl][0] = (sl == psl) ? beta : a[sl][0];
+        }
+        if (lane == 0) sh_tau[j & 1] = tau;
+      }
+      __syncthreads();
+      if (active) apply_reflector(a, vbuf[j & 1], sh_tau[j & 1], ((dmask >> hq) & 1) == 0);
+    }
+    // Column c was reduced at step c: rows below its diagonal are zero.
+    if (active) {
+#pragma unroll
+      for (int sl = 0; sl < 4; sl++) {
+        const int c = c0 + (sl ^ hq);
+        if (c < n && lane < n)
+          S[(int64_t)q * n + lane + lds * c] = lane > c ? R(0) : a[sl][0];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Interpolative decomposition by column-pivoted Householder QR, stopped at
 // the first |R_jj| < tol |R_00| (ie_mat.cpp:548-588).  The pivot rule,
-// Householder form and column-norm downdating follow LAPACK dgeqp3/dlaqp2;
-// logical column swaps are index swaps.
-//
-// Register-resident form: physical column c lives in VGPRs of wave
-// c % NW (slot c / NW), row i on lane i % 64 (register i / 64).  Per step
-// every wave picks the pivot redundantly from the LDS norms, the owner
-// wave forms the Householder vector and publishes it through LDS, and all
-// waves update their columns in registers.
-// ---------------------------------------------------------------------------
-
-template <typename R, int RPL, int CPW, int NW>
-__global__ __launch_bounds__(NW * 64) void k_qrcp_reg(DevState s, int start,
-                                                      double tol_d) {
-  constexpr int MAXM = RPL * 64;
-  constexpr int MAXN = CPW * NW;
+// Householder form and column-norm downdating follow LAPACK dgeqp3/dlaqp2.
+// Lanes 0..3 of each wave keep the norms, the logical position and the
+// pivot key of the wave's four columns; the pivot is the largest key
+// (pivot_key), found by one wave maximum.
+template <typename R, int RPL>
+__global__ __launch_bounds__(kXThreads) void k_qrcp_id(DevState s, int start, double tol_d) {
+  constexpr int MAXM = 64 * RPL;
   __shared__ R vbuf[MAXM];
-  __shared__ R vn1[MAXN];
-  __shared__ R vn2[MAXN];
-  __shared__ int col[MAXN];
-  __shared__ R rbuf[MAXN * MAXN];
+  __shared__ R rowj[kRegMaxCols];
+  __shared__ double keys[kRegMaxCols];
+  __shared__ R rbuf[kRegMaxCols * kRegMaxCols];
+  __shared__ int colmap[kRegMaxCols];
   __shared__ R sh_tau, sh_thresh;
   __shared__ int sh_stop;
 
   const int b = start + blockIdx.x;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int n = s.act_cnt[b], m = s.m_rows[b];
-  if (!qrcp_in_registers(m, n) || !pools_ok(s)) return;
-  const R tol = static_cast<R>(tol_d);
+  const int kind = s.qkind[b];
+  if ((kind != kQrcpReg && kind != kQrcpStack) || !pools_ok(s)) return;
+  const int n = s.act_cnt[b];
   const R* A = reinterpret_cast<const R*>(s.id_pool) + s.id_off[b];
+  int m = s.m_rows[b];
+  if (kind == kQrcpStack) {
+    A += (int64_t)max(m, 2 * n) * n;
+    m = id_chunks(m) * n;
+  }
+  const R tol = static_cast<R>(tol_d);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int hq = quad_class(lane);
+  const int c0 = 4 * wv;
+  const bool active = c0 < n;
 
-  R a[CPW][RPL];
-  bool done[CPW];
+  R a[4][RPL];
+  load_columns(A, m, 0, m, n, a);
+  // Lanes 0..3: column c0 + lane.
+  const int myc = c0 + (lane & 3);
+  R vn1 = 0, vn2 = 0;
+  int pos = myc;
+  bool avail = lane < 4 && myc < n;
+  if (active) {
+    R p[4];
 #pragma unroll
-  for (int sl = 0; sl < CPW; sl++) {
-    const int c = wv + NW * sl;
-    done[sl] = false;
+    for (int sl = 0; sl < 4; sl++) {
+      p[sl] = 0;
 #pragma unroll
-    for (int r = 0; r < RPL; r++) {
-      const int i = lane + 64 * r;
-      a[sl][r] = (c < n && i < m) ? A[i + (int64_t)m * c] : R(0);
+      for (int r = 0; r < RPL; r++) p[sl] += a[sl][r] * a[sl][r];
     }
+    vn1 = sqrt(class_by_lane(wave_total4(p)));
+    vn2 = vn1;
   }
-#pragma unroll
-  for (int sl = 0; sl < CPW; sl++) {
-    const int c = wv + NW * sl;
-    R ss = 0;
-#pragma unroll
-    for (int r = 0; r < RPL; r++) ss += a[sl][r] * a[sl][r];
-    ss = wave_total(ss);
-    if (lane == 0 && c < n) {
-      vn1[c] = sqrt(ss);
-      vn2[c] = vn1[c];
-    }
-  }
-  for (int c = tid; c < n; c += NW * 64) col[c] = c;
+  if (lane < 4) keys[myc] = avail ? pivot_key(vn1, pos) : -1.0;
   __syncthreads();
 
   const int kmax = min(m, n);
-  int k = 0;
+  int k = kmax;
   R thresh = 0;
+  int dmask = 0;  // classes of this wave already pivoted
   for (int j = 0; j < kmax; j++) {
-    // Pivot: first logical position q >= j with the largest norm (n <= 64:
-    // one position per lane).
-    const int q = j + lane;
-    int bi = wave_argmax<R>(q < n ? vn1[col[q]] : R(-1), q < n ? q : 0x7fffffff);
-    if (bi >= n) bi = j;
-    const int pvt = bi;
-    const int pc = col[pvt];
-    const int owner = pc % NW, pslot = pc / NW;
-    if (wv == owner) {
-      // Branch-free selects keep every register index static (a dynamic
-      // a[pslot] would move the whole matrix to scratch memory).
+    const double kv = keys[lane];
+    const double best = wave_max(kv);
+    if (!(best >= 0)) {  // every wave sees the same keys
+      k = j;
+      break;
+    }
+    const int p = __ffsll((unsigned long long)__ballot(kv == best)) - 1;
+    const int q = pivot_key_pos(best);
+    if (lane < 4) {
+      if (myc == p) {
+        pos = j;
+        avail = false;
+      } else if (pos == j) {
+        pos = q;
+      }
+    }
+    if (wv == (p >> 2)) {
+      const int psl = (p & 3) ^ hq;
       R x[RPL];
-#pragma unroll
-      for (int r = 0; r < RPL; r++) x[r] = a[0][r];
-#pragma unroll
-      for (int sl = 1; sl < CPW; sl++) {
-        const bool me = sl == pslot;
-#pragma unroll
-        for (int r = 0; r < RPL; r++) x[r] = me ? a[sl][r] : x[r];
-      }
-      const R alpha = wave_row<R, RPL>(x, j);
-      R ss = 0;
-#pragma unroll
-      for (int r = 0; r < RPL; r++) {
-        const int i = lane + 64 * r;
-        if (i > j && i < m) ss += x[r] * x[r];
-      }
-      ss = wave_total(ss);
-      const R xnorm = sqrt(ss);
-      R beta, tau, sc;
-      if (xnorm == 0) {
-        tau = 0;
-        beta = alpha;
-        sc = 0;
-      } else {
-        beta = -copysign(hypot(alpha, xnorm), alpha);
-        tau = (beta - alpha) / beta;
-        sc = 1 / (alpha - beta);
-      }
-#pragma unroll
-      for (int r = 0; r < RPL; r++) {
-        const int i = lane + 64 * r;
-        vbuf[i] = (i > j && i < m) ? x[r] * sc : (i == j ? R(1) : R(0));
-      }
+      pick_slot(a, psl, x);
+      R beta, tau;
+      householder(x, j, vbuf, &beta, &tau);
       const R th = (j == 0) ? fabs(tol * beta) : thresh;
       const int stop = (j >= 1 && fabs(beta) < th) ? 1 : 0;
       if (!stop) {
-        // R_jj of the pivot column; rows below the diagonal are not kept.
+        dmask |= 1 << (p & 3);
+        if (lane == j) {
 #pragma unroll
-        for (int sl = 0; sl < CPW; sl++) {
-          const bool me = sl == pslot;
-          done[sl] = done[sl] || me;
-#pragma unroll
-          for (int r = 0; r < RPL; r++)
-            a[sl][r] = (me && lane + 64 * r == j) ? beta : a[sl][r];
+          for (int sl = 0; sl < 4; sl++) a[sl][0] = (sl == psl) ? beta : a[sl][0];
         }
       }
       if (lane == 0) {
@@ -456,93 +583,71 @@ __global__ __launch_bounds__(NW * 64) void k_qrcp_reg(DevState s, int start,
       break;
     }
     if (j == 0) thresh = sh_thresh;
-    const R tau = sh_tau;
-    if (tid == 0 && pvt != j) {
-      const int t0 = col[j];
-      col[j] = col[pvt];
-      col[pvt] = t0;
-    }
-    R vr[RPL];
+    if (active) {
+      apply_reflector(a, vbuf, sh_tau, ((dmask >> hq) & 1) == 0);
+      if (lane == j) {
 #pragma unroll
-    for (int r = 0; r < RPL; r++) vr[r] = vbuf[lane + 64 * r];
-    // All dot products first, so their reductions can overlap.
-    R part[CPW];
-#pragma unroll
-    for (int sl = 0; sl < CPW; sl++) {
-      part[sl] = 0;
-#pragma unroll
-      for (int r = 0; r < RPL; r++) part[sl] += vr[r] * a[sl][r];
-    }
-#pragma unroll
-    for (int sl = 0; sl < CPW; sl++) {
-      const int c = wv + NW * sl;
-      if (c < n && !done[sl]) part[sl] = wave_total(part[sl]);
-    }
-#pragma unroll
-    for (int sl = 0; sl < CPW; sl++) {
-      const int c = wv + NW * sl;
-      if (c >= n || done[sl]) continue;
-      const R tw = tau * part[sl];
-#pragma unroll
-      for (int r = 0; r < RPL; r++) a[sl][r] -= tw * vr[r];
-      const R rjc = wave_row<R, RPL>(a[sl], j);
-      const R v1 = vn1[c];
-      if (v1 != 0) {
-        R temp = fabs(rjc) / v1;
+        for (int sl = 0; sl < 4; sl++) rowj[c0 + (sl ^ hq)] = a[sl][0];
+      }
+      wave_sync();
+      bool need = false;
+      if (avail && vn1 != 0) {
+        R temp = fabs(rowj[myc]) / vn1;
         temp = 1 - temp * temp;
         temp = temp > 0 ? temp : R(0);
-        const R rat = v1 / vn2[c];
-        if (temp * rat * rat <= Eps<R>::tol3z) {
-          R ss2 = 0;
+        const R rat = vn1 / vn2;
+        if (temp * rat * rat <= Eps<R>::tol3z) need = true;
+        else vn1 = vn1 * sqrt(temp);
+      }
+      if (__ballot(need)) {
+        R p2[4];
 #pragma unroll
-          for (int r = 0; r < RPL; r++) {
-            const int i = lane + 64 * r;
-            if (i > j && i < m) ss2 += a[sl][r] * a[sl][r];
-          }
-          ss2 = wave_total(ss2);
-          const R nv = (j < m - 1) ? sqrt(ss2) : R(0);
-          if (lane == 0) {
-            vn1[c] = nv;
-            vn2[c] = nv;
-          }
-        } else if (lane == 0) {
-          vn1[c] = v1 * sqrt(temp);
+        for (int sl = 0; sl < 4; sl++) {
+          p2[sl] = lane > j ? a[sl][0] * a[sl][0] : R(0);
+#pragma unroll
+          for (int r = 1; r < RPL; r++) p2[sl] += a[sl][r] * a[sl][r];
+        }
+        const R t = class_by_lane(wave_total4(p2));
+        if (need) {
+          const R nv = (j < m - 1) ? sqrt(t) : R(0);
+          vn1 = nv;
+          vn2 = nv;
         }
       }
+      if (lane < 4) keys[myc] = avail ? pivot_key(vn1, pos) : -1.0;
     }
     __syncthreads();
   }
 
+  // Logical order and rows 0..k-1 of R (physical column order, ld k).
+  if (lane < 4 && myc < n) colmap[pos] = myc;
+  if (active) {
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) {
+      const int c = c0 + (sl ^ hq);
+      if (c < n && lane < k) rbuf[lane + k * c] = a[sl][0];
+    }
+  }
+  __syncthreads();
+
   const int* act = s.act_list + (int64_t)b * s.ncap;
   int* out = s.out_list + (int64_t)b * s.ncap;
   if (k == 0) {
-    for (int i = tid; i < n; i += NW * 64) out[i] = act[i];
+    for (int i = tid; i < n; i += kXThreads) out[i] = act[i];
     if (tid == 0) {
       s.rank[b] = 0;
       s.out_cnt[b] = n;
     }
     return;
   }
-  // Rows 0..k-1 of R (physical column order) to LDS.
-#pragma unroll
-  for (int sl = 0; sl < CPW; sl++) {
-    const int c = wv + NW * sl;
-    if (c < n)
-#pragma unroll
-      for (int r = 0; r < RPL; r++) {
-        const int i = lane + 64 * r;
-        if (i < k) rbuf[i + k * c] = a[sl][r];
-      }
-  }
-  __syncthreads();
   // T = R11^{-1} R12: one wave per column, lane l holds t_l (k <= 64).
   const int r = n - k;
   R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
-  for (int cp = wv; cp < r; cp += NW) {
-    const int pcol = col[k + cp];
+  for (int cp = wv; cp < r; cp += kXThreads / 64) {
+    const int pcol = colmap[k + cp];
     R t = (lane < k) ? rbuf[lane + k * pcol] : R(0);
     for (int i = k - 1; i >= 0; i--) {
-      const int ci = col[i];
+      const int ci = colmap[i];
       const R ti = lane_value(t, i) / rbuf[i + k * ci];
       if (lane == i) t = ti;
       if (lane < i) t -= rbuf[lane + k * ci] * ti;
@@ -551,8 +656,8 @@ __global__ __launch_bounds__(NW * 64) void k_qrcp_reg(DevState s, int start,
   }
   int* red = s.red_list + (int64_t)b * s.ncap;
   int* pm = s.perm + (int64_t)b * s.ncap;
-  for (int i = tid; i < n; i += NW * 64) {
-    const int c = col[i];
+  for (int i = tid; i < n; i += kXThreads) {
+    const int c = colmap[i];
     pm[i] = c;
     if (i < k) out[i] = act[c];
     else red[i - k] = act[c];
@@ -580,7 +685,8 @@ __global__ __launch_bounds__(NT) void k_qrcp(DevState s, int start, double tol_d
   const int b = start + blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int n = s.act_cnt[b], m = s.m_rows[b];
-  if (pools_ok(s) && qrcp_in_registers(m, n)) return;  // k_qrcp_reg's box
+  const int kind = s.qkind[b];
+  if (pools_ok(s) && (kind == kQrcpReg || kind == kQrcpStack)) return;  // k_qrcp_id's box
   const int* act = s.act_list + (int64_t)b * s.ncap;
   int* out = s.out_list + (int64_t)b * s.ncap;
   if (m == 0 || n < kMinCompress || !pools_ok(s)) {
@@ -1443,8 +1549,8 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       k_assemble<R><<<count, 256, 0, st>>>(g, t, s, start, level);
       break;
     case kStageQrcp:
-      k_qrcp_reg<R, kRegRows, kRegCols, kRegWaves>
-          <<<count, kRegWaves * 64, 0, st>>>(s, start, tol);
+      k_rowcomp<R, kXRows><<<min(2 * count, 512), kXThreads, 0, st>>>(s, start, level);
+      k_qrcp_id<R, kXRows><<<count, kXThreads, 0, st>>>(s, start, tol);
       k_qrcp<R, 512><<<count, 512, 0, st>>>(s, start, tol);
       break;
     case kStageDecouple:
