@@ -178,7 +178,7 @@ struct iegpu_ctx {
   bool factored = false;
   DevBuf<int> s_act_cnt, s_act_list, s_chpos, s_out_cnt, s_out_list,
       s_red_list, s_perm, s_rank, s_upd_kind, s_m_rows, s_row_pool, s_err,
-      s_root_piv, s_qkind, s_jobs, s_njobs;
+      s_root_piv, s_qkind, s_jobs, s_njobs, s_lists, s_lcnt;
   DevBuf<int64_t> s_id_sz, s_row_sz, s_fac_sz, s_id_off, s_row_off, s_fac_off,
       s_fac_top;
   DevBuf<unsigned char> s_id_pool, s_fac_pool, s_root_mat;
@@ -237,6 +237,9 @@ struct iegpu_ctx {
     s.upd_kind = s_upd_kind.p;
     s.m_rows = s_m_rows.p;
     s.qkind = s_qkind.p;
+    s.nbox = tree.n_boxes;
+    s.lists = s_lists.p;
+    s.lcnt = s_lcnt.p;
     s.jobs = s_jobs.p;
     s.njobs = s_njobs.p;
     s.id_sz = s_id_sz.p;
@@ -346,6 +349,8 @@ void alloc_state(iegpu_ctx* c) {
   c->s_upd_kind.alloc(nb);
   c->s_m_rows.alloc(nb);
   c->s_qkind.alloc(nb);
+  c->s_lists.alloc((size_t)kNumLists * nb);
+  c->s_lcnt.alloc((size_t)2 * kMaxLevels * kNumLists);
   c->s_jobs.alloc((size_t)2 * kMaxChunks * nb);
   c->s_njobs.alloc(kMaxLevels);
   c->s_id_sz.alloc(nb);
@@ -357,7 +362,7 @@ void alloc_state(iegpu_ctx* c) {
   c->s_fac_top.alloc(1);
   c->s_err.alloc(4);
   c->s_root_piv.alloc(c->ncap + 64);
-  c->s_root_mat.alloc((size_t)c->ncap * c->ncap * 8);
+  c->s_root_mat.alloc((size_t)2 * c->ncap * c->ncap * 8);  // matrix + inverse
   c->s_id_pool.alloc((size_t)c->id_cap * c->elem());
   c->s_row_pool.alloc((size_t)c->row_cap);
   c->s_fac_pool.alloc((size_t)c->fac_cap * c->elem());
@@ -638,7 +643,10 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
     }
     c->precision = precision;
     if (c->ncap == 0) c->ncap = 256;
-    if (c->id_cap == 0) c->id_cap = std::max<int64_t>(nd * 640, 1 << 22);
+    // Register-QRCP boxes need no ID scratch; the pool holds the few
+    // row-compressed or general-QRCP boxes of one level (grown to the
+    // measured need if a level overflows it).
+    if (c->id_cap == 0) c->id_cap = std::max<int64_t>(nd * 16, 1 << 22);
     if (c->row_cap == 0) c->row_cap = std::max<int64_t>(nd * 8, 1 << 16);
     if (c->fac_cap == 0) c->fac_cap = std::max<int64_t>(nd * 160, 1 << 22);
     hipEvent_t e0, e1;
@@ -687,8 +695,10 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
         }
         c->ncap = std::min(kMaxBoxDofs, c->ncap * 2);
       }
-      if (errbits & kErrIdPool) c->id_cap *= 2;
-      if (errbits & kErrRowPool) c->row_cap *= 2;
+      if (errbits & kErrIdPool)
+        c->id_cap = std::max<int64_t>(2 * c->id_cap, ((int64_t)h_err[2] << 10) * 5 / 4);
+      if (errbits & kErrRowPool)
+        c->row_cap = std::max<int64_t>(2 * c->row_cap, ((int64_t)h_err[3] << 10) * 5 / 4);
       if (errbits & kErrFacPool) c->fac_cap *= 2;
       if (attempt == 7) {
         (void)hipEventDestroy(e0);

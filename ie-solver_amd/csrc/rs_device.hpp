@@ -36,6 +36,16 @@ constexpr int kRegMaxRows = 512;
 constexpr int kRegMaxCols = 64;
 constexpr int kMaxChunks = 8;
 constexpr int kMaxLevels = 64;  // tree depth bound (per-level job counters)
+constexpr int kSmallBox = 64;     // decouple with every operand in LDS
+constexpr int kRootRegMax = 192;  // root blocks inverted in registers
+
+// Per-level lists of the boxes each QRCP kernel takes, built by prefix
+// sums (k_scan) so that a kernel's workgroups map onto exactly its boxes.
+enum : int {
+  kLstReg = 0,   // register QRCP (kQrcpReg, kQrcpStack)
+  kLstLds = 1,   // general QRCP (kQrcpLds)
+  kNumLists = 2,
+};
 
 // Row chunks of a tall ID matrix: ceil(m / 512), as even as possible.
 __host__ __device__ inline int id_chunks(int m) {
@@ -84,6 +94,9 @@ struct DevState {
   int* upd_kind;       // 0: no carried update, 1: carried update stored
   int* m_rows;         // ID matrix height
   int* qkind;          // ID route of the box (QrcpKind)
+  int nbox;            // box slots
+  int* lists;          // kNumLists x nbox: a level's list L starts at its first slot
+  int* lcnt;           // kMaxLevels x kNumLists x 2: list lengths (second word spare)
   int* jobs;           // (box, chunk) row-compression jobs, kMaxChunks per box slot
   int* njobs;          // per level: number of row-compression jobs
   int64_t* id_sz;      // scratch doubles/floats needed (per box, level-local)
@@ -99,13 +112,24 @@ struct DevState {
   int64_t row_cap;
   int64_t fac_cap;
   int64_t* fac_top;    // running factor-pool offset across levels
-  int* err;            // [0] error bits, [1] max |active| seen
+  int* err;            // [0] error bits, [1] max |active| seen, [2] / [3] ID / row
+                       // pool size a level needed on overflow (1024 elements)
   void* root_mat;      // ncap*ncap
   int* root_piv;       // ncap
 };
 
+__device__ inline int* list_slot(const DevState& s, int level, int L) {
+  return s.lcnt + 2 * (level * kNumLists + L);
+}
+__device__ inline int list_count(const DevState& s, int level, int L) {
+  return list_slot(s, level, L)[0];
+}
+__device__ inline int list_at(const DevState& s, int start, int L, int idx) {
+  return s.lists[(int64_t)L * s.nbox + start + idx];
+}
+
 // Offsets of the factor blocks inside one compressed box's factor slot.
-// Layout: T (k x r) | L (k x r) | U (r x k) | X_rr LU (r x r) | piv (r) |
+// Layout: T (k x r) | L (k x r) | U (r x k) | X_rr^{-1} (r x r) | r spare |
 //         carried Schur block (k x k).  Column-major throughout.
 struct FacLayout {
   int64_t T, L, U, X, piv, S;
@@ -338,44 +362,62 @@ __device__ inline double wave_max(double v) {
   return lane_value(v, 63);
 }
 
-// Laplace double layer, ie_solver/kernel/kernel.cpp:123-165.
-__device__ inline double laplace_entry(const DevGeom& g, int t, int s) {
+// Geometry of one DOF as a kernel target or source: its point, normal,
+// quadrature weight, curvature and (Stokes) velocity component.
+struct DofGeo {
+  double x, y, n1, n2, w, k;
+  int c;
+};
+
+__device__ inline DofGeo dof_geo(const DevGeom& g, int dof) {
+  const bool stokes = g.pde == kPdeStokes;
+  const int p = stokes ? dof >> 1 : dof;
+  DofGeo d;
+  d.x = g.pts[2 * p];
+  d.y = g.pts[2 * p + 1];
+  d.n1 = g.nrm[2 * p];
+  d.n2 = g.nrm[2 * p + 1];
+  d.w = g.wts[p];
+  d.k = g.curv[p];
+  d.c = stokes ? dof & 1 : 0;
+  return d;
+}
+
+// Laplace double layer, ie_solver/kernel/kernel.cpp:123-165.  The
+// coincident-point (diagonal) value applies only between boundary DOFs.
+__device__ inline double laplace_entry(const DofGeo& t, const DofGeo& s, bool diag_ok) {
   const double scale = 1.0 / (2 * M_PI);
-  const double sp1 = g.pts[2 * s], sp2 = g.pts[2 * s + 1];
-  const double tp1 = g.pts[2 * t], tp2 = g.pts[2 * t + 1];
-  const double sw = g.wts[s];
-  if (tp1 == sp1 && tp2 == sp2) return 0.5 + 0.5 * g.curv[s] * sw * scale;
-  const double r0 = tp1 - sp1, r1 = tp2 - sp2;
-  return -sw * scale * (r0 * g.nrm[2 * s] + r1 * g.nrm[2 * s + 1]) /
-         (r0 * r0 + r1 * r1);
+  if (diag_ok && t.x == s.x && t.y == s.y) return 0.5 + 0.5 * s.k * s.w * scale;
+  const double r0 = t.x - s.x, r1 = t.y - s.y;
+  return -s.w * scale * (r0 * s.n1 + r1 * s.n2) / (r0 * r0 + r1 * r1);
 }
 
 // Stokes velocity double layer (2 DOFs per point, DOF d -> point d/2,
 // component d%2), ie_solver/kernel/kernel.cpp:212-290.
-__device__ inline double stokes_entry(const DevGeom& g, int t, int s) {
+__device__ inline double stokes_entry(const DofGeo& t, const DofGeo& s, bool diag_ok) {
   const double scale = 1.0 / M_PI;
-  const int tp = t >> 1, sp = s >> 1, a = t & 1, b = s & 1;
-  const double sp1 = g.pts[2 * sp], sp2 = g.pts[2 * sp + 1];
-  const double tp1 = g.pts[2 * tp], tp2 = g.pts[2 * tp + 1];
-  const double sn1 = g.nrm[2 * sp], sn2 = g.nrm[2 * sp + 1];
-  const double tna = g.nrm[2 * tp + a];
-  const double snb = b ? sn2 : sn1;
-  const double sw = g.wts[sp];
-  if (tp1 == sp1 && tp2 == sp2) {
-    const double pot = -0.5 * g.curv[sp] * sw * scale;
-    if (a == 0 && b == 0) return -0.5 + pot * sn2 * sn2 + sw * tna * sn1;
-    if (a == 1 && b == 1) return -0.5 + pot * sn1 * sn1 + sw * tna * sn2;
-    return -pot * sn1 * sn2 + sw * g.nrm[2 * tp + 1] * sn1;
+  const int a = t.c, b = s.c;
+  const double tna = a ? t.n2 : t.n1;
+  const double snb = b ? s.n2 : s.n1;
+  if (diag_ok && t.x == s.x && t.y == s.y) {
+    const double pot = -0.5 * s.k * s.w * scale;
+    if (a == 0 && b == 0) return -0.5 + pot * s.n2 * s.n2 + s.w * tna * s.n1;
+    if (a == 1 && b == 1) return -0.5 + pot * s.n1 * s.n1 + s.w * tna * s.n2;
+    return -pot * s.n1 * s.n2 + s.w * t.n2 * s.n1;
   }
-  const double r0 = tp1 - sp1, r1 = tp2 - sp2;
+  const double r0 = t.x - s.x, r1 = t.y - s.y;
   const double d2 = r0 * r0 + r1 * r1;
-  const double pot = sw * scale * (r0 * sn1 + r1 * sn2) / (d2 * d2);
+  const double pot = s.w * scale * (r0 * s.n1 + r1 * s.n2) / (d2 * d2);
   const double ra = a ? r1 : r0, rb = b ? r1 : r0;
-  return pot * ra * rb + sw * tna * snb;
+  return pot * ra * rb + s.w * tna * snb;
 }
 
+__device__ inline double kernel_entry(int pde, const DofGeo& t, const DofGeo& s,
+                                      bool diag_ok = true) {
+  return pde == kPdeStokes ? stokes_entry(t, s, diag_ok) : laplace_entry(t, s, diag_ok);
+}
 __device__ inline double kernel_entry(const DevGeom& g, int t, int s) {
-  return g.pde == kPdeStokes ? stokes_entry(g, t, s) : laplace_entry(g, t, s);
+  return kernel_entry(g.pde, dof_geo(g, t), dof_geo(g, s));
 }
 
 // Proxy point p of the circle radius r about (cx, cy):
@@ -392,53 +434,49 @@ __device__ inline void proxy_point(double cx, double cy, double r, int p,
   *n2 = sn;
 }
 
-// Entry of the proxy block (rows 0..2P-1) at box DOF `d`
-// (ie_solver/kernel/kernel.cpp:459-591).  P = 128 (Laplace) or 256 (Stokes).
-__device__ inline double proxy_entry(const DevGeom& g, double cx, double cy,
-                                     double r, int row, int d) {
-  if (g.pde != kPdeStokes) {
-    const double scale = 1.0 / (2 * M_PI);
-    const bool to_proxy = row < kNumProxy;
-    const int p = to_proxy ? row : row - kNumProxy;
-    double px, py, n1, n2;
-    proxy_point(cx, cy, r, p, &px, &py, &n1, &n2);
-    const double bx = g.pts[2 * d], by = g.pts[2 * d + 1];
-    if (to_proxy) {
-      const double r0 = px - bx, r1 = py - by;
-      return -g.wts[d] * scale * (r0 * g.nrm[2 * d] + r1 * g.nrm[2 * d + 1]) /
-             (r0 * r0 + r1 * r1);
-    }
-    const double pw = 2.0 * M_PI * r / kNumProxy;
-    const double r0 = bx - px, r1 = by - py;
-    return -pw * scale * (r0 * n1 + r1 * n2) / (r0 * r0 + r1 * r1);
+// One row of a box's ID matrix (ie_solver/kernel/kernel.cpp:293-423 and
+// 459-591): rows [0, cnt) are K(row DOF, box DOF), [cnt, 2 cnt)
+// K(box DOF, row DOF), then the proxy block: for Laplace P = 128 rows
+// box -> proxy point and P rows proxy point -> box; for Stokes the two
+// components of each proxy point interleave (2P rows each way).  A proxy
+// point acts as a DOF with the circle's outward normal and weight
+// 2 pi r / 128.  `q` is the row side, `target` whether it is the target.
+struct IdRow {
+  DofGeo q;
+  bool target;
+  bool diag_ok;
+};
+
+__device__ inline IdRow id_row(const DevGeom& g, const int* rows, int cnt,
+                               const double (*pxy)[kNumProxy], double pw, int i) {
+  IdRow r;
+  if (i < 2 * cnt) {
+    const bool first = i < cnt;
+    r.q = dof_geo(g, rows[first ? i : i - cnt]);
+    r.target = first;
+    r.diag_ok = true;
+    return r;
   }
-  const double scale = 1.0 / M_PI;
-  const int P = 2 * kNumProxy;
-  const bool to_proxy = row < P;
-  const int rr = to_proxy ? row : row - P;
-  const int p = rr >> 1, c = rr & 1;
-  double px, py, n1, n2;
-  proxy_point(cx, cy, r, p, &px, &py, &n1, &n2);
-  const int bp = d >> 1, bc = d & 1;
-  const double bx = g.pts[2 * bp], by = g.pts[2 * bp + 1];
-  const double bn1 = g.nrm[2 * bp], bn2 = g.nrm[2 * bp + 1];
-  if (to_proxy) {  // box DOF (source comp bc) -> proxy (target comp c)
-    const double sw = g.wts[bp];
-    const double r0 = px - bx, r1 = py - by;
-    const double d2 = r0 * r0 + r1 * r1;
-    const double pot = sw * scale * (r0 * bn1 + r1 * bn2) / (d2 * d2);
-    const double ra = c ? r1 : r0, rb = bc ? r1 : r0;
-    const double tn = c ? n2 : n1, sn = bc ? bn2 : bn1;
-    return pot * ra * rb + sw * tn * sn;
-  }
-  // proxy (source comp c) -> box DOF (target comp bc)
-  const double pw = 2.0 * M_PI * r / kNumProxy;
-  const double r0 = bx - px, r1 = by - py;
-  const double d2 = r0 * r0 + r1 * r1;
-  const double pot = pw * scale * (r0 * n1 + r1 * n2) / (d2 * d2);
-  const double ra = bc ? r1 : r0, rb = c ? r1 : r0;
-  const double tn = bc ? bn2 : bn1, sn = c ? n2 : n1;
-  return pot * ra * rb + pw * tn * sn;
+  const bool stokes = g.pde == kPdeStokes;
+  const int P = stokes ? 2 * kNumProxy : kNumProxy;
+  const int row = i - 2 * cnt;
+  r.target = row < P;
+  const int rr = r.target ? row : row - P;
+  const int p = stokes ? rr >> 1 : rr;
+  r.q.x = pxy[0][p];
+  r.q.y = pxy[1][p];
+  r.q.n1 = pxy[2][p];
+  r.q.n2 = pxy[3][p];
+  r.q.w = pw;
+  r.q.k = 0;
+  r.q.c = stokes ? rr & 1 : 0;
+  r.diag_ok = false;
+  return r;
+}
+
+__device__ inline double id_entry(int pde, const IdRow& r, const DofGeo& col) {
+  return r.target ? kernel_entry(pde, r.q, col, r.diag_ok)
+                  : kernel_entry(pde, col, r.q, r.diag_ok);
 }
 
 // Inner-circle test of ie_solver/kernel/kernel.cpp:384-394, evaluated

@@ -52,9 +52,12 @@ __global__ void k_reset(DevState s) {
   if (threadIdx.x == 0) {
     s.err[0] = 0;
     s.err[1] = 0;
+    s.err[2] = 0;
+    s.err[3] = 0;
     *s.fac_top = 0;
   }
   if (threadIdx.x < kMaxLevels) s.njobs[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < 2 * kMaxLevels * kNumLists; i += blockDim.x) s.lcnt[i] = 0;
 }
 
 // active(box) = concatenation of the children's outgoing lists (skeleton if
@@ -78,7 +81,7 @@ __global__ __launch_bounds__(64) void k_active(DevTree t, DevState s,
     int pos = 0;
     for (int q = 0; q < 4; q++) {
       const int c = t.child[4 * b + q];
-      const int cnt = s.out_cnt[c];
+      const int cnt = min(max(s.out_cnt[c], 0), s.ncap);
       if (lane == 0) s.chpos[4 * b + q] = pos;
       if (pos + cnt <= s.ncap) {
         const int* src = s.out_list + (int64_t)c * s.ncap;
@@ -153,23 +156,25 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
   if (threadIdx.x == 0) {
     s.m_rows[b] = m;
     s.qkind[b] = kind;
+    // Scratch: the assembled matrix (and the general decouple's operands)
+    // for the general QRCP, the stacked triangles for row compression;
+    // register-QRCP boxes evaluate their entries in place.
     const int64_t ld = m > 2 * n ? m : 2 * n;
-    int64_t stack = 0;
-    if (kind == kQrcpStack) {
-      // Row-compression jobs, one per chunk, and room for the stacked
-      // triangles after the ID matrix.
-      const int nch = id_chunks(m);
-      stack = (int64_t)nch * n * n;
-      const int base = atomicAdd(s.njobs + level, nch);
-      int* J = s.jobs + (int64_t)2 * kMaxChunks * start + 2 * base;
-      for (int q = 0; q < nch; q++) {
-        J[2 * q] = b;
-        J[2 * q + 1] = q;
-      }
-    }
-    s.id_sz[b] = (m > 0) ? ld * n + stack : 0;
-    s.row_sz[b] = (m > 0) ? (m - proxy_rows(g, level)) / 2 : 0;
+    int64_t scratch = kind == kQrcpLds ? ld * n : 0;
+    if (kind == kQrcpStack) scratch = (int64_t)id_chunks(m) * n * n;
+    s.id_sz[b] = scratch;
+    s.row_sz[b] = kind == kQrcpLds ? (m - proxy_rows(g, level)) / 2 : 0;
     s.fac_sz[b] = (n > 0) ? fac_slot_size(n) : 0;
+  }
+  // Until its ID succeeds a box passes its active DOFs on uncompressed:
+  // boxes a failed (retried) attempt never reaches still hold a consistent
+  // outgoing list for the next level's k_active.
+  const int* act = s.act_list + (int64_t)b * s.ncap;
+  int* out = s.out_list + (int64_t)b * s.ncap;
+  for (int i = threadIdx.x; i < n; i += 64) out[i] = act[i];
+  if (threadIdx.x == 0) {
+    s.rank[b] = 0;
+    s.out_cnt[b] = n;
   }
 }
 
@@ -177,50 +182,79 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
 // ID scratch and row lists are level-local; factor slots append to the
 // running pool offset.
 __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
-                                               int count) {
-  __shared__ int64_t sh[3][1024];
+                                               int count, int level) {
+  // Scanned per box: ID scratch, row-list and factor sizes, register-QRCP
+  // and general-QRCP list entries, row-compression jobs.
+  constexpr int NQ = 6;
+  __shared__ int64_t sh[NQ][1024];
   const int tid = threadIdx.x;
   const int chunk = (count + 1023) / 1024;
   const int lo = min(count, tid * chunk), hi = min(count, lo + chunk);
-  int64_t a = 0, b = 0, c = 0;
+  int64_t own[NQ] = {0, 0, 0, 0, 0, 0};
   for (int i = lo; i < hi; i++) {
-    a += s.id_sz[start + i];
-    b += s.row_sz[start + i];
-    c += s.fac_sz[start + i];
+    const int b = start + i;
+    const int kind = s.qkind[b];
+    own[0] += s.id_sz[b];
+    own[1] += s.row_sz[b];
+    own[2] += s.fac_sz[b];
+    own[3] += (kind == kQrcpReg || kind == kQrcpStack) ? 1 : 0;
+    own[4] += (kind == kQrcpLds) ? 1 : 0;
+    own[5] += (kind == kQrcpStack) ? id_chunks(s.m_rows[b]) : 0;
   }
-  sh[0][tid] = a;
-  sh[1][tid] = b;
-  sh[2][tid] = c;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) sh[q][tid] = own[q];
   __syncthreads();
   for (int o = 1; o < 1024; o <<= 1) {
-    int64_t x0 = 0, x1 = 0, x2 = 0;
-    if (tid >= o) {
-      x0 = sh[0][tid - o];
-      x1 = sh[1][tid - o];
-      x2 = sh[2][tid - o];
-    }
+    int64_t x[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) x[q] = (tid >= o) ? sh[q][tid - o] : 0;
     __syncthreads();
-    sh[0][tid] += x0;
-    sh[1][tid] += x1;
-    sh[2][tid] += x2;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) sh[q][tid] += x[q];
     __syncthreads();
   }
   const int64_t fbase = *s.fac_top;
-  int64_t oa = sh[0][tid] - a, ob = sh[1][tid] - b, oc = sh[2][tid] - c;
+  int64_t off[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) off[q] = sh[q][tid] - own[q];
+  int* lreg = s.lists + (int64_t)kLstReg * s.nbox + start;
+  int* llds = s.lists + (int64_t)kLstLds * s.nbox + start;
+  int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
   for (int i = lo; i < hi; i++) {
-    s.id_off[start + i] = oa;
-    s.row_off[start + i] = ob;
-    s.fac_off[start + i] = fbase + oc;
-    oa += s.id_sz[start + i];
-    ob += s.row_sz[start + i];
-    oc += s.fac_sz[start + i];
+    const int b = start + i;
+    const int kind = s.qkind[b];
+    s.id_off[b] = off[0];
+    s.row_off[b] = off[1];
+    s.fac_off[b] = fbase + off[2];
+    off[0] += s.id_sz[b];
+    off[1] += s.row_sz[b];
+    off[2] += s.fac_sz[b];
+    if (kind == kQrcpReg || kind == kQrcpStack) lreg[off[3]++] = b;
+    if (kind == kQrcpLds) llds[off[4]++] = b;
+    if (kind == kQrcpStack) {
+      const int nch = id_chunks(s.m_rows[b]);
+      for (int q = 0; q < nch; q++) {
+        jobs[2 * off[5]] = b;
+        jobs[2 * off[5] + 1] = q;
+        off[5]++;
+      }
+    }
   }
   __syncthreads();
   if (tid == 0) {
     const int64_t ta = sh[0][1023], tb = sh[1][1023], tc = sh[2][1023];
+    list_slot(s, level, kLstReg)[0] = (int)sh[3][1023];
+    list_slot(s, level, kLstLds)[0] = (int)sh[4][1023];
+    s.njobs[level] = (int)sh[5][1023];
     int e = 0;
-    if (ta > s.id_cap) e |= kErrIdPool;
-    if (tb > s.row_cap) e |= kErrRowPool;
+    if (ta > s.id_cap) {
+      e |= kErrIdPool;
+      atomicMax(s.err + 2, (int)((ta + 1023) >> 10));  // size needed, in 1024 elements
+    }
+    if (tb > s.row_cap) {
+      e |= kErrRowPool;
+      atomicMax(s.err + 3, (int)((tb + 1023) >> 10));
+    }
     if (fbase + tc > s.fac_cap) e |= kErrFacPool;
     if (e) atomicOr(s.err, e);
     *s.fac_top = fbase + tc;
@@ -233,89 +267,58 @@ __device__ inline bool pools_ok(const DevState& s) {
 
 // ---------------------------------------------------------------------------
 // ID matrix assembly: [K(ic, box); K(box, ic)^T; proxy] (m x n,
-// column-major, ld m), kernel.cpp:293-423.  Consecutive threads write
-// consecutive rows of a column.
+// column-major, ld m), kernel.cpp:293-423.  The register QR kernels
+// evaluate their boxes' entries in place; this kernel writes the matrix of
+// the boxes the general QRCP takes.
 // ---------------------------------------------------------------------------
 
-template <typename R>
-__global__ __launch_bounds__(256) void k_assemble(DevGeom g, DevTree t,
-                                                  DevState s, int start,
-                                                  int level) {
-  const int b = start + blockIdx.x;
-  const int m = s.m_rows[b];
-  if (m == 0 || !pools_ok(s)) return;
-  const int n = s.act_cnt[b];
+// Row DOFs and proxy points of box b into LDS (all threads; the caller
+// synchronises).  Returns the number of row DOFs (cnt).
+__device__ inline int id_setup(const DevGeom& g, const DevTree& t, const DevState& s,
+                               int b, int level, int m, int* rows,
+                               double (*pxy)[kNumProxy]) {
   const int tid = threadIdx.x;
-  const int* act = s.act_list + (int64_t)b * s.ncap;
-  int* rows = s.row_pool + s.row_off[b];
-  R* A = reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
-  const int P = proxy_rows(g, level);
-  const int cnt = (m - P) / 2;
   if (tid < 64)
-    for_each_row_dof(g, t, s, b, level >= 3,
-                     [&](int pos, int dof) { rows[pos] = dof; });
-  __shared__ double pxy[4][kNumProxy];
-  const double cx = t.cx[b], cy = t.cy[b];
-  const double rad = t.side[b] * kRadiusRatio;
-  if (P > 0 && tid < kNumProxy) {
+    for_each_row_dof(g, t, s, b, level >= 3, [&](int pos, int dof) { rows[pos] = dof; });
+  if (level >= 3 && tid < kNumProxy) {
     double px, py, n1, n2;
-    proxy_point(cx, cy, rad, tid, &px, &py, &n1, &n2);
+    proxy_point(t.cx[b], t.cy[b], t.side[b] * kRadiusRatio, tid, &px, &py, &n1, &n2);
     pxy[0][tid] = px;
     pxy[1][tid] = py;
     pxy[2][tid] = n1;
     pxy[3][tid] = n2;
   }
-  __syncthreads();
-  const double pw = 2.0 * M_PI * rad / kNumProxy;
-  for (int j = 0; j < n; j++) {
-    const int sd = act[j];
-    R* col = A + (int64_t)m * j;
+  return (m - proxy_rows(g, level)) / 2;
+}
+
+__device__ inline double proxy_weight(const DevTree& t, int b) {
+  return 2.0 * M_PI * (t.side[b] * kRadiusRatio) / kNumProxy;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_assemble(DevGeom g, DevTree t,
+                                                  DevState s, int start,
+                                                  int level) {
+  __shared__ double pxy[4][kNumProxy];
+  if (!pools_ok(s)) return;
+  const int nl = list_count(s, level, kLstLds);
+  for (int li = blockIdx.x; li < nl; li += gridDim.x) {
+    const int b = list_at(s, start, kLstLds, li);
+    const int m = s.m_rows[b];
+    const int n = s.act_cnt[b];
+    const int tid = threadIdx.x;
+    const int* act = s.act_list + (int64_t)b * s.ncap;
+    int* rows = s.row_pool + s.row_off[b];
+    R* A = reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
+    const int cnt = id_setup(g, t, s, b, level, m, rows, pxy);
+    __syncthreads();
+    const double pw = proxy_weight(t, b);
     for (int i = tid; i < m; i += 256) {
-      double v;
-      if (i < cnt) {
-        v = kernel_entry(g, rows[i], sd);
-      } else if (i < 2 * cnt) {
-        v = kernel_entry(g, sd, rows[i - cnt]);
-      } else {
-        const int row = i - 2 * cnt;
-        if (g.pde != kPdeStokes) {
-          const double scale = 1.0 / (2 * M_PI);
-          const double bx = g.pts[2 * sd], by = g.pts[2 * sd + 1];
-          if (row < kNumProxy) {
-            const double r0 = pxy[0][row] - bx, r1 = pxy[1][row] - by;
-            v = -g.wts[sd] * scale * (r0 * g.nrm[2 * sd] + r1 * g.nrm[2 * sd + 1]) /
-                (r0 * r0 + r1 * r1);
-          } else {
-            const int p = row - kNumProxy;
-            const double r0 = bx - pxy[0][p], r1 = by - pxy[1][p];
-            v = -pw * scale * (r0 * pxy[2][p] + r1 * pxy[3][p]) / (r0 * r0 + r1 * r1);
-          }
-        } else {
-          const double scale = 1.0 / M_PI;
-          const int PP = 2 * kNumProxy;
-          const bool to_proxy = row < PP;
-          const int rr = to_proxy ? row : row - PP;
-          const int p = rr >> 1, c = rr & 1;
-          const double px = pxy[0][p], py = pxy[1][p], n1 = pxy[2][p], n2 = pxy[3][p];
-          const int bp = sd >> 1, bc = sd & 1;
-          const double bx = g.pts[2 * bp], by = g.pts[2 * bp + 1];
-          const double bn1 = g.nrm[2 * bp], bn2 = g.nrm[2 * bp + 1];
-          if (to_proxy) {
-            const double sw = g.wts[bp];
-            const double r0 = px - bx, r1 = py - by;
-            const double d2 = r0 * r0 + r1 * r1;
-            const double pot = sw * scale * (r0 * bn1 + r1 * bn2) / (d2 * d2);
-            v = pot * (c ? r1 : r0) * (bc ? r1 : r0) + sw * (c ? n2 : n1) * (bc ? bn2 : bn1);
-          } else {
-            const double r0 = bx - px, r1 = by - py;
-            const double d2 = r0 * r0 + r1 * r1;
-            const double pot = pw * scale * (r0 * n1 + r1 * n2) / (d2 * d2);
-            v = pot * (bc ? r1 : r0) * (c ? r1 : r0) + pw * (bc ? bn2 : bn1) * (c ? n2 : n1);
-          }
-        }
-      }
-      col[i] = static_cast<R>(v);
+      const IdRow rw = id_row(g, rows, cnt, pxy, pw, i);
+      for (int j = 0; j < n; j++)
+        A[i + (int64_t)m * j] = static_cast<R>(id_entry(g.pde, rw, dof_geo(g, act[j])));
     }
+    __syncthreads();
   }
 }
 
@@ -419,6 +422,31 @@ __device__ inline void load_columns(const R* A, int64_t ld, int row0, int rows, 
   }
 }
 
+// Entries of rows [row0, row0 + rows) of a box's ID matrix, evaluated into
+// the interleave (range row i on lane i % 64, register i / 64).
+template <typename R, int RPL>
+__device__ inline void eval_columns(const DevGeom& g, const int* rowdofs, int cnt,
+                                    const double (*pxy)[kNumProxy], double pw,
+                                    const DofGeo* colg, int row0, int rows, int n,
+                                    R (&a)[4][RPL]) {
+  const int lane = threadIdx.x & 63, c0 = 4 * (threadIdx.x >> 6);
+  const int hq = quad_class(lane);
+#pragma unroll
+  for (int r = 0; r < RPL; r++) {
+    const int i = lane + 64 * r;
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) a[sl][r] = R(0);
+    if (i < rows) {
+      const IdRow rw = id_row(g, rowdofs, cnt, pxy, pw, row0 + i);
+#pragma unroll
+      for (int sl = 0; sl < 4; sl++) {
+        const int c = c0 + (sl ^ hq);
+        if (c < n) a[sl][r] = static_cast<R>(id_entry(g.pde, rw, colg[c]));
+      }
+    }
+  }
+}
+
 // Row compression of a tall ID matrix: chunk q of box b (rows
 // [q ch, q ch + rows), ch = ceil(m / chunks)) is reduced by unpivoted
 // Householder QR to its n x n triangle R_q, written to rows [q n, q n + n)
@@ -427,10 +455,14 @@ __device__ inline void load_columns(const R* A, int64_t ld, int row0, int rows, 
 // the same R and column order as that of A.  Grid-strided over the
 // level's jobs; one barrier per step (double-buffered reflector).
 template <typename R, int RPL>
-__global__ __launch_bounds__(kXThreads) void k_rowcomp(DevState s, int start, int level) {
+__global__ __launch_bounds__(kXThreads) void k_rowcomp(DevGeom g, DevTree t, DevState s,
+                                                       int start, int level) {
   constexpr int MAXM = 64 * RPL;
   __shared__ R vbuf[2][MAXM];
   __shared__ R sh_tau[2];
+  __shared__ int rowdofs[kMaxChunks * kRegMaxRows / 2];
+  __shared__ double pxy[4][kNumProxy];
+  __shared__ DofGeo colg[kRegMaxCols];
   if (!pools_ok(s)) return;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int hq = quad_class(lane);
@@ -443,12 +475,15 @@ __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevState s, int start, in
     const int nch = id_chunks(m);
     const int ch = (m + nch - 1) / nch;
     const int r0 = q * ch, rows = min(ch, m - r0);
-    R* A = reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
-    R* S = A + (int64_t)max(m, 2 * n) * n;
+    R* S = reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
     const int64_t lds = (int64_t)nch * n;
     const bool active = c0 < n;
+    const int* act = s.act_list + (int64_t)b * s.ncap;
+    const int cnt = id_setup(g, t, s, b, level, m, rowdofs, pxy);
+    for (int c = tid; c < n; c += kXThreads) colg[c] = dof_geo(g, act[c]);
+    __syncthreads();
     R a[4][RPL];
-    load_columns(A, m, r0, rows, n, a);
+    eval_columns(g, rowdofs, cnt, pxy, proxy_weight(t, b), colg, r0, rows, n, a);
     const int steps = min(rows, n);
     int dmask = 0;
     for (int j = 0; j < steps; j++) {
@@ -488,9 +523,13 @@ __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevState s, int start, in
 // pivot key of the wave's four columns; the pivot is the largest key
 // (pivot_key), found by one wave maximum.
 template <typename R, int RPL>
-__global__ __launch_bounds__(kXThreads) void k_qrcp_id(DevState s, int start, double tol_d) {
+__device__ void qrcp_id_box(const DevGeom& g, const DevTree& t, const DevState& s,
+                            int start, int level, int b, double tol_d) {
   constexpr int MAXM = 64 * RPL;
   __shared__ R vbuf[MAXM];
+  __shared__ int rowdofs[kRegMaxRows / 2];
+  __shared__ double pxy[4][kNumProxy];
+  __shared__ DofGeo colg[kRegMaxCols];
   __shared__ R rowj[kRegMaxCols];
   __shared__ double keys[kRegMaxCols];
   __shared__ R rbuf[kRegMaxCols * kRegMaxCols];
@@ -498,24 +537,27 @@ __global__ __launch_bounds__(kXThreads) void k_qrcp_id(DevState s, int start, do
   __shared__ R sh_tau, sh_thresh;
   __shared__ int sh_stop;
 
-  const int b = start + blockIdx.x;
   const int kind = s.qkind[b];
-  if ((kind != kQrcpReg && kind != kQrcpStack) || !pools_ok(s)) return;
   const int n = s.act_cnt[b];
-  const R* A = reinterpret_cast<const R*>(s.id_pool) + s.id_off[b];
   int m = s.m_rows[b];
-  if (kind == kQrcpStack) {
-    A += (int64_t)max(m, 2 * n) * n;
-    m = id_chunks(m) * n;
-  }
   const R tol = static_cast<R>(tol_d);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int hq = quad_class(lane);
   const int c0 = 4 * wv;
   const bool active = c0 < n;
+  const int* act = s.act_list + (int64_t)b * s.ncap;
 
   R a[4][RPL];
-  load_columns(A, m, 0, m, n, a);
+  if (kind == kQrcpReg) {
+    const int cnt = id_setup(g, t, s, b, level, m, rowdofs, pxy);
+    for (int c = tid; c < n; c += kXThreads) colg[c] = dof_geo(g, act[c]);
+    __syncthreads();
+    eval_columns(g, rowdofs, cnt, pxy, proxy_weight(t, b), colg, 0, m, n, a);
+  } else {
+    // The stacked triangles of the row compression.
+    m = id_chunks(m) * n;
+    load_columns(reinterpret_cast<const R*>(s.id_pool) + s.id_off[b], m, 0, m, n, a);
+  }
   // Lanes 0..3: column c0 + lane.
   const int myc = c0 + (lane & 3);
   R vn1 = 0, vn2 = 0;
@@ -630,7 +672,6 @@ __global__ __launch_bounds__(kXThreads) void k_qrcp_id(DevState s, int start, do
   }
   __syncthreads();
 
-  const int* act = s.act_list + (int64_t)b * s.ncap;
   int* out = s.out_list + (int64_t)b * s.ncap;
   if (k == 0) {
     for (int i = tid; i < n; i += kXThreads) out[i] = act[i];
@@ -668,11 +709,21 @@ __global__ __launch_bounds__(kXThreads) void k_qrcp_id(DevState s, int start, do
   }
 }
 
+// One workgroup per entry of the level's register-QRCP list (a loop over
+// boxes would keep more values live across the step loop than the 128
+// registers of a 1024-thread workgroup hold).
+template <typename R, int RPL>
+__global__ __launch_bounds__(kXThreads) void k_qrcp_id(DevGeom g, DevTree t, DevState s,
+                                                       int start, int level, double tol_d) {
+  if (!pools_ok(s) || (int)blockIdx.x >= list_count(s, level, kLstReg)) return;
+  qrcp_id_box<R, RPL>(g, t, s, start, level, list_at(s, start, kLstReg, blockIdx.x), tol_d);
+}
+
 // General form (any box the register form does not take): the matrix lives
 // in LDS when it fits, else in the box's scratch slot; each wave owns a
 // strided set of columns.
 template <typename R, int NT>
-__global__ __launch_bounds__(NT) void k_qrcp(DevState s, int start, double tol_d) {
+__device__ void qrcp_general_box(const DevState& s, int start, int level, int b, double tol_d) {
   constexpr int NW = NT / 64;
   constexpr int kLdsElems = 131072 / sizeof(R);
   __shared__ R smat[kLdsElems];
@@ -682,21 +733,10 @@ __global__ __launch_bounds__(NT) void k_qrcp(DevState s, int start, double tol_d
   __shared__ R sh_tau;
   __shared__ int sh_stop;
 
-  const int b = start + blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int n = s.act_cnt[b], m = s.m_rows[b];
-  const int kind = s.qkind[b];
-  if (pools_ok(s) && (kind == kQrcpReg || kind == kQrcpStack)) return;  // k_qrcp_id's box
   const int* act = s.act_list + (int64_t)b * s.ncap;
   int* out = s.out_list + (int64_t)b * s.ncap;
-  if (m == 0 || n < kMinCompress || !pools_ok(s)) {
-    for (int i = tid; i < n; i += NT) out[i] = act[i];
-    if (tid == 0) {
-      s.rank[b] = 0;
-      s.out_cnt[b] = n;
-    }
-    return;
-  }
   const R tol = static_cast<R>(tol_d);
   R* A = reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
   const int64_t mn = (int64_t)m * n;
@@ -851,70 +891,85 @@ __global__ __launch_bounds__(NT) void k_qrcp(DevState s, int start, double tol_d
   }
 }
 
+template <typename R, int NT>
+__global__ __launch_bounds__(NT) void k_qrcp(DevState s, int start, int level, double tol_d) {
+  if (!pools_ok(s)) return;
+  const int nl = list_count(s, level, kLstLds);
+  for (int li = blockIdx.x; li < nl; li += gridDim.x) {
+    qrcp_general_box<R, NT>(s, start, level, list_at(s, start, kLstLds, li), tol_d);
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Block LU with partial pivoting (LAPACK dgetrf pivot rule: first index of
-// the largest |entry|), in place, column-major with leading dimension ld.
+// Gauss-Jordan inversion with partial pivoting.  Step j pivots on the row
+// with the largest |entry| of column j among rows not yet pivoted (first
+// index on ties, LAPACK dgetrf's choice) without moving rows: with pivot
+// row prow[j] at step j the result M satisfies
+//   X^{-1}(j, prow[jj]) = M(prow[j], jj).
+// Steps alternate between two n x n buffers (ld n) starting in `a`, so M
+// lands in `a` for even n and in `b` for odd n.  All NT threads take part;
+// every wave picks the pivot itself, so a step needs one barrier.
+// Returns 0, or j + 1 for the first zero pivot.  n <= 2048.
 // ---------------------------------------------------------------------------
 
 template <typename R, int NT>
-__device__ int block_lu(R* X, int n, int ld, int* ipiv) {
-  __shared__ int sh_p;
+__device__ int gj_invert(R* a, R* b, int n, int* prow) {
   const int tid = threadIdx.x, lane = tid & 63;
+  uint32_t used = 0;  // bit u: row lane + 64 u already pivoted
   int info = 0;
+  R* cur = a;
+  R* nxt = b;
+  const int i0 = tid % n, c0 = tid / n, di = NT % n, dc = NT / n;
   for (int j = 0; j < n; j++) {
-    if (tid < 64) {
-      R best = -1;
-      int bi = n;
-      for (int i = j + lane; i < n; i += 64) {
-        const R v = fabs(X[i + (int64_t)ld * j]);
-        if (v > best) {
-          best = v;
-          bi = i;
-        }
-      }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const R ob = __shfl_xor(best, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
-      if (bi >= n) bi = j;  // NaN column: no interchange
-      if (lane == 0) {
-        sh_p = bi;
-        ipiv[j] = bi;
+    R best = -1;
+    int bi = 0x7fffffff;
+    for (int u = 0; lane + 64 * u < n; u++) {
+      if ((used >> u) & 1) continue;
+      const int i = lane + 64 * u;
+      const R v = fabs(cur[i + (int64_t)n * j]);
+      const R key = (v == v) ? v : R(-0.5);  // NaN rows after all others
+      if (key > best) {
+        best = key;
+        bi = i;
       }
     }
-    __syncthreads();
-    const int p = sh_p;
-    if (p != j) {
-      for (int c = tid; c < n; c += NT) {
-        const R a = X[j + (int64_t)ld * c];
-        X[j + (int64_t)ld * c] = X[p + (int64_t)ld * c];
-        X[p + (int64_t)ld * c] = a;
+    const int p = wave_argmax<R>(best, bi);
+    if ((p & 63) == lane) used |= 1u << (p >> 6);
+    if (tid == 0) prow[j] = p;
+    const R piv = cur[p + (int64_t)n * j];
+    if (piv == R(0) && info == 0) info = j + 1;
+    const R inv = R(1) / piv;
+    int i = i0, c = c0;
+    while (c < n) {
+      const int64_t idx = i + (int64_t)n * c;
+      const R rp = ((c == j) ? R(1) : cur[p + (int64_t)n * c]) * inv;
+      nxt[idx] = (i == p) ? rp : ((c == j) ? R(0) : cur[idx]) - cur[i + (int64_t)n * j] * rp;
+      i += di;
+      c += dc;
+      if (i >= n) {
+        i -= n;
+        c++;
       }
     }
     __syncthreads();
-    const R piv = X[j + (int64_t)ld * j];
-    if (piv != 0) {
-      const R inv = 1 / piv;
-      for (int i = j + 1 + tid; i < n; i += NT) X[i + (int64_t)ld * j] *= inv;
-    } else if (info == 0) {
-      info = j + 1;
-    }
-    __syncthreads();
-    const int len = n - j - 1;
-    const int64_t tot = (int64_t)len * len;
-    for (int64_t idx = tid; idx < tot; idx += NT) {
-      const int i = j + 1 + (int)(idx % len);
-      const int c = j + 1 + (int)(idx / len);
-      X[i + (int64_t)ld * c] -= X[i + (int64_t)ld * j] * X[j + (int64_t)ld * c];
-    }
-    __syncthreads();
+    R* t = cur;
+    cur = nxt;
+    nxt = t;
   }
   return info;
+}
+
+// Inverts X (r x r) in place through the scratch buffer W (>= r^2): the
+// steps start in W when r is odd, so M ends in X.
+template <typename R, int NT>
+__device__ int gj_invert_into(R* X, R* W, int r, int* prow) {
+  if (r & 1) {
+    for (int idx = threadIdx.x; idx < r * r; idx += NT) W[idx] = X[idx];
+    __syncthreads();
+    return gj_invert<R, NT>(W, X, r, prow);
+  }
+  return gj_invert<R, NT>(X, W, r, prow);
 }
 
 // Carried Schur block of child q at (local i, local j); 0 outside the
@@ -943,7 +998,6 @@ __device__ const R* carried_block(const DevState& s, int c) {
   return F + FacLayout::make(k, s.act_cnt[c] - k).S;
 }
 
-constexpr int kSmallBox = 64;  // decouple in LDS with lane-per-row solves
 
 // Children's carried blocks and the position map of a box's active list.
 template <typename R>
@@ -974,30 +1028,110 @@ __device__ void load_child_blocks(const DevTree& t, const DevState& s, int b,
   }
 }
 
-// Decouple of a compressed box with |active| <= 64: every operand in LDS;
-// LU by one wave, triangular solves with one matrix row (or column) per
-// lane and the pivots broadcast by shuffles.  Same operations as
-// k_decouple below.
+// ---------------------------------------------------------------------------
+// Decouple one box (skel_factorization.cpp:77-128): with s/r the skeleton
+// and redundant positions and K_BN = K(act, act) - (children's carried
+// Schur blocks),
+//   K_rs = K(r,s) - T^T K(s,s)
+//   K_sr = K(s,r) - K(s,s) T
+//   X_rr = K(r,r) - T^T K(s,r) - K_rs T
+//   L = K_sr X_rr^{-1},  U = X_rr^{-1} K_rs,  schur = K_sr X_rr^{-1} K_rs
+// and the box's carried block = (children's update on s,s) + schur.  X_rr
+// is inverted by Gauss-Jordan (the reference LU-factors it and solves): the
+// factor slot keeps X_rr^{-1}, so L, U, the carried block and the solve's
+// diagonal step are plain products.  Uncompressed boxes carry their
+// children's blocks unchanged.
+// ---------------------------------------------------------------------------
+
+// P (n x n, skeleton positions first) -> K_rs, K_sr, X_rr.
+template <typename R, int NT>
+__device__ void decouple_blocks(const R* P, const R* T, int n, int k, R* Krs, R* Ksr, R* X) {
+  const int tid = threadIdx.x, r = n - k;
+  for (int idx = tid; idx < r * k; idx += NT) {
+    const int i = idx % r, a = idx / r;
+    R acc = P[(k + i) + (int64_t)n * a];
+    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + (int64_t)n * a];
+    Krs[idx] = acc;
+  }
+  for (int idx = tid; idx < k * r; idx += NT) {
+    const int a = idx % k, j = idx / k;
+    R acc = P[a + (int64_t)n * (k + j)];
+    for (int l = 0; l < k; l++) acc -= P[a + (int64_t)n * l] * T[l + k * j];
+    Ksr[idx] = acc;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < r * r; idx += NT) {
+    const int i = idx % r, j = idx / r;
+    R acc = P[(k + i) + (int64_t)n * (k + j)];
+    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + (int64_t)n * (k + j)];
+    for (int l = 0; l < k; l++) acc -= Krs[i + r * l] * T[l + k * j];
+    X[idx] = acc;
+  }
+  __syncthreads();
+}
+
+// From M / prow of gj_invert on X_rr: U, L, X_rr^{-1} and the carried block
+// into the factor slot.  W has room for r k entries.
+template <typename R, int NT, typename Upd>
+__device__ void decouple_tail(const R* M, const int* prow, const R* Krs, const R* Ksr,
+                              R* W, int r, int k, R* F, const FacLayout& fl,
+                              const Upd& upd, const short* sperm) {
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < r * k; idx += NT) {  // U = X^{-1} K_rs
+    const int j = idx % r, a = idx / r;
+    const R* mrow = M + prow[j];
+    R acc = 0;
+    for (int jj = 0; jj < r; jj++) acc += mrow[(int64_t)r * jj] * Krs[prow[jj] + (int64_t)r * a];
+    W[idx] = acc;
+    F[fl.U + idx] = acc;
+  }
+  for (int idx = tid; idx < k * r; idx += NT) {  // L = K_sr X^{-1}
+    const int a = idx % k, jj = idx / k;
+    const R* mcol = M + (int64_t)r * jj;
+    R acc = 0;
+    for (int j = 0; j < r; j++) acc += Ksr[a + (int64_t)k * j] * mcol[prow[j]];
+    F[fl.L + a + (int64_t)k * prow[jj]] = acc;
+  }
+  for (int idx = tid; idx < r * r; idx += NT) {
+    const int j = idx % r, jj = idx / r;
+    F[fl.X + j + (int64_t)r * prow[jj]] = M[prow[j] + (int64_t)r * jj];
+  }
+  __syncthreads();
+  for (int idx = tid; idx < k * k; idx += NT) {  // carried block
+    const int a = idx % k, c = idx / k;
+    R acc = 0;
+    for (int j = 0; j < r; j++) acc += Ksr[a + (int64_t)k * j] * W[j + (int64_t)r * c];
+    F[fl.S + idx] = upd(sperm[a], sperm[c]) + acc;
+  }
+}
+
+// Compressed boxes with |active| <= 64, every operand in LDS.
 template <typename R>
 __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
                                                         DevState s, int start) {
-  constexpr int NT = 256, NW = 4;
+  constexpr int NT = 256;
   constexpr int kBuf = 8704;  // >= n^2 + r^2 + 3 k r for n <= 64
   __shared__ R buf[kBuf];
   __shared__ signed char pch[kSmallBox];
   __shared__ short ploc[kSmallBox];
   __shared__ short sperm[kSmallBox];
-  __shared__ int spiv[kSmallBox], idxf[kSmallBox], idxb[kSmallBox];
+  __shared__ int prow[kSmallBox];
+  __shared__ DofGeo geo[kSmallBox];
   __shared__ const R* chS[4];
   __shared__ int chN[4];
-  __shared__ int sh_info;
 
   const int b = start + blockIdx.x;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x;
+  if (!pools_ok(s)) return;
   const int n = s.act_cnt[b], k = s.rank[b];
-  if (!pools_ok(s) || n == 0 || n > kSmallBox || k == 0) return;
+  if (n == 0 || n > kSmallBox || k == 0) return;  // k_decouple's box
   load_child_blocks<R>(t, s, b, n, chS, chN, pch, ploc, NT);
-  for (int i = tid; i < n; i += NT) sperm[i] = (short)s.perm[(int64_t)b * s.ncap + i];
+  const int* act = s.act_list + (int64_t)b * s.ncap;
+  for (int i = tid; i < n; i += NT) {
+    const int q = s.perm[(int64_t)b * s.ncap + i];
+    sperm[i] = (short)q;
+    geo[i] = dof_geo(g, act[q]);
+  }
   __syncthreads();
   ChildUpdates<R> upd;
   upd.S = chS;
@@ -1007,144 +1141,27 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
   const int r = n - k;
   const FacLayout fl = FacLayout::make(k, r);
   R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
-  const int* act = s.act_list + (int64_t)b * s.ncap;
   R* P = buf;
   R* X = P + n * n;
   R* Krs = X + r * r;
   R* Ksr = Krs + r * k;
   R* T = Ksr + k * r;
-  R* Lm = P;  // P is dead once X, K_rs and K_sr are formed
   for (int i = tid; i < k * r; i += NT) T[i] = F[fl.T + i];
   for (int idx = tid; idx < n * n; idx += NT) {
-    const int ia = sperm[idx % n], ic = sperm[idx / n];
-    P[idx] = static_cast<R>(kernel_entry(g, act[ia], act[ic])) - upd(ia, ic);
+    const int i = idx % n, c = idx / n;
+    P[idx] = static_cast<R>(kernel_entry(g.pde, geo[i], geo[c])) - upd(sperm[i], sperm[c]);
   }
   __syncthreads();
-  for (int idx = tid; idx < r * k; idx += NT) {
-    const int i = idx % r, a = idx / r;
-    R acc = P[(k + i) + n * a];
-    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + n * a];
-    Krs[idx] = acc;
-  }
-  for (int idx = tid; idx < k * r; idx += NT) {
-    const int a = idx % k, j = idx / k;
-    R acc = P[a + n * (k + j)];
-    for (int l = 0; l < k; l++) acc -= P[a + n * l] * T[l + k * j];
-    Ksr[idx] = acc;
-  }
-  __syncthreads();
-  for (int idx = tid; idx < r * r; idx += NT) {
-    const int i = idx % r, j = idx / r;
-    R acc = P[(k + i) + n * (k + j)];
-    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + n * (k + j)];
-    for (int l = 0; l < k; l++) acc -= Krs[i + r * l] * T[l + k * j];
-    X[idx] = acc;
-  }
-  __syncthreads();
-  if (wv == 0) {
-    // LU with partial pivoting, lane i = row i.
-    int info = 0;
-    for (int j = 0; j < r; j++) {
-      const bool live = lane >= j && lane < r;
-      const int bi = wave_argmax<R>(live ? fabs(X[lane + r * j]) : R(-1),
-                                    live ? lane : 0x7fffffff);
-      const int p = (bi >= r) ? j : bi;
-      if (lane == 0) spiv[j] = p;
-      if (p != j && lane < r) {
-        const R tmp = X[j + r * lane];
-        X[j + r * lane] = X[p + r * lane];
-        X[p + r * lane] = tmp;
-      }
-      wave_sync();
-      const R piv = X[j + r * j];
-      if (piv == 0) {
-        if (info == 0) info = j + 1;
-      } else if (lane > j && lane < r) {
-        X[lane + r * j] *= 1 / piv;
-      }
-      wave_sync();
-      if (lane > j && lane < r) {
-        const R lij = X[lane + r * j];
-        for (int c = j + 1; c < r; c++) X[lane + r * c] -= lij * X[j + r * c];
-      }
-      wave_sync();
-    }
-    if (lane == 0) {
-      sh_info = info;
-      for (int i = 0; i < r; i++) {
-        idxf[i] = i;
-        idxb[i] = i;
-      }
-      for (int i = 0; i < r; i++) {
-        const int p = spiv[i], q = idxf[i];
-        idxf[i] = idxf[p];
-        idxf[p] = q;
-      }
-      for (int i = r - 1; i >= 0; i--) {
-        const int p = spiv[i], q = idxb[i];
-        idxb[i] = idxb[p];
-        idxb[p] = q;
-      }
-    }
-  }
-  __syncthreads();
-  if (sh_info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
-  // U = X^{-1} K_rs: wave per column, lane i = row i.
-  for (int a = wv; a < k; a += NW) {
-    R u = (lane < r) ? Krs[idxf[lane] + r * a] : R(0);
-    for (int l = 0; l < r; l++) {
-      const R ul = lane_value(u, l);
-      if (lane > l && lane < r) u -= X[lane + r * l] * ul;
-    }
-    for (int i = r - 1; i >= 0; i--) {
-      const R ui = lane_value(u, i) / X[i + r * i];
-      if (lane == i) u = ui;
-      if (lane < i) u -= X[lane + r * i] * ui;
-    }
-    if (lane < r) F[fl.U + lane + (int64_t)r * a] = u;
-  }
-  // L = K_sr X^{-1}: wave per row, lane j = column j (X^T z^T = row^T).
-  for (int a = wv; a < k; a += NW) {
-    R acc = (lane < r) ? Ksr[a + k * lane] : R(0);
-    for (int i = 0; i < r; i++) {
-      const R yi = lane_value(acc, i) / X[i + r * i];
-      if (lane == i) acc = yi;
-      if (lane > i && lane < r) acc -= X[i + r * lane] * yi;
-    }
-    for (int i = r - 1; i >= 0; i--) {
-      const R zi = lane_value(acc, i);
-      if (lane < i) acc -= X[i + r * lane] * zi;
-    }
-    const R z = __shfl(acc, lane < r ? idxb[lane] : 0, 64);
-    if (lane < r) {
-      Lm[a + k * lane] = z;
-      F[fl.L + a + (int64_t)k * lane] = z;
-    }
-  }
-  __syncthreads();
-  for (int idx = tid; idx < k * k; idx += NT) {
-    const int a = idx % k, c = idx / k;
-    R acc = 0;
-    for (int j = 0; j < r; j++) acc += Lm[a + k * j] * Krs[j + r * c];
-    F[fl.S + idx] = upd(sperm[a], sperm[c]) + acc;
-  }
-  for (int idx = tid; idx < r * r; idx += NT) F[fl.X + idx] = X[idx];
-  for (int i = tid; i < r; i += NT) F[fl.piv + i] = static_cast<R>(spiv[i]);
+  decouple_blocks<R, NT>(P, T, n, k, Krs, Ksr, X);
+  const int info = gj_invert_into<R, NT>(X, P, r, prow);
+  if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
+  decouple_tail<R, NT>(X, prow, Krs, Ksr, P, r, k, F, fl, upd, sperm);
   if (tid == 0) s.upd_kind[b] = 1;
 }
 
-// ---------------------------------------------------------------------------
-// Decouple one box (skel_factorization.cpp:77-128): with s/r the skeleton
-// and redundant positions and K_BN = K(act, act) - (children's carried
-// Schur blocks),
-//   K_rs = K(r,s) - T^T K(s,s)
-//   X_rr = K(r,r) - T^T K(s,r) - K_rs T          (LU, partial pivoting)
-//   K_sr = K(s,r) - K(s,s) T
-//   L = K_sr X_rr^{-1},  U = X_rr^{-1} K_rs,  schur = L K_rs
-// and the box's carried block = (children's update on s,s) + schur.
-// Uncompressed boxes carry their children's blocks unchanged.
-// ---------------------------------------------------------------------------
-
+// Every other box: uncompressed boxes pass their children's blocks on;
+// compressed boxes with |active| > 64 keep P and X_rr in LDS when they fit
+// and in the box's scratch slot otherwise.
 template <typename R, int NT>
 __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
                                                  DevState s, int start) {
@@ -1153,40 +1170,20 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   __shared__ signed char pch[kMaxBoxDofs];
   __shared__ short ploc[kMaxBoxDofs];
   __shared__ short sperm[kMaxBoxDofs];
-  __shared__ int spiv[kMaxBoxDofs];
+  __shared__ int prow[kMaxBoxDofs];
   __shared__ const R* chS[4];
   __shared__ int chN[4];
 
   const int b = start + blockIdx.x;
   const int tid = threadIdx.x;
-  const int n = s.act_cnt[b];
   if (!pools_ok(s)) return;
+  const int n = s.act_cnt[b];
   if (n == 0) {
     if (tid == 0) s.upd_kind[b] = 0;
     return;
   }
   if (n <= kSmallBox && s.rank[b] > 0) return;  // k_decouple_small's box
-  const bool leaf = t.leaf[b] != 0;
-  if (tid < 4) {
-    const int c = leaf ? -1 : t.child[4 * b + tid];
-    chS[tid] = (c >= 0) ? carried_block<R>(s, c) : nullptr;
-    chN[tid] = (c >= 0) ? s.out_cnt[c] : 0;
-  }
-  __syncthreads();
-  for (int i = tid; i < n; i += NT) {
-    int q = -1, loc = 0;
-    if (!leaf) {
-      for (int u = 0; u < 4; u++) {
-        const int p0 = s.chpos[4 * b + u];
-        if (i >= p0 && i < p0 + chN[u]) {
-          q = u;
-          loc = i - p0;
-        }
-      }
-    }
-    pch[i] = (signed char)q;
-    ploc[i] = (short)loc;
-  }
+  load_child_blocks<R>(t, s, b, n, chS, chN, pch, ploc, NT);
   __syncthreads();
   ChildUpdates<R> upd;
   upd.S = chS;
@@ -1210,6 +1207,7 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   const int* act = s.act_list + (int64_t)b * s.ncap;
   for (int i = tid; i < n; i += NT) sperm[i] = (short)s.perm[(int64_t)b * s.ncap + i];
   __syncthreads();
+  // n^2 + r^2 + 2 r k <= 2 n^2 entries.
   R* P = (2 * (int64_t)n * n <= kLdsElems)
              ? sbuf
              : reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
@@ -1221,92 +1219,15 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
     P[idx] = static_cast<R>(kernel_entry(g, act[ia], act[ic])) - upd(ia, ic);
   }
   __syncthreads();
-  const R* T = F + fl.T;
-  for (int idx = tid; idx < r * k; idx += NT) {
-    const int i = idx % r, a = idx / r;
-    R acc = P[(k + i) + (int64_t)n * a];
-    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + (int64_t)n * a];
-    Krs[idx] = acc;
-  }
-  for (int idx = tid; idx < k * r; idx += NT) {
-    const int a = idx % k, j = idx / k;
-    R acc = P[a + (int64_t)n * (k + j)];
-    for (int l = 0; l < k; l++) acc -= P[a + (int64_t)n * l] * T[l + k * j];
-    Ksr[idx] = acc;
-  }
-  __syncthreads();
-  for (int idx = tid; idx < r * r; idx += NT) {
-    const int i = idx % r, j = idx / r;
-    R acc = P[(k + i) + (int64_t)n * (k + j)];
-    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + (int64_t)n * (k + j)];
-    for (int l = 0; l < k; l++) acc -= Krs[i + r * l] * T[l + k * j];
-    X[idx] = acc;
-  }
-  __syncthreads();
-  const int info = block_lu<R, NT>(X, r, r, spiv);
+  decouple_blocks<R, NT>(P, F + fl.T, n, k, Krs, Ksr, X);
+  const int info = gj_invert_into<R, NT>(X, P, r, prow);
   if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
-  R* FX = F + fl.X;
-  R* Fp = F + fl.piv;
-  for (int idx = tid; idx < r * r; idx += NT) FX[idx] = X[idx];
-  for (int i = tid; i < r; i += NT) Fp[i] = static_cast<R>(spiv[i]);
-  // U = X^{-1} K_rs, one thread per column (dgetrs 'N').
-  R* U = F + fl.U;
-  for (int a = tid; a < k; a += NT) {
-    R* u = U + (int64_t)r * a;
-    for (int i = 0; i < r; i++) u[i] = Krs[i + r * a];
-    for (int i = 0; i < r; i++) {
-      const int p = spiv[i];
-      if (p != i) {
-        const R tmp = u[i];
-        u[i] = u[p];
-        u[p] = tmp;
-      }
-    }
-    for (int l = 0; l < r; l++) {
-      const R ul = u[l];
-      for (int i = l + 1; i < r; i++) u[i] -= X[i + r * l] * ul;
-    }
-    for (int i = r - 1; i >= 0; i--) {
-      u[i] /= X[i + r * i];
-      const R ui = u[i];
-      for (int l = 0; l < i; l++) u[l] -= X[l + r * i] * ui;
-    }
-  }
-  // L = K_sr X^{-1}, one thread per row (dgetrs 'T' on the transpose).
-  R* Lm = F + fl.L;
-  for (int a = tid; a < k; a += NT) {
-    for (int i = 0; i < r; i++) {
-      R acc = Ksr[a + k * i];
-      for (int l = 0; l < i; l++) acc -= X[l + r * i] * Lm[a + k * l];
-      Lm[a + k * i] = acc / X[i + r * i];
-    }
-    for (int i = r - 1; i >= 0; i--) {
-      R acc = Lm[a + k * i];
-      for (int l = i + 1; l < r; l++) acc -= X[l + r * i] * Lm[a + k * l];
-      Lm[a + k * i] = acc;
-    }
-    for (int i = r - 1; i >= 0; i--) {
-      const int p = spiv[i];
-      if (p != i) {
-        const R tmp = Lm[a + k * i];
-        Lm[a + k * i] = Lm[a + k * p];
-        Lm[a + k * p] = tmp;
-      }
-    }
-  }
-  __syncthreads();
-  R* S = F + fl.S;
-  for (int idx = tid; idx < k * k; idx += NT) {
-    const int a = idx % k, c = idx / k;
-    R acc = 0;
-    for (int j = 0; j < r; j++) acc += Lm[a + k * j] * Krs[j + r * c];
-    S[idx] = upd(sperm[a], sperm[c]) + acc;
-  }
+  decouple_tail<R, NT>(X, prow, Krs, Ksr, P, r, k, F, fl, upd, sperm);
   if (tid == 0) s.upd_kind[b] = 1;
 }
 
 // ---------------------------------------------------------------------------
-// Root: K(allskel, allskel) - carried blocks of the level-1 boxes, LU.
+// Root: K(allskel, allskel) - carried blocks of the level-1 boxes, inverted.
 // ---------------------------------------------------------------------------
 
 template <typename R, int NT>
@@ -1359,10 +1280,154 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
     A[idx] = static_cast<R>(kernel_entry(g, act[i], act[j])) - upd(i, j);
   }
   __syncthreads();
-  if (factor_lu) {
-    const int info = block_lu<R, NT>(A, n, n, s.root_piv);
+  if (factor_lu && n > kRootRegMax) {
+    // Gauss-Jordan inverse (the reference LU-factors the root block) for
+    // blocks too large for k_root_inv; the inverse, unpermuted, goes after
+    // the assembled matrix.
+    R* B = A + (int64_t)n * n;
+    const int* prow = s.root_piv;
+    const int info = gj_invert_into<R, NT>(A, B, n, s.root_piv);
     if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
+    for (int64_t idx = tid; idx < (int64_t)n * n; idx += NT) {
+      const int j = (int)(idx % n), jj = (int)(idx / n);
+      B[j + (int64_t)n * prow[jj]] = A[prow[j] + (int64_t)n * jj];
+    }
   }
+}
+
+// Gauss-Jordan inverse of the root block with the matrix in registers:
+// 512 threads on a 32 x 16 grid, thread (tr, tc) holding rows tr + 32 u and
+// columns tc + 16 v (n <= 192).  Per step the pivot column (kept in LDS by
+// its owners) picks the pivot row as in gj_invert, the pivot row's owners
+// publish it, and every thread updates its entries.  The inverse,
+// unpermuted, goes after the assembled matrix.
+constexpr int kRootTR = 32, kRootTC = 16, kRootRU = 6, kRootCV = 12;
+
+template <typename R>
+__global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int factor_lu) {
+  constexpr int NMAX = kRootRegMax;
+  __shared__ R colbuf[2][NMAX];
+  __shared__ R rowbuf[NMAX];
+  __shared__ int prow[NMAX];
+  __shared__ int invrow[NMAX];
+  if (!factor_lu || !pools_ok(s)) return;
+  const int n = s.act_cnt[0];
+  if (n == 0 || n > NMAX) return;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int tr = tid % kRootTR, tc = tid / kRootTR;
+  R* A = reinterpret_cast<R*>(s.root_mat);
+  R a[kRootRU][kRootCV];
+#pragma unroll
+  for (int u = 0; u < kRootRU; u++)
+#pragma unroll
+    for (int v = 0; v < kRootCV; v++) {
+      const int i = tr + kRootTR * u, c = tc + kRootTC * v;
+      a[u][v] = (i < n && c < n) ? A[i + (int64_t)n * c] : R(0);
+    }
+  if (tc == 0) {
+#pragma unroll
+    for (int u = 0; u < kRootRU; u++) {
+      const int i = tr + kRootTR * u;
+      if (i < n) colbuf[0][i] = a[u][0];
+    }
+  }
+  __syncthreads();
+  uint32_t used = 0;
+  int info = 0;
+  for (int j = 0; j < n; j++) {
+    const R* col = colbuf[j & 1];
+    R* coln = colbuf[(j + 1) & 1];
+    R best = -1;
+    int bi = 0x7fffffff;
+    for (int uu = 0; lane + 64 * uu < n; uu++) {
+      if ((used >> uu) & 1) continue;
+      const int i = lane + 64 * uu;
+      const R v = fabs(col[i]);
+      const R key = (v == v) ? v : R(-0.5);
+      if (key > best) {
+        best = key;
+        bi = i;
+      }
+    }
+    const int p = wave_argmax<R>(best, bi);
+    if ((p & 63) == lane) used |= 1u << (p >> 6);
+    if (tid == 0) prow[j] = p;
+    const R piv = col[p];
+    if (piv == R(0) && info == 0) info = j + 1;
+    const R inv = R(1) / piv;
+    if (tr == p % kRootTR) {
+      const int up = p / kRootTR;
+#pragma unroll
+      for (int u = 0; u < kRootRU; u++) {
+        if (u != up) continue;
+#pragma unroll
+        for (int v = 0; v < kRootCV; v++) {
+          const int c = tc + kRootTC * v;
+          if (c < n) rowbuf[c] = (c == j) ? R(1) : a[u][v];
+        }
+      }
+    }
+    __syncthreads();
+    R cv[kRootRU], rv[kRootCV];
+#pragma unroll
+    for (int u = 0; u < kRootRU; u++) {
+      const int i = tr + kRootTR * u;
+      cv[u] = (i < n) ? col[i] : R(0);
+    }
+#pragma unroll
+    for (int v = 0; v < kRootCV; v++) {
+      const int c = tc + kRootTC * v;
+      rv[v] = (c < n) ? rowbuf[c] * inv : R(0);
+    }
+    // Column j restarts from zero, then one FMA per entry; the pivot row
+    // is overwritten with the scaled pivot row.
+    if (tc == j % kRootTC) {
+      const int vj = j / kRootTC;
+#pragma unroll
+      for (int v = 0; v < kRootCV; v++) {
+        if (v != vj) continue;
+#pragma unroll
+        for (int u = 0; u < kRootRU; u++) a[u][v] = R(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRootRU; u++)
+#pragma unroll
+      for (int v = 0; v < kRootCV; v++) a[u][v] -= cv[u] * rv[v];
+    if (tr == p % kRootTR) {
+      const int up = p / kRootTR;
+#pragma unroll
+      for (int u = 0; u < kRootRU; u++) {
+        if (u != up) continue;
+#pragma unroll
+        for (int v = 0; v < kRootCV; v++) a[u][v] = rv[v];
+      }
+    }
+    if (j + 1 < n && tc == (j + 1) % kRootTC) {
+      const int vn = (j + 1) / kRootTC;
+#pragma unroll
+      for (int v = 0; v < kRootCV; v++) {
+        if (v != vn) continue;
+#pragma unroll
+        for (int u = 0; u < kRootRU; u++) {
+          const int i = tr + kRootTR * u;
+          if (i < n) coln[i] = a[u][v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < n) invrow[prow[tid]] = tid;
+  __syncthreads();
+  R* B = A + (int64_t)n * n;
+#pragma unroll
+  for (int u = 0; u < kRootRU; u++)
+#pragma unroll
+    for (int v = 0; v < kRootCV; v++) {
+      const int i = tr + kRootTR * u, c = tc + kRootTC * v;
+      if (i < n && c < n) B[invrow[i] + (int64_t)n * prow[c]] = a[u][v];
+    }
+  if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
 }
 
 // ---------------------------------------------------------------------------
@@ -1371,32 +1436,19 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
 // scattered back (boxes of one level touch disjoint DOFs).
 // ---------------------------------------------------------------------------
 
+// x <- X^{-1} x with an explicit inverse (r x r, ld r): one wave, x and
+// tmp in LDS.
 template <typename R>
-__device__ void wave_lu_solve(const R* X, const R* piv, int r, double* x) {
+__device__ void wave_inv_apply(const R* Xi, int r, double* x, double* tmp) {
   const int lane = threadIdx.x & 63;
-  if (lane == 0) {
-    for (int i = 0; i < r; i++) {
-      const int p = (int)piv[i];
-      if (p != i) {
-        const double tmp = x[i];
-        x[i] = x[p];
-        x[p] = tmp;
-      }
-    }
+  for (int i = lane; i < r; i += 64) {
+    double acc = 0;
+    for (int j = 0; j < r; j++) acc += (double)Xi[i + (int64_t)r * j] * x[j];
+    tmp[i] = acc;
   }
   wave_sync();
-  for (int l = 0; l < r; l++) {
-    const double xl = x[l];
-    for (int i = l + 1 + lane; i < r; i += 64) x[i] -= (double)X[i + r * l] * xl;
-    wave_sync();
-  }
-  for (int i = r - 1; i >= 0; i--) {
-    if (lane == 0) x[i] /= (double)X[i + r * i];
-    wave_sync();
-    const double xi = x[i];
-    for (int l = lane; l < i; l += 64) x[l] -= (double)X[l + r * i] * xi;
-    wave_sync();
-  }
+  for (int i = lane; i < r; i += 64) x[i] = tmp[i];
+  wave_sync();
 }
 
 template <typename R>
@@ -1405,6 +1457,7 @@ __global__ __launch_bounds__(64) void k_sweep_up(DevState s, int start,
                                                  int diag) {
   __shared__ double xs[kMaxBoxDofs];
   __shared__ double xr[kMaxBoxDofs];
+  __shared__ double tmp[kMaxBoxDofs];
   const int b = start + blockIdx.x;
   const int k = s.rank[b];
   if (k == 0) return;
@@ -1438,7 +1491,7 @@ __global__ __launch_bounds__(64) void k_sweep_up(DevState s, int start,
       xs[a] = acc;
     }
     wave_sync();
-    if (diag) wave_lu_solve(F + fl.X, F + fl.piv, r, xr);
+    if (diag) wave_inv_apply(F + fl.X, r, xr, tmp);
     for (int a = lane; a < k; a += 64) x[sk[a]] = xs[a];
     for (int i = lane; i < r; i += 64) x[rd[i]] = xr[i];
     wave_sync();
@@ -1448,6 +1501,7 @@ __global__ __launch_bounds__(64) void k_sweep_up(DevState s, int start,
 template <typename R>
 __global__ __launch_bounds__(64) void k_diag(DevState s, int start, SolveVec v) {
   __shared__ double xr[kMaxBoxDofs];
+  __shared__ double tmp[kMaxBoxDofs];
   const int b = start + blockIdx.x;
   const int k = s.rank[b];
   if (k == 0) return;
@@ -1460,7 +1514,7 @@ __global__ __launch_bounds__(64) void k_diag(DevState s, int start, SolveVec v) 
     double* x = v.x + v.ldx * c;
     for (int i = lane; i < r; i += 64) xr[i] = x[rd[i]];
     wave_sync();
-    wave_lu_solve(F + fl.X, F + fl.piv, r, xr);
+    wave_inv_apply(F + fl.X, r, xr, tmp);
     for (int i = lane; i < r; i += 64) x[rd[i]] = xr[i];
     wave_sync();
   }
@@ -1508,17 +1562,16 @@ __global__ __launch_bounds__(64) void k_sweep_down(DevState s, int start,
 template <typename R>
 __global__ __launch_bounds__(64) void k_root_solve(DevState s, SolveVec v) {
   __shared__ double xa[kMaxBoxDofs];
-  __shared__ R piv[kMaxBoxDofs];
+  __shared__ double tmp[kMaxBoxDofs];
   const int lane = threadIdx.x;
   const int n = s.act_cnt[0];
   const int* act = s.act_list;
-  const R* A = reinterpret_cast<const R*>(s.root_mat);
-  for (int i = lane; i < n; i += 64) piv[i] = static_cast<R>(s.root_piv[i]);
+  const R* Ai = reinterpret_cast<const R*>(s.root_mat) + (int64_t)n * n;
   for (int c = 0; c < v.nrhs; c++) {
     double* x = v.x + v.ldx * c;
     for (int i = lane; i < n; i += 64) xa[i] = x[act[i]];
     wave_sync();
-    wave_lu_solve(A, piv, n, xa);
+    wave_inv_apply(Ai, n, xa, tmp);
     for (int i = lane; i < n; i += 64) x[act[i]] = xa[i];
     wave_sync();
   }
@@ -1543,15 +1596,15 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       k_rows<<<count, 64, 0, st>>>(g, t, s, start, level);
       break;
     case kStageScan:
-      k_scan<<<1, 1024, 0, st>>>(s, start, count);
+      k_scan<<<1, 1024, 0, st>>>(s, start, count, level);
       break;
     case kStageAssemble:
-      k_assemble<R><<<count, 256, 0, st>>>(g, t, s, start, level);
+      k_assemble<R><<<min(count, 64), 256, 0, st>>>(g, t, s, start, level);
       break;
     case kStageQrcp:
-      k_rowcomp<R, kXRows><<<min(2 * count, 512), kXThreads, 0, st>>>(s, start, level);
-      k_qrcp_id<R, kXRows><<<count, kXThreads, 0, st>>>(s, start, tol);
-      k_qrcp<R, 512><<<count, 512, 0, st>>>(s, start, tol);
+      k_rowcomp<R, kXRows><<<min(2 * count, 256), kXThreads, 0, st>>>(g, t, s, start, level);
+      k_qrcp_id<R, kXRows><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
+      k_qrcp<R, 512><<<min(count, 64), 512, 0, st>>>(s, start, level, tol);
       break;
     case kStageDecouple:
       k_decouple_small<R><<<count, 256, 0, st>>>(g, t, s, start);
@@ -1567,6 +1620,7 @@ void launch_root(const DevGeom& g, const DevTree& t, const DevState& s,
                  bool factor_lu, hipStream_t st) {
   k_active<<<1, 64, 0, st>>>(t, s, 0);
   k_root<R, 1024><<<1, 1024, 0, st>>>(g, t, s, factor_lu ? 1 : 0);
+  k_root_inv<R><<<1, kRootTR * kRootTC, 0, st>>>(s, factor_lu ? 1 : 0);
 }
 
 template <typename R>
