@@ -31,9 +31,11 @@ enum QrcpKind : int {
   kQrcpReg = 1,     // register QRCP on the ID matrix (m <= 512, n <= 64)
   kQrcpStack = 2,   // row-compressed in chunks first, then register QRCP
   kQrcpLds = 3,     // general QRCP (LDS or the box's scratch slot)
+  kQrcpReg8 = 4,    // wide register QRCP (m <= 256, n <= 128)
+  kQrcpStack8 = 5,  // row-compressed, then wide register QRCP
 };
-constexpr int kRegMaxRows = 512;
-constexpr int kRegMaxCols = 64;
+constexpr int kRegMaxRows = 512, kRegMaxCols = 64;
+constexpr int kRegMaxRows8 = 256, kRegMaxCols8 = 128;
 constexpr int kMaxChunks = 8;
 constexpr int kMaxLevels = 64;  // tree depth bound (per-level job counters)
 constexpr int kSmallBox = 64;     // decouple with every operand in LDS
@@ -44,12 +46,13 @@ constexpr int kRootRegMax = 192;  // root blocks inverted in registers
 enum : int {
   kLstReg = 0,   // register QRCP (kQrcpReg, kQrcpStack)
   kLstLds = 1,   // general QRCP (kQrcpLds)
-  kNumLists = 2,
+  kLstReg8 = 2,  // wide register QRCP (kQrcpReg8, kQrcpStack8)
+  kNumLists = 3,
 };
 
-// Row chunks of a tall ID matrix: ceil(m / 512), as even as possible.
-__host__ __device__ inline int id_chunks(int m) {
-  return (m + kRegMaxRows - 1) / kRegMaxRows;
+// Row chunks of a tall ID matrix, at most `rows` rows each.
+__host__ __device__ inline int id_chunks(int m, int rows) {
+  return (m + rows - 1) / rows;
 }
 
 struct DevGeom {
@@ -121,6 +124,14 @@ struct DevState {
 __device__ inline int* list_slot(const DevState& s, int level, int L) {
   return s.lcnt + 2 * (level * kNumLists + L);
 }
+// Row-compression jobs (chunks) of box b.
+__device__ inline int row_jobs(const DevState& s, int b) {
+  const int kind = s.qkind[b];
+  if (kind == kQrcpStack) return id_chunks(s.m_rows[b], kRegMaxRows);
+  if (kind == kQrcpStack8) return id_chunks(s.m_rows[b], kRegMaxRows8);
+  return 0;
+}
+
 __device__ inline int list_count(const DevState& s, int level, int L) {
   return list_slot(s, level, L)[0];
 }
@@ -321,6 +332,69 @@ __device__ inline void spread4(R t, R (&out)[4]) {
 template <typename R>
 __device__ inline R class_by_lane(R t) {
   return dpp_any<R, dpp::kQuadBitrev, 0xF>(t);
+}
+
+// Eight-column interleave (same scheme, three exchange steps): slot sl of
+// lane l carries class sl ^ oct_class(l), the bit-reversed (l & 7).  The
+// third exchange takes the lane 4 away within the row (row_ror:4), whose
+// low three bits differ in bit 2 only.
+__device__ inline int oct_class(int lane) {
+  return ((lane & 1) << 2) | (lane & 2) | ((lane >> 2) & 1);
+}
+
+template <typename R>
+__device__ inline R wave_total8(const R (&p)[8]) {
+  R s[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) s[i] = p[i] + dpp_any<R, dpp::kQuadSwap1, 0xF>(p[i + 4]);
+  const R u0 = s[0] + dpp_any<R, dpp::kQuadSwap2, 0xF>(s[2]);
+  const R u1 = s[1] + dpp_any<R, dpp::kQuadSwap2, 0xF>(s[3]);
+  R t = u0 + dpp_any<R, dpp::kRowRor4, 0xF>(u1);
+  t += dpp_any<R, dpp::kRowRor8, 0xF>(t);
+  t += shfl_xor_any(t, 16);
+  t += shfl_xor_any(t, 32);
+  return t;
+}
+
+template <typename R>
+__device__ inline void spread8(R t, R (&out)[8]) {
+  const R r4 = dpp_any<R, dpp::kRowRor4, 0xF>(t);
+  out[0] = t;
+  out[1] = r4;
+  out[2] = dpp_any<R, dpp::kQuadSwap2, 0xF>(t);
+  out[3] = dpp_any<R, dpp::kQuadSwap2, 0xF>(r4);
+  out[4] = dpp_any<R, dpp::kQuadSwap1, 0xF>(t);
+  out[5] = dpp_any<R, dpp::kQuadSwap1, 0xF>(r4);
+  out[6] = dpp_any<R, dpp::kQuadSwap3, 0xF>(t);
+  out[7] = dpp_any<R, dpp::kQuadSwap3, 0xF>(r4);
+}
+
+template <typename R>
+__device__ inline R class_by_lane8(R t) {
+  const int lane = threadIdx.x & 63;
+  return __shfl(t, (lane & ~7) | oct_class(lane & 7), 64);
+}
+
+// The interleave for CPW = 4 or 8 columns per wave.
+template <int CPW>
+__device__ inline int slot_class(int lane) {
+  if constexpr (CPW == 4) return quad_class(lane);
+  else return oct_class(lane);
+}
+template <typename R, int CPW>
+__device__ inline R wave_total_cols(const R (&p)[CPW]) {
+  if constexpr (CPW == 4) return wave_total4(p);
+  else return wave_total8(p);
+}
+template <typename R, int CPW>
+__device__ inline void spread_cols(R t, R (&out)[CPW]) {
+  if constexpr (CPW == 4) spread4(t, out);
+  else spread8(t, out);
+}
+template <typename R, int CPW>
+__device__ inline R class_by_lane_cols(R t) {
+  if constexpr (CPW == 4) return class_by_lane(t);
+  else return class_by_lane8(t);
 }
 
 // Pivot key: a column norm v >= 0 with its logical position in the low

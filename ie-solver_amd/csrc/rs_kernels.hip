@@ -148,9 +148,13 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
   }
   int kind = kQrcpNone;
   if (m > 0) {
-    const int nch = id_chunks(m);
+    const int nch4 = id_chunks(m, kRegMaxRows), nch8 = id_chunks(m, kRegMaxRows8);
     if (n <= kRegMaxCols && m <= kRegMaxRows) kind = kQrcpReg;
-    else if (n <= kRegMaxCols && nch <= kMaxChunks && nch * n <= kRegMaxRows) kind = kQrcpStack;
+    else if (n <= kRegMaxCols && nch4 <= kMaxChunks && nch4 * n <= kRegMaxRows)
+      kind = kQrcpStack;
+    else if (n <= kRegMaxCols8 && m <= kRegMaxRows8) kind = kQrcpReg8;
+    else if (n <= kRegMaxCols8 && nch8 <= kMaxChunks && nch8 * n <= kRegMaxRows8)
+      kind = kQrcpStack8;
     else kind = kQrcpLds;
   }
   if (threadIdx.x == 0) {
@@ -161,7 +165,8 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
     // register-QRCP boxes evaluate their entries in place.
     const int64_t ld = m > 2 * n ? m : 2 * n;
     int64_t scratch = kind == kQrcpLds ? ld * n : 0;
-    if (kind == kQrcpStack) scratch = (int64_t)id_chunks(m) * n * n;
+    if (kind == kQrcpStack || kind == kQrcpStack8)
+      scratch = (int64_t)id_chunks(m, kind == kQrcpStack ? kRegMaxRows : kRegMaxRows8) * n * n;
     s.id_sz[b] = scratch;
     s.row_sz[b] = kind == kQrcpLds ? (m - proxy_rows(g, level)) / 2 : 0;
     s.fac_sz[b] = (n > 0) ? fac_slot_size(n) : 0;
@@ -185,12 +190,12 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
                                                int count, int level) {
   // Scanned per box: ID scratch, row-list and factor sizes, register-QRCP
   // and general-QRCP list entries, row-compression jobs.
-  constexpr int NQ = 6;
+  constexpr int NQ = 7;
   __shared__ int64_t sh[NQ][1024];
   const int tid = threadIdx.x;
   const int chunk = (count + 1023) / 1024;
   const int lo = min(count, tid * chunk), hi = min(count, lo + chunk);
-  int64_t own[NQ] = {0, 0, 0, 0, 0, 0};
+  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0};
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
     const int kind = s.qkind[b];
@@ -199,7 +204,8 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     own[2] += s.fac_sz[b];
     own[3] += (kind == kQrcpReg || kind == kQrcpStack) ? 1 : 0;
     own[4] += (kind == kQrcpLds) ? 1 : 0;
-    own[5] += (kind == kQrcpStack) ? id_chunks(s.m_rows[b]) : 0;
+    own[5] += row_jobs(s, b);
+    own[6] += (kind == kQrcpReg8 || kind == kQrcpStack8) ? 1 : 0;
   }
 #pragma unroll
   for (int q = 0; q < NQ; q++) sh[q][tid] = own[q];
@@ -219,6 +225,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
   for (int q = 0; q < NQ; q++) off[q] = sh[q][tid] - own[q];
   int* lreg = s.lists + (int64_t)kLstReg * s.nbox + start;
   int* llds = s.lists + (int64_t)kLstLds * s.nbox + start;
+  int* lreg8 = s.lists + (int64_t)kLstReg8 * s.nbox + start;
   int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
@@ -231,8 +238,9 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     off[2] += s.fac_sz[b];
     if (kind == kQrcpReg || kind == kQrcpStack) lreg[off[3]++] = b;
     if (kind == kQrcpLds) llds[off[4]++] = b;
-    if (kind == kQrcpStack) {
-      const int nch = id_chunks(s.m_rows[b]);
+    if (kind == kQrcpReg8 || kind == kQrcpStack8) lreg8[off[6]++] = b;
+    {
+      const int nch = row_jobs(s, b);
       for (int q = 0; q < nch; q++) {
         jobs[2 * off[5]] = b;
         jobs[2 * off[5] + 1] = q;
@@ -245,6 +253,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     const int64_t ta = sh[0][1023], tb = sh[1][1023], tc = sh[2][1023];
     list_slot(s, level, kLstReg)[0] = (int)sh[3][1023];
     list_slot(s, level, kLstLds)[0] = (int)sh[4][1023];
+    list_slot(s, level, kLstReg8)[0] = (int)sh[6][1023];
     s.njobs[level] = (int)sh[5][1023];
     int e = 0;
     if (ta > s.id_cap) {
@@ -324,96 +333,167 @@ __global__ __launch_bounds__(256) void k_assemble(DevGeom g, DevTree t,
 
 // ---------------------------------------------------------------------------
 // Register-resident QR kernels (1024 threads, 16 waves).  Wave w holds
-// columns 4w .. 4w+3 of an m x n matrix (m <= 64 RPL, n <= 64) in the
-// four-column interleave of rs_device.hpp: row i on lane i % 64, register
-// i / 64, column 4w + (sl ^ quad_class(lane)) in slot sl.  A Householder
-// step: the wave owning the pivot column forms the reflector and publishes
-// it through LDS; every wave then updates its columns in registers with
-// one four-column reduction.  Waves past the last column only take part
-// in the barriers.
+// columns CPW w .. CPW w + CPW - 1 of an m x n matrix (m <= 64 RPL,
+// n <= 16 CPW) in the column interleave of rs_device.hpp: row i on lane
+// i % 64, register i / 64, column CPW w + (sl ^ slot_class(lane)) in slot
+// sl.  Two shapes: 4 columns x 8 row registers (n <= 64, m <= 512) and
+// 8 columns x 4 row registers (n <= 128, m <= 256).  A Householder step:
+// the wave owning the pivot column publishes the reflector through LDS;
+// every wave then updates its columns in registers with one interleaved
+// column reduction.  Waves past the last column only take part in the
+// barriers.
 // ---------------------------------------------------------------------------
 
 constexpr int kXThreads = 1024;
-constexpr int kXRows = kRegMaxRows / 64;  // registers per lane per column
+
+template <int CPW>
+struct RegShape {
+  static constexpr int RPL = CPW == 4 ? 8 : 4;  // row registers per lane
+  static constexpr int MAXM = 64 * RPL;
+  static constexpr int MAXN = 16 * CPW;
+};
 
 // Entries of the column in slot psl (per lane) of the interleave.
-template <typename R, int RPL>
-__device__ inline void pick_slot(const R (&a)[4][RPL], int psl, R (&x)[RPL]) {
+template <typename R, int RPL, int CPW>
+__device__ inline void pick_slot(const R (&a)[CPW][RPL], int psl, R (&x)[RPL]) {
 #pragma unroll
   for (int r = 0; r < RPL; r++) x[r] = a[0][r];
 #pragma unroll
-  for (int sl = 1; sl < 4; sl++) {
+  for (int sl = 1; sl < CPW; sl++) {
     const bool me = psl == sl;
 #pragma unroll
     for (int r = 0; r < RPL; r++) x[r] = me ? a[sl][r] : x[r];
   }
 }
 
-// Householder reflector of column x at step j (j < 64), LAPACK dlarfg:
-// beta = -sign(alpha) ||x(j:)||, tau = (beta - alpha) / beta,
-// v = x(j+1:) / (alpha - beta), v_j = 1, v(0:j-1) = 0.  Writes v to vb
-// (all 64 RPL rows; rows past the matrix hold zeros in x).
+// Row i (wave-uniform) of a column held as x[RPL], in every lane.  Each
+// register is read out before the select: selecting first turns into a
+// dynamically indexed extract, which the compiler lowers through scratch.
 template <typename R, int RPL>
-__device__ inline void householder(const R (&x)[RPL], int j, R* vb, R* beta_out,
-                                   R* tau_out) {
-  const int lane = threadIdx.x & 63;
-  const R alpha = lane_value(x[0], j);
-  R ss = lane > j ? x[0] * x[0] : R(0);
+__device__ inline R row_value(const R (&x)[RPL], int i) {
+  const int ri = i >> 6, li = i & 63;
+  R v = lane_value(x[0], li);
 #pragma unroll
-  for (int r = 1; r < RPL; r++) ss += x[r] * x[r];
-  ss = wave_total(ss);
-  const R xnorm = sqrt(ss);
-  R beta, tau, sc;
-  if (xnorm == 0) {
-    tau = 0;
-    beta = alpha;
-    sc = 0;
-  } else {
-    beta = -copysign(hypot(alpha, xnorm), alpha);
-    tau = (beta - alpha) / beta;
-    sc = 1 / (alpha - beta);
+  for (int r = 1; r < RPL; r++) {
+    const R t = lane_value(x[r], li);
+    v = (r == ri) ? t : v;
   }
-  vb[lane] = lane > j ? x[0] * sc : (lane == j ? R(1) : R(0));
+  return v;
+}
+
+// LAPACK dlarfg on (alpha, x) with ||x||^2 = xn2: beta = -sign(alpha)
+// ||(alpha, x)||, tau = (beta - alpha) / beta, scale 1 / (alpha - beta).
+template <typename R>
+__device__ inline void householder_scalars(R alpha, R xn2, R* beta, R* tau, R* sc) {
+  const R xnorm = sqrt(xn2);
+  if (xnorm == 0) {
+    *tau = 0;
+    *beta = alpha;
+    *sc = 0;
+  } else {
+    *beta = -copysign(hypot(alpha, xnorm), alpha);
+    *tau = (*beta - alpha) / *beta;
+    *sc = 1 / (alpha - *beta);
+  }
+}
+
+// Reflector v of column x at step j: v(0:j-1) = 0, v_j = 1,
+// v(j+1:) = x(j+1:) sc; all 64 RPL rows (rows past the matrix hold zeros).
+template <typename R, int RPL>
+__device__ inline void write_reflector(const R (&x)[RPL], int j, R sc, R* vb) {
+  const int lane = threadIdx.x & 63;
 #pragma unroll
-  for (int r = 1; r < RPL; r++) vb[lane + 64 * r] = x[r] * sc;
-  *beta_out = beta;
-  *tau_out = tau;
+  for (int r = 0; r < RPL; r++) {
+    const int i = lane + 64 * r;
+    vb[i] = i > j ? x[r] * sc : (i == j ? R(1) : R(0));
+  }
 }
 
 // a <- (I - tau v v^T) a on this wave's columns; `live` says whether the
-// class quad_class(lane) is still to be updated (pivoted columns keep
+// class slot_class(lane) is still to be updated (pivoted columns keep
 // their R entries).
-template <typename R, int RPL>
-__device__ inline void apply_reflector(R (&a)[4][RPL], const R* vb, R tau, bool live) {
+template <typename R, int RPL, int CPW>
+__device__ inline void apply_reflector(R (&a)[CPW][RPL], const R* vb, R tau, bool live) {
   const int lane = threadIdx.x & 63;
   R vr[RPL];
 #pragma unroll
   for (int r = 0; r < RPL; r++) vr[r] = vb[lane + 64 * r];
-  R p[4];
+  R p[CPW];
 #pragma unroll
-  for (int sl = 0; sl < 4; sl++) {
+  for (int sl = 0; sl < CPW; sl++) {
     p[sl] = 0;
 #pragma unroll
     for (int r = 0; r < RPL; r++) p[sl] += vr[r] * a[sl][r];
   }
-  R t = wave_total4(p);
+  R t = wave_total_cols<R, CPW>(p);
   t = live ? tau * t : R(0);
-  R tw[4];
-  spread4(t, tw);
+  R tw[CPW];
+  spread_cols<R, CPW>(t, tw);
 #pragma unroll
-  for (int sl = 0; sl < 4; sl++)
+  for (int sl = 0; sl < CPW; sl++)
 #pragma unroll
     for (int r = 0; r < RPL; r++) a[sl][r] -= tw[sl] * vr[r];
 }
 
-template <typename R, int RPL>
-__device__ inline void load_columns(const R* A, int64_t ld, int row0, int rows, int n,
-                                    R (&a)[4][RPL]) {
-  const int lane = threadIdx.x & 63, c0 = 4 * (threadIdx.x >> 6);
-  const int hq = quad_class(lane);
+// Sum of squares of rows > i of each of this wave's columns (i = -1: all
+// rows); lanes 0..CPW-1 return column CPW w + lane.
+template <typename R, int RPL, int CPW>
+__device__ inline R tail_sq(const R (&a)[CPW][RPL], int i) {
+  const int lane = threadIdx.x & 63;
+  R p[CPW];
 #pragma unroll
-  for (int sl = 0; sl < 4; sl++) {
-    const int c = c0 + (sl ^ hq);
+  for (int sl = 0; sl < CPW; sl++) {
+    p[sl] = 0;
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+      // With four columns per wave, i < 64: registers past the first are
+      // below row i.
+      if (CPW == 4 && r > 0) p[sl] += a[sl][r] * a[sl][r];
+      else p[sl] += (lane + 64 * r > i) ? a[sl][r] * a[sl][r] : R(0);
+    }
+  }
+  return class_by_lane_cols<R, CPW>(wave_total_cols<R, CPW>(p));
+}
+
+// Row i of this wave's columns into dst[column].  With four columns per
+// wave (n <= 64) row i < 64 sits in register 0 of lane i; the wide shape
+// reads the row out of every register that can hold it (rows < 128) and
+// selects, so that no register is indexed at run time (that would move the
+// whole matrix to scratch memory).
+template <typename R, int RPL, int CPW>
+__device__ inline void publish_row(const R (&a)[CPW][RPL], int i, R* dst) {
+  const int lane = threadIdx.x & 63, c0 = CPW * (threadIdx.x >> 6);
+  if constexpr (CPW == 4) {
+    const int cls = slot_class<CPW>(i);  // lane i's class: uniform addresses
+    if (lane == i) {
+#pragma unroll
+      for (int sl = 0; sl < CPW; sl++) dst[c0 + (sl ^ cls)] = a[sl][0];
+    }
+  } else {
+    constexpr int NR = (RegShape<CPW>::MAXN + 63) / 64 < RPL ? (RegShape<CPW>::MAXN + 63) / 64 : RPL;
+    const int li = i & 63, ri = i >> 6;
+    const int cls = slot_class<CPW>(li);
+#pragma unroll
+    for (int sl = 0; sl < CPW; sl++) {
+      R v = lane_value(a[sl][0], li);
+#pragma unroll
+      for (int r = 1; r < NR; r++) {
+        const R t = lane_value(a[sl][r], li);
+        v = (r == ri) ? t : v;
+      }
+      if (lane == 0) dst[c0 + (sl ^ cls)] = v;
+    }
+  }
+}
+
+template <typename R, int RPL, int CPW>
+__device__ inline void load_columns(const R* A, int64_t ld, int row0, int rows, int n,
+                                    R (&a)[CPW][RPL]) {
+  const int lane = threadIdx.x & 63, c0 = CPW * (threadIdx.x >> 6);
+  const int cls = slot_class<CPW>(lane);
+#pragma unroll
+  for (int sl = 0; sl < CPW; sl++) {
+    const int c = c0 + (sl ^ cls);
 #pragma unroll
     for (int r = 0; r < RPL; r++) {
       const int i = lane + 64 * r;
@@ -424,23 +504,23 @@ __device__ inline void load_columns(const R* A, int64_t ld, int row0, int rows, 
 
 // Entries of rows [row0, row0 + rows) of a box's ID matrix, evaluated into
 // the interleave (range row i on lane i % 64, register i / 64).
-template <typename R, int RPL>
+template <typename R, int RPL, int CPW>
 __device__ inline void eval_columns(const DevGeom& g, const int* rowdofs, int cnt,
                                     const double (*pxy)[kNumProxy], double pw,
                                     const DofGeo* colg, int row0, int rows, int n,
-                                    R (&a)[4][RPL]) {
-  const int lane = threadIdx.x & 63, c0 = 4 * (threadIdx.x >> 6);
-  const int hq = quad_class(lane);
+                                    R (&a)[CPW][RPL]) {
+  const int lane = threadIdx.x & 63, c0 = CPW * (threadIdx.x >> 6);
+  const int cls = slot_class<CPW>(lane);
 #pragma unroll
   for (int r = 0; r < RPL; r++) {
     const int i = lane + 64 * r;
 #pragma unroll
-    for (int sl = 0; sl < 4; sl++) a[sl][r] = R(0);
+    for (int sl = 0; sl < CPW; sl++) a[sl][r] = R(0);
     if (i < rows) {
       const IdRow rw = id_row(g, rowdofs, cnt, pxy, pw, row0 + i);
 #pragma unroll
-      for (int sl = 0; sl < 4; sl++) {
-        const int c = c0 + (sl ^ hq);
+      for (int sl = 0; sl < CPW; sl++) {
+        const int c = c0 + (sl ^ cls);
         if (c < n) a[sl][r] = static_cast<R>(id_entry(g.pde, rw, colg[c]));
       }
     }
@@ -450,29 +530,33 @@ __device__ inline void eval_columns(const DevGeom& g, const int* rowdofs, int cn
 // Row compression of a tall ID matrix: chunk q of box b (rows
 // [q ch, q ch + rows), ch = ceil(m / chunks)) is reduced by unpivoted
 // Householder QR to its n x n triangle R_q, written to rows [q n, q n + n)
-// of the box's stack (ld chunks * n, after the ID matrix in its scratch
-// slot).  The stack is Q^T A for an orthogonal Q, so its pivoted QR has
-// the same R and column order as that of A.  Grid-strided over the
-// level's jobs; one barrier per step (double-buffered reflector).
-template <typename R, int RPL>
+// of the box's stack (ld chunks * n, in its scratch slot).  The stack is
+// Q^T A for an orthogonal Q, so its pivoted QR has the same R and column
+// order as that of A.  Grid-strided over the level's jobs of this shape;
+// one barrier per step (double-buffered reflector).
+template <typename R, int CPW>
 __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevGeom g, DevTree t, DevState s,
                                                        int start, int level) {
-  constexpr int MAXM = 64 * RPL;
+  constexpr int RPL = RegShape<CPW>::RPL, MAXM = RegShape<CPW>::MAXM;
+  constexpr int MAXN = RegShape<CPW>::MAXN;
+  constexpr int kind = CPW == 4 ? kQrcpStack : kQrcpStack8;
   __shared__ R vbuf[2][MAXM];
   __shared__ R sh_tau[2];
   __shared__ int rowdofs[kMaxChunks * kRegMaxRows / 2];
   __shared__ double pxy[4][kNumProxy];
-  __shared__ DofGeo colg[kRegMaxCols];
+  __shared__ DofGeo colg[MAXN];
+  __shared__ R rdiag[MAXN];  // R_jj of step j (the register keeps alpha)
   if (!pools_ok(s)) return;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int hq = quad_class(lane);
-  const int c0 = 4 * wv;
+  const int cls = slot_class<CPW>(lane);
+  const int c0 = CPW * wv;
   const int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
   const int njobs = s.njobs[level];
   for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
     const int b = jobs[2 * job], q = jobs[2 * job + 1];
+    if (s.qkind[b] != kind) continue;
     const int n = s.act_cnt[b], m = s.m_rows[b];
-    const int nch = id_chunks(m);
+    const int nch = id_chunks(m, MAXM);
     const int ch = (m + nch - 1) / nch;
     const int r0 = q * ch, rows = min(ch, m - r0);
     R* S = reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
@@ -482,34 +566,44 @@ __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevGeom g, DevTree t, Dev
     const int cnt = id_setup(g, t, s, b, level, m, rowdofs, pxy);
     for (int c = tid; c < n; c += kXThreads) colg[c] = dof_geo(g, act[c]);
     __syncthreads();
-    R a[4][RPL];
-    eval_columns(g, rowdofs, cnt, pxy, proxy_weight(t, b), colg, r0, rows, n, a);
+    R a[CPW][RPL];
+    eval_columns<R, RPL, CPW>(g, rowdofs, cnt, pxy, proxy_weight(t, b), colg, r0, rows, n, a);
     const int steps = min(rows, n);
     int dmask = 0;
     for (int j = 0; j < steps; j++) {
-      if (wv == (j >> 2)) {
-        const int psl = (j & 3) ^ hq;
+      if (wv == j / CPW) {
+        const int pcls = j % CPW, psl = pcls ^ cls;
         R x[RPL];
-        pick_slot(a, psl, x);
-        R beta, tau;
-        householder(x, j, vbuf[j & 1], &beta, &tau);
-        dmask |= 1 << (j & 3);
-        if (lane == j) {
+        pick_slot<R, RPL, CPW>(a, psl, x);
+        const R alpha = row_value<R, RPL>(x, j);
+        R ss = 0;
 #pragma unroll
-          for (int sl = 0; sl < 4; sl++) a[sl][0] = (sl == psl) ? beta : a[sl][0];
+        for (int r = 0; r < RPL; r++) ss += (lane + 64 * r > j) ? x[r] * x[r] : R(0);
+        R beta, tau, sc;
+        householder_scalars(alpha, wave_total(ss), &beta, &tau, &sc);
+        write_reflector<R, RPL>(x, j, sc, vbuf[j & 1]);
+        dmask |= 1 << pcls;
+        if (lane == 0) {
+          sh_tau[j & 1] = tau;
+          rdiag[j] = beta;
         }
-        if (lane == 0) sh_tau[j & 1] = tau;
       }
       __syncthreads();
-      if (active) apply_reflector(a, vbuf[j & 1], sh_tau[j & 1], ((dmask >> hq) & 1) == 0);
+      if (active)
+        apply_reflector<R, RPL, CPW>(a, vbuf[j & 1], sh_tau[j & 1], ((dmask >> cls) & 1) == 0);
     }
     // Column c was reduced at step c: rows below its diagonal are zero.
     if (active) {
 #pragma unroll
-      for (int sl = 0; sl < 4; sl++) {
-        const int c = c0 + (sl ^ hq);
-        if (c < n && lane < n)
-          S[(int64_t)q * n + lane + lds * c] = lane > c ? R(0) : a[sl][0];
+      for (int sl = 0; sl < CPW; sl++) {
+        const int c = c0 + (sl ^ cls);
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+          const int i = lane + 64 * r;
+          if (c < n && i < n)
+            S[(int64_t)q * n + i + lds * c] =
+                i > c ? R(0) : (i == c && c < steps ? rdiag[c] : a[sl][r]);
+        }
       }
     }
     __syncthreads();
@@ -519,62 +613,59 @@ __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevGeom g, DevTree t, Dev
 // Interpolative decomposition by column-pivoted Householder QR, stopped at
 // the first |R_jj| < tol |R_00| (ie_mat.cpp:548-588).  The pivot rule,
 // Householder form and column-norm downdating follow LAPACK dgeqp3/dlaqp2.
-// Lanes 0..3 of each wave keep the norms, the logical position and the
-// pivot key of the wave's four columns; the pivot is the largest key
-// (pivot_key), found by one wave maximum.
-template <typename R, int RPL>
-__device__ void qrcp_id_box(const DevGeom& g, const DevTree& t, const DevState& s,
+// Lanes 0..CPW-1 of each wave keep the downdated norms, logical position
+// and pivot key of the wave's columns; the pivot is the largest key
+// (pivot_key), found by one wave maximum; the wave owning the pivot column
+// forms the reflector and the stop decision.
+template <typename R, int CPW>
+__device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, const DevState& s,
                             int start, int level, int b, double tol_d) {
-  constexpr int MAXM = 64 * RPL;
+  constexpr int RPL = RegShape<CPW>::RPL, MAXM = RegShape<CPW>::MAXM;
+  constexpr int MAXN = RegShape<CPW>::MAXN, KH = MAXN / 64;
   __shared__ R vbuf[MAXM];
-  __shared__ int rowdofs[kRegMaxRows / 2];
-  __shared__ double pxy[4][kNumProxy];
-  __shared__ DofGeo colg[kRegMaxCols];
-  __shared__ R rowj[kRegMaxCols];
-  __shared__ double keys[kRegMaxCols];
-  __shared__ R rbuf[kRegMaxCols * kRegMaxCols];
-  __shared__ int colmap[kRegMaxCols];
+  __shared__ R rowj[MAXN];    // row j after step j (downdating)
+  __shared__ R rdiag[MAXN];   // R_jj of step j (the register keeps alpha)
   __shared__ R sh_tau, sh_thresh;
   __shared__ int sh_stop;
+  __shared__ double keys[MAXN];
+  __shared__ R rbuf[MAXN * MAXN];
+  __shared__ int colmap[MAXN];
+  __shared__ int rowdofs[MAXM / 2];
+  __shared__ double pxy[4][kNumProxy];
+  __shared__ DofGeo colg[MAXN];
 
   const int kind = s.qkind[b];
   const int n = s.act_cnt[b];
   int m = s.m_rows[b];
   const R tol = static_cast<R>(tol_d);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int hq = quad_class(lane);
-  const int c0 = 4 * wv;
+  const int cls = slot_class<CPW>(lane);
+  const int c0 = CPW * wv;
   const bool active = c0 < n;
   const int* act = s.act_list + (int64_t)b * s.ncap;
 
-  R a[4][RPL];
-  if (kind == kQrcpReg) {
+  R a[CPW][RPL];
+  if (kind == kQrcpReg || kind == kQrcpReg8) {
     const int cnt = id_setup(g, t, s, b, level, m, rowdofs, pxy);
     for (int c = tid; c < n; c += kXThreads) colg[c] = dof_geo(g, act[c]);
     __syncthreads();
-    eval_columns(g, rowdofs, cnt, pxy, proxy_weight(t, b), colg, 0, m, n, a);
+    eval_columns<R, RPL, CPW>(g, rowdofs, cnt, pxy, proxy_weight(t, b), colg, 0, m, n, a);
   } else {
     // The stacked triangles of the row compression.
-    m = id_chunks(m) * n;
-    load_columns(reinterpret_cast<const R*>(s.id_pool) + s.id_off[b], m, 0, m, n, a);
+    m = id_chunks(m, MAXM) * n;
+    load_columns<R, RPL, CPW>(reinterpret_cast<const R*>(s.id_pool) + s.id_off[b], m, 0, m,
+                              n, a);
   }
-  // Lanes 0..3: column c0 + lane.
-  const int myc = c0 + (lane & 3);
+  // Lanes 0..CPW-1: column c0 + lane.
+  const int myc = c0 + (lane % CPW);
   R vn1 = 0, vn2 = 0;
   int pos = myc;
-  bool avail = lane < 4 && myc < n;
+  bool avail = lane < CPW && myc < n;
   if (active) {
-    R p[4];
-#pragma unroll
-    for (int sl = 0; sl < 4; sl++) {
-      p[sl] = 0;
-#pragma unroll
-      for (int r = 0; r < RPL; r++) p[sl] += a[sl][r] * a[sl][r];
-    }
-    vn1 = sqrt(class_by_lane(wave_total4(p)));
+    vn1 = sqrt(tail_sq<R, RPL, CPW>(a, -1));
     vn2 = vn1;
   }
-  if (lane < 4) keys[myc] = avail ? pivot_key(vn1, pos) : -1.0;
+  if (lane < CPW) keys[myc] = avail ? pivot_key(vn1, pos) : -1.0;
   __syncthreads();
 
   const int kmax = min(m, n);
@@ -582,15 +673,26 @@ __device__ void qrcp_id_box(const DevGeom& g, const DevTree& t, const DevState& 
   R thresh = 0;
   int dmask = 0;  // classes of this wave already pivoted
   for (int j = 0; j < kmax; j++) {
-    const double kv = keys[lane];
-    const double best = wave_max(kv);
+    double kv[KH];
+    double best = -1.0;
+#pragma unroll
+    for (int h = 0; h < KH; h++) {
+      kv[h] = keys[lane + 64 * h];
+      best = fmax(best, kv[h]);
+    }
+    best = wave_max(best);
     if (!(best >= 0)) {  // every wave sees the same keys
       k = j;
       break;
     }
-    const int p = __ffsll((unsigned long long)__ballot(kv == best)) - 1;
+    int p = -1;
+#pragma unroll
+    for (int h = KH - 1; h >= 0; h--) {
+      const uint64_t hit = __ballot(kv[h] == best);
+      if (hit) p = 64 * h + __ffsll((unsigned long long)hit) - 1;
+    }
     const int q = pivot_key_pos(best);
-    if (lane < 4) {
+    if (lane < CPW) {
       if (myc == p) {
         pos = j;
         avail = false;
@@ -598,24 +700,28 @@ __device__ void qrcp_id_box(const DevGeom& g, const DevTree& t, const DevState& 
         pos = q;
       }
     }
-    if (wv == (p >> 2)) {
-      const int psl = (p & 3) ^ hq;
+    if (wv == p / CPW) {
+      const int psl = (p % CPW) ^ cls;
       R x[RPL];
-      pick_slot(a, psl, x);
-      R beta, tau;
-      householder(x, j, vbuf, &beta, &tau);
+      pick_slot<R, RPL, CPW>(a, psl, x);
+      // With four columns per wave j < 64: row j sits in register 0.
+      const R alpha = CPW == 4 ? lane_value(x[0], j) : row_value<R, RPL>(x, j);
+      R ss = 0;
+#pragma unroll
+      for (int r = 0; r < RPL; r++) {
+        if (CPW == 4 && r > 0) ss += x[r] * x[r];
+        else ss += (lane + 64 * r > j) ? x[r] * x[r] : R(0);
+      }
+      R beta, tau, sc;
+      householder_scalars(alpha, wave_total(ss), &beta, &tau, &sc);
+      write_reflector<R, RPL>(x, j, sc, vbuf);
       const R th = (j == 0) ? fabs(tol * beta) : thresh;
       const int stop = (j >= 1 && fabs(beta) < th) ? 1 : 0;
-      if (!stop) {
-        dmask |= 1 << (p & 3);
-        if (lane == j) {
-#pragma unroll
-          for (int sl = 0; sl < 4; sl++) a[sl][0] = (sl == psl) ? beta : a[sl][0];
-        }
-      }
+      if (!stop) dmask |= 1 << (p % CPW);
       if (lane == 0) {
         sh_tau = tau;
         sh_stop = stop;
+        rdiag[j] = beta;
         if (j == 0) sh_thresh = th;
       }
     }
@@ -626,11 +732,8 @@ __device__ void qrcp_id_box(const DevGeom& g, const DevTree& t, const DevState& 
     }
     if (j == 0) thresh = sh_thresh;
     if (active) {
-      apply_reflector(a, vbuf, sh_tau, ((dmask >> hq) & 1) == 0);
-      if (lane == j) {
-#pragma unroll
-        for (int sl = 0; sl < 4; sl++) rowj[c0 + (sl ^ hq)] = a[sl][0];
-      }
+      apply_reflector<R, RPL, CPW>(a, vbuf, sh_tau, ((dmask >> cls) & 1) == 0);
+      publish_row<R, RPL, CPW>(a, j, rowj);
       wave_sync();
       bool need = false;
       if (avail && vn1 != 0) {
@@ -641,35 +744,34 @@ __device__ void qrcp_id_box(const DevGeom& g, const DevTree& t, const DevState& 
         if (temp * rat * rat <= Eps<R>::tol3z) need = true;
         else vn1 = vn1 * sqrt(temp);
       }
-      if (__ballot(need)) {
-        R p2[4];
-#pragma unroll
-        for (int sl = 0; sl < 4; sl++) {
-          p2[sl] = lane > j ? a[sl][0] * a[sl][0] : R(0);
-#pragma unroll
-          for (int r = 1; r < RPL; r++) p2[sl] += a[sl][r] * a[sl][r];
-        }
-        const R t = class_by_lane(wave_total4(p2));
+      if (__ballot(need)) {  // dlaqp2 recomputes the norm of rows j+1..
+        const R t2 = tail_sq<R, RPL, CPW>(a, j);
         if (need) {
-          const R nv = (j < m - 1) ? sqrt(t) : R(0);
+          const R nv = (j < m - 1) ? sqrt(t2) : R(0);
           vn1 = nv;
           vn2 = nv;
         }
       }
-      if (lane < 4) keys[myc] = avail ? pivot_key(vn1, pos) : -1.0;
+      if (lane < CPW) keys[myc] = avail ? pivot_key(vn1, pos) : -1.0;
     }
     __syncthreads();
   }
 
   // Logical order and rows 0..k-1 of R (physical column order, ld k).
-  if (lane < 4 && myc < n) colmap[pos] = myc;
+  if (lane < CPW && myc < n) colmap[pos] = myc;
   if (active) {
 #pragma unroll
-    for (int sl = 0; sl < 4; sl++) {
-      const int c = c0 + (sl ^ hq);
-      if (c < n && lane < k) rbuf[lane + k * c] = a[sl][0];
+    for (int sl = 0; sl < CPW; sl++) {
+      const int c = c0 + (sl ^ cls);
+#pragma unroll
+      for (int r = 0; r < RPL; r++) {
+        const int i = lane + 64 * r;
+        if (c < n && i < k) rbuf[i + k * c] = a[sl][r];
+      }
     }
   }
+  __syncthreads();
+  if (tid < k) rbuf[tid + k * colmap[tid]] = rdiag[tid];
   __syncthreads();
 
   int* out = s.out_list + (int64_t)b * s.ncap;
@@ -681,19 +783,32 @@ __device__ void qrcp_id_box(const DevGeom& g, const DevTree& t, const DevState& 
     }
     return;
   }
-  // T = R11^{-1} R12: one wave per column, lane l holds t_l (k <= 64).
+  // T = R11^{-1} R12: one wave per column, lane l holds rows l, l + 64, ...
   const int r = n - k;
   R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
   for (int cp = wv; cp < r; cp += kXThreads / 64) {
     const int pcol = colmap[k + cp];
-    R t = (lane < k) ? rbuf[lane + k * pcol] : R(0);
+    R tv[KH];
+#pragma unroll
+    for (int h = 0; h < KH; h++) {
+      const int i = lane + 64 * h;
+      tv[h] = (i < k) ? rbuf[i + k * pcol] : R(0);
+    }
     for (int i = k - 1; i >= 0; i--) {
       const int ci = colmap[i];
-      const R ti = lane_value(t, i) / rbuf[i + k * ci];
-      if (lane == i) t = ti;
-      if (lane < i) t -= rbuf[lane + k * ci] * ti;
+      const R ti = row_value<R, KH>(tv, i) / rbuf[i + k * ci];
+#pragma unroll
+      for (int h = 0; h < KH; h++) {
+        const int row = lane + 64 * h;
+        if (row == i) tv[h] = ti;
+        if (row < i) tv[h] -= rbuf[row + k * ci] * ti;
+      }
     }
-    if (lane < k) F[lane + (int64_t)k * cp] = t;
+#pragma unroll
+    for (int h = 0; h < KH; h++) {
+      const int row = lane + 64 * h;
+      if (row < k) F[row + (int64_t)k * cp] = tv[h];
+    }
   }
   int* red = s.red_list + (int64_t)b * s.ncap;
   int* pm = s.perm + (int64_t)b * s.ncap;
@@ -709,21 +824,31 @@ __device__ void qrcp_id_box(const DevGeom& g, const DevTree& t, const DevState& 
   }
 }
 
-// One workgroup per entry of the level's register-QRCP list (a loop over
-// boxes would keep more values live across the step loop than the 128
-// registers of a 1024-thread workgroup hold).
-template <typename R, int RPL>
+// One workgroup per entry of the level's register-QRCP list of this shape
+// (a loop over boxes would keep more values live across the step loop than
+// the 128 registers of a 1024-thread workgroup hold).
+template <typename R, int CPW>
 __global__ __launch_bounds__(kXThreads) void k_qrcp_id(DevGeom g, DevTree t, DevState s,
                                                        int start, int level, double tol_d) {
-  if (!pools_ok(s) || (int)blockIdx.x >= list_count(s, level, kLstReg)) return;
-  qrcp_id_box<R, RPL>(g, t, s, start, level, list_at(s, start, kLstReg, blockIdx.x), tol_d);
+  constexpr int L = CPW == 4 ? kLstReg : kLstReg8;
+  if (!pools_ok(s)) return;
+  const int nl = list_count(s, level, L);
+  if constexpr (CPW == 4) {
+    if ((int)blockIdx.x < nl)
+      qrcp_id_box<R, CPW>(g, t, s, start, level, list_at(s, start, L, blockIdx.x), tol_d);
+  } else {  // few boxes: a small grid strides over them
+    for (int li = blockIdx.x; li < nl; li += gridDim.x) {
+      qrcp_id_box<R, CPW>(g, t, s, start, level, list_at(s, start, L, li), tol_d);
+      __syncthreads();
+    }
+  }
 }
 
 // General form (any box the register form does not take): the matrix lives
 // in LDS when it fits, else in the box's scratch slot; each wave owns a
 // strided set of columns.
 template <typename R, int NT>
-__device__ void qrcp_general_box(const DevState& s, int start, int level, int b, double tol_d) {
+__device__ __forceinline__ void qrcp_general_box(const DevState& s, int start, int level, int b, double tol_d) {
   constexpr int NW = NT / 64;
   constexpr int kLdsElems = 131072 / sizeof(R);
   __shared__ R smat[kLdsElems];
@@ -1602,8 +1727,10 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       k_assemble<R><<<min(count, 64), 256, 0, st>>>(g, t, s, start, level);
       break;
     case kStageQrcp:
-      k_rowcomp<R, kXRows><<<min(2 * count, 256), kXThreads, 0, st>>>(g, t, s, start, level);
-      k_qrcp_id<R, kXRows><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
+      k_rowcomp<R, 4><<<min(2 * count, 256), kXThreads, 0, st>>>(g, t, s, start, level);
+      k_rowcomp<R, 8><<<min(2 * count, 64), kXThreads, 0, st>>>(g, t, s, start, level);
+      k_qrcp_id<R, 4><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
+      k_qrcp_id<R, 8><<<min(count, 64), kXThreads, 0, st>>>(g, t, s, start, level, tol);
       k_qrcp<R, 512><<<min(count, 64), 512, 0, st>>>(s, start, level, tol);
       break;
     case kStageDecouple:
