@@ -397,22 +397,25 @@ __device__ inline R class_by_lane_cols(R t) {
   else return class_by_lane8(t);
 }
 
-// Pivot key: a column norm v >= 0 with its logical position in the low
-// seven bits, ordered as a non-negative double.  The largest key is the
-// largest norm, ties (equal to 2^-45 relative in FP64) going to the
-// smaller position, as LAPACK's first-maximum rule.  Unavailable columns
+// Pivot key: a magnitude v >= 0 with its position in the low BITS bits,
+// ordered as a non-negative double.  The largest key is the largest
+// magnitude, ties (equal to 2^(BITS-52) relative in FP64) going to the
+// smaller position, as LAPACK's first-maximum rule.  Unavailable entries
 // carry -1.
-template <typename R>
+template <typename R, int BITS = 7>
 __device__ inline double pivot_key(R v, int pos) {
+  constexpr uint64_t kMask = (uint64_t(1) << BITS) - 1;
   uint64_t b;
   if constexpr (sizeof(R) == 8)
-    b = (uint64_t)__double_as_longlong(v) & ~uint64_t(127);
+    b = (uint64_t)__double_as_longlong(v) & ~kMask;
   else
     b = (uint64_t)__float_as_uint(v) << 32;
-  return __longlong_as_double((long long)(b | (uint64_t)(127 - pos)));
+  return __longlong_as_double((long long)(b | (kMask - (uint64_t)pos)));
 }
+template <int BITS = 7>
 __device__ inline int pivot_key_pos(double key) {
-  return 127 - (int)(__double_as_longlong(key) & 127);
+  constexpr int kMask = (1 << BITS) - 1;
+  return kMask - (int)(__double_as_longlong(key) & kMask);
 }
 
 // DPP move that leaves lanes outside the row mask with their own value.

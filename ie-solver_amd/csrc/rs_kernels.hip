@@ -771,7 +771,10 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
     }
   }
   __syncthreads();
-  if (tid < k) rbuf[tid + k * colmap[tid]] = rdiag[tid];
+  if (tid < k) {
+    rbuf[tid + k * colmap[tid]] = rdiag[tid];
+    rdiag[tid] = R(1) / rdiag[tid];  // reciprocal diagonal for the solve below
+  }
   __syncthreads();
 
   int* out = s.out_list + (int64_t)b * s.ncap;
@@ -796,7 +799,7 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
     }
     for (int i = k - 1; i >= 0; i--) {
       const int ci = colmap[i];
-      const R ti = row_value<R, KH>(tv, i) / rbuf[i + k * ci];
+      const R ti = row_value<R, KH>(tv, i) * rdiag[i];
 #pragma unroll
       for (int h = 0; h < KH; h++) {
         const int row = lane + 64 * h;
@@ -1085,6 +1088,107 @@ __device__ int gj_invert(R* a, R* b, int n, int* prow) {
   return info;
 }
 
+// Gauss-Jordan inverse of X (r x r, r <= 64, in LDS, ld r) with the
+// matrix in the registers of 256 threads on a 16 x 16 grid, thread
+// (tr, tc) holding rows tr + 16 u and columns tc + 16 v.  Per step the
+// pivot column (kept in LDS by its owners) gives the pivot row by one wave
+// maximum over packed keys (same rule as gj_invert), the pivot row's
+// owners publish it, and every thread updates its entries with one FMA
+// each.  M (as gj_invert) is written back to X.  colbuf holds 2 x 64 and
+// rowbuf 64 entries.  Returns 0, or j + 1 for the first zero pivot.
+template <typename R>
+__device__ int gj_invert_regs(R* X, int r, int* prow, R* colbuf, R* rowbuf) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int tr = tid & 15, tc = tid >> 4;
+  R a[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = tr + 16 * u, c = tc + 16 * v;
+      a[u][v] = (i < r && c < r) ? X[i + r * c] : R(0);
+    }
+  if (tc == 0) {
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (tr + 16 * u < r) colbuf[tr + 16 * u] = a[u][0];
+  }
+  __syncthreads();
+  bool used = false;  // row `lane` already pivoted (every wave keeps a copy)
+  int info = 0;
+  for (int j = 0; j < r; j++) {
+    const R* col = colbuf + 64 * (j & 1);
+    R* coln = colbuf + 64 * ((j + 1) & 1);
+    const bool cand = lane < r && !used;
+    const R mag = cand ? fabs(col[lane]) : R(0);
+    const double best = wave_max((cand && mag == mag) ? pivot_key(mag, lane) : -1.0);
+    const int p = best >= 0 ? pivot_key_pos(best)
+                            : __ffsll((unsigned long long)__ballot(cand)) - 1;  // NaN column
+    if (lane == p) used = true;
+    if (tid == 0) prow[j] = p;
+    const R piv = col[p];
+    if (piv == R(0) && info == 0) info = j + 1;
+    const R inv = R(1) / piv;
+    const int up = p >> 4;
+    if (tr == (p & 15)) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (u != up) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int c = tc + 16 * v;
+          if (c < r) rowbuf[c] = (c == j) ? R(1) : a[u][v];
+        }
+      }
+    }
+    __syncthreads();
+    R cv[4], rv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) cv[u] = (tr + 16 * u < r) ? col[tr + 16 * u] : R(0);
+#pragma unroll
+    for (int v = 0; v < 4; v++) rv[v] = (tc + 16 * v < r) ? rowbuf[tc + 16 * v] * inv : R(0);
+    if (tc == (j & 15)) {  // column j restarts from zero
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        if (v != (j >> 4)) continue;
+#pragma unroll
+        for (int u = 0; u < 4; u++) a[u][v] = R(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) a[u][v] -= cv[u] * rv[v];
+    if (tr == (p & 15)) {  // the pivot row becomes the scaled pivot row
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (u != up) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) a[u][v] = rv[v];
+      }
+    }
+    if (j + 1 < r && tc == ((j + 1) & 15)) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        if (v != ((j + 1) >> 4)) continue;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (tr + 16 * u < r) coln[tr + 16 * u] = a[u][v];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = tr + 16 * u, c = tc + 16 * v;
+      if (i < r && c < r) X[i + r * c] = a[u][v];
+    }
+  __syncthreads();
+  return info;
+}
+
 // Inverts X (r x r) in place through the scratch buffer W (>= r^2): the
 // steps start in W when r is odd, so M ends in X.
 template <typename R, int NT>
@@ -1278,7 +1382,7 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
   }
   __syncthreads();
   decouple_blocks<R, NT>(P, T, n, k, Krs, Ksr, X);
-  const int info = gj_invert_into<R, NT>(X, P, r, prow);
+  const int info = gj_invert_regs<R>(X, r, prow, P, P + 128);  // P is dead
   if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
   decouple_tail<R, NT>(X, prow, Krs, Ksr, P, r, k, F, fl, upd, sperm);
   if (tid == 0) s.upd_kind[b] = 1;
@@ -1462,19 +1566,26 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
   for (int j = 0; j < n; j++) {
     const R* col = colbuf[j & 1];
     R* coln = colbuf[(j + 1) & 1];
-    R best = -1;
-    int bi = 0x7fffffff;
+    double key = -1.0;
+    uint32_t cand = 0;
     for (int uu = 0; lane + 64 * uu < n; uu++) {
       if ((used >> uu) & 1) continue;
       const int i = lane + 64 * uu;
+      cand |= 1u << uu;
       const R v = fabs(col[i]);
-      const R key = (v == v) ? v : R(-0.5);
-      if (key > best) {
-        best = key;
-        bi = i;
+      if (v == v) key = fmax(key, pivot_key<R, 8>(v, i));
+    }
+    const double best = wave_max(key);
+    int p;
+    if (best >= 0) {
+      p = pivot_key_pos<8>(best);
+    } else {  // NaN column: the first row not yet pivoted
+      p = 0x7fffffff;
+      for (int uu = 0; uu < 4; uu++) {
+        const uint64_t m = __ballot((cand >> uu) & 1);
+        if (m && p == 0x7fffffff) p = 64 * uu + __ffsll((unsigned long long)m) - 1;
       }
     }
-    const int p = wave_argmax<R>(best, bi);
     if ((p & 63) == lane) used |= 1u << (p >> 6);
     if (tid == 0) prow[j] = p;
     const R piv = col[p];
