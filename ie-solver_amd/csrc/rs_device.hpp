@@ -216,6 +216,12 @@ __device__ inline int dpp_i32(int v) {
   return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
 }
 
+// Index of the calling wave in its workgroup, as a wave-uniform (scalar)
+// value: the compiler cannot tell that threadIdx.x / 64 is uniform.
+__device__ inline int wave_id() {
+  return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
 // Value of `v` in lane `lane` (wave-uniform index), for every lane.
 __device__ inline double lane_value(double v, int lane) {
   const long long b = __double_as_longlong(v);
@@ -460,13 +466,21 @@ __device__ inline DofGeo dof_geo(const DevGeom& g, int dof) {
   return d;
 }
 
+// 1 / d from the hardware reciprocal refined by one Newton step (within an
+// ulp or two of the rounded quotient; the kernel entries take it in place
+// of a full IEEE division).
+__device__ inline double recip(double d) {
+  const double y = __builtin_amdgcn_rcp(d);
+  return fma(y, fma(-d, y, 1.0), y);
+}
+
 // Laplace double layer, ie_solver/kernel/kernel.cpp:123-165.  The
 // coincident-point (diagonal) value applies only between boundary DOFs.
 __device__ inline double laplace_entry(const DofGeo& t, const DofGeo& s, bool diag_ok) {
   const double scale = 1.0 / (2 * M_PI);
   if (diag_ok && t.x == s.x && t.y == s.y) return 0.5 + 0.5 * s.k * s.w * scale;
   const double r0 = t.x - s.x, r1 = t.y - s.y;
-  return -s.w * scale * (r0 * s.n1 + r1 * s.n2) / (r0 * r0 + r1 * r1);
+  return -s.w * scale * (r0 * s.n1 + r1 * s.n2) * recip(r0 * r0 + r1 * r1);
 }
 
 // Stokes velocity double layer (2 DOFs per point, DOF d -> point d/2,
@@ -484,7 +498,8 @@ __device__ inline double stokes_entry(const DofGeo& t, const DofGeo& s, bool dia
   }
   const double r0 = t.x - s.x, r1 = t.y - s.y;
   const double d2 = r0 * r0 + r1 * r1;
-  const double pot = s.w * scale * (r0 * s.n1 + r1 * s.n2) / (d2 * d2);
+  const double i2 = recip(d2);
+  const double pot = s.w * scale * (r0 * s.n1 + r1 * s.n2) * (i2 * i2);
   const double ra = a ? r1 : r0, rb = b ? r1 : r0;
   return pot * ra * rb + s.w * tna * snb;
 }

@@ -462,7 +462,7 @@ __device__ inline R tail_sq(const R (&a)[CPW][RPL], int i) {
 // whole matrix to scratch memory).
 template <typename R, int RPL, int CPW>
 __device__ inline void publish_row(const R (&a)[CPW][RPL], int i, R* dst) {
-  const int lane = threadIdx.x & 63, c0 = CPW * (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, c0 = CPW * wave_id();
   if constexpr (CPW == 4) {
     const int cls = slot_class<CPW>(i);  // lane i's class: uniform addresses
     if (lane == i) {
@@ -489,7 +489,7 @@ __device__ inline void publish_row(const R (&a)[CPW][RPL], int i, R* dst) {
 template <typename R, int RPL, int CPW>
 __device__ inline void load_columns(const R* A, int64_t ld, int row0, int rows, int n,
                                     R (&a)[CPW][RPL]) {
-  const int lane = threadIdx.x & 63, c0 = CPW * (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, c0 = CPW * wave_id();
   const int cls = slot_class<CPW>(lane);
 #pragma unroll
   for (int sl = 0; sl < CPW; sl++) {
@@ -503,26 +503,38 @@ __device__ inline void load_columns(const R* A, int64_t ld, int row0, int rows, 
 }
 
 // Entries of rows [row0, row0 + rows) of a box's ID matrix, evaluated into
-// the interleave (range row i on lane i % 64, register i / 64).
+// the interleave (range row i on lane i % 64, register i / 64).  The wave's
+// columns are read once as wave-uniform values (scalar loads); each row's
+// entries are formed in natural column order and then placed in the
+// interleaved slots.
 template <typename R, int RPL, int CPW>
-__device__ inline void eval_columns(const DevGeom& g, const int* rowdofs, int cnt,
-                                    const double (*pxy)[kNumProxy], double pw,
-                                    const DofGeo* colg, int row0, int rows, int n,
-                                    R (&a)[CPW][RPL]) {
-  const int lane = threadIdx.x & 63, c0 = CPW * (threadIdx.x >> 6);
+__device__ inline void eval_columns(const DevGeom& g, const int* act, const int* rowdofs,
+                                    int cnt, const double (*pxy)[kNumProxy], double pw,
+                                    int row0, int rows, int n, R (&a)[CPW][RPL]) {
+  const int lane = threadIdx.x & 63, c0 = CPW * wave_id();
   const int cls = slot_class<CPW>(lane);
+  DofGeo cg[CPW];
+#pragma unroll
+  for (int q = 0; q < CPW; q++) cg[q] = dof_geo(g, act[min(c0 + q, n - 1)]);
 #pragma unroll
   for (int r = 0; r < RPL; r++) {
     const int i = lane + 64 * r;
+    R nat[CPW];
 #pragma unroll
-    for (int sl = 0; sl < CPW; sl++) a[sl][r] = R(0);
+    for (int q = 0; q < CPW; q++) nat[q] = R(0);
     if (i < rows) {
       const IdRow rw = id_row(g, rowdofs, cnt, pxy, pw, row0 + i);
 #pragma unroll
-      for (int sl = 0; sl < CPW; sl++) {
-        const int c = c0 + (sl ^ cls);
-        if (c < n) a[sl][r] = static_cast<R>(id_entry(g.pde, rw, colg[c]));
-      }
+      for (int q = 0; q < CPW; q++)
+        if (c0 + q < n) nat[q] = static_cast<R>(id_entry(g.pde, rw, cg[q]));
+    }
+#pragma unroll
+    for (int sl = 0; sl < CPW; sl++) {
+      const int want = sl ^ cls;
+      R v = nat[0];
+#pragma unroll
+      for (int q = 1; q < CPW; q++) v = (want == q) ? nat[q] : v;
+      a[sl][r] = v;
     }
   }
 }
@@ -544,10 +556,9 @@ __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevGeom g, DevTree t, Dev
   __shared__ R sh_tau[2];
   __shared__ int rowdofs[kMaxChunks * kRegMaxRows / 2];
   __shared__ double pxy[4][kNumProxy];
-  __shared__ DofGeo colg[MAXN];
   __shared__ R rdiag[MAXN];  // R_jj of step j (the register keeps alpha)
   if (!pools_ok(s)) return;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int cls = slot_class<CPW>(lane);
   const int c0 = CPW * wv;
   const int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
@@ -564,10 +575,9 @@ __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevGeom g, DevTree t, Dev
     const bool active = c0 < n;
     const int* act = s.act_list + (int64_t)b * s.ncap;
     const int cnt = id_setup(g, t, s, b, level, m, rowdofs, pxy);
-    for (int c = tid; c < n; c += kXThreads) colg[c] = dof_geo(g, act[c]);
     __syncthreads();
     R a[CPW][RPL];
-    eval_columns<R, RPL, CPW>(g, rowdofs, cnt, pxy, proxy_weight(t, b), colg, r0, rows, n, a);
+    eval_columns<R, RPL, CPW>(g, act, rowdofs, cnt, pxy, proxy_weight(t, b), r0, rows, n, a);
     const int steps = min(rows, n);
     int dmask = 0;
     for (int j = 0; j < steps; j++) {
@@ -632,13 +642,12 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
   __shared__ int colmap[MAXN];
   __shared__ int rowdofs[MAXM / 2];
   __shared__ double pxy[4][kNumProxy];
-  __shared__ DofGeo colg[MAXN];
 
   const int kind = s.qkind[b];
   const int n = s.act_cnt[b];
   int m = s.m_rows[b];
   const R tol = static_cast<R>(tol_d);
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int cls = slot_class<CPW>(lane);
   const int c0 = CPW * wv;
   const bool active = c0 < n;
@@ -647,9 +656,8 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
   R a[CPW][RPL];
   if (kind == kQrcpReg || kind == kQrcpReg8) {
     const int cnt = id_setup(g, t, s, b, level, m, rowdofs, pxy);
-    for (int c = tid; c < n; c += kXThreads) colg[c] = dof_geo(g, act[c]);
     __syncthreads();
-    eval_columns<R, RPL, CPW>(g, rowdofs, cnt, pxy, proxy_weight(t, b), colg, 0, m, n, a);
+    eval_columns<R, RPL, CPW>(g, act, rowdofs, cnt, pxy, proxy_weight(t, b), 0, m, n, a);
   } else {
     // The stacked triangles of the row compression.
     m = id_chunks(m, MAXM) * n;
@@ -1334,18 +1342,21 @@ __device__ void decouple_tail(const R* M, const int* prow, const R* Krs, const R
   }
 }
 
-// Compressed boxes with |active| <= 64, every operand in LDS.
+// Compressed boxes with |active| <= 64, every operand in LDS.  Only the
+// skeleton rows of K_BN are stored (K(s,s), K(s,r)); the redundant rows
+// are evaluated where K_rs and X_rr need them, which keeps the buffer at
+// n^2 + 2 k r <= 6144 entries and three boxes per CU.
 template <typename R>
 __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
                                                         DevState s, int start) {
   constexpr int NT = 256;
-  constexpr int kBuf = 8704;  // >= n^2 + r^2 + 3 k r for n <= 64
+  constexpr int kBuf = 6144;  // >= n^2 + 2 k r for n <= 64
   __shared__ R buf[kBuf];
+  __shared__ DofGeo geo[kSmallBox];
   __shared__ signed char pch[kSmallBox];
   __shared__ short ploc[kSmallBox];
   __shared__ short sperm[kSmallBox];
   __shared__ int prow[kSmallBox];
-  __shared__ DofGeo geo[kSmallBox];
   __shared__ const R* chS[4];
   __shared__ int chN[4];
 
@@ -1370,21 +1381,48 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
   const int r = n - k;
   const FacLayout fl = FacLayout::make(k, r);
   R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
-  R* P = buf;
-  R* X = P + n * n;
-  R* Krs = X + r * r;
-  R* Ksr = Krs + r * k;
-  R* T = Ksr + k * r;
-  for (int i = tid; i < k * r; i += NT) T[i] = F[fl.T + i];
-  for (int idx = tid; idx < n * n; idx += NT) {
-    const int i = idx % n, c = idx / n;
-    P[idx] = static_cast<R>(kernel_entry(g.pde, geo[i], geo[c])) - upd(sperm[i], sperm[c]);
+  // K_BN entry at permuted positions (skeleton first).
+  auto kbn = [&](int i, int c) -> R {
+    return static_cast<R>(kernel_entry(g.pde, geo[i], geo[c])) - upd(sperm[i], sperm[c]);
+  };
+  R* Pss = buf;          // k x k
+  R* Psr = Pss + k * k;  // k x r
+  R* T = Psr + k * r;    // k x r
+  R* X = T + k * r;      // r x r
+  R* Krs = X + r * r;    // r x k
+  R* Ksr = Krs + r * k;  // k x r
+  for (int idx = tid; idx < k * r; idx += NT) T[idx] = F[fl.T + idx];
+  for (int idx = tid; idx < k * n; idx += NT) {
+    const int a = idx % k, c = idx / k;  // Pss and Psr are contiguous (ld k)
+    Pss[idx] = kbn(a, c);
   }
   __syncthreads();
-  decouple_blocks<R, NT>(P, T, n, k, Krs, Ksr, X);
-  const int info = gj_invert_regs<R>(X, r, prow, P, P + 128);  // P is dead
+  for (int idx = tid; idx < r * k; idx += NT) {  // K_rs = K(r,s) - T^T K(s,s)
+    const int i = idx % r, a = idx / r;
+    R acc = kbn(k + i, a);
+    for (int l = 0; l < k; l++) acc -= T[l + k * i] * Pss[l + k * a];
+    Krs[idx] = acc;
+  }
+  for (int idx = tid; idx < k * r; idx += NT) {  // K_sr = K(s,r) - K(s,s) T
+    const int a = idx % k, j = idx / k;
+    R acc = Psr[idx];
+    for (int l = 0; l < k; l++) acc -= Pss[a + k * l] * T[l + k * j];
+    Ksr[idx] = acc;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < r * r; idx += NT) {  // X_rr = K(r,r) - T^T K(s,r) - K_rs T
+    const int i = idx % r, j = idx / r;
+    R acc = kbn(k + i, k + j);
+    for (int l = 0; l < k; l++) acc -= T[l + k * i] * Psr[l + k * j];
+    for (int l = 0; l < k; l++) acc -= Krs[i + r * l] * T[l + k * j];
+    X[idx] = acc;
+  }
+  __syncthreads();
+  // The DOF geometry is dead: its space holds the pivot column and row.
+  R* colbuf = reinterpret_cast<R*>(geo);
+  const int info = gj_invert_regs<R>(X, r, prow, colbuf, colbuf + 128);
   if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
-  decouple_tail<R, NT>(X, prow, Krs, Ksr, P, r, k, F, fl, upd, sperm);
+  decouple_tail<R, NT>(X, prow, Krs, Ksr, buf, r, k, F, fl, upd, sperm);
   if (tid == 0) s.upd_kind[b] = 1;
 }
 
