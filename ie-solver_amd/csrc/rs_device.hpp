@@ -44,11 +44,13 @@ constexpr int kRootRegMax = 192;  // root blocks inverted in registers
 // Per-level lists of the boxes each QRCP kernel takes, built by prefix
 // sums (k_scan) so that a kernel's workgroups map onto exactly its boxes.
 enum : int {
-  kLstReg = 0,   // register QRCP (kQrcpReg, kQrcpStack)
-  kLstLds = 1,   // general QRCP (kQrcpLds)
-  kLstReg8 = 2,  // wide register QRCP (kQrcpReg8, kQrcpStack8)
-  kNumLists = 3,
+  kLstReg = 0,      // register QRCP (kQrcpReg, kQrcpStack), n > 32
+  kLstLds = 1,      // general QRCP (kQrcpLds)
+  kLstReg8 = 2,     // wide register QRCP (kQrcpReg8, kQrcpStack8)
+  kLstRegHalf = 3,  // register QRCP, n <= 32 (8-wave workgroups)
+  kNumLists = 4,
 };
+constexpr int kHalfCols = 32;
 
 // Row chunks of a tall ID matrix, at most `rows` rows each.
 __host__ __device__ inline int id_chunks(int m, int rows) {

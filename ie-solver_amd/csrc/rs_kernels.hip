@@ -190,19 +190,20 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
                                                int count, int level) {
   // Scanned per box: ID scratch, row-list and factor sizes, register-QRCP
   // and general-QRCP list entries, row-compression jobs.
-  constexpr int NQ = 7;
+  constexpr int NQ = 8;
   __shared__ int64_t sh[NQ][1024];
   const int tid = threadIdx.x;
   const int chunk = (count + 1023) / 1024;
   const int lo = min(count, tid * chunk), hi = min(count, lo + chunk);
-  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0};
+  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
     const int kind = s.qkind[b];
     own[0] += s.id_sz[b];
     own[1] += s.row_sz[b];
     own[2] += s.fac_sz[b];
-    own[3] += (kind == kQrcpReg || kind == kQrcpStack) ? 1 : 0;
+    own[3] += ((kind == kQrcpReg || kind == kQrcpStack) && s.act_cnt[b] > kHalfCols) ? 1 : 0;
+    own[7] += ((kind == kQrcpReg || kind == kQrcpStack) && s.act_cnt[b] <= kHalfCols) ? 1 : 0;
     own[4] += (kind == kQrcpLds) ? 1 : 0;
     own[5] += row_jobs(s, b);
     own[6] += (kind == kQrcpReg8 || kind == kQrcpStack8) ? 1 : 0;
@@ -226,6 +227,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
   int* lreg = s.lists + (int64_t)kLstReg * s.nbox + start;
   int* llds = s.lists + (int64_t)kLstLds * s.nbox + start;
   int* lreg8 = s.lists + (int64_t)kLstReg8 * s.nbox + start;
+  int* lhalf = s.lists + (int64_t)kLstRegHalf * s.nbox + start;
   int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
@@ -236,7 +238,10 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     off[0] += s.id_sz[b];
     off[1] += s.row_sz[b];
     off[2] += s.fac_sz[b];
-    if (kind == kQrcpReg || kind == kQrcpStack) lreg[off[3]++] = b;
+    if (kind == kQrcpReg || kind == kQrcpStack) {
+      if (s.act_cnt[b] > kHalfCols) lreg[off[3]++] = b;
+      else lhalf[off[7]++] = b;
+    }
     if (kind == kQrcpLds) llds[off[4]++] = b;
     if (kind == kQrcpReg8 || kind == kQrcpStack8) lreg8[off[6]++] = b;
     {
@@ -254,6 +259,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     list_slot(s, level, kLstReg)[0] = (int)sh[3][1023];
     list_slot(s, level, kLstLds)[0] = (int)sh[4][1023];
     list_slot(s, level, kLstReg8)[0] = (int)sh[6][1023];
+    list_slot(s, level, kLstRegHalf)[0] = (int)sh[7][1023];
     s.njobs[level] = (int)sh[5][1023];
     int e = 0;
     if (ta > s.id_cap) {
@@ -383,17 +389,24 @@ __device__ inline R row_value(const R (&x)[RPL], int i) {
 
 // LAPACK dlarfg on (alpha, x) with ||x||^2 = xn2: beta = -sign(alpha)
 // ||(alpha, x)||, tau = (beta - alpha) / beta, scale 1 / (alpha - beta).
+// (The ID matrices hold moderate entries, so ||(alpha, x)|| is formed
+// directly rather than through dlapy2's scaling, and the two quotients use
+// independent reciprocals.)
+template <typename R>
+__device__ inline R recip_any(R d) {
+  if constexpr (sizeof(R) == 8) return recip(d);
+  else return R(1) / d;
+}
 template <typename R>
 __device__ inline void householder_scalars(R alpha, R xn2, R* beta, R* tau, R* sc) {
-  const R xnorm = sqrt(xn2);
-  if (xnorm == 0) {
+  if (xn2 == 0) {
     *tau = 0;
     *beta = alpha;
     *sc = 0;
   } else {
-    *beta = -copysign(hypot(alpha, xnorm), alpha);
-    *tau = (*beta - alpha) / *beta;
-    *sc = 1 / (alpha - *beta);
+    *beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+    *tau = (*beta - alpha) * recip_any(*beta);
+    *sc = recip_any(alpha - *beta);
   }
 }
 
@@ -421,9 +434,13 @@ __device__ inline void apply_reflector(R (&a)[CPW][RPL], const R* vb, R tau, boo
   R p[CPW];
 #pragma unroll
   for (int sl = 0; sl < CPW; sl++) {
-    p[sl] = 0;
+    R p0 = 0, p1 = 0;  // two chains halve the dependent FMA latency
 #pragma unroll
-    for (int r = 0; r < RPL; r++) p[sl] += vr[r] * a[sl][r];
+    for (int r = 0; r < RPL; r += 2) {
+      p0 += vr[r] * a[sl][r];
+      if (r + 1 < RPL) p1 += vr[r + 1] * a[sl][r + 1];
+    }
+    p[sl] = p0 + p1;
   }
   R t = wave_total_cols<R, CPW>(p);
   t = live ? tau * t : R(0);
@@ -627,11 +644,11 @@ __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevGeom g, DevTree t, Dev
 // and pivot key of the wave's columns; the pivot is the largest key
 // (pivot_key), found by one wave maximum; the wave owning the pivot column
 // forms the reflector and the stop decision.
-template <typename R, int CPW>
+template <typename R, int CPW, int NW>
 __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, const DevState& s,
                             int start, int level, int b, double tol_d) {
   constexpr int RPL = RegShape<CPW>::RPL, MAXM = RegShape<CPW>::MAXM;
-  constexpr int MAXN = RegShape<CPW>::MAXN, KH = MAXN / 64;
+  constexpr int NT = 64 * NW, MAXN = CPW * NW, KH = (MAXN + 63) / 64;
   __shared__ R vbuf[MAXM];
   __shared__ R rowj[MAXN];    // row j after step j (downdating)
   __shared__ R rdiag[MAXN];   // R_jj of step j (the register keeps alpha)
@@ -685,7 +702,7 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
     double best = -1.0;
 #pragma unroll
     for (int h = 0; h < KH; h++) {
-      kv[h] = keys[lane + 64 * h];
+      kv[h] = (lane + 64 * h < MAXN) ? keys[lane + 64 * h] : -1.0;
       best = fmax(best, kv[h]);
     }
     best = wave_max(best);
@@ -745,10 +762,10 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
       wave_sync();
       bool need = false;
       if (avail && vn1 != 0) {
-        R temp = fabs(rowj[myc]) / vn1;
+        R temp = fabs(rowj[myc]) * recip_any(vn1);
         temp = 1 - temp * temp;
         temp = temp > 0 ? temp : R(0);
-        const R rat = vn1 / vn2;
+        const R rat = vn1 * recip_any(vn2);
         if (temp * rat * rat <= Eps<R>::tol3z) need = true;
         else vn1 = vn1 * sqrt(temp);
       }
@@ -787,7 +804,7 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
 
   int* out = s.out_list + (int64_t)b * s.ncap;
   if (k == 0) {
-    for (int i = tid; i < n; i += kXThreads) out[i] = act[i];
+    for (int i = tid; i < n; i += NT) out[i] = act[i];
     if (tid == 0) {
       s.rank[b] = 0;
       s.out_cnt[b] = n;
@@ -797,7 +814,7 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
   // T = R11^{-1} R12: one wave per column, lane l holds rows l, l + 64, ...
   const int r = n - k;
   R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
-  for (int cp = wv; cp < r; cp += kXThreads / 64) {
+  for (int cp = wv; cp < r; cp += NW) {
     const int pcol = colmap[k + cp];
     R tv[KH];
 #pragma unroll
@@ -823,7 +840,7 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
   }
   int* red = s.red_list + (int64_t)b * s.ncap;
   int* pm = s.perm + (int64_t)b * s.ncap;
-  for (int i = tid; i < n; i += kXThreads) {
+  for (int i = tid; i < n; i += NT) {
     const int c = colmap[i];
     pm[i] = c;
     if (i < k) out[i] = act[c];
@@ -837,19 +854,20 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
 
 // One workgroup per entry of the level's register-QRCP list of this shape
 // (a loop over boxes would keep more values live across the step loop than
-// the 128 registers of a 1024-thread workgroup hold).
-template <typename R, int CPW>
-__global__ __launch_bounds__(kXThreads) void k_qrcp_id(DevGeom g, DevTree t, DevState s,
-                                                       int start, int level, double tol_d) {
-  constexpr int L = CPW == 4 ? kLstReg : kLstReg8;
+// the 128 registers of a 16-wave workgroup hold).  Boxes with at most 32
+// columns take 8 waves, so that two of them share a CU.
+template <typename R, int CPW, int NW>
+__global__ __launch_bounds__(64 * NW, 4) void k_qrcp_id(  // 4 waves per SIMD
+    DevGeom g, DevTree t, DevState s, int start, int level, double tol_d) {
+  constexpr int L = CPW == 8 ? kLstReg8 : (NW == 8 ? kLstRegHalf : kLstReg);
   if (!pools_ok(s)) return;
   const int nl = list_count(s, level, L);
   if constexpr (CPW == 4) {
     if ((int)blockIdx.x < nl)
-      qrcp_id_box<R, CPW>(g, t, s, start, level, list_at(s, start, L, blockIdx.x), tol_d);
+      qrcp_id_box<R, CPW, NW>(g, t, s, start, level, list_at(s, start, L, blockIdx.x), tol_d);
   } else {  // few boxes: a small grid strides over them
     for (int li = blockIdx.x; li < nl; li += gridDim.x) {
-      qrcp_id_box<R, CPW>(g, t, s, start, level, list_at(s, start, L, li), tol_d);
+      qrcp_id_box<R, CPW, NW>(g, t, s, start, level, list_at(s, start, L, li), tol_d);
       __syncthreads();
     }
   }
@@ -1878,8 +1896,9 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
     case kStageQrcp:
       k_rowcomp<R, 4><<<min(2 * count, 256), kXThreads, 0, st>>>(g, t, s, start, level);
       k_rowcomp<R, 8><<<min(2 * count, 64), kXThreads, 0, st>>>(g, t, s, start, level);
-      k_qrcp_id<R, 4><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
-      k_qrcp_id<R, 8><<<min(count, 64), kXThreads, 0, st>>>(g, t, s, start, level, tol);
+      k_qrcp_id<R, 4, 8><<<count, 512, 0, st>>>(g, t, s, start, level, tol);
+      k_qrcp_id<R, 4, 16><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
+      k_qrcp_id<R, 8, 16><<<min(count, 64), kXThreads, 0, st>>>(g, t, s, start, level, tol);
       k_qrcp<R, 512><<<min(count, 64), 512, 0, st>>>(s, start, level, tol);
       break;
     case kStageDecouple:
