@@ -454,7 +454,7 @@ __device__ inline void apply_reflector(R (&a)[CPW][RPL], const R* vb, R tau, boo
 
 // Sum of squares of rows > i of each of this wave's columns (i = -1: all
 // rows); lanes 0..CPW-1 return column CPW w + lane.
-template <typename R, int RPL, int CPW>
+template <typename R, int RPL, int CPW, int MAXN>
 __device__ inline R tail_sq(const R (&a)[CPW][RPL], int i) {
   const int lane = threadIdx.x & 63;
   R p[CPW];
@@ -463,31 +463,31 @@ __device__ inline R tail_sq(const R (&a)[CPW][RPL], int i) {
     p[sl] = 0;
 #pragma unroll
     for (int r = 0; r < RPL; r++) {
-      // With four columns per wave, i < 64: registers past the first are
+      // With at most 64 columns, i < 64: registers past the first are
       // below row i.
-      if (CPW == 4 && r > 0) p[sl] += a[sl][r] * a[sl][r];
+      if (MAXN <= 64 && r > 0) p[sl] += a[sl][r] * a[sl][r];
       else p[sl] += (lane + 64 * r > i) ? a[sl][r] * a[sl][r] : R(0);
     }
   }
   return class_by_lane_cols<R, CPW>(wave_total_cols<R, CPW>(p));
 }
 
-// Row i of this wave's columns into dst[column].  With four columns per
-// wave (n <= 64) row i < 64 sits in register 0 of lane i; the wide shape
+// Row i of this wave's columns into dst[column].  With at most 64 columns
+// row i < 64 sits in register 0 of lane i; the wide shape
 // reads the row out of every register that can hold it (rows < 128) and
 // selects, so that no register is indexed at run time (that would move the
 // whole matrix to scratch memory).
-template <typename R, int RPL, int CPW>
+template <typename R, int RPL, int CPW, int MAXN>
 __device__ inline void publish_row(const R (&a)[CPW][RPL], int i, R* dst) {
   const int lane = threadIdx.x & 63, c0 = CPW * wave_id();
-  if constexpr (CPW == 4) {
+  if constexpr (MAXN <= 64) {
     const int cls = slot_class<CPW>(i);  // lane i's class: uniform addresses
     if (lane == i) {
 #pragma unroll
       for (int sl = 0; sl < CPW; sl++) dst[c0 + (sl ^ cls)] = a[sl][0];
     }
   } else {
-    constexpr int NR = (RegShape<CPW>::MAXN + 63) / 64 < RPL ? (RegShape<CPW>::MAXN + 63) / 64 : RPL;
+    constexpr int NR = (MAXN + 63) / 64 < RPL ? (MAXN + 63) / 64 : RPL;
     const int li = i & 63, ri = i >> 6;
     const int cls = slot_class<CPW>(li);
 #pragma unroll
@@ -644,10 +644,10 @@ __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevGeom g, DevTree t, Dev
 // and pivot key of the wave's columns; the pivot is the largest key
 // (pivot_key), found by one wave maximum; the wave owning the pivot column
 // forms the reflector and the stop decision.
-template <typename R, int CPW, int NW>
+template <typename R, int CPW, int RPL, int NW>
 __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, const DevState& s,
                             int start, int level, int b, double tol_d) {
-  constexpr int RPL = RegShape<CPW>::RPL, MAXM = RegShape<CPW>::MAXM;
+  constexpr int MAXM = 64 * RPL;
   constexpr int NT = 64 * NW, MAXN = CPW * NW, KH = (MAXN + 63) / 64;
   __shared__ R vbuf[MAXM];
   __shared__ R rowj[MAXN];    // row j after step j (downdating)
@@ -687,7 +687,7 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
   int pos = myc;
   bool avail = lane < CPW && myc < n;
   if (active) {
-    vn1 = sqrt(tail_sq<R, RPL, CPW>(a, -1));
+    vn1 = sqrt(tail_sq<R, RPL, CPW, MAXN>(a, -1));
     vn2 = vn1;
   }
   if (lane < CPW) keys[myc] = avail ? pivot_key(vn1, pos) : -1.0;
@@ -730,11 +730,11 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
       R x[RPL];
       pick_slot<R, RPL, CPW>(a, psl, x);
       // With four columns per wave j < 64: row j sits in register 0.
-      const R alpha = CPW == 4 ? lane_value(x[0], j) : row_value<R, RPL>(x, j);
+      const R alpha = MAXN <= 64 ? lane_value(x[0], j) : row_value<R, RPL>(x, j);
       R ss = 0;
 #pragma unroll
       for (int r = 0; r < RPL; r++) {
-        if (CPW == 4 && r > 0) ss += x[r] * x[r];
+        if (MAXN <= 64 && r > 0) ss += x[r] * x[r];
         else ss += (lane + 64 * r > j) ? x[r] * x[r] : R(0);
       }
       R beta, tau, sc;
@@ -758,7 +758,7 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
     if (j == 0) thresh = sh_thresh;
     if (active) {
       apply_reflector<R, RPL, CPW>(a, vbuf, sh_tau, ((dmask >> cls) & 1) == 0);
-      publish_row<R, RPL, CPW>(a, j, rowj);
+      publish_row<R, RPL, CPW, MAXN>(a, j, rowj);
       wave_sync();
       bool need = false;
       if (avail && vn1 != 0) {
@@ -770,7 +770,7 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
         else vn1 = vn1 * sqrt(temp);
       }
       if (__ballot(need)) {  // dlaqp2 recomputes the norm of rows j+1..
-        const R t2 = tail_sq<R, RPL, CPW>(a, j);
+        const R t2 = tail_sq<R, RPL, CPW, MAXN>(a, j);
         if (need) {
           const R nv = (j < m - 1) ? sqrt(t2) : R(0);
           vn1 = nv;
@@ -852,22 +852,33 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
   }
 }
 
-// One workgroup per entry of the level's register-QRCP list of this shape
-// (a loop over boxes would keep more values live across the step loop than
-// the 128 registers of a 16-wave workgroup hold).  Boxes with at most 32
-// columns take 8 waves, so that two of them share a CU.
-template <typename R, int CPW, int NW>
-__global__ __launch_bounds__(64 * NW, 4) void k_qrcp_id(  // 4 waves per SIMD
+// Register shapes (columns per wave x row registers per lane x waves):
+// 4 x 8 x 8 for n <= 32 (two boxes share a CU), 4 x 8 x 16 for n <= 64,
+// 8 x 4 x 16 for n <= 128 with m <= 256.  Four waves per SIMD are needed to
+// hide the FP64 latency of the step loop: an 8 x 8 x 8 shape (256
+// registers, two waves per SIMD) measured 22% slower than 4 x 8 x 16.
+template <int CPW, int RPL, int NW>
+struct QrcpShape {
+  static constexpr int kMaxN = CPW * NW;
+  static constexpr int kList = kMaxN > 64 ? kLstReg8 : (kMaxN <= 32 ? kLstRegHalf : kLstReg);
+  static constexpr int kWavesPerSimd = CPW * RPL > 32 ? 2 : 4;  // register budget
+};
+
+// One workgroup per list entry for the n <= 64 shapes (a loop over boxes
+// would keep more values live across the step loop than the registers
+// hold); the few n <= 128 boxes share a small strided grid.
+template <typename R, int CPW, int RPL, int NW>
+__global__ __launch_bounds__(64 * NW, (QrcpShape<CPW, RPL, NW>::kWavesPerSimd)) void k_qrcp_id(
     DevGeom g, DevTree t, DevState s, int start, int level, double tol_d) {
-  constexpr int L = CPW == 8 ? kLstReg8 : (NW == 8 ? kLstRegHalf : kLstReg);
+  constexpr int L = QrcpShape<CPW, RPL, NW>::kList;
   if (!pools_ok(s)) return;
   const int nl = list_count(s, level, L);
-  if constexpr (CPW == 4) {
+  if constexpr (L != kLstReg8) {
     if ((int)blockIdx.x < nl)
-      qrcp_id_box<R, CPW, NW>(g, t, s, start, level, list_at(s, start, L, blockIdx.x), tol_d);
-  } else {  // few boxes: a small grid strides over them
+      qrcp_id_box<R, CPW, RPL, NW>(g, t, s, start, level, list_at(s, start, L, blockIdx.x), tol_d);
+  } else {
     for (int li = blockIdx.x; li < nl; li += gridDim.x) {
-      qrcp_id_box<R, CPW, NW>(g, t, s, start, level, list_at(s, start, L, li), tol_d);
+      qrcp_id_box<R, CPW, RPL, NW>(g, t, s, start, level, list_at(s, start, L, li), tol_d);
       __syncthreads();
     }
   }
@@ -1896,9 +1907,9 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
     case kStageQrcp:
       k_rowcomp<R, 4><<<min(2 * count, 256), kXThreads, 0, st>>>(g, t, s, start, level);
       k_rowcomp<R, 8><<<min(2 * count, 64), kXThreads, 0, st>>>(g, t, s, start, level);
-      k_qrcp_id<R, 4, 8><<<count, 512, 0, st>>>(g, t, s, start, level, tol);
-      k_qrcp_id<R, 4, 16><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
-      k_qrcp_id<R, 8, 16><<<min(count, 64), kXThreads, 0, st>>>(g, t, s, start, level, tol);
+      k_qrcp_id<R, 4, 8, 8><<<count, 512, 0, st>>>(g, t, s, start, level, tol);
+      k_qrcp_id<R, 4, 8, 16><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
+      k_qrcp_id<R, 8, 4, 16><<<min(count, 64), kXThreads, 0, st>>>(g, t, s, start, level, tol);
       k_qrcp<R, 512><<<min(count, 64), 512, 0, st>>>(s, start, level, tol);
       break;
     case kStageDecouple:
