@@ -178,11 +178,14 @@ struct iegpu_ctx {
   bool factored = false;
   DevBuf<int> s_act_cnt, s_act_list, s_chpos, s_out_cnt, s_out_list,
       s_red_list, s_perm, s_rank, s_upd_kind, s_m_rows, s_row_pool, s_err,
-      s_root_piv, s_qkind, s_jobs, s_njobs, s_lists, s_lcnt;
+      s_root_piv, s_qkind, s_jobs, s_njobs, s_lists, s_lcnt, s_sk_w, s_rid_done;
   DevBuf<int64_t> s_id_sz, s_row_sz, s_fac_sz, s_id_off, s_row_off, s_fac_off,
       s_fac_top;
   DevBuf<unsigned char> s_id_pool, s_fac_pool, s_root_mat;
   int64_t id_cap = 0, row_cap = 0, fac_cap = 0;
+  // randomized ID
+  int rid_mode = 0, rid_width = 0;
+  DevBuf<double> d_omega;
   DevBuf<double> d_work;  // solve vector for host-pointer solves
   // iterative refinement: b, x, residual / correction (ndof), border parts,
   // reduction partials
@@ -258,6 +261,11 @@ struct iegpu_ctx {
     s.err = s_err.p;
     s.root_mat = s_root_mat.p;
     s.root_piv = s_root_piv.p;
+    s.rid_mode = rid_mode;
+    s.rid_width = rid_width;
+    s.sk_w = s_sk_w.p;
+    s.rid_done = s_rid_done.p;
+    s.omega = d_omega.p;
     return s;
   }
 };
@@ -349,6 +357,8 @@ void alloc_state(iegpu_ctx* c) {
   c->s_upd_kind.alloc(nb);
   c->s_m_rows.alloc(nb);
   c->s_qkind.alloc(nb);
+  c->s_sk_w.alloc(nb);
+  c->s_rid_done.alloc(nb);
   c->s_lists.alloc((size_t)kNumLists * nb);
   c->s_lcnt.alloc((size_t)2 * kMaxLevels * kNumLists);
   c->s_jobs.alloc((size_t)2 * kMaxChunks * nb);
@@ -626,6 +636,47 @@ int iegpu_set_holes(iegpu_ctx* c, int n_holes, const double* centers,
   return IEGPU_OK;
 }
 
+int iegpu_set_id_method(iegpu_ctx* c, int method, int sketch_rows) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (method == IEGPU_ID_QRCP) {
+    c->rid_mode = 0;
+    c->rid_width = 0;
+    return IEGPU_OK;
+  }
+  if (method != IEGPU_ID_RANDOMIZED)
+    return fail(c, IEGPU_ERR_INVALID, "unknown ID method");
+  if (sketch_rows < 16 || sketch_rows > kRidRows || sketch_rows % 16 != 0)
+    return fail(c, IEGPU_ERR_INVALID, "sketch_rows must be 16 or 32");
+  c->rid_mode = 1;
+  c->rid_width = sketch_rows;
+  return IEGPU_OK;
+}
+
+int iegpu_id_stats(iegpu_ctx* c, int* n_sketched, int* n_fallback) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!c->factored) return fail(c, IEGPU_ERR_INVALID, "factor first");
+  int ns = 0, nf = 0;
+  if (c->rid_mode != 0) {
+    try {
+      HIP_TRY(hipSetDevice(c->device));
+      const int lo = c->tree.level_start[1], nb = c->tree.n_boxes;
+      std::vector<int> w(nb), d(nb);
+      HIP_TRY(hipMemcpy(w.data(), c->s_sk_w.p, sizeof(int) * nb, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(d.data(), c->s_rid_done.p, sizeof(int) * nb, hipMemcpyDeviceToHost));
+      for (int b = lo; b < nb; b++) {
+        if (w[b] <= 0) continue;
+        if (d[b]) ns++;
+        else nf++;
+      }
+    } catch (const HipError& e) {
+      return from_hip(c, e);
+    }
+  }
+  if (n_sketched) *n_sketched = ns;
+  if (n_fallback) *n_fallback = nf;
+  return IEGPU_OK;
+}
+
 int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
   if (!c) return IEGPU_ERR_INVALID;
   if (!c->have_tree) return fail(c, IEGPU_ERR_INVALID, "build_tree first");
@@ -649,6 +700,11 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
     if (c->id_cap == 0) c->id_cap = std::max<int64_t>(nd * 16, 1 << 22);
     if (c->row_cap == 0) c->row_cap = std::max<int64_t>(nd * 8, 1 << 16);
     if (c->fac_cap == 0) c->fac_cap = std::max<int64_t>(nd * 160, 1 << 22);
+    if (c->rid_mode != 0 && !c->d_omega.p) {
+      const int64_t no = (int64_t)kRidTableRows * kRidRows;
+      c->d_omega.alloc(no);
+      launch_gaussian(c->d_omega.p, no, 0x5EEDull, c->stream);
+    }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

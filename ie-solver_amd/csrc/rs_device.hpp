@@ -48,9 +48,19 @@ enum : int {
   kLstLds = 1,      // general QRCP (kQrcpLds)
   kLstReg8 = 2,     // wide register QRCP (kQrcpReg8, kQrcpStack8)
   kLstRegHalf = 3,  // register QRCP, n <= 32 (8-wave workgroups)
-  kNumLists = 4,
+  kLstRid = 4,      // randomized ID (sketch width > 0)
+  kNumLists = 5,
 };
 constexpr int kHalfCols = 32;
+
+// Randomized ID: a Gaussian sketch of at most kRidRows rows (a multiple of
+// 16 per box); a cut must show within kRidRows - kRidOver pivots, else the
+// box goes through the full QRCP.  Boxes with at most kRidMaxNear row DOFs
+// and 64 columns take it.
+constexpr int kRidRows = 32;
+constexpr int kRidOver = 6;
+constexpr int kRidMaxNear = 256;
+constexpr int kRidTableRows = 2 * kRidMaxNear + 4 * kNumProxy;  // Omega rows
 
 // Row chunks of a tall ID matrix, at most `rows` rows each.
 __host__ __device__ inline int id_chunks(int m, int rows) {
@@ -121,6 +131,13 @@ struct DevState {
                        // pool size a level needed on overflow (1024 elements)
   void* root_mat;      // ncap*ncap
   int* root_piv;       // ncap
+  // Randomized ID (rid_mode 0: off; 1: sketch width rid_width for every
+  // eligible box; 2: width from the rank model).
+  int rid_mode;
+  int rid_width;
+  int* sk_w;           // per box: sketch rows (0: full QRCP)
+  int* rid_done;       // per box: 1 once the randomized ID succeeded
+  const double* omega; // kRidTableRows x kRidRows Gaussian table (row-major)
 };
 
 __device__ inline int* list_slot(const DevState& s, int level, int L) {

@@ -170,6 +170,12 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
     s.id_sz[b] = scratch;
     s.row_sz[b] = kind == kQrcpLds ? (m - proxy_rows(g, level)) / 2 : 0;
     s.fac_sz[b] = (n > 0) ? fac_slot_size(n) : 0;
+    int w = 0;
+    if (s.rid_mode != 0 && (kind == kQrcpReg || kind == kQrcpStack) &&
+        (m - proxy_rows(g, level)) / 2 <= kRidMaxNear)
+      w = s.rid_width;
+    s.sk_w[b] = w;
+    s.rid_done[b] = 0;
   }
   // Until its ID succeeds a box passes its active DOFs on uncompressed:
   // boxes a failed (retried) attempt never reaches still hold a consistent
@@ -190,12 +196,12 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
                                                int count, int level) {
   // Scanned per box: ID scratch, row-list and factor sizes, register-QRCP
   // and general-QRCP list entries, row-compression jobs.
-  constexpr int NQ = 8;
+  constexpr int NQ = 9;
   __shared__ int64_t sh[NQ][1024];
   const int tid = threadIdx.x;
   const int chunk = (count + 1023) / 1024;
   const int lo = min(count, tid * chunk), hi = min(count, lo + chunk);
-  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
     const int kind = s.qkind[b];
@@ -207,6 +213,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     own[4] += (kind == kQrcpLds) ? 1 : 0;
     own[5] += row_jobs(s, b);
     own[6] += (kind == kQrcpReg8 || kind == kQrcpStack8) ? 1 : 0;
+    own[8] += (s.sk_w[b] > 0) ? 1 : 0;
   }
 #pragma unroll
   for (int q = 0; q < NQ; q++) sh[q][tid] = own[q];
@@ -228,6 +235,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
   int* llds = s.lists + (int64_t)kLstLds * s.nbox + start;
   int* lreg8 = s.lists + (int64_t)kLstReg8 * s.nbox + start;
   int* lhalf = s.lists + (int64_t)kLstRegHalf * s.nbox + start;
+  int* lrid = s.lists + (int64_t)kLstRid * s.nbox + start;
   int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
@@ -244,6 +252,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     }
     if (kind == kQrcpLds) llds[off[4]++] = b;
     if (kind == kQrcpReg8 || kind == kQrcpStack8) lreg8[off[6]++] = b;
+    if (s.sk_w[b] > 0) lrid[off[8]++] = b;
     {
       const int nch = row_jobs(s, b);
       for (int q = 0; q < nch; q++) {
@@ -260,6 +269,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     list_slot(s, level, kLstLds)[0] = (int)sh[4][1023];
     list_slot(s, level, kLstReg8)[0] = (int)sh[6][1023];
     list_slot(s, level, kLstRegHalf)[0] = (int)sh[7][1023];
+    list_slot(s, level, kLstRid)[0] = (int)sh[8][1023];
     s.njobs[level] = (int)sh[5][1023];
     int e = 0;
     if (ta > s.id_cap) {
@@ -582,7 +592,7 @@ __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevGeom g, DevTree t, Dev
   const int njobs = s.njobs[level];
   for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
     const int b = jobs[2 * job], q = jobs[2 * job + 1];
-    if (s.qkind[b] != kind) continue;
+    if (s.qkind[b] != kind || s.rid_done[b]) continue;
     const int n = s.act_cnt[b], m = s.m_rows[b];
     const int nch = id_chunks(m, MAXM);
     const int ch = (m + nch - 1) / nch;
@@ -874,13 +884,318 @@ __global__ __launch_bounds__(64 * NW, (QrcpShape<CPW, RPL, NW>::kWavesPerSimd)) 
   if (!pools_ok(s)) return;
   const int nl = list_count(s, level, L);
   if constexpr (L != kLstReg8) {
-    if ((int)blockIdx.x < nl)
-      qrcp_id_box<R, CPW, RPL, NW>(g, t, s, start, level, list_at(s, start, L, blockIdx.x), tol_d);
+    if ((int)blockIdx.x < nl) {
+      const int b = list_at(s, start, L, blockIdx.x);
+      if (!s.rid_done[b]) qrcp_id_box<R, CPW, RPL, NW>(g, t, s, start, level, b, tol_d);
+    }
   } else {
     for (int li = blockIdx.x; li < nl; li += gridDim.x) {
       qrcp_id_box<R, CPW, RPL, NW>(g, t, s, start, level, list_at(s, start, L, li), tol_d);
       __syncthreads();
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Randomized interpolative decomposition.  The sketch Y = Omega A (l x n,
+// l = sk_w[b] in {16, 32}) is formed on the matrix cores: A's entries are
+// evaluated straight into the B operand of v_mfma_{f64,f32}_16x16x4 (lane:
+// row k0 + lane / 16, column lane % 16 of a 16-column tile) and Omega, a
+// fixed Gaussian table indexed by the ID row, is the A operand.  Y moves
+// through LDS to one column per lane and is factored by column-pivoted
+// Householder QR with the reference's rank rule (ie_mat.cpp:561-568)
+// applied to the sketch, |R_jj| corrected for the dimensions the sketch
+// has left; T = R11^{-1} R12 of the sketch is the ID of A (Halko,
+// Martinsson & Tropp 2011, sec. 5.2).  A box whose sketch shows no cut
+// within l - kRidOver pivots leaves rid_done = 0 and the full QRCP kernels
+// take it.
+// ---------------------------------------------------------------------------
+
+// Output layout of the 16x16x4 MFMAs: lane l, element v holds D(i, l % 16)
+// with i = l / 16 + 4 v (f64) or 4 (l / 16) + v (f32).
+template <typename R>
+struct Mfma16 {
+  typedef R V4 __attribute__((ext_vector_type(4)));
+  __device__ static int row(int q, int v) { return sizeof(R) == 8 ? q + 4 * v : 4 * q + v; }
+};
+__device__ inline Mfma16<double>::V4 mfma16x16x4(double a, double b, Mfma16<double>::V4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline Mfma16<float>::V4 mfma16x16x4(float a, float b, Mfma16<float>::V4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ inline DofGeo pick(bool first, const DofGeo& a, const DofGeo& b) {
+  DofGeo d;
+  d.x = first ? a.x : b.x;
+  d.y = first ? a.y : b.y;
+  d.n1 = first ? a.n1 : b.n1;
+  d.n2 = first ? a.n2 : b.n2;
+  d.w = first ? a.w : b.w;
+  d.k = first ? a.k : b.k;
+  d.c = first ? a.c : b.c;
+  return d;
+}
+
+// Rows [s0, s1) of one segment of a box's ID matrix (all rows of a
+// segment share their role: TG the row point is the target, NEAR a
+// boundary DOF) times Omega into the sketch tiles acc0 (rows 0..15) and
+// acc1 (rows 16..31, when two).  Row geometry comes from the LDS table at
+// index idx0 + (r - s0) (>> 1 for proxy rows of Stokes); the next 16
+// rows' Omega operands are loaded before this iteration's entries.
+template <typename R, int PDE, bool TG, bool NEAR>
+__device__ __forceinline__ void rid_segment(const double (*tab)[kRidMaxNear + kNumProxy],
+                                            const int* rowdofs, const double* omega, int s0,
+                                            int s1, int idx0, const DofGeo& cg, bool cv,
+                                            bool two, typename Mfma16<R>::V4& acc0,
+                                            typename Mfma16<R>::V4& acc1) {
+  constexpr int LM = kRidRows;
+  constexpr bool stokes = PDE == kPdeStokes;
+  const int lane = threadIdx.x & 63, cl = lane & 15, kq = lane >> 4;
+  auto load_omega = [&](int k0, R* w0, R* w1) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int r = k0 + 4 * u + kq;
+      const double* om = omega + (int64_t)min(r, s1 - 1) * LM + cl;
+      w0[u] = r < s1 ? static_cast<R>(om[0]) : R(0);
+      w1[u] = (two && r < s1) ? static_cast<R>(om[16]) : R(0);
+    }
+  };
+  R nw0[4], nw1[4];
+  load_omega(s0, nw0, nw1);
+  for (int k0 = s0; k0 < s1; k0 += 16) {
+    R w0[4], w1[4], a[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      w0[u] = nw0[u];
+      w1[u] = nw1[u];
+    }
+    if (k0 + 16 < s1) load_omega(k0 + 16, nw0, nw1);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int r = min(k0 + 4 * u + kq, s1 - 1);
+      const int rel = r - s0;
+      const int idx = idx0 + ((stokes && !NEAR) ? rel >> 1 : rel);
+      DofGeo rg;
+      rg.x = tab[0][idx];
+      rg.y = tab[1][idx];
+      rg.n1 = tab[2][idx];
+      rg.n2 = tab[3][idx];
+      rg.w = tab[4][idx];
+      rg.k = tab[5][idx];
+      rg.c = stokes ? (NEAR ? rowdofs[rel] & 1 : rel & 1) : 0;
+      const double e = TG ? kernel_entry(PDE, rg, cg, NEAR) : kernel_entry(PDE, cg, rg, NEAR);
+      a[u] = (cv && k0 + 4 * u + kq < s1) ? static_cast<R>(e) : R(0);
+    }
+    if (two) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        acc0 = mfma16x16x4(w0[u], a[u], acc0);
+        acc1 = mfma16x16x4(w1[u], a[u], acc1);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; u++) acc0 = mfma16x16x4(w0[u], a[u], acc0);
+    }
+  }
+}
+
+// One workgroup (4 waves) per box.  The box's row geometry (near DOFs,
+// then the proxy circle) is staged in LDS; wave w sketches the 16-column
+// tile w (n <= 64) segment by segment (near target rows, near source rows,
+// proxy target rows, proxy source rows).  The tiles of Y go to LDS, and
+// wave 0 alone runs the pivoted QR of Y, the other waves ending there.
+// QR layout: lane c holds column c's active rows, shifted so that the
+// step's row is always register 0; row j of R leaves the registers for
+// LDS (rr[j][c]) at step j, where the T solve finds R11 and R12.
+template <typename R, int PDE>
+__global__ __launch_bounds__(256, 3) void k_rid(DevGeom g, DevTree t, DevState s, int start,
+                                             int level, double tol_d) {
+  constexpr int LM = kRidRows, YLD = 65, TR = kRidMaxNear + kNumProxy;
+  using V4 = typename Mfma16<R>::V4;
+  // Row geometry x, y, n1, n2, w, k during the sketch; Y (row-major, ld
+  // YLD), then the rows of R, afterwards.
+  __shared__ double tab[6][TR];
+  __shared__ int rowdofs[kRidMaxNear];
+  __shared__ R vbuf[LM];    // Householder vector of the step (shifted rows)
+  __shared__ int spiv[64];  // pivot column (lane) of each step
+  static_assert(LM * YLD * sizeof(R) <= sizeof(tab), "Y fits the row table");
+  R* ybuf = reinterpret_cast<R*>(&tab[0][0]);
+  if (!pools_ok(s)) return;
+  if ((int)blockIdx.x >= list_count(s, level, kLstRid)) return;
+  const int b = list_at(s, start, kLstRid, blockIdx.x);
+  const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
+  const int n = s.act_cnt[b], m = s.m_rows[b], l = s.sk_w[b];
+  const int P = proxy_rows(g, level) / 2;  // rows per proxy block
+  const int cnt = (m - 2 * P) / 2;
+  const int* act = s.act_list + (int64_t)b * s.ncap;
+  if (wv == 0)
+    for_each_row_dof(g, t, s, b, level >= 3, [&](int pos, int dof) { rowdofs[pos] = dof; });
+  if (level >= 3 && tid < kNumProxy) {
+    double px, py, n1, n2;
+    proxy_point(t.cx[b], t.cy[b], t.side[b] * kRadiusRatio, tid, &px, &py, &n1, &n2);
+    tab[0][cnt + tid] = px;
+    tab[1][cnt + tid] = py;
+    tab[2][cnt + tid] = n1;
+    tab[3][cnt + tid] = n2;
+    tab[4][cnt + tid] = proxy_weight(t, b);
+    tab[5][cnt + tid] = 0;
+  }
+  __syncthreads();
+  for (int i = tid; i < cnt; i += 256) {
+    const DofGeo d = dof_geo(g, rowdofs[i]);
+    tab[0][i] = d.x;
+    tab[1][i] = d.y;
+    tab[2][i] = d.n1;
+    tab[3][i] = d.n2;
+    tab[4][i] = d.w;
+    tab[5][i] = d.k;
+  }
+  __syncthreads();
+
+  // Sketch: wave wv takes columns 16 wv .. 16 wv + 15.
+  const int kq = lane >> 4;
+  const int c = 16 * wv + (lane & 15);
+  V4 acc0 = V4{0, 0, 0, 0}, acc1 = V4{0, 0, 0, 0};
+  if (16 * wv < n) {
+    const bool cv = c < n;
+    const DofGeo cg = dof_geo(g, act[cv ? c : n - 1]);
+    const bool two = l > 16;
+    rid_segment<R, PDE, true, true>(tab, rowdofs, s.omega, 0, cnt, 0, cg, cv, two, acc0, acc1);
+    rid_segment<R, PDE, false, true>(tab, rowdofs, s.omega, cnt, 2 * cnt, 0, cg, cv, two, acc0,
+                                     acc1);
+    if (P > 0) {
+      rid_segment<R, PDE, true, false>(tab, rowdofs, s.omega, 2 * cnt, 2 * cnt + P, cnt, cg, cv,
+                                       two, acc0, acc1);
+      rid_segment<R, PDE, false, false>(tab, rowdofs, s.omega, 2 * cnt + P, m, cnt, cg, cv, two,
+                                        acc0, acc1);
+    }
+  }
+  __syncthreads();  // the row table becomes Y
+  // acc[v] = Y[16 it + row(kq, v)][c]; tiles past n and rows past l are 0.
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    ybuf[Mfma16<R>::row(kq, v) * YLD + c] = acc0[v];
+    ybuf[(16 + Mfma16<R>::row(kq, v)) * YLD + c] = acc1[v];
+  }
+  __syncthreads();
+  if (wv != 0) return;
+
+  // Pivoted QR of the sketch (wave 0).
+  R y[LM];
+#pragma unroll
+  for (int r = 0; r < LM; r++) y[r] = ybuf[r * YLD + lane];
+  R* rr = ybuf;  // rows of R, rr[j * YLD + c]
+  const R tol = static_cast<R>(tol_d);
+  bool avail = lane < n;
+  int k = -1;
+  R thresh = 0;
+  R tl = 0;  // squared norm of the active rows past the first
+  {
+    R p4[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 1; r < LM; r++) p4[r & 3] += y[r] * y[r];
+    tl = (p4[0] + p4[1]) + (p4[2] + p4[3]);
+  }
+  const int jlim = min(n, l - kRidOver);
+  int j = 0;
+  for (; j < jlim; j++) {
+    const double key = avail ? pivot_key<R, 6>(y[0] * y[0] + tl, lane) : -1.0;
+    const double best = wave_max(key);
+    const int p = __ffsll((unsigned long long)__ballot(key == best)) - 1;
+    const R alpha = lane_value(y[0], p), xt = lane_value(tl, p);
+    R beta, tau, sc;
+    householder_scalars(alpha, xt, &beta, &tau, &sc);
+    // A sketch of l rows keeps l - j dimensions at step j, which shrinks
+    // |R_jj| by about sqrt((l - j) / l) against |R_00|: the cut compares
+    // |R_jj| sqrt(l / (l - j)), with a factor 2 margin, to tol |R_00|.
+    if (j == 0) {
+      thresh = fabs(tol * beta);
+    } else if (2 * fabs(beta) * sqrt(static_cast<R>(l) / static_cast<R>(l - j)) < thresh) {
+      k = j;
+      break;
+    }
+    if (lane == p) {
+      vbuf[0] = R(1);
+#pragma unroll
+      for (int r = 1; r < LM; r++) vbuf[r] = y[r] * sc;
+      spiv[j] = p;
+      avail = false;
+    }
+    wave_sync();
+    R d4[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < LM; r++) d4[r & 3] += vbuf[r] * y[r];
+    const R w = tau * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
+    // Row j of R leaves; the rest shift up one register.
+    rr[j * YLD + lane] = (lane == p) ? beta : y[0] - w;
+    R t4[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < LM - 1; r++) {
+      y[r] = y[r + 1] - w * vbuf[r + 1];
+      if (r > 0) t4[r & 3] += y[r] * y[r];
+    }
+    y[LM - 1] = 0;
+    tl = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+    wave_sync();
+  }
+  if (k < 0) {
+    if (j < n) return;  // the sketch ran out before a cut: full QRCP
+    k = n;              // every column taken: nothing redundant
+  }
+
+  // T = R11^{-1} R12, column sweep in place on the rows of R: R11(i, q) =
+  // rr[i][spiv[q]], the redundant lanes' columns become T.
+  const bool red = avail;
+  for (int q = k - 1; q >= 0; q--) {
+    const int pq = spiv[q];
+    const R dinv = recip_any(rr[q * YLD + pq]);
+    if (red) {
+      const R tq = rr[q * YLD + lane] * dinv;
+      rr[q * YLD + lane] = tq;
+      for (int i = 0; i < q; i++) rr[i * YLD + lane] -= rr[i * YLD + pq] * tq;
+    }
+    wave_sync();
+  }
+  const uint64_t redmask = __ballot(red);
+  const int cp = __popcll(redmask & ((1ull << lane) - 1));  // redundant index
+  R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
+  if (red)
+    for (int i = 0; i < k; i++) F[i + (int64_t)k * cp] = rr[i * YLD + lane];
+  int* out = s.out_list + (int64_t)b * s.ncap;
+  int* rl = s.red_list + (int64_t)b * s.ncap;
+  int* pm = s.perm + (int64_t)b * s.ncap;
+  if (lane < k) {
+    const int cc = spiv[lane];
+    pm[lane] = cc;
+    out[lane] = act[cc];
+  }
+  if (red) {
+    pm[k + cp] = lane;
+    rl[cp] = act[lane];
+  }
+  if (lane == 0) {
+    s.rank[b] = k;
+    s.out_cnt[b] = k;
+    s.rid_done[b] = 1;
+  }
+}
+
+// Gaussian table for the sketches: z = sqrt(-2 ln u1) cos(2 pi u2) (Box and
+// Muller 1958) from two 53-bit uniforms of a SplitMix64 hash of the index.
+__device__ inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__global__ __launch_bounds__(256) void k_gaussian(double* out, int64_t n, uint64_t seed) {
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const uint64_t h1 = splitmix64(seed ^ (2 * (uint64_t)i));
+    const uint64_t h2 = splitmix64(seed ^ (2 * (uint64_t)i + 1));
+    const double u1 = ((h1 >> 11) + 1) * (1.0 / 9007199254740992.0);  // (0, 1]
+    const double u2 = (h2 >> 11) * (1.0 / 9007199254740992.0);        // [0, 1)
+    out[i] = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
   }
 }
 
@@ -1886,6 +2201,10 @@ void launch_reset(const DevState& s, hipStream_t st) {
   k_reset<<<1, 64, 0, st>>>(s);
 }
 
+void launch_gaussian(double* out, int64_t n, uint64_t seed, hipStream_t st) {
+  k_gaussian<<<256, 256, 0, st>>>(out, n, seed);
+}
+
 template <typename R>
 void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
                          const DevState& s, int level, int start, int count,
@@ -1905,6 +2224,12 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       k_assemble<R><<<min(count, 64), 256, 0, st>>>(g, t, s, start, level);
       break;
     case kStageQrcp:
+      if (s.rid_mode != 0) {
+        if (g.pde == kPdeStokes)
+          k_rid<R, kPdeStokes><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
+        else
+          k_rid<R, kPdeLaplace><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
+      }
       k_rowcomp<R, 4><<<min(2 * count, 256), kXThreads, 0, st>>>(g, t, s, start, level);
       k_rowcomp<R, 8><<<min(2 * count, 64), kXThreads, 0, st>>>(g, t, s, start, level);
       k_qrcp_id<R, 4, 8, 8><<<count, 512, 0, st>>>(g, t, s, start, level, tol);

@@ -14,6 +14,8 @@ struct SolveVec {
 };
 
 void launch_reset(const DevState& s, hipStream_t st);
+// Standard normal table (n entries) for the randomized-ID sketches.
+void launch_gaussian(double* out, int64_t n, uint64_t seed, hipStream_t st);
 
 // Stages of one tree level of the factorization, in launch order.
 enum FactorStage : int {

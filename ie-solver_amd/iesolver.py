@@ -26,6 +26,8 @@ PDE_LAPLACE = 0
 PDE_STOKES = 2
 FP64 = 0
 FP32 = 1
+ID_QRCP = 0
+ID_RANDOMIZED = 1
 
 STATUS = {
     0: "OK", 1: "INVALID", 2: "NO_COMPRESSION", 3: "SINGULAR", 4: "OOM",
@@ -37,7 +39,7 @@ EXPORTS = [
     "iegpu_build_tree", "iegpu_set_holes", "iegpu_factor", "iegpu_solve",
     "iegpu_solve_device", "iegpu_tree_info", "iegpu_box_info", "iegpu_box_list",
     "iegpu_last_timings", "iegpu_set_profiling", "iegpu_profile", "iegpu_evaluate",
-    "iegpu_solve_refined", "iegpu_apply",
+    "iegpu_solve_refined", "iegpu_apply", "iegpu_set_id_method", "iegpu_id_stats",
 ]
 
 STAGES = ["active", "rows", "scan", "assemble", "qrcp", "decouple", "root",
@@ -82,6 +84,8 @@ def load():
     L.iegpu_apply.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.iegpu_set_profiling.argtypes = [vp, ctypes.c_int]
     L.iegpu_profile.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _ip]
+    L.iegpu_set_id_method.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.iegpu_id_stats.argtypes = [vp, _ip, _ip]
     _lib = L
     return L
 
@@ -143,6 +147,18 @@ class RSSolver:
 
     def factor(self, id_tol, precision=FP64):
         self._check(self.L.iegpu_factor(self.h, id_tol, precision))
+
+    def set_id_method(self, method, sketch_rows=32):
+        """ID_QRCP (the reference's full pivoted QR) or ID_RANDOMIZED (Gaussian
+        sketch of sketch_rows rows, full-QRCP fallback)."""
+        self._check(self.L.iegpu_set_id_method(self.h, method, sketch_rows))
+
+    def id_stats(self):
+        """(boxes whose ID came from the sketch, boxes that fell back)."""
+        a = ctypes.c_int(0)
+        b = ctypes.c_int(0)
+        self._check(self.L.iegpu_id_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def solve(self, b):
         b = _d(b)

@@ -152,6 +152,24 @@ int iegpu_set_profiling(iegpu_ctx* ctx, int on);
 int iegpu_profile(iegpu_ctx* ctx, int stage, int level, double* ms,
                   int* launches);
 
+/* Interpolative decomposition used by the next iegpu_factor.
+ * IEGPU_ID_QRCP (default): the reference's full column-pivoted QR of each
+ * box's ID matrix with the rank cut |R_ii| < tol |R_00|
+ * (ie_solver/ie_mat/ie_mat.cpp:548-588).  IEGPU_ID_RANDOMIZED: boxes with
+ * at most 64 active DOFs (and at most 256 near-field row DOFs) take the ID
+ * of a Gaussian sketch Omega A of l = sketch_rows rows (16 or 32) with the
+ * same cut applied to the sketch's R, |R_jj| scaled by 2 sqrt(l / (l - j))
+ * (a sketch keeps l - j dimensions at step j); a box whose sketch shows no
+ * cut within l - 6 pivots falls back to the full QRCP.  No reference
+ * counterpart (the reference always factors the full ID matrix). */
+enum { IEGPU_ID_QRCP = 0, IEGPU_ID_RANDOMIZED = 1 };
+int iegpu_set_id_method(iegpu_ctx* ctx, int method, int sketch_rows);
+
+/* After a factorization with IEGPU_ID_RANDOMIZED: boxes whose ID came from
+ * the sketch, and boxes that fell back to the full QRCP.  Either pointer
+ * may be NULL. */
+int iegpu_id_stats(iegpu_ctx* ctx, int* n_sketched, int* n_fallback);
+
 #ifdef __cplusplus
 }
 #endif

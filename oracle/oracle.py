@@ -243,3 +243,92 @@ def dense_matrix(geom, pde=PDE_LAPLACE):
     n = geom["points"].shape[0] * (2 if pde == PDE_STOKES else 1)
     idx = np.arange(n, dtype=np.int32)
     return kernel_block(geom, idx, idx, pde)
+
+
+# ---------------------------------------------------------------------------
+# Randomized interpolative decomposition.  No reference counterpart: the
+# reference always factors the full ID matrix (ie_mat.cpp:548-588).  This is
+# a numpy restatement of the product's IEGPU_ID_RANDOMIZED form (Gaussian
+# sketch, pivoted QR of the sketch with the reference's rank rule), written
+# as the checker of that form.
+# ---------------------------------------------------------------------------
+RID_ROWS = 32          # sketch rows held per box
+RID_OVER = 6           # sketch rows kept beyond the last accepted pivot
+RID_TABLE_ROWS = 1024  # Omega rows (ID rows of an eligible box)
+RID_SEED = 0x5EED
+
+
+def _splitmix64(x):
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def gaussian_table(rows=RID_TABLE_ROWS, cols=RID_ROWS, seed=RID_SEED):
+    """Omega, row-major (rows x cols): entry i = r * cols + c is
+    sqrt(-2 ln u1) cos(2 pi u2) (Box-Muller) with u1 = (h1 >> 11 + 1) 2^-53,
+    u2 = (h2 >> 11) 2^-53 from SplitMix64 hashes of seed ^ 2i, seed ^ (2i+1)."""
+    with np.errstate(over="ignore"):
+        i = np.arange(rows * cols, dtype=np.uint64)
+        s = np.uint64(seed)
+        h1 = _splitmix64(s ^ (np.uint64(2) * i))
+        h2 = _splitmix64(s ^ (np.uint64(2) * i + np.uint64(1)))
+    u1 = ((h1 >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * 2.0 ** -53
+    u2 = (h2 >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return (np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)).reshape(rows, cols)
+
+
+def randomized_id(A, tol, rows, table=None, over=RID_OVER):
+    """ID of A (m x n) from its sketch Y = Omega[:m, :rows]^T A.
+
+    Column-pivoted Householder QR of Y: the pivot is the largest remaining
+    column norm (first index on ties); stop at the first j >= 1 with
+    2 |R_jj| sqrt(rows / (rows - j)) < tol |R_00| (the sketch keeps rows - j
+    dimensions at step j); at most min(n, rows - over) pivots.  Returns
+    (k, perm, T) with perm = skeleton columns in pivot order, then the
+    redundant columns in increasing order, and T = R11^{-1} R12 (k x n-k);
+    (n, perm, empty) when every column is taken; None when the sketch runs
+    out before a cut (the full QRCP then takes the box)."""
+    if table is None:
+        table = gaussian_table()
+    A = np.asarray(A, dtype=np.float64)
+    m, n = A.shape
+    Y = table[:m, :rows].T @ A
+    avail = np.ones(n, bool)
+    piv = []
+    thresh = 0.0
+    k = None
+    jlim = min(n, rows - over)
+    for j in range(jlim):
+        nrm = np.where(avail, np.sum(Y[j:, :] ** 2, axis=0), -1.0)
+        p = int(np.argmax(nrm))
+        x = Y[j:, p].copy()
+        alpha, xt = x[0], float(np.sum(x[1:] ** 2))
+        beta = -np.copysign(np.sqrt(alpha * alpha + xt), alpha) if xt > 0 else alpha
+        if j == 0:
+            thresh = abs(tol * beta)
+        elif 2 * abs(beta) * np.sqrt(rows / (rows - j)) < thresh:
+            k = j
+            break
+        if xt > 0:
+            tau = (beta - alpha) / beta
+            v = x / (alpha - beta)
+            v[0] = 1.0
+            Y[j:, :] -= tau * np.outer(v, v @ Y[j:, :])
+        Y[j, p] = beta
+        Y[j + 1:, p] = 0.0
+        piv.append(p)
+        avail[p] = False
+    if k is None:
+        if jlim < n:
+            return None
+        k = n
+    red = [c for c in range(n) if avail[c]]
+    perm = np.array(piv[:k] + red, dtype=np.int64)
+    if k == n:
+        return k, perm, np.zeros((k, 0))
+    R11 = Y[:k, :][:, perm[:k]]
+    R12 = Y[:k, :][:, red]
+    T = np.linalg.solve(np.triu(R11), R12)
+    return k, perm, T

@@ -86,6 +86,24 @@ def test_error_codes():
     assert L.iegpu_destroy(None) == 1
 
 
+def test_id_method_arguments():
+    s = iesolver.RSSolver(device=-1)
+    s.set_id_method(iesolver.ID_RANDOMIZED, 32)
+    s.set_id_method(iesolver.ID_RANDOMIZED, 16)
+    s.set_id_method(iesolver.ID_QRCP)
+    for method, rows in [(7, 32), (iesolver.ID_RANDOMIZED, 24), (iesolver.ID_RANDOMIZED, 48),
+                         (iesolver.ID_RANDOMIZED, -16), (iesolver.ID_RANDOMIZED, 0)]:
+        with pytest.raises(iesolver.IeGpuError) as e:
+            s.set_id_method(method, rows)
+        assert e.value.code == 1, (method, rows)
+    with pytest.raises(iesolver.IeGpuError) as e:
+        s.id_stats()  # nothing factored
+    assert e.value.code == 1
+    L = iesolver.load()
+    assert L.iegpu_set_id_method(None, 0, 0) == 1
+    assert L.iegpu_id_stats(None, None, None) == 1
+
+
 def test_geometry_circle_closed_form():
     g = G.circle(1000, (0.5, 0.5), 0.5)
     assert np.allclose(np.hypot(*(g["points"] - 0.5).T), 0.5, atol=1e-15)
