@@ -130,3 +130,49 @@ def test_rid_fp32_factor():
     er = _rel(s.solve(f)[0], mo)
     print(f"fp32 plain {e32:.3e}, fp32 randomized {er:.3e}, stats {s.id_stats()}", flush=True)
     assert er <= max(10 * e32, 10 * tol)
+
+
+def _rid_eligible(m, n, level, pde):
+    # Boxes the register QRCP takes (n <= 64, m <= 512 or row-compressible)
+    # with at most 256 near-field row DOFs.
+    P = (512 if pde == iesolver.PDE_STOKES else 256) if level >= 3 else 0
+    nch = -(-m // 512)
+    reg = n <= 64 and (m <= 512 or (nch <= 8 and nch * n <= 512))
+    return m > 0 and reg and (m - P) // 2 <= 256
+
+
+@pytest.mark.parametrize("pde", [iesolver.PDE_LAPLACE, iesolver.PDE_STOKES])
+def test_rid_ranks_match_oracle_restatement(pde):
+    # Per box: the GPU's randomized-ID rank against oracle.randomized_id on
+    # the oracle's ID matrix with the same Omega table.  Equal ranks except
+    # where rounding moves a diagonal across the cut (then within 1).
+    tol = 1e-10
+    g = G.star(8192) if pde == iesolver.PDE_LAPLACE else G.circle_with_holes(8192, 8, 0)
+    s = _solver(g, tol, 32, pde)
+    gb = s.box_info()
+    o = oracle.OracleProblem(g, pde=pde, record_id_mats=True)
+    o.factor(tol, 4)
+    ob = o.box_info()
+    table = oracle.gaussian_table()
+    same = near = sketched = 0
+    for b in range(o.num_boxes()):
+        if ob["rank"][b] <= 0:
+            continue
+        A = o.id_mat(b)
+        m, n = A.shape
+        if not _rid_eligible(m, n, ob["level"][b], pde):
+            continue
+        r = oracle.randomized_id(A, tol, 32, table)
+        if r is None:
+            continue
+        sketched += 1
+        d = abs(r[0] - gb["rank"][b])
+        same += d == 0
+        near += d <= 1
+    gs, gf = s.id_stats()
+    print(f"pde {pde}: restated sketched {sketched}, gpu sketched {gs}; ranks equal "
+          f"{same}, within 1 {near}", flush=True)
+    assert sketched >= 50
+    assert near == sketched
+    assert same >= 0.9 * sketched
+    assert abs(gs - sketched) <= 0.05 * sketched
