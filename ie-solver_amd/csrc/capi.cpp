@@ -16,9 +16,14 @@
 #include "rs_eval.hpp"
 #include "rs_kernels.hpp"
 #include "rs_matvec.hpp"
+#include "rs_mlp.hpp"
 #include "tree.hpp"
 
 using namespace iegpu;
+
+static_assert(IEGPU_RANK_FEATURES == kFeat && IEGPU_RANK_MODEL_LEN == kMlpLen &&
+                  IEGPU_RANK_PARAMS == kMlpParams,
+              "rank model layout of iegpu.h");
 
 namespace {
 
@@ -186,6 +191,10 @@ struct iegpu_ctx {
   // randomized ID
   int rid_mode = 0, rid_width = 0;
   DevBuf<double> d_omega;
+  // rank model
+  bool have_model = false, record_features = false;
+  int rid_margin = 0;
+  DevBuf<float> d_model, d_feat, m_X, m_y, m_work;
   DevBuf<double> d_work;  // solve vector for host-pointer solves
   // iterative refinement: b, x, residual / correction (ndof), border parts,
   // reduction partials
@@ -266,6 +275,9 @@ struct iegpu_ctx {
     s.sk_w = s_sk_w.p;
     s.rid_done = s_rid_done.p;
     s.omega = d_omega.p;
+    s.rank_model = d_model.p;
+    s.rid_margin = rid_margin;
+    s.feat = record_features ? d_feat.p : nullptr;
     return s;
   }
 };
@@ -358,6 +370,10 @@ void alloc_state(iegpu_ctx* c) {
   c->s_m_rows.alloc(nb);
   c->s_qkind.alloc(nb);
   c->s_sk_w.alloc(nb);
+  if (c->record_features && c->d_feat.n != (size_t)nb * kFeat) {
+    c->d_feat.alloc((size_t)nb * kFeat);
+    HIP_TRY(hipMemset(c->d_feat.p, 0, sizeof(float) * nb * kFeat));
+  }
   c->s_rid_done.alloc(nb);
   c->s_lists.alloc((size_t)kNumLists * nb);
   c->s_lcnt.alloc((size_t)2 * kMaxLevels * kNumLists);
@@ -645,28 +661,189 @@ int iegpu_set_id_method(iegpu_ctx* c, int method, int sketch_rows) {
   }
   if (method != IEGPU_ID_RANDOMIZED)
     return fail(c, IEGPU_ERR_INVALID, "unknown ID method");
+  if (sketch_rows == 0) {
+    if (!c->have_model) return fail(c, IEGPU_ERR_INVALID, "sketch_rows 0 needs a rank model");
+    c->rid_mode = 2;
+    c->rid_width = 0;
+    return IEGPU_OK;
+  }
   if (sketch_rows < 16 || sketch_rows > kRidRows || sketch_rows % 16 != 0)
-    return fail(c, IEGPU_ERR_INVALID, "sketch_rows must be 16 or 32");
+    return fail(c, IEGPU_ERR_INVALID, "sketch_rows must be 0, 16 or 32");
   c->rid_mode = 1;
   c->rid_width = sketch_rows;
   return IEGPU_OK;
 }
 
-int iegpu_id_stats(iegpu_ctx* c, int* n_sketched, int* n_fallback) {
+int iegpu_record_features(iegpu_ctx* c, int on) {
+  if (!c) return IEGPU_ERR_INVALID;
+  c->record_features = on != 0;
+  return IEGPU_OK;
+}
+
+int iegpu_box_features(iegpu_ctx* c, float* feat, int cap) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!c->factored || !c->d_feat.p)
+    return fail(c, IEGPU_ERR_INVALID, "no factorization recorded features");
+  const HostTree& T = c->tree;
+  const int64_t need = (int64_t)T.n_boxes * kFeat;
+  if (!feat || cap < need) return fail(c, IEGPU_ERR_INVALID, "feature buffer too small");
+  std::vector<float> dev(need);
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpy(dev.data(), c->d_feat.p, sizeof(float) * need, hipMemcpyDeviceToHost));
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  // device rows are level-major slots; the ABI's boxes are creation ids
+  for (int id = 0; id < T.n_boxes; id++)
+    for (int q = 0; q < kFeat; q++)
+      feat[(int64_t)id * kFeat + q] = dev[(int64_t)T.slot_of_id[id] * kFeat + q];
+  return IEGPU_OK;
+}
+
+int iegpu_rank_model_init(iegpu_ctx* c, const float* X, const float* y, int n, unsigned seed,
+                          float* model) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!X || !y || !model || n < 2) return fail(c, IEGPU_ERR_INVALID, "bad model arguments");
+  std::vector<double> mu(kFeat, 0.0), sd(kFeat, 0.0);
+  double ym = 0, ys = 0;
+  for (int i = 0; i < n; i++) {
+    for (int q = 0; q < kFeat; q++) mu[q] += X[(int64_t)i * kFeat + q];
+    ym += y[i];
+  }
+  for (int q = 0; q < kFeat; q++) mu[q] /= n;
+  ym /= n;
+  for (int i = 0; i < n; i++) {
+    for (int q = 0; q < kFeat; q++) {
+      const double d = X[(int64_t)i * kFeat + q] - mu[q];
+      sd[q] += d * d;
+    }
+    ys += (y[i] - ym) * (y[i] - ym);
+  }
+  for (int q = 0; q < kFeat; q++) {
+    const double v = std::sqrt(sd[q] / n);
+    model[kMlpMu + q] = (float)mu[q];
+    model[kMlpSd + q] = (float)(v > 1e-6 ? v : 1.0);  // constant features pass through
+  }
+  model[kMlpYMu] = (float)ym;
+  model[kMlpYSd] = (float)(std::sqrt(ys / n) > 1e-6 ? std::sqrt(ys / n) : 1.0);
+  // Glorot-uniform weights from a 32-bit xorshift stream; zero biases.
+  uint32_t st = seed ? seed : 0x9E3779B9u;
+  auto uni = [&st]() {
+    st ^= st << 13;
+    st ^= st >> 17;
+    st ^= st << 5;
+    return (st >> 8) * (1.0 / 16777216.0);
+  };
+  auto fill = [&](int off, int count, int fan_in, int fan_out) {
+    const double a = std::sqrt(6.0 / (fan_in + fan_out));
+    for (int i = 0; i < count; i++) model[off + i] = (float)((2 * uni() - 1) * a);
+  };
+  fill(kMlpW1, kFeat * kHid, kFeat, kHid);
+  fill(kMlpW2, kHid * kHid, kHid, kHid);
+  fill(kMlpW3, kHid, kHid, 1);
+  for (int j = 0; j < kHid; j++) model[kMlpB1 + j] = model[kMlpB2 + j] = 0;
+  model[kMlpB3] = 0;
+  return IEGPU_OK;
+}
+
+namespace {
+int upload_samples(iegpu_ctx* c, const float* X, const float* y, int n, const float* model) {
+  c->m_X.upload(X, (size_t)n * kFeat, c->stream);
+  c->m_y.upload(y, (size_t)n, c->stream);
+  c->m_work.alloc(kMlpLen + kMlpParams + 8);
+  HIP_TRY(hipMemcpyAsync(c->m_work.p, model, sizeof(float) * kMlpLen, hipMemcpyHostToDevice,
+                         c->stream));
+  return IEGPU_OK;
+}
+}  // namespace
+
+int iegpu_rank_model_train(iegpu_ctx* c, const float* X, const float* y, int n, float* model,
+                           int steps, float lr, unsigned seed, float* loss) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
+  if (!X || !y || !model || n < 1 || steps < 1 || !(lr > 0))
+    return fail(c, IEGPU_ERR_INVALID, "bad training arguments");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    upload_samples(c, X, y, n, model);
+    float* dl = c->m_work.p + kMlpLen + kMlpParams;
+    launch_mlp_train(c->m_X.p, c->m_y.p, n, c->m_work.p, steps, lr, seed, dl, c->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(model, c->m_work.p, sizeof(float) * kMlpLen, hipMemcpyDeviceToHost,
+                           c->stream));
+    float l = 0;
+    HIP_TRY(hipMemcpyAsync(&l, dl, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (loss) *loss = l;
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_rank_model_grad(iegpu_ctx* c, const float* model, const float* X, const float* y,
+                          int n, float* grad, float* loss) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
+  if (!X || !y || !model || !grad || n < 1) return fail(c, IEGPU_ERR_INVALID, "bad arguments");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    upload_samples(c, X, y, n, model);
+    float* g = c->m_work.p + kMlpLen;
+    float* dl = g + kMlpParams;
+    launch_mlp_grad(c->m_X.p, c->m_y.p, n, c->m_work.p, g, dl, c->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(grad, g, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, c->stream));
+    float l = 0;
+    HIP_TRY(hipMemcpyAsync(&l, dl, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (loss) *loss = l;
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_set_rank_model(iegpu_ctx* c, const float* model, int margin) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
+  if (!model || margin < 0 || margin > 16) return fail(c, IEGPU_ERR_INVALID, "bad rank model");
+  for (int i = 0; i < kMlpLen; i++)
+    if (!std::isfinite(model[i])) return fail(c, IEGPU_ERR_INVALID, "rank model not finite");
+  for (int q = 0; q < kFeat; q++)
+    if (!(model[kMlpSd + q] > 0)) return fail(c, IEGPU_ERR_INVALID, "rank model scale <= 0");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    c->d_model.upload(model, kMlpLen, c->stream);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  c->have_model = true;
+  c->rid_margin = margin;
+  return IEGPU_OK;
+}
+
+int iegpu_id_stats(iegpu_ctx* c, int* n_sketched, int* n_fallback, double* sketch_flops) {
   if (!c) return IEGPU_ERR_INVALID;
   if (!c->factored) return fail(c, IEGPU_ERR_INVALID, "factor first");
   int ns = 0, nf = 0;
+  double fl = 0;
   if (c->rid_mode != 0) {
     try {
       HIP_TRY(hipSetDevice(c->device));
       const int lo = c->tree.level_start[1], nb = c->tree.n_boxes;
-      std::vector<int> w(nb), d(nb);
+      std::vector<int> w(nb), d(nb), m(nb), n(nb);
       HIP_TRY(hipMemcpy(w.data(), c->s_sk_w.p, sizeof(int) * nb, hipMemcpyDeviceToHost));
       HIP_TRY(hipMemcpy(d.data(), c->s_rid_done.p, sizeof(int) * nb, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(m.data(), c->s_m_rows.p, sizeof(int) * nb, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(n.data(), c->s_act_cnt.p, sizeof(int) * nb, hipMemcpyDeviceToHost));
       for (int b = lo; b < nb; b++) {
         if (w[b] <= 0) continue;
         if (d[b]) ns++;
         else nf++;
+        fl += 2.0 * w[b] * m[b] * (16 * ((n[b] + 15) / 16));
       }
     } catch (const HipError& e) {
       return from_hip(c, e);
@@ -674,6 +851,7 @@ int iegpu_id_stats(iegpu_ctx* c, int* n_sketched, int* n_fallback) {
   }
   if (n_sketched) *n_sketched = ns;
   if (n_fallback) *n_fallback = nf;
+  if (sketch_flops) *sketch_flops = fl;
   return IEGPU_OK;
 }
 

@@ -62,6 +62,27 @@ constexpr int kRidOver = 6;
 constexpr int kRidMaxNear = 256;
 constexpr int kRidTableRows = 2 * kRidMaxNear + 4 * kNumProxy;  // Omega rows
 
+// Rank model: an MLP kFeat -> kHid (tanh) -> kHid (tanh) -> 1 on per-box
+// features, predicting the ID rank.  Parameter blob (floats): feature
+// means and scales (kFeat each), W1 [kFeat][kHid], b1, W2 [kHid][kHid], b2,
+// w3 [kHid], b3, then the target mean and scale.
+constexpr int kFeat = 9;
+constexpr int kHid = 32;
+enum : int {
+  kMlpMu = 0,
+  kMlpSd = kMlpMu + kFeat,
+  kMlpW1 = kMlpSd + kFeat,
+  kMlpB1 = kMlpW1 + kFeat * kHid,
+  kMlpW2 = kMlpB1 + kHid,
+  kMlpB2 = kMlpW2 + kHid * kHid,
+  kMlpW3 = kMlpB2 + kHid,
+  kMlpB3 = kMlpW3 + kHid,
+  kMlpYMu = kMlpB3 + 1,
+  kMlpYSd = kMlpYMu + 1,
+  kMlpLen = kMlpYSd + 1,
+  kMlpParams = kMlpYMu - kMlpW1,  // trained parameters (W1 .. b3)
+};
+
 // Row chunks of a tall ID matrix, at most `rows` rows each.
 __host__ __device__ inline int id_chunks(int m, int rows) {
   return (m + rows - 1) / rows;
@@ -138,6 +159,9 @@ struct DevState {
   int* sk_w;           // per box: sketch rows (0: full QRCP)
   int* rid_done;       // per box: 1 once the randomized ID succeeded
   const double* omega; // kRidTableRows x kRidRows Gaussian table (row-major)
+  const float* rank_model;  // kMlpLen floats (rid_mode 2)
+  int rid_margin;           // ranks added to the prediction before sizing
+  float* feat;              // kFeat per box, written by k_rows when non-null
 };
 
 __device__ inline int* list_slot(const DevState& s, int level, int L) {

@@ -136,8 +136,60 @@ __device__ inline int proxy_rows(const DevGeom& g, int level) {
   return (g.pde == kPdeStokes) ? 4 * kNumProxy : 2 * kNumProxy;
 }
 
+// Rank-model features of box b (every lane returns them): log2 |active|,
+// log2 ID rows, level, log2(near rows + 1), side x mean and max |curvature|
+// of the active DOFs, 1 - |mean normal| (how far the curve turns inside
+// the box), -log10 tol, Stokes flag.
+__device__ inline void box_features(const DevGeom& g, const DevTree& t, const DevState& s, int b,
+                                    int level, int n, int m, int cnt, double tol, float* f) {
+  const int lane = threadIdx.x & 63;
+  const int* act = s.act_list + (int64_t)b * s.ncap;
+  double ks = 0, km = 0, nx = 0, ny = 0;
+  for (int i = lane; i < n; i += 64) {
+    const DofGeo d = dof_geo(g, act[i]);
+    const double k = fabs(d.k);
+    ks += k;
+    km = fmax(km, k);
+    nx += d.n1;
+    ny += d.n2;
+  }
+  ks = wave_total(ks);
+  km = wave_max(km);
+  nx = wave_total(nx);
+  ny = wave_total(ny);
+  const double side = t.side[b], nn = n > 0 ? n : 1;
+  f[0] = (float)log2((double)nn);
+  f[1] = (float)log2((double)(m > 0 ? m : 1));
+  f[2] = (float)level;
+  f[3] = (float)log2((double)cnt + 1.0);
+  f[4] = (float)(side * ks / nn);
+  f[5] = (float)(side * km);
+  f[6] = (float)(1.0 - sqrt(nx * nx + ny * ny) / nn);
+  f[7] = (float)(-log10(tol));
+  f[8] = g.pde == kPdeStokes ? 1.0f : 0.0f;
+}
+
+// Rank predicted by the model for features f (lanes 0..kHid-1 own the
+// hidden units; every lane returns the prediction).
+__device__ inline float mlp_rank(const float* M, const float* f, float* hb) {
+  const int lane = threadIdx.x & 63;
+  const int j = lane & (kHid - 1);
+  float z = M[kMlpB1 + j];
+#pragma unroll
+  for (int q = 0; q < kFeat; q++)
+    z += M[kMlpW1 + q * kHid + j] * ((f[q] - M[kMlpMu + q]) / M[kMlpSd + q]);
+  if (lane < kHid) hb[j] = tanhf(z);
+  __syncthreads();
+  float z2 = M[kMlpB2 + j];
+  for (int i = 0; i < kHid; i++) z2 += M[kMlpW2 + i * kHid + j] * hb[i];
+  double o = lane < kHid ? (double)(M[kMlpW3 + j] * tanhf(z2)) : 0.0;
+  o = wave_total(o) + M[kMlpB3];
+  return (float)o * M[kMlpYSd] + M[kMlpYMu];
+}
+
 __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
-                                             int start, int level) {
+                                             int start, int level, double tol) {
+  __shared__ float hb[kHid];
   const int b = start + blockIdx.x;
   const int n = s.act_cnt[b];
   int m = 0;
@@ -170,10 +222,23 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
     s.id_sz[b] = scratch;
     s.row_sz[b] = kind == kQrcpLds ? (m - proxy_rows(g, level)) / 2 : 0;
     s.fac_sz[b] = (n > 0) ? fac_slot_size(n) : 0;
-    int w = 0;
-    if (s.rid_mode != 0 && (kind == kQrcpReg || kind == kQrcpStack) &&
-        (m - proxy_rows(g, level)) / 2 <= kRidMaxNear)
-      w = s.rid_width;
+  }
+  // Sketch width: fixed, or from the rank model (0: straight to the full
+  // QRCP when the predicted rank does not fit the widest sketch).
+  const bool elig = s.rid_mode != 0 && (kind == kQrcpReg || kind == kQrcpStack) &&
+                    (m - proxy_rows(g, level)) / 2 <= kRidMaxNear;
+  int w = elig ? s.rid_width : 0;
+  if (m > 0 && (s.feat != nullptr || (elig && s.rid_mode == 2))) {
+    float f[kFeat];
+    box_features(g, t, s, b, level, n, m, (m - proxy_rows(g, level)) / 2, tol, f);
+    if (s.feat != nullptr && threadIdx.x == 0)
+      for (int q = 0; q < kFeat; q++) s.feat[(int64_t)b * kFeat + q] = f[q];
+    if (elig && s.rid_mode == 2) {
+      const float kp = mlp_rank(s.rank_model, f, hb) + (float)s.rid_margin;
+      w = kp <= 16 - kRidOver ? 16 : (kp <= kRidRows - kRidOver ? kRidRows : 0);
+    }
+  }
+  if (threadIdx.x == 0) {
     s.sk_w[b] = w;
     s.rid_done[b] = 0;
   }
@@ -2215,7 +2280,7 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       k_active<<<count, 64, 0, st>>>(t, s, start);
       break;
     case kStageRows:
-      k_rows<<<count, 64, 0, st>>>(g, t, s, start, level);
+      k_rows<<<count, 64, 0, st>>>(g, t, s, start, level, tol);
       break;
     case kStageScan:
       k_scan<<<1, 1024, 0, st>>>(s, start, count, level);

@@ -40,7 +40,12 @@ EXPORTS = [
     "iegpu_solve_device", "iegpu_tree_info", "iegpu_box_info", "iegpu_box_list",
     "iegpu_last_timings", "iegpu_set_profiling", "iegpu_profile", "iegpu_evaluate",
     "iegpu_solve_refined", "iegpu_apply", "iegpu_set_id_method", "iegpu_id_stats",
+    "iegpu_record_features", "iegpu_box_features", "iegpu_rank_model_init",
+    "iegpu_rank_model_train", "iegpu_rank_model_grad", "iegpu_set_rank_model",
 ]
+RANK_FEATURES = 9
+RANK_MODEL_LEN = 1429
+RANK_PARAMS = 1409
 
 STAGES = ["active", "rows", "scan", "assemble", "qrcp", "decouple", "root",
           "sweep_up", "root_solve", "sweep_down", "border"]
@@ -85,7 +90,15 @@ def load():
     L.iegpu_set_profiling.argtypes = [vp, ctypes.c_int]
     L.iegpu_profile.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _ip]
     L.iegpu_set_id_method.argtypes = [vp, ctypes.c_int, ctypes.c_int]
-    L.iegpu_id_stats.argtypes = [vp, _ip, _ip]
+    L.iegpu_id_stats.argtypes = [vp, _ip, _ip, _dp]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.iegpu_record_features.argtypes = [vp, ctypes.c_int]
+    L.iegpu_box_features.argtypes = [vp, fp, ctypes.c_int]
+    L.iegpu_rank_model_init.argtypes = [vp, fp, fp, ctypes.c_int, ctypes.c_uint, fp]
+    L.iegpu_rank_model_train.argtypes = [vp, fp, fp, ctypes.c_int, fp, ctypes.c_int,
+                                         ctypes.c_float, ctypes.c_uint, fp]
+    L.iegpu_rank_model_grad.argtypes = [vp, fp, fp, fp, ctypes.c_int, fp, fp]
+    L.iegpu_set_rank_model.argtypes = [vp, fp, ctypes.c_int]
     _lib = L
     return L
 
@@ -100,6 +113,14 @@ def _pd(a):
 
 def _pi(a):
     return a.ctypes.data_as(_ip)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _pf(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 class RSSolver:
@@ -153,12 +174,52 @@ class RSSolver:
         sketch of sketch_rows rows, full-QRCP fallback)."""
         self._check(self.L.iegpu_set_id_method(self.h, method, sketch_rows))
 
-    def id_stats(self):
-        """(boxes whose ID came from the sketch, boxes that fell back)."""
+    def record_features(self, on=True):
+        self._check(self.L.iegpu_record_features(self.h, 1 if on else 0))
+
+    def box_features(self):
+        """(n_boxes, RANK_FEATURES) float32 features of the last factorization."""
+        nb = self.tree_info()[1]
+        out = np.zeros(nb * RANK_FEATURES, np.float32)
+        self._check(self.L.iegpu_box_features(self.h, _pf(out), out.size))
+        return out.reshape(nb, RANK_FEATURES)
+
+    def rank_model_init(self, X, y, seed=1):
+        X, y = _f(X), _f(y)
+        m = np.zeros(RANK_MODEL_LEN, np.float32)
+        self._check(self.L.iegpu_rank_model_init(self.h, _pf(X), _pf(y), len(y), seed, _pf(m)))
+        return m
+
+    def rank_model_train(self, model, X, y, steps=4000, lr=3e-3, seed=1):
+        """Trains a copy of model on the GPU; returns (model, final loss)."""
+        X, y, m = _f(X), _f(y), _f(model).copy()
+        loss = ctypes.c_float(0)
+        self._check(self.L.iegpu_rank_model_train(self.h, _pf(X), _pf(y), len(y), _pf(m), steps,
+                                                  lr, seed, ctypes.byref(loss)))
+        return m, loss.value
+
+    def rank_model_grad(self, model, X, y):
+        """(loss, gradient) of the mean squared error, from the GPU kernels."""
+        X, y, m = _f(X), _f(y), _f(model)
+        g = np.zeros(RANK_PARAMS, np.float32)
+        loss = ctypes.c_float(0)
+        self._check(self.L.iegpu_rank_model_grad(self.h, _pf(m), _pf(X), _pf(y), len(y), _pf(g),
+                                                 ctypes.byref(loss)))
+        return loss.value, g
+
+    def set_rank_model(self, model, margin=2):
+        m = _f(model)
+        self._check(self.L.iegpu_set_rank_model(self.h, _pf(m), margin))
+
+    def id_stats(self, with_flops=False):
+        """(boxes whose ID came from the sketch, boxes that fell back[, sketch
+        matrix-core flops])."""
         a = ctypes.c_int(0)
         b = ctypes.c_int(0)
-        self._check(self.L.iegpu_id_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+        f = ctypes.c_double(0)
+        self._check(self.L.iegpu_id_stats(self.h, ctypes.byref(a), ctypes.byref(b),
+                                          ctypes.byref(f)))
+        return (a.value, b.value, f.value) if with_flops else (a.value, b.value)
 
     def solve(self, b):
         b = _d(b)

@@ -157,7 +157,8 @@ int iegpu_profile(iegpu_ctx* ctx, int stage, int level, double* ms,
  * box's ID matrix with the rank cut |R_ii| < tol |R_00|
  * (ie_solver/ie_mat/ie_mat.cpp:548-588).  IEGPU_ID_RANDOMIZED: boxes with
  * at most 64 active DOFs (and at most 256 near-field row DOFs) take the ID
- * of a Gaussian sketch Omega A of l = sketch_rows rows (16 or 32) with the
+ * of a Gaussian sketch Omega A of l = sketch_rows rows (16 or 32; 0: per
+ * box from the rank model, iegpu_set_rank_model) with the
  * same cut applied to the sketch's R, |R_jj| scaled by 2 sqrt(l / (l - j))
  * (a sketch keeps l - j dimensions at step j); a box whose sketch shows no
  * cut within l - 6 pivots falls back to the full QRCP.  No reference
@@ -166,9 +167,50 @@ enum { IEGPU_ID_QRCP = 0, IEGPU_ID_RANDOMIZED = 1 };
 int iegpu_set_id_method(iegpu_ctx* ctx, int method, int sketch_rows);
 
 /* After a factorization with IEGPU_ID_RANDOMIZED: boxes whose ID came from
- * the sketch, and boxes that fell back to the full QRCP.  Either pointer
- * may be NULL. */
-int iegpu_id_stats(iegpu_ctx* ctx, int* n_sketched, int* n_fallback);
+ * the sketch, boxes that fell back to the full QRCP (their sketch wasted),
+ * and the sketches' matrix-core work in flops, sum of 2 l m n16 over the
+ * sketched and fallen-back boxes (l sketch rows, m ID rows, n16 = |active|
+ * rounded up to 16).  Any pointer may be NULL. */
+int iegpu_id_stats(iegpu_ctx* ctx, int* n_sketched, int* n_fallback, double* sketch_flops);
+
+/* Rank model for the randomized ID (no reference counterpart).  An MLP
+ * IEGPU_RANK_FEATURES -> 32 (tanh) -> 32 (tanh) -> 1 predicts each box's
+ * ID rank from features the factorization computes on the GPU: log2
+ * |active|, log2 ID rows, level, log2(near-field rows + 1), side x mean and
+ * max |curvature|, 1 - |mean normal| of the active DOFs, -log10 id_tol,
+ * Stokes flag.  The model is a float blob of IEGPU_RANK_MODEL_LEN: feature
+ * means and scales, W1 [9][32], b1, W2 [32][32], b2, w3, b3 (the
+ * IEGPU_RANK_PARAMS trained parameters), target mean and scale. */
+#define IEGPU_RANK_FEATURES 9
+#define IEGPU_RANK_MODEL_LEN 1429
+#define IEGPU_RANK_PARAMS 1409
+
+/* on != 0: the next factorizations record every box's features. */
+int iegpu_record_features(iegpu_ctx* ctx, int on);
+/* Features of every box (n_boxes x IEGPU_RANK_FEATURES, box creation
+ * order) from the last factorization that recorded them; boxes without an
+ * ID (m_rows 0 in iegpu_box_info) hold no meaningful row.  cap = floats
+ * available in feat. */
+int iegpu_box_features(iegpu_ctx* ctx, float* feat, int cap);
+/* A fresh model for samples X (n x IEGPU_RANK_FEATURES) with ranks y:
+ * normalization from the samples, weights uniform in +-sqrt(6/(in+out))
+ * from seed.  Host only (ctx may be a host-only context). */
+int iegpu_rank_model_init(iegpu_ctx* ctx, const float* X, const float* y, int n,
+                          unsigned seed, float* model);
+/* Trains model in place on the GPU: `steps` Adam steps (learning rate lr)
+ * on batches of 64 samples drawn with replacement, loss = mean squared
+ * error of the standardized rank; *loss (nullable) receives the mean loss
+ * of the last 64 steps. */
+int iegpu_rank_model_train(iegpu_ctx* ctx, const float* X, const float* y, int n, float* model,
+                           int steps, float lr, unsigned seed, float* loss);
+/* Loss over all n samples and its gradient (IEGPU_RANK_PARAMS floats, blob
+ * order) computed by the GPU kernels the training uses. */
+int iegpu_rank_model_grad(iegpu_ctx* ctx, const float* model, const float* X, const float* y,
+                          int n, float* grad, float* loss);
+/* Model used by iegpu_set_id_method(ctx, IEGPU_ID_RANDOMIZED, 0): a box's
+ * sketch has 16 rows if (predicted rank + margin) <= 10, 32 rows if it is
+ * <= 26, and none otherwise (the full QRCP takes the box directly). */
+int iegpu_set_rank_model(iegpu_ctx* ctx, const float* model, int margin);
 
 #ifdef __cplusplus
 }

@@ -332,3 +332,59 @@ def randomized_id(A, tol, rows, table=None, over=RID_OVER):
     R12 = Y[:k, :][:, red]
     T = np.linalg.solve(np.triu(R11), R12)
     return k, perm, T
+
+
+# ---------------------------------------------------------------------------
+# Rank model (no reference counterpart): numpy restatement of the product's
+# MLP (9 features -> 32 tanh -> 32 tanh -> 1) and of its training loss, the
+# checker of the GPU forward / backward kernels.  Blob layout as iegpu.h.
+# ---------------------------------------------------------------------------
+RANK_FEATURES = 9
+RANK_HID = 32
+_F, _H = RANK_FEATURES, RANK_HID
+_MU, _SD = 0, _F
+_W1 = 2 * _F
+_B1 = _W1 + _F * _H
+_W2 = _B1 + _H
+_B2 = _W2 + _H * _H
+_W3 = _B2 + _H
+_B3 = _W3 + _H
+_YMU = _B3 + 1
+_YSD = _YMU + 1
+RANK_MODEL_LEN = _YSD + 1
+RANK_PARAMS = _YMU - _W1
+
+
+def _mlp_parts(model):
+    m = np.asarray(model, dtype=np.float64)
+    return dict(mu=m[_MU:_SD], sd=m[_SD:_W1], W1=m[_W1:_B1].reshape(_F, _H), b1=m[_B1:_W2],
+                W2=m[_W2:_B2].reshape(_H, _H), b2=m[_B2:_W3], w3=m[_W3:_B3], b3=m[_B3],
+                ymu=m[_YMU], ysd=m[_YSD])
+
+
+def _mlp_forward(p, X):
+    x = (np.asarray(X, np.float64) - p["mu"]) / p["sd"]
+    h1 = np.tanh(x @ p["W1"] + p["b1"])
+    h2 = np.tanh(h1 @ p["W2"] + p["b2"])
+    return x, h1, h2, h2 @ p["w3"] + p["b3"]
+
+
+def mlp_predict(model, X):
+    """Predicted ranks of feature rows X."""
+    p = _mlp_parts(model)
+    return _mlp_forward(p, X)[3] * p["ysd"] + p["ymu"]
+
+
+def mlp_loss_grad(model, X, y):
+    """Mean squared error of the standardized rank and its gradient with
+    respect to the trained parameters (blob order W1, b1, W2, b2, w3, b3)."""
+    p = _mlp_parts(model)
+    x, h1, h2, out = _mlp_forward(p, X)
+    e = out - (np.asarray(y, np.float64) - p["ymu"]) / p["ysd"]
+    n = len(e)
+    dout = 2 * e / n
+    d2 = np.outer(dout, p["w3"]) * (1 - h2 ** 2)
+    d1 = (d2 @ p["W2"].T) * (1 - h1 ** 2)
+    g = np.concatenate([(x.T @ d1).ravel(), d1.sum(0), (h1.T @ d2).ravel(), d2.sum(0),
+                        h2.T @ dout, [dout.sum()]])
+    return float(np.mean(e ** 2)), g

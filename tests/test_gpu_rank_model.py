@@ -1,0 +1,97 @@
+"""GPU tests of the rank model (iegpu_rank_model_*): the training kernels'
+loss and gradient against the numpy oracle, training itself, and what the
+learned sketch widths gain over fixed widths on boundaries the model never
+saw.  No reference counterpart (the reference has no randomized ID)."""
+import numpy as np
+import pytest
+
+import geometry as G
+import iesolver
+import oracle
+import rank_model
+
+pytestmark = pytest.mark.gpu
+
+
+def _synthetic(n, seed):
+    rng = np.random.default_rng(seed)
+    X = (rng.normal(size=(n, oracle.RANK_FEATURES)) * [1, 2, 3, 1, .5, 2, .1, 1, .5] + 4)
+    y = 15 + 3 * np.tanh(X[:, 0] - 4) + X[:, 2] - 4 + 0.5 * np.sin(X[:, 1])
+    return X.astype(np.float32), y.astype(np.float32)
+
+
+def test_gradient_matches_oracle():
+    X, y = _synthetic(500, 0)
+    s = iesolver.RSSolver(0)
+    m = s.rank_model_init(X, y, seed=3)
+    m[2 * oracle.RANK_FEATURES:-2] += np.random.default_rng(4).normal(
+        size=oracle.RANK_PARAMS + 0).astype(np.float32)[:oracle.RANK_PARAMS] * 0.1
+    loss, g = s.rank_model_grad(m, X, y)
+    lo, go = oracle.mlp_loss_grad(m.astype(np.float64), X.astype(np.float64), y.astype(np.float64))
+    print(f"loss gpu {loss:.6e} oracle {lo:.6e}; grad max abs diff "
+          f"{np.abs(g - go).max():.3e} of max {np.abs(go).max():.3e}", flush=True)
+    assert abs(loss - lo) <= 1e-4 * abs(lo)
+    assert np.abs(g - go).max() <= 1e-4 * np.abs(go).max()
+
+
+def test_training_fits_a_known_function():
+    X, y = _synthetic(4000, 1)
+    Xt, yt = _synthetic(500, 2)
+    s = iesolver.RSSolver(0)
+    m0 = s.rank_model_init(X, y, seed=5)
+    m, loss = s.rank_model_train(m0, X, y, steps=3000, lr=3e-3, seed=5)
+    l0 = oracle.mlp_loss_grad(m0, X, y)[0]
+    l1 = oracle.mlp_loss_grad(m, X, y)[0]
+    mae = np.mean(np.abs(oracle.mlp_predict(m, Xt) - yt))
+    print(f"loss {l0:.3e} -> {l1:.3e} (kernel tail {loss:.3e}), held-out MAE {mae:.3f}", flush=True)
+    assert l1 <= 0.05 * l0
+    assert mae <= 0.3
+
+
+def test_learned_widths_beat_fixed_widths():
+    X, y = rank_model.collect(rank_model.training_cases())
+    model, loss = rank_model.train(X, y, steps=6000)
+    print(f"trained on {len(y)} boxes, loss {loss:.3e}", flush=True)
+    held = [("wavy 1e-10", G.wavy(65536, seed=0), iesolver.PDE_LAPLACE, 1e-10),
+            ("wavy 1e-6", G.wavy(65536, seed=0), iesolver.PDE_LAPLACE, 1e-6),
+            ("stokes holes 1e-10", G.circle_with_holes(8192, 8, 0), iesolver.PDE_STOKES, 1e-10)]
+    tot = dict(fixed=0.0, learned=0.0)
+    cost_fb = 2 * 32 * 1000 * 64.0  # a wasted 32-row sketch of a large box, in flops
+    for name, g, pde, tol in held:
+        Xh, yh = rank_model.collect([(g, pde, tol)])
+        mae = np.mean(np.abs(oracle.mlp_predict(model, Xh) - yh))
+        res = {}
+        for form in ("fixed", "learned"):
+            s = iesolver.RSSolver(0)
+            s.set_geometry(g, pde)
+            s.build_tree(64)
+            if g.get("holes") is not None:
+                s.set_holes(g["holes"]["centers"], g["holes"]["radii"])
+            if form == "learned":
+                s.set_rank_model(model, margin=2)
+                s.set_id_method(iesolver.ID_RANDOMIZED, 0)
+            else:
+                s.set_id_method(iesolver.ID_RANDOMIZED, 32)
+            s.factor(tol)
+            b = np.random.default_rng(0).standard_normal(s.ndof)
+            x, a = s.solve(b)
+            yv, z = s.apply(x, a)
+            bw = np.sqrt(np.sum((yv - b) ** 2) + (np.sum(z ** 2) if z is not None else 0))
+            res[form] = s.id_stats(with_flops=True) + (bw / np.linalg.norm(b),)
+            s.close()
+        (fs, ff, fw, fb), (ls, lf, lw, lb) = res["fixed"], res["learned"]
+        print(f"{name}: rank MAE {mae:.2f}; fixed 32: sketched {fs} fallback {ff} sketch "
+              f"{fw / 1e9:.2f} GF bw {fb:.1e}; learned: sketched {ls} fallback {lf} sketch "
+              f"{lw / 1e9:.2f} GF bw {lb:.1e}", flush=True)
+        assert lb <= 50 * tol
+        assert lf <= ff + 0.01 * (fs + ff)  # never more fallbacks ...
+        assert lw <= fw                      # ... nor more sketch work
+        tot["fixed"] += fw + cost_fb * ff
+        tot["learned"] += lw + cost_fb * lf
+    # Over the held-out cases the model saves sketch work where ranks are
+    # low (16-row sketches) and wasted sketches where they are high (a
+    # fallback pays its sketch and the full QRCP); at tol 1e-10 on the
+    # Laplace boundary every box needs 32 rows either way.
+    print(f"total sketch + wasted work: fixed {tot['fixed'] / 1e9:.2f} GF, learned "
+          f"{tot['learned'] / 1e9:.2f} GF", flush=True)
+    assert tot["learned"] < 0.8 * tot["fixed"]

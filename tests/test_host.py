@@ -101,7 +101,7 @@ def test_id_method_arguments():
     assert e.value.code == 1
     L = iesolver.load()
     assert L.iegpu_set_id_method(None, 0, 0) == 1
-    assert L.iegpu_id_stats(None, None, None) == 1
+    assert L.iegpu_id_stats(None, None, None, None) == 1
 
 
 def test_geometry_circle_closed_form():
