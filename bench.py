@@ -2,12 +2,20 @@
 """Factor + solve throughput of the RS path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config wavy262k]
+                    [--form learned|randomized|plain]
 
 A step is one recursive-skeletonization factorization plus one solve
 (SkelFactorization::skeletonize + linear_solve, timed as in
 bie_time_trial, ie_solver/linear_solve.cpp:156-185): tree build, geometry
 and data generation happen before the timed region, and the geometry and
 right-hand side are resident in HBM when it starts.  Prints one JSON line.
+
+Forms: "plain" is the reference's algorithm in FP64 (full pivoted-QR IDs);
+"randomized" takes each box's ID from a 32-row Gaussian sketch on the
+matrix cores (full-QRCP fallback); "learned" (default) sizes each box's
+sketch from the rank model, trained on the GPU before the timed region on
+other boundaries (ie-solver_amd/rank_model.py).  The plain form is timed
+beside the chosen one as its comparator.
 
 The CPU baseline runs the oracle (oracle/, a from-scratch restatement of
 the reference using LAPACK from scipy's OpenBLAS) on the same workload on
@@ -108,6 +116,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="wavy262k", choices=sorted(CONFIGS))
+    ap.add_argument("--form", default="learned", choices=["learned", "randomized", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-profile", action="store_true",
@@ -137,21 +146,55 @@ def main():
         s.factor(cfg["tol"], iesolver.FP64)
         s.solve_device(f_dev.data_ptr(), mu_dev.data_ptr(), None, None)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    if not args.no_profile:
-        s.set_profiling(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    prof = s.profile_table() if not args.no_profile else {}
-    s.set_profiling(False)
+    def timed(profile):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        if profile:
+            s.set_profiling(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        prof = s.profile_table() if profile else {}
+        s.set_profiling(False)
+        return (t1 - t0) * 1e3 / args.steps, prof
+
+    # Comparator: the plain FP64 form on the same workload.
+    s.set_id_method(iesolver.ID_QRCP)
+    plain_ms, plain_prof = timed(args.form == "plain" and not args.no_profile)
+    mu_plain = mu_dev.cpu().numpy()
+    model_info = None
+    if args.form == "plain":
+        ms_per_step, prof = plain_ms, plain_prof
+    else:
+        if args.form == "learned":
+            import rank_model
+            t0 = time.perf_counter()
+            X, y = rank_model.collect(rank_model.training_cases())
+            model, loss = rank_model.train(X, y)
+            s.set_rank_model(model, margin=2)
+            s.set_id_method(iesolver.ID_RANDOMIZED, 0)
+            model_info = dict(training_boxes=int(len(y)), final_loss=round(float(loss), 5),
+                              margin=2, seconds_untimed=round(time.perf_counter() - t0, 2),
+                              trained_on="stars k=3,5,9 N=16384, wavy seeds 1,2 N=65536, "
+                                         "Stokes holes N=16384 and star N=8192, tol 1e-6/8/10")
+        else:
+            s.set_id_method(iesolver.ID_RANDOMIZED, 32)
+        ms_per_step, prof = timed(not args.no_profile)
     mu = mu_dev.cpu().numpy()
+    # Accuracy against the exact discrete solution (untimed): this form's
+    # factorization as the preconditioner of FP64 iterative refinement with
+    # the matrix-free FP64 operator, to a relative residual of 1e-15.
+    mu_exact, _, ref_its, ref_res = s.solve_refined(f, 1e-15, 4)
+    err_exact = float(np.linalg.norm(mu - mu_exact) / np.linalg.norm(mu_exact))
+    err_exact_plain = float(np.linalg.norm(mu_plain - mu_exact) / np.linalg.norm(mu_exact))
+    id_stats = None
+    if args.form != "plain":
+        ns, nf, sf = s.id_stats(with_flops=True)
+        id_stats = dict(sketched=ns, fallback=nf, sketch_gflop=round(sf / 1e9, 3))
     info = s.box_info()
     nl, nb, nd, _ = s.tree_info()
 
@@ -172,8 +215,16 @@ def main():
         for name in ("assemble", "qrcp", "decouple", "sweep_up", "sweep_down"):
             ms, launches = prof[name]
             work, unit = stage_work(info, name)
+            if name == "qrcp" and id_stats is not None:
+                # The randomized ID's work: the sketches on the matrix cores
+                # (the boxes' full-QRCP flops are not executed).
+                work, unit = id_stats["sketch_gflop"] * 1e9, "flops"
             if launches and work:
                 cands.append((ms, name, launches, work, unit))
+        defs = dict(assemble="8 m n bytes per box", decouple="6 s^2 r + 8 r^2 s + 2/3 r^3 flops",
+                    sweep_up="8 (3 s r + r^2) factor bytes", sweep_down="8 (3 s r + r^2) factor bytes",
+                    qrcp="2 m n^2 - 2/3 n^3 + k^2 r flops (full pivoted QR)" if id_stats is None
+                    else "2 l m n16 sketch flops on the matrix cores")
         if cands:
             ms, name, launches, work, unit = max(cands)
             per_launch = work * args.steps / launches   # algorithmic work per launch
@@ -184,14 +235,14 @@ def main():
                                 peak=PEAK_HBM_GBS, unit="GB/s", frac=round(ach / PEAK_HBM_GBS, 6),
                                 traffic=None, launches=launches,
                                 avg_launch_us=round(avg_s * 1e6, 3),
-                                work_per_launch=per_launch)
+                                work_per_launch=per_launch, work_def=defs[name])
             else:
                 ach = per_launch / avg_s / 1e12
                 roofline = dict(kernel=name, bound="mfma", achieved=round(ach, 4),
                                 peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                                 frac=round(ach / PEAK_FP64_TFLOPS, 6), traffic=None,
                                 launches=launches, avg_launch_us=round(avg_s * 1e6, 3),
-                                work_per_launch=per_launch)
+                                work_per_launch=per_launch, work_def=defs[name])
 
     cpu = None
     parity = None
@@ -217,10 +268,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"laplace_{args.config}_plain_fp64", "pde": "laplace",
+        "config": {"workload": f"laplace_{args.config}_{args.form}_fp64", "pde": "laplace",
                    "n_unknowns": N, "id_tol": cfg["tol"], "leaf_dofs": cfg["leaf"],
-                   "levels": nl, "boxes": nb, "form": "plain_fp64",
+                   "levels": nl, "boxes": nb, "form": args.form + "_fp64",
                    "parallelism": "single_gpu"},
+        "plain_fp64_ms_per_step": round(plain_ms, 4),
+        "speedup_vs_plain_fp64": round(plain_ms / ms_per_step, 4),
+        "rel_err_vs_exact": err_exact,
+        "plain_fp64_rel_err_vs_exact": err_exact_plain,
+        "exact": f"FP64 refinement to relative residual {ref_res:.1e} ({ref_its} corrections)",
+        "rel_err_vs_plain_fp64": float(np.linalg.norm(mu - mu_plain) / np.linalg.norm(mu_plain)),
+        "id_stats": id_stats,
+        "rank_model": model_info,
         "rel_err_interior": interior_err,
         "rel_err_vs_cpu_oracle": parity,
         "stages_ms": stages_ms,
