@@ -117,6 +117,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="wavy262k", choices=sorted(CONFIGS))
     ap.add_argument("--form", default="learned", choices=["learned", "randomized", "plain"])
+    ap.add_argument("--rank-model", default=None,
+                    help="load a trained model (.npy from rank_model.py) instead of training")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 comparator (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-profile", action="store_true",
@@ -147,25 +151,34 @@ def main():
         s.solve_device(f_dev.data_ptr(), mu_dev.data_ptr(), None, None)
 
     def timed(profile):
+        """ms per step of K steps after W warm-up steps, timed without HIP
+        events; with profile, K more steps with per-stage HIP events around
+        every launch give the per-stage device times (the events add gaps
+        between launches, so they stay out of the timed steps)."""
         for _ in range(args.warmup):
             step()
-        torch.cuda.synchronize()
-        if profile:
-            s.set_profiling(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             step()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        prof = s.profile_table() if profile else {}
-        s.set_profiling(False)
+        prof = {}
+        if profile:
+            s.set_profiling(True)
+            for _ in range(args.steps):
+                step()
+            torch.cuda.synchronize()
+            prof = s.profile_table()
+            s.set_profiling(False)
         return (t1 - t0) * 1e3 / args.steps, prof
 
     # Comparator: the plain FP64 form on the same workload.
-    s.set_id_method(iesolver.ID_QRCP)
-    plain_ms, plain_prof = timed(args.form == "plain" and not args.no_profile)
-    mu_plain = mu_dev.cpu().numpy()
+    plain_ms, plain_prof, mu_plain = None, {}, None
+    if args.form == "plain" or not args.no_plain:
+        s.set_id_method(iesolver.ID_QRCP)
+        plain_ms, plain_prof = timed(args.form == "plain" and not args.no_profile)
+        mu_plain = mu_dev.cpu().numpy()
     model_info = None
     if args.form == "plain":
         ms_per_step, prof = plain_ms, plain_prof
@@ -173,14 +186,19 @@ def main():
         if args.form == "learned":
             import rank_model
             t0 = time.perf_counter()
-            X, y = rank_model.collect(rank_model.training_cases())
-            model, loss = rank_model.train(X, y)
+            if args.rank_model:
+                model = np.load(args.rank_model)
+                model_info = dict(loaded=args.rank_model, margin=2)
+            else:
+                X, y = rank_model.collect(rank_model.training_cases())
+                model, loss = rank_model.train(X, y)
+                model_info = dict(training_boxes=int(len(y)), final_loss=round(float(loss), 5),
+                                  margin=2, seconds_untimed=round(time.perf_counter() - t0, 2),
+                                  trained_on="stars k=3,5,9 N=16384, wavy seeds 1,2 N=65536, "
+                                             "Stokes holes N=16384 and star N=8192, "
+                                             "tol 1e-6/8/10")
             s.set_rank_model(model, margin=2)
             s.set_id_method(iesolver.ID_RANDOMIZED, 0)
-            model_info = dict(training_boxes=int(len(y)), final_loss=round(float(loss), 5),
-                              margin=2, seconds_untimed=round(time.perf_counter() - t0, 2),
-                              trained_on="stars k=3,5,9 N=16384, wavy seeds 1,2 N=65536, "
-                                         "Stokes holes N=16384 and star N=8192, tol 1e-6/8/10")
         else:
             s.set_id_method(iesolver.ID_RANDOMIZED, 32)
         ms_per_step, prof = timed(not args.no_profile)
@@ -190,7 +208,8 @@ def main():
     # the matrix-free FP64 operator, to a relative residual of 1e-15.
     mu_exact, _, ref_its, ref_res = s.solve_refined(f, 1e-15, 4)
     err_exact = float(np.linalg.norm(mu - mu_exact) / np.linalg.norm(mu_exact))
-    err_exact_plain = float(np.linalg.norm(mu_plain - mu_exact) / np.linalg.norm(mu_exact))
+    err_exact_plain = (float(np.linalg.norm(mu_plain - mu_exact) / np.linalg.norm(mu_exact))
+                       if mu_plain is not None else None)
     id_stats = None
     if args.form != "plain":
         ns, nf, sf = s.id_stats(with_flops=True)
@@ -272,12 +291,13 @@ def main():
                    "n_unknowns": N, "id_tol": cfg["tol"], "leaf_dofs": cfg["leaf"],
                    "levels": nl, "boxes": nb, "form": args.form + "_fp64",
                    "parallelism": "single_gpu"},
-        "plain_fp64_ms_per_step": round(plain_ms, 4),
-        "speedup_vs_plain_fp64": round(plain_ms / ms_per_step, 4),
+        "plain_fp64_ms_per_step": round(plain_ms, 4) if plain_ms else None,
+        "speedup_vs_plain_fp64": round(plain_ms / ms_per_step, 4) if plain_ms else None,
         "rel_err_vs_exact": err_exact,
         "plain_fp64_rel_err_vs_exact": err_exact_plain,
         "exact": f"FP64 refinement to relative residual {ref_res:.1e} ({ref_its} corrections)",
-        "rel_err_vs_plain_fp64": float(np.linalg.norm(mu - mu_plain) / np.linalg.norm(mu_plain)),
+        "rel_err_vs_plain_fp64": (float(np.linalg.norm(mu - mu_plain) / np.linalg.norm(mu_plain))
+                                  if mu_plain is not None else None),
         "id_stats": id_stats,
         "rank_model": model_info,
         "rel_err_interior": interior_err,

@@ -49,7 +49,8 @@ enum : int {
   kLstReg8 = 2,     // wide register QRCP (kQrcpReg8, kQrcpStack8)
   kLstRegHalf = 3,  // register QRCP, n <= 32 (8-wave workgroups)
   kLstRid = 4,      // randomized ID (sketch width > 0)
-  kNumLists = 5,
+  kLstFallback = 5, // randomized mode: register-QRCP boxes without a successful sketch
+  kNumLists = 6,
 };
 constexpr int kHalfCols = 32;
 

@@ -257,16 +257,23 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
 // Exclusive scans of the per-box sizes of one level (single workgroup).
 // ID scratch and row lists are level-local; factor slots append to the
 // running pool offset.
+// With the randomized ID, a register-QRCP box that gets no sketch (not
+// eligible, or sent straight to the full QRCP by the rank model) starts the
+// level's full-QRCP list; k_rid appends the boxes whose sketch fails.
+__device__ inline bool rid_full_qrcp(const DevState& s, int b, int kind) {
+  return s.rid_mode != 0 && (kind == kQrcpReg || kind == kQrcpStack) && s.sk_w[b] == 0;
+}
+
 __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
                                                int count, int level) {
   // Scanned per box: ID scratch, row-list and factor sizes, register-QRCP
   // and general-QRCP list entries, row-compression jobs.
-  constexpr int NQ = 9;
+  constexpr int NQ = 10;
   __shared__ int64_t sh[NQ][1024];
   const int tid = threadIdx.x;
   const int chunk = (count + 1023) / 1024;
   const int lo = min(count, tid * chunk), hi = min(count, lo + chunk);
-  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
     const int kind = s.qkind[b];
@@ -279,6 +286,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     own[5] += row_jobs(s, b);
     own[6] += (kind == kQrcpReg8 || kind == kQrcpStack8) ? 1 : 0;
     own[8] += (s.sk_w[b] > 0) ? 1 : 0;
+    own[9] += rid_full_qrcp(s, b, kind) ? 1 : 0;
   }
 #pragma unroll
   for (int q = 0; q < NQ; q++) sh[q][tid] = own[q];
@@ -301,6 +309,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
   int* lreg8 = s.lists + (int64_t)kLstReg8 * s.nbox + start;
   int* lhalf = s.lists + (int64_t)kLstRegHalf * s.nbox + start;
   int* lrid = s.lists + (int64_t)kLstRid * s.nbox + start;
+  int* lfb = s.lists + (int64_t)kLstFallback * s.nbox + start;
   int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
@@ -318,6 +327,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     if (kind == kQrcpLds) llds[off[4]++] = b;
     if (kind == kQrcpReg8 || kind == kQrcpStack8) lreg8[off[6]++] = b;
     if (s.sk_w[b] > 0) lrid[off[8]++] = b;
+    if (rid_full_qrcp(s, b, kind)) lfb[off[9]++] = b;
     {
       const int nch = row_jobs(s, b);
       for (int q = 0; q < nch; q++) {
@@ -335,6 +345,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     list_slot(s, level, kLstReg8)[0] = (int)sh[6][1023];
     list_slot(s, level, kLstRegHalf)[0] = (int)sh[7][1023];
     list_slot(s, level, kLstRid)[0] = (int)sh[8][1023];
+    list_slot(s, level, kLstFallback)[0] = (int)sh[9][1023];  // k_rid appends
     s.njobs[level] = (int)sh[5][1023];
     int e = 0;
     if (ta > s.id_cap) {
@@ -654,10 +665,21 @@ __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevGeom g, DevTree t, Dev
   const int cls = slot_class<CPW>(lane);
   const int c0 = CPW * wv;
   const int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
-  const int njobs = s.njobs[level];
-  for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
-    const int b = jobs[2 * job], q = jobs[2 * job + 1];
-    if (s.qkind[b] != kind || s.rid_done[b]) continue;
+  // With the randomized ID only its fallbacks need row compression: their
+  // (box, chunk) pairs come from the fallback list.
+  const bool fb = kind == kQrcpStack && s.rid_mode != 0;
+  const int total = fb ? list_count(s, level, kLstFallback) * kMaxChunks : s.njobs[level];
+  for (int job = blockIdx.x; job < total; job += gridDim.x) {
+    int b, q;
+    if (fb) {
+      b = list_at(s, start, kLstFallback, job / kMaxChunks);
+      q = job % kMaxChunks;
+      if (s.qkind[b] != kind || q >= id_chunks(s.m_rows[b], MAXM)) continue;
+    } else {
+      b = jobs[2 * job];
+      q = jobs[2 * job + 1];
+      if (s.qkind[b] != kind) continue;
+    }
     const int n = s.act_cnt[b], m = s.m_rows[b];
     const int nch = id_chunks(m, MAXM);
     const int ch = (m + nch - 1) / nch;
@@ -941,21 +963,26 @@ struct QrcpShape {
 
 // One workgroup per list entry for the n <= 64 shapes (a loop over boxes
 // would keep more values live across the step loop than the registers
-// hold); the few n <= 128 boxes share a small strided grid.
-template <typename R, int CPW, int RPL, int NW>
+// hold); the few n <= 128 boxes share a small strided grid, and so do the
+// randomized ID's few fallbacks (STRIDE: the loop's spills stay there).
+template <typename R, int CPW, int RPL, int NW, bool STRIDE>
 __global__ __launch_bounds__(64 * NW, (QrcpShape<CPW, RPL, NW>::kWavesPerSimd)) void k_qrcp_id(
     DevGeom g, DevTree t, DevState s, int start, int level, double tol_d) {
   constexpr int L = QrcpShape<CPW, RPL, NW>::kList;
   if (!pools_ok(s)) return;
   const int nl = list_count(s, level, L);
-  if constexpr (L != kLstReg8) {
-    if ((int)blockIdx.x < nl) {
-      const int b = list_at(s, start, L, blockIdx.x);
-      if (!s.rid_done[b]) qrcp_id_box<R, CPW, RPL, NW>(g, t, s, start, level, b, tol_d);
-    }
+  if constexpr (!STRIDE) {
+    if ((int)blockIdx.x < nl)
+      qrcp_id_box<R, CPW, RPL, NW>(g, t, s, start, level, list_at(s, start, L, blockIdx.x), tol_d);
   } else {
-    for (int li = blockIdx.x; li < nl; li += gridDim.x) {
-      qrcp_id_box<R, CPW, RPL, NW>(g, t, s, start, level, list_at(s, start, L, li), tol_d);
+    // With the randomized ID the n <= 64 shapes take only its fallbacks.
+    const bool fb = L != kLstReg8 && s.rid_mode != 0;
+    const int LL = fb ? kLstFallback : L;
+    const int cnt = list_count(s, level, LL);
+    for (int li = blockIdx.x; li < cnt; li += gridDim.x) {
+      const int b = list_at(s, start, LL, li);
+      if (fb && (L == kLstRegHalf) != (s.act_cnt[b] <= kHalfCols)) continue;
+      qrcp_id_box<R, CPW, RPL, NW>(g, t, s, start, level, b, tol_d);
       __syncthreads();
     }
   }
@@ -1205,8 +1232,14 @@ __global__ __launch_bounds__(256, 3) void k_rid(DevGeom g, DevTree t, DevState s
     wave_sync();
   }
   if (k < 0) {
-    if (j < n) return;  // the sketch ran out before a cut: full QRCP
-    k = n;              // every column taken: nothing redundant
+    if (j < n) {  // the sketch ran out before a cut: the full QRCP takes it
+      if (lane == 0) {
+        const int pos = atomicAdd(list_slot(s, level, kLstFallback), 1);
+        s.lists[(int64_t)kLstFallback * s.nbox + start + pos] = b;
+      }
+      return;
+    }
+    k = n;  // every column taken: nothing redundant
   }
 
   // T = R11^{-1} R12, column sweep in place on the rows of R: R11(i, q) =
@@ -2294,12 +2327,19 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
           k_rid<R, kPdeStokes><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
         else
           k_rid<R, kPdeLaplace><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
+        // Only the boxes without a successful sketch reach the full QRCP.
+        const int grid = min(count, 64);
+        k_rowcomp<R, 4><<<grid, kXThreads, 0, st>>>(g, t, s, start, level);
+        k_qrcp_id<R, 4, 8, 8, true><<<grid, 512, 0, st>>>(g, t, s, start, level, tol);
+        k_qrcp_id<R, 4, 8, 16, true><<<grid, kXThreads, 0, st>>>(g, t, s, start, level, tol);
+      } else {
+        k_rowcomp<R, 4><<<min(2 * count, 256), kXThreads, 0, st>>>(g, t, s, start, level);
+        k_qrcp_id<R, 4, 8, 8, false><<<count, 512, 0, st>>>(g, t, s, start, level, tol);
+        k_qrcp_id<R, 4, 8, 16, false><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
       }
-      k_rowcomp<R, 4><<<min(2 * count, 256), kXThreads, 0, st>>>(g, t, s, start, level);
       k_rowcomp<R, 8><<<min(2 * count, 64), kXThreads, 0, st>>>(g, t, s, start, level);
-      k_qrcp_id<R, 4, 8, 8><<<count, 512, 0, st>>>(g, t, s, start, level, tol);
-      k_qrcp_id<R, 4, 8, 16><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
-      k_qrcp_id<R, 8, 4, 16><<<min(count, 64), kXThreads, 0, st>>>(g, t, s, start, level, tol);
+      k_qrcp_id<R, 8, 4, 16, true><<<min(count, 64), kXThreads, 0, st>>>(g, t, s, start, level,
+                                                                         tol);
       k_qrcp<R, 512><<<min(count, 64), 512, 0, st>>>(s, start, level, tol);
       break;
     case kStageDecouple:

@@ -58,3 +58,22 @@ def train(X, y, steps=6000, lr=3e-3, seed=1, device=0):
     m, loss = s.rank_model_train(m0, X, y, steps, lr, seed)
     s.close()
     return m, loss
+
+
+def main(argv=None):
+    """python ie-solver_amd/rank_model.py --out FILE: collect the training
+    boxes, train on GPU 0, save the model blob (float32 .npy)."""
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--steps", type=int, default=6000)
+    args = ap.parse_args(argv)
+    X, y = collect(training_cases())
+    model, loss = train(X, y, steps=args.steps)
+    np.save(args.out, model.astype(np.float32))
+    print(f"trained on {len(y)} boxes, final loss {loss:.4f} -> {args.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
