@@ -41,6 +41,15 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 
+# HBM bytes per launch of the dominant kernels of the learned form at
+# wavy262k, from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
+# calibrated on a copy kernel of known size: profiles/*_hbm_traffic.md.
+try:
+    with open(os.path.join(ROOT, "profiles", "hbm_traffic_wavy262k_learned.json")) as _f:
+        TRAFFIC = {k: v["bytes_per_launch"] for k, v in json.load(_f).items()}
+except (OSError, ValueError, KeyError):
+    TRAFFIC = {}
+
 CONFIGS = {
     # Laplace N=262144 on the seeded wavy boundary (the headline workload).
     "wavy262k": dict(shape="wavy", n=262144, seed=0, tol=1e-10, leaf=64),
@@ -231,35 +240,40 @@ def main():
             if launches:
                 stages_ms[name] = round(ms / args.steps, 4)
         cands = []
-        for name in ("assemble", "qrcp", "decouple", "sweep_up", "sweep_down"):
+        for name in ("assemble", "rid", "qrcp", "decouple", "sweep_up", "sweep_down"):
             ms, launches = prof[name]
-            work, unit = stage_work(info, name)
-            if name == "qrcp" and id_stats is not None:
-                # The randomized ID's work: the sketches on the matrix cores
-                # (the boxes' full-QRCP flops are not executed).
-                work, unit = id_stats["sketch_gflop"] * 1e9, "flops"
+            if name == "rid":
+                # k_rid's work: the sketches on the matrix cores.
+                work, unit = (id_stats["sketch_gflop"] * 1e9, "flops") if id_stats else (None, None)
+            elif name == "qrcp" and id_stats is not None:
+                # What is left to the full QRCP: its boxes' flops are not
+                # separated from the sketched boxes', so it is not a candidate.
+                work, unit = None, None
+            else:
+                work, unit = stage_work(info, name)
             if launches and work:
                 cands.append((ms, name, launches, work, unit))
         defs = dict(assemble="8 m n bytes per box", decouple="6 s^2 r + 8 r^2 s + 2/3 r^3 flops",
                     sweep_up="8 (3 s r + r^2) factor bytes", sweep_down="8 (3 s r + r^2) factor bytes",
-                    qrcp="2 m n^2 - 2/3 n^3 + k^2 r flops (full pivoted QR)" if id_stats is None
-                    else "2 l m n16 sketch flops on the matrix cores")
+                    qrcp="2 m n^2 - 2/3 n^3 + k^2 r flops (full pivoted QR)",
+                    rid="2 l m n16 sketch flops on the matrix cores (k_rid)")
         if cands:
             ms, name, launches, work, unit = max(cands)
             per_launch = work * args.steps / launches   # algorithmic work per launch
             avg_s = ms / launches * 1e-3
+            traffic = TRAFFIC.get(name) if args.form == "learned" else None
             if unit == "bytes":
                 ach = per_launch / avg_s / 1e9
                 roofline = dict(kernel=name, bound="hbm", achieved=round(ach, 3),
                                 peak=PEAK_HBM_GBS, unit="GB/s", frac=round(ach / PEAK_HBM_GBS, 6),
-                                traffic=None, launches=launches,
+                                traffic=traffic, launches=launches,
                                 avg_launch_us=round(avg_s * 1e6, 3),
                                 work_per_launch=per_launch, work_def=defs[name])
             else:
                 ach = per_launch / avg_s / 1e12
                 roofline = dict(kernel=name, bound="mfma", achieved=round(ach, 4),
                                 peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                                frac=round(ach / PEAK_FP64_TFLOPS, 6), traffic=None,
+                                frac=round(ach / PEAK_FP64_TFLOPS, 6), traffic=traffic,
                                 launches=launches, avg_launch_us=round(avg_s * 1e6, 3),
                                 work_per_launch=per_launch, work_def=defs[name])
 

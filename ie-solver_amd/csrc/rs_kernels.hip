@@ -2321,12 +2321,16 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
     case kStageAssemble:
       k_assemble<R><<<min(count, 64), 256, 0, st>>>(g, t, s, start, level);
       break;
-    case kStageQrcp:
+    case kStageRid:
       if (s.rid_mode != 0) {
         if (g.pde == kPdeStokes)
           k_rid<R, kPdeStokes><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
         else
           k_rid<R, kPdeLaplace><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
+      }
+      break;
+    case kStageQrcp:
+      if (s.rid_mode != 0) {
         // Only the boxes without a successful sketch reach the full QRCP.
         const int grid = min(count, 64);
         k_rowcomp<R, 4><<<grid, kXThreads, 0, st>>>(g, t, s, start, level);

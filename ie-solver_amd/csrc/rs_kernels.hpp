@@ -23,9 +23,10 @@ enum FactorStage : int {
   kStageRows = 1,     // ID row counts and slot sizes
   kStageScan = 2,     // slot offsets
   kStageAssemble = 3, // ID matrices
-  kStageQrcp = 4,     // pivoted QR, rank, T
-  kStageDecouple = 5, // Schur complement, X_rr LU, L, U
-  kNumFactorStages = 6
+  kStageRid = 4,      // randomized ID (Gaussian sketch + QR of the sketch)
+  kStageQrcp = 5,     // pivoted QR, rank, T
+  kStageDecouple = 6, // Schur complement, X_rr LU, L, U
+  kNumFactorStages = 7
 };
 
 template <typename R>

@@ -47,7 +47,7 @@ RANK_FEATURES = 9
 RANK_MODEL_LEN = 1429
 RANK_PARAMS = 1409
 
-STAGES = ["active", "rows", "scan", "assemble", "qrcp", "decouple", "root",
+STAGES = ["active", "rows", "scan", "assemble", "rid", "qrcp", "decouple", "root",
           "sweep_up", "root_solve", "sweep_down", "border"]
 
 _dp = ctypes.POINTER(ctypes.c_double)

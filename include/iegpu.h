@@ -145,9 +145,9 @@ int iegpu_last_timings(const iegpu_ctx* ctx, double* t, int nt);
  * the launching stream (stands in for the reference's omp_get_wtime
  * accumulators, skel_factorization.cpp:45-56,155-166).  on != 0 enables
  * and resets the counters.  Stages: 0 active lists, 1 ID row counts,
- * 2 slot offsets, 3 ID-matrix assembly, 4 pivoted QR, 5 decouple, 6 root,
- * 7 upward sweep, 8 root / bordered solve, 9 downward sweep, 10 bordered
- * factor.  level = -1 sums all levels. */
+ * 2 slot offsets, 3 ID-matrix assembly, 4 randomized ID (sketches), 5 full
+ * pivoted QR, 6 decouple, 7 root, 8 upward sweep, 9 root / bordered solve,
+ * 10 downward sweep, 11 bordered factor.  level = -1 sums all levels. */
 int iegpu_set_profiling(iegpu_ctx* ctx, int on);
 int iegpu_profile(iegpu_ctx* ctx, int stage, int level, double* ms,
                   int* launches);
