@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--form", default="learned", choices=["learned", "randomized", "plain"])
     ap.add_argument("--rank-model", default=None,
                     help="load a trained model (.npy from rank_model.py) instead of training")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the factorization kernel by kernel instead of from a HIP graph")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 comparator (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -151,6 +153,7 @@ def main():
     s = iesolver.RSSolver(0)
     s.set_geometry(g, iesolver.PDE_LAPLACE)
     s.build_tree(cfg["leaf"])
+    s.set_graphs(not args.no_graphs)  # both forms: factorization replayed from a HIP graph
     f_dev = torch.from_numpy(f).to(dev)
     mu_dev = torch.empty_like(f_dev)
     torch.cuda.synchronize()
@@ -197,16 +200,16 @@ def main():
             t0 = time.perf_counter()
             if args.rank_model:
                 model = np.load(args.rank_model)
-                model_info = dict(loaded=args.rank_model, margin=2)
+                model_info = dict(loaded=args.rank_model, margin=3)
             else:
                 X, y = rank_model.collect(rank_model.training_cases())
                 model, loss = rank_model.train(X, y)
                 model_info = dict(training_boxes=int(len(y)), final_loss=round(float(loss), 5),
-                                  margin=2, seconds_untimed=round(time.perf_counter() - t0, 2),
+                                  margin=3, seconds_untimed=round(time.perf_counter() - t0, 2),
                                   trained_on="stars k=3,5,9 N=16384, wavy seeds 1,2 N=65536, "
                                              "Stokes holes N=16384 and star N=8192, "
                                              "tol 1e-6/8/10")
-            s.set_rank_model(model, margin=2)
+            s.set_rank_model(model, margin=3)
             s.set_id_method(iesolver.ID_RANDOMIZED, 0)
         else:
             s.set_id_method(iesolver.ID_RANDOMIZED, 32)
@@ -312,6 +315,7 @@ def main():
         "exact": f"FP64 refinement to relative residual {ref_res:.1e} ({ref_its} corrections)",
         "rel_err_vs_plain_fp64": (float(np.linalg.norm(mu - mu_plain) / np.linalg.norm(mu_plain))
                                   if mu_plain is not None else None),
+        "hip_graph": not args.no_graphs,
         "id_stats": id_stats,
         "rank_model": model_info,
         "rel_err_interior": interior_err,

@@ -191,6 +191,12 @@ struct iegpu_ctx {
   // randomized ID
   int rid_mode = 0, rid_width = 0;
   DevBuf<double> d_omega;
+  // factorization replayed as a captured HIP graph (iegpu_set_graphs)
+  bool use_graphs = false;
+  hipGraphExec_t fgraph = nullptr;
+  DevState fgraph_state{};
+  double fgraph_tol = 0;
+  int fgraph_prec = -1;
   // rank model
   bool have_model = false, record_features = false;
   int rid_margin = 0;
@@ -236,7 +242,7 @@ struct iegpu_ctx {
     return t;
   }
   DevState dstate() {
-    DevState s;
+    DevState s{};  // zeroed padding: states are compared bytewise
     s.ncap = ncap;
     s.act_cnt = s_act_cnt.p;
     s.act_list = s_act_list.p;
@@ -519,6 +525,7 @@ int iegpu_destroy(iegpu_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
   }
   c->prof.release();
+  if (c->fgraph) (void)hipGraphExecDestroy(c->fgraph);
   hipStream_t st = c->stream;
   delete c;
   if (st) (void)hipStreamDestroy(st);
@@ -671,6 +678,12 @@ int iegpu_set_id_method(iegpu_ctx* c, int method, int sketch_rows) {
     return fail(c, IEGPU_ERR_INVALID, "sketch_rows must be 0, 16 or 32");
   c->rid_mode = 1;
   c->rid_width = sketch_rows;
+  return IEGPU_OK;
+}
+
+int iegpu_set_graphs(iegpu_ctx* c, int on) {
+  if (!c) return IEGPU_ERR_INVALID;
+  c->use_graphs = on != 0;
   return IEGPU_OK;
 }
 
@@ -896,7 +909,27 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
         c->d_ybuf.alloc(ns_max);
       }
       HIP_TRY(hipEventRecord(e0, c->stream));
-      if (precision == IEGPU_FP32) {
+      if (c->use_graphs && !c->prof.on && c->hole_dim == 0) {
+        // Capture the launch sequence once per state; replay it after.
+        const DevState cur = c->dstate();
+        if (!c->fgraph || std::memcmp(&cur, &c->fgraph_state, sizeof(DevState)) != 0 ||
+            c->fgraph_tol != id_tol || c->fgraph_prec != precision) {
+          if (c->fgraph) HIP_TRY(hipGraphExecDestroy(c->fgraph));
+          c->fgraph = nullptr;
+          HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+          if (precision == IEGPU_FP32) enqueue_factor<float>(c, id_tol);
+          else enqueue_factor<double>(c, id_tol);
+          hipGraph_t gr = nullptr;
+          HIP_TRY(hipStreamEndCapture(c->stream, &gr));
+          const hipError_t ie = hipGraphInstantiate(&c->fgraph, gr, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(gr);
+          HIP_TRY(ie);
+          c->fgraph_state = cur;
+          c->fgraph_tol = id_tol;
+          c->fgraph_prec = precision;
+        }
+        HIP_TRY(hipGraphLaunch(c->fgraph, c->stream));
+      } else if (precision == IEGPU_FP32) {
         enqueue_factor<float>(c, id_tol);
       } else {
         enqueue_factor<double>(c, id_tol);

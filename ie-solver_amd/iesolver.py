@@ -40,7 +40,7 @@ EXPORTS = [
     "iegpu_solve_device", "iegpu_tree_info", "iegpu_box_info", "iegpu_box_list",
     "iegpu_last_timings", "iegpu_set_profiling", "iegpu_profile", "iegpu_evaluate",
     "iegpu_solve_refined", "iegpu_apply", "iegpu_set_id_method", "iegpu_id_stats",
-    "iegpu_record_features", "iegpu_box_features", "iegpu_rank_model_init",
+    "iegpu_set_graphs", "iegpu_record_features", "iegpu_box_features", "iegpu_rank_model_init",
     "iegpu_rank_model_train", "iegpu_rank_model_grad", "iegpu_set_rank_model",
 ]
 RANK_FEATURES = 9
@@ -93,6 +93,7 @@ def load():
     L.iegpu_id_stats.argtypes = [vp, _ip, _ip, _dp]
     fp = ctypes.POINTER(ctypes.c_float)
     L.iegpu_record_features.argtypes = [vp, ctypes.c_int]
+    L.iegpu_set_graphs.argtypes = [vp, ctypes.c_int]
     L.iegpu_box_features.argtypes = [vp, fp, ctypes.c_int]
     L.iegpu_rank_model_init.argtypes = [vp, fp, fp, ctypes.c_int, ctypes.c_uint, fp]
     L.iegpu_rank_model_train.argtypes = [vp, fp, fp, ctypes.c_int, fp, ctypes.c_int,
@@ -173,6 +174,10 @@ class RSSolver:
         """ID_QRCP (the reference's full pivoted QR) or ID_RANDOMIZED (Gaussian
         sketch of sketch_rows rows, full-QRCP fallback)."""
         self._check(self.L.iegpu_set_id_method(self.h, method, sketch_rows))
+
+    def set_graphs(self, on=True):
+        """Replay the factorization as a captured HIP graph (iegpu_set_graphs)."""
+        self._check(self.L.iegpu_set_graphs(self.h, 1 if on else 0))
 
     def record_features(self, on=True):
         self._check(self.L.iegpu_record_features(self.h, 1 if on else 0))

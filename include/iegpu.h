@@ -152,6 +152,13 @@ int iegpu_set_profiling(iegpu_ctx* ctx, int on);
 int iegpu_profile(iegpu_ctx* ctx, int stage, int level, double* ms,
                   int* launches);
 
+/* on != 0: iegpu_factor captures its launch sequence (every level's
+ * kernels, the root) into a HIP graph and replays it on later calls while
+ * the tolerance, precision, ID method and workspaces are unchanged (a
+ * change, or a workspace growth, recaptures).  Not used while per-stage
+ * profiling is on or with holes.  No reference counterpart. */
+int iegpu_set_graphs(iegpu_ctx* ctx, int on);
+
 /* Interpolative decomposition used by the next iegpu_factor.
  * IEGPU_ID_QRCP (default): the reference's full column-pivoted QR of each
  * box's ID matrix with the rank cut |R_ii| < tol |R_00|
