@@ -2152,16 +2152,31 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
 // scattered back (boxes of one level touch disjoint DOFs).
 // ---------------------------------------------------------------------------
 
+// sum_j p[stride j] v[j], j < len: eight loads in flight, four partial
+// sums (a sweep's per-lane loop is otherwise one load latency per term).
+template <typename R>
+__device__ inline double dot_ld(const R* p, int64_t stride, const double* v, int len) {
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int j = 0;
+  for (; j + 8 <= len; j += 8) {
+    R q[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) q[u] = p[stride * (j + u)];
+    a0 += (double)q[0] * v[j] + (double)q[4] * v[j + 4];
+    a1 += (double)q[1] * v[j + 1] + (double)q[5] * v[j + 5];
+    a2 += (double)q[2] * v[j + 2] + (double)q[6] * v[j + 6];
+    a3 += (double)q[3] * v[j + 3] + (double)q[7] * v[j + 7];
+  }
+  for (; j < len; j++) a0 += (double)p[stride * j] * v[j];
+  return (a0 + a1) + (a2 + a3);
+}
+
 // x <- X^{-1} x with an explicit inverse (r x r, ld r): one wave, x and
 // tmp in LDS.
 template <typename R>
 __device__ void wave_inv_apply(const R* Xi, int r, double* x, double* tmp) {
   const int lane = threadIdx.x & 63;
-  for (int i = lane; i < r; i += 64) {
-    double acc = 0;
-    for (int j = 0; j < r; j++) acc += (double)Xi[i + (int64_t)r * j] * x[j];
-    tmp[i] = acc;
-  }
+  for (int i = lane; i < r; i += 64) tmp[i] = dot_ld(Xi + i, r, x, r);
   wave_sync();
   for (int i = lane; i < r; i += 64) x[i] = tmp[i];
   wave_sync();
@@ -2188,23 +2203,11 @@ __global__ __launch_bounds__(64) void k_sweep_up(DevState s, int start,
     for (int a = lane; a < k; a += 64) xs[a] = x[sk[a]];
     for (int i = lane; i < r; i += 64) xr[i] = x[rd[i]];
     wave_sync();
-    for (int i = lane; i < r; i += 64) {
-      double acc = xr[i];
-      const R* tc = F + fl.T + (int64_t)k * i;
-      for (int a = 0; a < k; a++) acc -= (double)tc[a] * xs[a];
-      xr[i] = acc;
-    }
+    for (int i = lane; i < r; i += 64) xr[i] -= dot_ld(F + fl.T + (int64_t)k * i, 1, xs, k);
     wave_sync();
     for (int a = lane; a < k; a += 64) {
-      double acc = xs[a];
-      if (use_Ut) {
-        const R* uc = F + fl.U + (int64_t)r * a;
-        for (int i = 0; i < r; i++) acc -= (double)uc[i] * xr[i];
-      } else {
-        const R* L = F + fl.L;
-        for (int i = 0; i < r; i++) acc -= (double)L[a + (int64_t)k * i] * xr[i];
-      }
-      xs[a] = acc;
+      if (use_Ut) xs[a] -= dot_ld(F + fl.U + (int64_t)r * a, 1, xr, r);
+      else xs[a] -= dot_ld(F + fl.L + a, k, xr, r);
     }
     wave_sync();
     if (diag) wave_inv_apply(F + fl.X, r, xr, tmp);
@@ -2255,19 +2258,9 @@ __global__ __launch_bounds__(64) void k_sweep_down(DevState s, int start,
     for (int a = lane; a < k; a += 64) xs[a] = x[sk[a]];
     for (int i = lane; i < r; i += 64) xr[i] = x[rd[i]];
     wave_sync();
-    for (int i = lane; i < r; i += 64) {
-      double acc = xr[i];
-      const R* U = F + fl.U;
-      for (int a = 0; a < k; a++) acc -= (double)U[i + (int64_t)r * a] * xs[a];
-      xr[i] = acc;
-    }
+    for (int i = lane; i < r; i += 64) xr[i] -= dot_ld(F + fl.U + i, r, xs, k);
     wave_sync();
-    for (int a = lane; a < k; a += 64) {
-      double acc = xs[a];
-      const R* T = F + fl.T;
-      for (int i = 0; i < r; i++) acc -= (double)T[a + (int64_t)k * i] * xr[i];
-      xs[a] = acc;
-    }
+    for (int a = lane; a < k; a += 64) xs[a] -= dot_ld(F + fl.T + a, k, xr, r);
     wave_sync();
     for (int a = lane; a < k; a += 64) x[sk[a]] = xs[a];
     for (int i = lane; i < r; i += 64) x[rd[i]] = xr[i];
@@ -2275,21 +2268,31 @@ __global__ __launch_bounds__(64) void k_sweep_down(DevState s, int start,
   }
 }
 
+// x_root <- (root inverse) x_root: 256 threads, one row each per pass.
 template <typename R>
-__global__ __launch_bounds__(64) void k_root_solve(DevState s, SolveVec v) {
+__global__ __launch_bounds__(256) void k_root_solve(DevState s, SolveVec v) {
   __shared__ double xa[kMaxBoxDofs];
-  __shared__ double tmp[kMaxBoxDofs];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   const int n = s.act_cnt[0];
   const int* act = s.act_list;
   const R* Ai = reinterpret_cast<const R*>(s.root_mat) + (int64_t)n * n;
   for (int c = 0; c < v.nrhs; c++) {
     double* x = v.x + v.ldx * c;
-    for (int i = lane; i < n; i += 64) xa[i] = x[act[i]];
-    wave_sync();
-    wave_inv_apply(Ai, n, xa, tmp);
-    for (int i = lane; i < n; i += 64) x[act[i]] = xa[i];
-    wave_sync();
+    for (int i = tid; i < n; i += 256) xa[i] = x[act[i]];
+    __syncthreads();
+    double y[kMaxBoxDofs / 256];
+#pragma unroll
+    for (int q = 0; q < kMaxBoxDofs / 256; q++) {
+      const int i = tid + 256 * q;
+      y[q] = i < n ? dot_ld(Ai + i, n, xa, n) : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMaxBoxDofs / 256; q++) {
+      const int i = tid + 256 * q;
+      if (i < n) x[act[i]] = y[q];
+    }
+    __syncthreads();
   }
 }
 
@@ -2333,7 +2336,9 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       if (s.rid_mode != 0) {
         // Only the boxes without a successful sketch reach the full QRCP.
         const int grid = min(count, 64);
-        k_rowcomp<R, 4><<<grid, kXThreads, 0, st>>>(g, t, s, start, level);
+        // (box, chunk) jobs of the listed boxes: up to kMaxChunks per box
+        k_rowcomp<R, 4><<<min(count * kMaxChunks, 256), kXThreads, 0, st>>>(g, t, s, start,
+                                                                           level);
         k_qrcp_id<R, 4, 8, 8, true><<<grid, 512, 0, st>>>(g, t, s, start, level, tol);
         k_qrcp_id<R, 4, 8, 16, true><<<grid, kXThreads, 0, st>>>(g, t, s, start, level, tol);
       } else {
@@ -2386,7 +2391,7 @@ void launch_sweep_down(const DevState& s, int start, int count, SolveVec v,
 
 template <typename R>
 void launch_root_solve(const DevState& s, SolveVec v, hipStream_t st) {
-  k_root_solve<R><<<1, 64, 0, st>>>(s, v);
+  k_root_solve<R><<<1, 256, 0, st>>>(s, v);
 }
 
 #define IEGPU_INSTANTIATE(R)                                                   \
