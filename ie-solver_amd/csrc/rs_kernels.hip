@@ -1538,43 +1538,57 @@ __device__ int gj_invert(R* a, R* b, int n, int* prow) {
   return info;
 }
 
-// Gauss-Jordan inverse of X (r x r, r <= 64, in LDS, ld r) with the
-// matrix in the registers of 256 threads on a 16 x 16 grid, thread
-// (tr, tc) holding rows tr + 16 u and columns tc + 16 v.  Per step the
-// pivot column (kept in LDS by its owners) gives the pivot row by one wave
-// maximum over packed keys (same rule as gj_invert), the pivot row's
-// owners publish it, and every thread updates its entries with one FMA
-// each.  M (as gj_invert) is written back to X.  colbuf holds 2 x 64 and
-// rowbuf 64 entries.  Returns 0, or j + 1 for the first zero pivot.
-template <typename R>
+// Gauss-Jordan inverse of X (r x r, r <= 16 U, in LDS or global memory,
+// ld r) with the matrix in the registers of 256 threads on a 16 x 16 grid,
+// thread (tr, tc) holding rows tr + 16 u and columns tc + 16 v (U x U
+// entries).  Per step the pivot column (kept in LDS by its owners) gives
+// the pivot row by one wave maximum over packed keys (same rule as
+// gj_invert), the pivot row's owners publish it, and every thread updates
+// its entries with one FMA each.  M (as gj_invert) is written back to X.
+// colbuf holds 2 x 16 U and rowbuf 16 U entries.  Returns 0, or j + 1 for
+// the first zero pivot.
+template <typename R, int U = 4>
 __device__ int gj_invert_regs(R* X, int r, int* prow, R* colbuf, R* rowbuf) {
+  constexpr int RM = 16 * U, KH = (RM + 63) / 64;
   const int tid = threadIdx.x, lane = tid & 63;
   const int tr = tid & 15, tc = tid >> 4;
-  R a[4][4];
+  R a[U][U];
 #pragma unroll
-  for (int u = 0; u < 4; u++)
+  for (int u = 0; u < U; u++)
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
+    for (int v = 0; v < U; v++) {
       const int i = tr + 16 * u, c = tc + 16 * v;
       a[u][v] = (i < r && c < r) ? X[i + r * c] : R(0);
     }
   if (tc == 0) {
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < U; u++)
       if (tr + 16 * u < r) colbuf[tr + 16 * u] = a[u][0];
   }
   __syncthreads();
-  bool used = false;  // row `lane` already pivoted (every wave keeps a copy)
+  bool used[KH];  // row lane + 64 h already pivoted (every wave keeps a copy)
+#pragma unroll
+  for (int h = 0; h < KH; h++) used[h] = false;
   int info = 0;
   for (int j = 0; j < r; j++) {
-    const R* col = colbuf + 64 * (j & 1);
-    R* coln = colbuf + 64 * ((j + 1) & 1);
-    const bool cand = lane < r && !used;
-    const R mag = cand ? fabs(col[lane]) : R(0);
-    const double best = wave_max((cand && mag == mag) ? pivot_key(mag, lane) : -1.0);
-    const int p = best >= 0 ? pivot_key_pos(best)
-                            : __ffsll((unsigned long long)__ballot(cand)) - 1;  // NaN column
-    if (lane == p) used = true;
+    const R* col = colbuf + RM * (j & 1);
+    R* coln = colbuf + RM * ((j + 1) & 1);
+    double key = -1.0;
+    int first = -1;  // first candidate row (a NaN column has no key)
+#pragma unroll
+    for (int h = KH - 1; h >= 0; h--) {
+      const int row = lane + 64 * h;
+      const bool cand = row < r && !used[h];
+      const R mag = cand ? fabs(col[row]) : R(0);
+      key = fmax(key, (cand && mag == mag) ? pivot_key(mag, row) : -1.0);
+      const uint64_t cb = __ballot(cand);
+      if (cb) first = 64 * h + __ffsll((unsigned long long)cb) - 1;
+    }
+    const double best = wave_max(key);
+    const int p = best >= 0 ? pivot_key_pos(best) : first;
+#pragma unroll
+    for (int h = 0; h < KH; h++)
+      if (lane + 64 * h == p) used[h] = true;
     if (tid == 0) prow[j] = p;
     const R piv = col[p];
     if (piv == R(0) && info == 0) info = j + 1;
@@ -1582,56 +1596,56 @@ __device__ int gj_invert_regs(R* X, int r, int* prow, R* colbuf, R* rowbuf) {
     const int up = p >> 4;
     if (tr == (p & 15)) {
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < U; u++) {
         if (u != up) continue;
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
+        for (int v = 0; v < U; v++) {
           const int c = tc + 16 * v;
           if (c < r) rowbuf[c] = (c == j) ? R(1) : a[u][v];
         }
       }
     }
     __syncthreads();
-    R cv[4], rv[4];
+    R cv[U], rv[U];
 #pragma unroll
-    for (int u = 0; u < 4; u++) cv[u] = (tr + 16 * u < r) ? col[tr + 16 * u] : R(0);
+    for (int u = 0; u < U; u++) cv[u] = (tr + 16 * u < r) ? col[tr + 16 * u] : R(0);
 #pragma unroll
-    for (int v = 0; v < 4; v++) rv[v] = (tc + 16 * v < r) ? rowbuf[tc + 16 * v] * inv : R(0);
+    for (int v = 0; v < U; v++) rv[v] = (tc + 16 * v < r) ? rowbuf[tc + 16 * v] * inv : R(0);
     if (tc == (j & 15)) {  // column j restarts from zero
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
+      for (int v = 0; v < U; v++) {
         if (v != (j >> 4)) continue;
 #pragma unroll
-        for (int u = 0; u < 4; u++) a[u][v] = R(0);
+        for (int u = 0; u < U; u++) a[u][v] = R(0);
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < U; u++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) a[u][v] -= cv[u] * rv[v];
+      for (int v = 0; v < U; v++) a[u][v] -= cv[u] * rv[v];
     if (tr == (p & 15)) {  // the pivot row becomes the scaled pivot row
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < U; u++) {
         if (u != up) continue;
 #pragma unroll
-        for (int v = 0; v < 4; v++) a[u][v] = rv[v];
+        for (int v = 0; v < U; v++) a[u][v] = rv[v];
       }
     }
     if (j + 1 < r && tc == ((j + 1) & 15)) {
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
+      for (int v = 0; v < U; v++) {
         if (v != ((j + 1) >> 4)) continue;
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int u = 0; u < U; u++)
           if (tr + 16 * u < r) coln[tr + 16 * u] = a[u][v];
       }
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int u = 0; u < 4; u++)
+  for (int u = 0; u < U; u++)
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
+    for (int v = 0; v < U; v++) {
       const int i = tr + 16 * u, c = tc + 16 * v;
       if (i < r && c < r) X[i + r * c] = a[u][v];
     }
@@ -1723,28 +1737,39 @@ __device__ void load_child_blocks(const DevTree& t, const DevState& s, int b,
 // ---------------------------------------------------------------------------
 
 // P (n x n, skeleton positions first) -> K_rs, K_sr, X_rr.
+// sum_l a[sa l] b[sb l], l < len, in four partial sums (a dependent FP64
+// FMA chain costs its latency per term; global operands add a load
+// latency per term besides).
+template <typename R>
+__device__ inline R dot2(const R* a, int64_t sa, const R* b, int64_t sb, int len) {
+  R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int l = 0;
+  for (; l + 4 <= len; l += 4) {
+    s0 += a[sa * l] * b[sb * l];
+    s1 += a[sa * (l + 1)] * b[sb * (l + 1)];
+    s2 += a[sa * (l + 2)] * b[sb * (l + 2)];
+    s3 += a[sa * (l + 3)] * b[sb * (l + 3)];
+  }
+  for (; l < len; l++) s0 += a[sa * l] * b[sb * l];
+  return (s0 + s1) + (s2 + s3);
+}
+
 template <typename R, int NT>
 __device__ void decouple_blocks(const R* P, const R* T, int n, int k, R* Krs, R* Ksr, R* X) {
   const int tid = threadIdx.x, r = n - k;
   for (int idx = tid; idx < r * k; idx += NT) {
     const int i = idx % r, a = idx / r;
-    R acc = P[(k + i) + (int64_t)n * a];
-    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + (int64_t)n * a];
-    Krs[idx] = acc;
+    Krs[idx] = P[(k + i) + (int64_t)n * a] - dot2(T + k * i, 1, P + (int64_t)n * a, 1, k);
   }
   for (int idx = tid; idx < k * r; idx += NT) {
     const int a = idx % k, j = idx / k;
-    R acc = P[a + (int64_t)n * (k + j)];
-    for (int l = 0; l < k; l++) acc -= P[a + (int64_t)n * l] * T[l + k * j];
-    Ksr[idx] = acc;
+    Ksr[idx] = P[a + (int64_t)n * (k + j)] - dot2(P + a, n, T + k * j, 1, k);
   }
   __syncthreads();
   for (int idx = tid; idx < r * r; idx += NT) {
     const int i = idx % r, j = idx / r;
-    R acc = P[(k + i) + (int64_t)n * (k + j)];
-    for (int l = 0; l < k; l++) acc -= T[l + k * i] * P[l + (int64_t)n * (k + j)];
-    for (int l = 0; l < k; l++) acc -= Krs[i + r * l] * T[l + k * j];
-    X[idx] = acc;
+    X[idx] = P[(k + i) + (int64_t)n * (k + j)] - dot2(T + k * i, 1, P + (int64_t)n * (k + j), 1, k) -
+             dot2(Krs + i, r, T + k * j, 1, k);
   }
   __syncthreads();
 }
@@ -1759,17 +1784,33 @@ __device__ void decouple_tail(const R* M, const int* prow, const R* Krs, const R
   for (int idx = tid; idx < r * k; idx += NT) {  // U = X^{-1} K_rs
     const int j = idx % r, a = idx / r;
     const R* mrow = M + prow[j];
-    R acc = 0;
-    for (int jj = 0; jj < r; jj++) acc += mrow[(int64_t)r * jj] * Krs[prow[jj] + (int64_t)r * a];
+    const R* kc = Krs + (int64_t)r * a;
+    R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int jj = 0;
+    for (; jj + 4 <= r; jj += 4) {
+      s0 += mrow[(int64_t)r * jj] * kc[prow[jj]];
+      s1 += mrow[(int64_t)r * (jj + 1)] * kc[prow[jj + 1]];
+      s2 += mrow[(int64_t)r * (jj + 2)] * kc[prow[jj + 2]];
+      s3 += mrow[(int64_t)r * (jj + 3)] * kc[prow[jj + 3]];
+    }
+    for (; jj < r; jj++) s0 += mrow[(int64_t)r * jj] * kc[prow[jj]];
+    const R acc = (s0 + s1) + (s2 + s3);
     W[idx] = acc;
     F[fl.U + idx] = acc;
   }
   for (int idx = tid; idx < k * r; idx += NT) {  // L = K_sr X^{-1}
     const int a = idx % k, jj = idx / k;
     const R* mcol = M + (int64_t)r * jj;
-    R acc = 0;
-    for (int j = 0; j < r; j++) acc += Ksr[a + (int64_t)k * j] * mcol[prow[j]];
-    F[fl.L + a + (int64_t)k * prow[jj]] = acc;
+    R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int j = 0;
+    for (; j + 4 <= r; j += 4) {
+      s0 += Ksr[a + (int64_t)k * j] * mcol[prow[j]];
+      s1 += Ksr[a + (int64_t)k * (j + 1)] * mcol[prow[j + 1]];
+      s2 += Ksr[a + (int64_t)k * (j + 2)] * mcol[prow[j + 2]];
+      s3 += Ksr[a + (int64_t)k * (j + 3)] * mcol[prow[j + 3]];
+    }
+    for (; j < r; j++) s0 += Ksr[a + (int64_t)k * j] * mcol[prow[j]];
+    F[fl.L + a + (int64_t)k * prow[jj]] = (s0 + s1) + (s2 + s3);
   }
   for (int idx = tid; idx < r * r; idx += NT) {
     const int j = idx % r, jj = idx / r;
@@ -1778,9 +1819,7 @@ __device__ void decouple_tail(const R* M, const int* prow, const R* Krs, const R
   __syncthreads();
   for (int idx = tid; idx < k * k; idx += NT) {  // carried block
     const int a = idx % k, c = idx / k;
-    R acc = 0;
-    for (int j = 0; j < r; j++) acc += Ksr[a + (int64_t)k * j] * W[j + (int64_t)r * c];
-    F[fl.S + idx] = upd(sperm[a], sperm[c]) + acc;
+    F[fl.S + idx] = upd(sperm[a], sperm[c]) + dot2(Ksr + a, k, W + (int64_t)r * c, 1, r);
   }
 }
 
@@ -1841,23 +1880,17 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
   __syncthreads();
   for (int idx = tid; idx < r * k; idx += NT) {  // K_rs = K(r,s) - T^T K(s,s)
     const int i = idx % r, a = idx / r;
-    R acc = kbn(k + i, a);
-    for (int l = 0; l < k; l++) acc -= T[l + k * i] * Pss[l + k * a];
-    Krs[idx] = acc;
+    Krs[idx] = kbn(k + i, a) - dot2(T + k * i, 1, Pss + k * a, 1, k);
   }
   for (int idx = tid; idx < k * r; idx += NT) {  // K_sr = K(s,r) - K(s,s) T
     const int a = idx % k, j = idx / k;
-    R acc = Psr[idx];
-    for (int l = 0; l < k; l++) acc -= Pss[a + k * l] * T[l + k * j];
-    Ksr[idx] = acc;
+    Ksr[idx] = Psr[idx] - dot2(Pss + a, k, T + k * j, 1, k);
   }
   __syncthreads();
   for (int idx = tid; idx < r * r; idx += NT) {  // X_rr = K(r,r) - T^T K(s,r) - K_rs T
     const int i = idx % r, j = idx / r;
-    R acc = kbn(k + i, k + j);
-    for (int l = 0; l < k; l++) acc -= T[l + k * i] * Psr[l + k * j];
-    for (int l = 0; l < k; l++) acc -= Krs[i + r * l] * T[l + k * j];
-    X[idx] = acc;
+    X[idx] = kbn(k + i, k + j) - dot2(T + k * i, 1, Psr + k * j, 1, k) -
+             dot2(Krs + i, r, T + k * j, 1, k);
   }
   __syncthreads();
   // The DOF geometry is dead: its space holds the pivot column and row.
@@ -1929,7 +1962,13 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   }
   __syncthreads();
   decouple_blocks<R, NT>(P, F + fl.T, n, k, Krs, Ksr, X);
-  const int info = gj_invert_into<R, NT>(X, P, r, prow);
+  // X_rr^{-1}: in registers for r <= 128 (the LDS buffer is free: the
+  // blocks of these boxes live in the scratch slot), else step by step.
+  int info;
+  if (NT == 256 && r <= 64 && P != sbuf) info = gj_invert_regs<R, 4>(X, r, prow, sbuf, sbuf + 128);
+  else if (NT == 256 && r <= 128 && P != sbuf)
+    info = gj_invert_regs<R, 8>(X, r, prow, sbuf, sbuf + 256);
+  else info = gj_invert_into<R, NT>(X, P, r, prow);
   if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
   decouple_tail<R, NT>(X, prow, Krs, Ksr, P, r, k, F, fl, upd, sperm);
   if (tid == 0) s.upd_kind[b] = 1;
