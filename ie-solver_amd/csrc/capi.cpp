@@ -899,7 +899,7 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    int errbits = 0, maxn = 0;
+    int errbits = 0;
     for (int attempt = 0; attempt < 8; attempt++) {
       alloc_state(c);
       if (c->hole_dim > 0) {
@@ -939,7 +939,6 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
       HIP_TRY(hipMemcpyAsync(h_err, c->s_err.p, sizeof(h_err), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
       errbits = h_err[0];
-      maxn = h_err[1];
       const int grow = errbits & (kErrBoxCap | kErrIdPool | kErrRowPool | kErrFacPool);
       if (!grow) {
         if (c->hole_dim > 0) {
@@ -981,7 +980,6 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
     c->fac_bytes = (double)top * c->elem();
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)maxn;
     if (errbits & kErrSingular) {
       c->factored = false;
       return fail(c, IEGPU_ERR_SINGULAR, "exact zero pivot in an LU factorization");

@@ -149,7 +149,7 @@ struct DevState {
   int64_t row_cap;
   int64_t fac_cap;
   int64_t* fac_top;    // running factor-pool offset across levels
-  int* err;            // [0] error bits, [1] max |active| seen, [2] / [3] ID / row
+  int* err;            // [0] error bits, [1] spare, [2] / [3] ID / row
                        // pool size a level needed on overflow (1024 elements)
   void* root_mat;      // ncap*ncap
   int* root_piv;       // ncap

@@ -95,10 +95,7 @@ __global__ __launch_bounds__(64) void k_active(DevTree t, DevState s,
       n = 0;
     }
   }
-  if (lane == 0) {
-    s.act_cnt[b] = n;
-    atomicMax(s.err + 1, n);
-  }
+  if (lane == 0) s.act_cnt[b] = n;
 }
 
 // Visits the ID row DOFs of box b in reference order (source boxes in
@@ -269,7 +266,8 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
   // Scanned per box: ID scratch, row-list and factor sizes, register-QRCP
   // and general-QRCP list entries, row-compression jobs.
   constexpr int NQ = 10;
-  __shared__ int64_t sh[NQ][1024];
+  __shared__ int64_t wtot[NQ][16];  // per-wave totals, then their exclusive scan
+  __shared__ int64_t tot[NQ];
   const int tid = threadIdx.x;
   const int chunk = (count + 1023) / 1024;
   const int lo = min(count, tid * chunk), hi = min(count, lo + chunk);
@@ -288,22 +286,36 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     own[8] += (s.sk_w[b] > 0) ? 1 : 0;
     own[9] += rid_full_qrcp(s, b, kind) ? 1 : 0;
   }
+  // Exclusive scan over the 1024 threads: inclusive scans within each wave
+  // (shuffles), then a scan of the 16 wave totals.
+  const int lane = tid & 63, wv = tid >> 6;
+  int64_t inc[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; q++) sh[q][tid] = own[q];
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    int64_t x[NQ];
+  for (int q = 0; q < NQ; q++) {
+    int64_t v = own[q];
 #pragma unroll
-    for (int q = 0; q < NQ; q++) x[q] = (tid >= o) ? sh[q][tid - o] : 0;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NQ; q++) sh[q][tid] += x[q];
-    __syncthreads();
+    for (int o = 1; o < 64; o <<= 1) {
+      const int64_t u = __shfl_up(v, o, 64);
+      if (lane >= o) v += u;
+    }
+    inc[q] = v;
+    if (lane == 63) wtot[q][wv] = v;
   }
+  __syncthreads();
+  if (tid < NQ) {
+    int64_t run = 0;
+    for (int w = 0; w < 16; w++) {
+      const int64_t t = wtot[tid][w];
+      wtot[tid][w] = run;
+      run += t;
+    }
+    tot[tid] = run;
+  }
+  __syncthreads();
   const int64_t fbase = *s.fac_top;
   int64_t off[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; q++) off[q] = sh[q][tid] - own[q];
+  for (int q = 0; q < NQ; q++) off[q] = wtot[q][wv] + inc[q] - own[q];
   int* lreg = s.lists + (int64_t)kLstReg * s.nbox + start;
   int* llds = s.lists + (int64_t)kLstLds * s.nbox + start;
   int* lreg8 = s.lists + (int64_t)kLstReg8 * s.nbox + start;
@@ -339,14 +351,14 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
   }
   __syncthreads();
   if (tid == 0) {
-    const int64_t ta = sh[0][1023], tb = sh[1][1023], tc = sh[2][1023];
-    list_slot(s, level, kLstReg)[0] = (int)sh[3][1023];
-    list_slot(s, level, kLstLds)[0] = (int)sh[4][1023];
-    list_slot(s, level, kLstReg8)[0] = (int)sh[6][1023];
-    list_slot(s, level, kLstRegHalf)[0] = (int)sh[7][1023];
-    list_slot(s, level, kLstRid)[0] = (int)sh[8][1023];
-    list_slot(s, level, kLstFallback)[0] = (int)sh[9][1023];  // k_rid appends
-    s.njobs[level] = (int)sh[5][1023];
+    const int64_t ta = tot[0], tb = tot[1], tc = tot[2];
+    list_slot(s, level, kLstReg)[0] = (int)tot[3];
+    list_slot(s, level, kLstLds)[0] = (int)tot[4];
+    list_slot(s, level, kLstReg8)[0] = (int)tot[6];
+    list_slot(s, level, kLstRegHalf)[0] = (int)tot[7];
+    list_slot(s, level, kLstRid)[0] = (int)tot[8];
+    list_slot(s, level, kLstFallback)[0] = (int)tot[9];  // k_rid appends
+    s.njobs[level] = (int)tot[5];
     int e = 0;
     if (ta > s.id_cap) {
       e |= kErrIdPool;
