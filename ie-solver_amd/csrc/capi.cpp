@@ -157,6 +157,10 @@ struct iegpu_ctx {
   int n_points = 0;
   std::vector<double> h_pts, h_nrm, h_curv, h_wts;
   DevBuf<double> d_pts, d_nrm, d_curv, d_wts;
+  // batch of independent problems: point offsets (n_prob + 1)
+  int n_prob = 1;
+  std::vector<int> h_poff;
+  DevBuf<int> d_poff;
 
   // tree
   bool have_tree = false;
@@ -221,6 +225,8 @@ struct iegpu_ctx {
     g.n_points = n_points;
     g.sol_dim = sol_dim;
     g.pde = pde;
+    g.n_prob = n_prob;
+    g.prob_off = d_poff.p;
     return g;
   }
   DevTree dtree() const {
@@ -393,8 +399,9 @@ void alloc_state(iegpu_ctx* c) {
   c->s_fac_off.alloc(nb);
   c->s_fac_top.alloc(1);
   c->s_err.alloc(4);
-  c->s_root_piv.alloc(c->ncap + 64);
-  c->s_root_mat.alloc((size_t)2 * c->ncap * c->ncap * 8);  // matrix + inverse
+  const size_t nr = c->tree.n_roots;
+  c->s_root_piv.alloc(nr * (c->ncap + 64));
+  c->s_root_mat.alloc(nr * 2 * c->ncap * c->ncap * 8);  // matrix + inverse per root
   c->s_id_pool.alloc((size_t)c->id_cap * c->elem());
   c->s_row_pool.alloc((size_t)c->row_cap);
   c->s_fac_pool.alloc((size_t)c->fac_cap * c->elem());
@@ -417,7 +424,7 @@ void enqueue_factor(iegpu_ctx* c, double tol) {
       });
   }
   c->prof.run(kProfRoot, 0, st,
-              [&] { launch_root<R>(g, t, s, c->hole_dim == 0, st); });
+              [&] { launch_root<R>(g, t, s, T.n_roots, c->hole_dim == 0, st); });
 }
 
 template <typename R>
@@ -471,7 +478,7 @@ void enqueue_solve(iegpu_ctx* c, double* x, double* alpha, hipStream_t st,
   }
   c->prof.run(kProfRootSolve, 0, st, [&] {
     if (!holes) {
-      launch_root_solve<R>(s, v, st);
+      launch_root_solve<R>(s, T.n_roots, v, st);
     } else {
       const int h = c->hole_dim;
       launch_border_gemv(c->d_mPsiT.p, x, c->d_mask.p, nd, h, c->d_partial.p,
@@ -536,10 +543,11 @@ const char* iegpu_last_error(const iegpu_ctx* c) {
   return c ? c->err.c_str() : "null context";
 }
 
-int iegpu_set_geometry(iegpu_ctx* c, int pde, int n_points,
-                       const double* points, const double* normals,
-                       const double* curvatures, const double* weights) {
-  if (!c) return IEGPU_ERR_INVALID;
+namespace {
+int set_geometry(iegpu_ctx* c, int pde, int n_problems, const int* point_off,
+                 const double* points, const double* normals,
+                 const double* curvatures, const double* weights) {
+  const int n_points = point_off[n_problems];
   if (pde != kPdeLaplace && pde != kPdeStokes)
     return fail(c, IEGPU_ERR_INVALID, "pde must be LAPLACE (0) or STOKES (2)");
   if (n_points <= 0 || !points || !normals || !curvatures || !weights)
@@ -558,12 +566,15 @@ int iegpu_set_geometry(iegpu_ctx* c, int pde, int n_points,
     c->h_nrm.assign(normals, normals + 2 * n_points);
     c->h_curv.assign(curvatures, curvatures + n_points);
     c->h_wts.assign(weights, weights + n_points);
+    c->n_prob = n_problems;
+    c->h_poff.assign(point_off, point_off + n_problems + 1);
     if (c->device >= 0) {
       HIP_TRY(hipSetDevice(c->device));
       c->d_pts.upload(c->h_pts, c->stream);
       c->d_nrm.upload(c->h_nrm, c->stream);
       c->d_curv.upload(c->h_curv, c->stream);
       c->d_wts.upload(c->h_wts, c->stream);
+      c->d_poff.upload(c->h_poff, c->stream);
       HIP_TRY(hipStreamSynchronize(c->stream));
     }
     c->have_tree = false;
@@ -575,6 +586,30 @@ int iegpu_set_geometry(iegpu_ctx* c, int pde, int n_points,
   }
   return IEGPU_OK;
 }
+}  // namespace
+
+int iegpu_set_geometry(iegpu_ctx* c, int pde, int n_points,
+                       const double* points, const double* normals,
+                       const double* curvatures, const double* weights) {
+  if (!c) return IEGPU_ERR_INVALID;
+  const int off[2] = {0, n_points};
+  return set_geometry(c, pde, 1, off, points, normals, curvatures, weights);
+}
+
+int iegpu_set_geometry_batch(iegpu_ctx* c, int pde, int n_problems,
+                             const int* point_offsets, const double* points,
+                             const double* normals, const double* curvatures,
+                             const double* weights) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (n_problems < 1 || !point_offsets)
+    return fail(c, IEGPU_ERR_INVALID, "n_problems must be >= 1 with point offsets");
+  if (point_offsets[0] != 0) return fail(c, IEGPU_ERR_INVALID, "point_offsets[0] must be 0");
+  for (int p = 0; p < n_problems; p++)
+    if (point_offsets[p + 1] <= point_offsets[p])
+      return fail(c, IEGPU_ERR_INVALID, "every problem needs at least one point");
+  return set_geometry(c, pde, n_problems, point_offsets, points, normals, curvatures,
+                      weights);
+}
 
 int iegpu_build_tree(iegpu_ctx* c, int leaf_cap) {
   if (!c) return IEGPU_ERR_INVALID;
@@ -583,7 +618,7 @@ int iegpu_build_tree(iegpu_ctx* c, int leaf_cap) {
     return fail(c, IEGPU_ERR_INVALID, "leaf_cap out of range");
   try {
     c->tree = HostTree();
-    c->tree.build(c->h_pts.data(), c->n_points, c->sol_dim, leaf_cap);
+    c->tree.build_forest(c->h_pts.data(), c->h_poff.data(), c->n_prob, c->sol_dim, leaf_cap);
     if (c->tree.n_levels > kMaxLevels)
       return fail(c, IEGPU_ERR_INVALID, "quadtree deeper than 64 levels");
     c->have_tree = true;
@@ -627,6 +662,8 @@ int iegpu_set_holes(iegpu_ctx* c, int n_holes, const double* centers,
   if (n_holes < 0 || (n_holes > 0 && (!centers || !radii)))
     return fail(c, IEGPU_ERR_INVALID, "bad hole arrays");
   if (n_holes > 64) return fail(c, IEGPU_ERR_INVALID, "at most 64 holes");
+  if (n_holes > 0 && c->n_prob > 1)
+    return fail(c, IEGPU_ERR_INVALID, "holes are not supported in a batch of problems");
   if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
   try {
     HIP_TRY(hipSetDevice(c->device));
@@ -1162,6 +1199,8 @@ int iegpu_solve_refined(iegpu_ctx* c, const double* b, double* x, double* alpha,
     return fail(c, IEGPU_ERR_INVALID, "bad refinement arguments");
   const int h = c->hole_dim;
   if (h > 0 && !alpha) return fail(c, IEGPU_ERR_INVALID, "alpha required with holes");
+  if (c->n_prob > 1 && c->pde == kPdeStokes)
+    return fail(c, IEGPU_ERR_INVALID, "the FP64 operator of a Stokes batch is not supported");
   try {
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t st = c->stream;
@@ -1237,6 +1276,8 @@ int iegpu_apply(iegpu_ctx* c, const double* x, const double* alpha, double* y,
   if (c->n_points == 0) return fail(c, IEGPU_ERR_INVALID, "set_geometry first");
   if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
   if (!x || !y) return fail(c, IEGPU_ERR_INVALID, "null vector");
+  if (c->n_prob > 1 && c->pde == kPdeStokes)
+    return fail(c, IEGPU_ERR_INVALID, "the FP64 operator of a Stokes batch is not supported");
   const int h = c->hole_dim;
   try {
     HIP_TRY(hipSetDevice(c->device));
@@ -1270,6 +1311,8 @@ int iegpu_evaluate(iegpu_ctx* c, int n_targets, const double* targets,
   if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
   if (n_targets < 0 || (n_targets > 0 && (!targets || !mu || !out)))
     return fail(c, IEGPU_ERR_INVALID, "bad evaluation arrays");
+  if (c->n_prob > 1)
+    return fail(c, IEGPU_ERR_INVALID, "interior evaluation takes one problem, not a batch");
   if (n_targets == 0) return IEGPU_OK;
   try {
     HIP_TRY(hipSetDevice(c->device));

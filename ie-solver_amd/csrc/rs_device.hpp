@@ -97,6 +97,10 @@ struct DevGeom {
   int n_points;
   int sol_dim;
   int pde;
+  // Independent problems (a batch): problem p owns points
+  // [prob_off[p], prob_off[p+1]); n_prob == 1 for a single boundary.
+  int n_prob;
+  const int* prob_off;
 };
 
 // Static tree, slot-indexed (level-major).
@@ -151,8 +155,8 @@ struct DevState {
   int64_t* fac_top;    // running factor-pool offset across levels
   int* err;            // [0] error bits, [1] spare, [2] / [3] ID / row
                        // pool size a level needed on overflow (1024 elements)
-  void* root_mat;      // ncap*ncap
-  int* root_piv;       // ncap
+  void* root_mat;      // per root: 2 ncap^2 elements (matrix + inverse)
+  int* root_piv;       // per root: ncap + 64
   // Randomized ID (rid_mode 0: off; 1: sketch width rid_width for every
   // eligible box; 2: width from the rank model).
   int rid_mode;
@@ -644,6 +648,15 @@ __device__ inline DofList source_dofs(const DevTree& t, const DevState& s,
     d.n = s.act_cnt[box];
   }
   return d;
+}
+
+// Dense block (matrix, then its inverse) and pivots of root r (slot r).
+template <typename R>
+__device__ inline R* root_block(const DevState& s, int r) {
+  return reinterpret_cast<R*>(s.root_mat) + 2 * (int64_t)s.ncap * s.ncap * r;
+}
+__device__ inline int* root_pivots(const DevState& s, int r) {
+  return s.root_piv + (int64_t)(s.ncap + 64) * r;
 }
 
 }  // namespace iegpu

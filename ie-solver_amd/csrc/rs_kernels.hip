@@ -1997,7 +1997,7 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
   __shared__ short ploc[kMaxBoxDofs];
   __shared__ const R* chS[4];
   __shared__ int chN[4];
-  const int b = 0;
+  const int b = blockIdx.x;  // root of problem b
   const int tid = threadIdx.x;
   if (tid == 0) {  // the root is never compressed
     s.rank[b] = 0;
@@ -2033,8 +2033,8 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
   upd.N = chN;
   upd.pch = pch;
   upd.ploc = ploc;
-  const int* act = s.act_list;
-  R* A = reinterpret_cast<R*>(s.root_mat);
+  const int* act = s.act_list + (int64_t)b * s.ncap;
+  R* A = root_block<R>(s, b);
   for (int64_t idx = tid; idx < (int64_t)n * n; idx += NT) {
     const int i = (int)(idx % n), j = (int)(idx / n);
     A[idx] = static_cast<R>(kernel_entry(g, act[i], act[j])) - upd(i, j);
@@ -2045,8 +2045,8 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
     // blocks too large for k_root_inv; the inverse, unpermuted, goes after
     // the assembled matrix.
     R* B = A + (int64_t)n * n;
-    const int* prow = s.root_piv;
-    const int info = gj_invert_into<R, NT>(A, B, n, s.root_piv);
+    int* prow = root_pivots(s, b);
+    const int info = gj_invert_into<R, NT>(A, B, n, prow);
     if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
     for (int64_t idx = tid; idx < (int64_t)n * n; idx += NT) {
       const int j = (int)(idx % n), jj = (int)(idx / n);
@@ -2071,11 +2071,11 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
   __shared__ int prow[NMAX];
   __shared__ int invrow[NMAX];
   if (!factor_lu || !pools_ok(s)) return;
-  const int n = s.act_cnt[0];
+  const int n = s.act_cnt[blockIdx.x];
   if (n == 0 || n > NMAX) return;
   const int tid = threadIdx.x, lane = tid & 63;
   const int tr = tid % kRootTR, tc = tid / kRootTR;
-  R* A = reinterpret_cast<R*>(s.root_mat);
+  R* A = root_block<R>(s, blockIdx.x);
   R a[kRootRU][kRootCV];
 #pragma unroll
   for (int u = 0; u < kRootRU; u++)
@@ -2324,9 +2324,9 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_root_solve(DevState s, SolveVec v) {
   __shared__ double xa[kMaxBoxDofs];
   const int tid = threadIdx.x;
-  const int n = s.act_cnt[0];
-  const int* act = s.act_list;
-  const R* Ai = reinterpret_cast<const R*>(s.root_mat) + (int64_t)n * n;
+  const int n = s.act_cnt[blockIdx.x];
+  const int* act = s.act_list + (int64_t)blockIdx.x * s.ncap;
+  const R* Ai = root_block<R>(s, blockIdx.x) + (int64_t)n * n;
   for (int c = 0; c < v.nrhs; c++) {
     double* x = v.x + v.ldx * c;
     for (int i = tid; i < n; i += 256) xa[i] = x[act[i]];
@@ -2413,10 +2413,10 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
 
 template <typename R>
 void launch_root(const DevGeom& g, const DevTree& t, const DevState& s,
-                 bool factor_lu, hipStream_t st) {
-  k_active<<<1, 64, 0, st>>>(t, s, 0);
-  k_root<R, 1024><<<1, 1024, 0, st>>>(g, t, s, factor_lu ? 1 : 0);
-  k_root_inv<R><<<1, kRootTR * kRootTC, 0, st>>>(s, factor_lu ? 1 : 0);
+                 int n_roots, bool factor_lu, hipStream_t st) {
+  k_active<<<n_roots, 64, 0, st>>>(t, s, 0);
+  k_root<R, 1024><<<n_roots, 1024, 0, st>>>(g, t, s, factor_lu ? 1 : 0);
+  k_root_inv<R><<<n_roots, kRootTR * kRootTC, 0, st>>>(s, factor_lu ? 1 : 0);
 }
 
 template <typename R>
@@ -2441,8 +2441,8 @@ void launch_sweep_down(const DevState& s, int start, int count, SolveVec v,
 }
 
 template <typename R>
-void launch_root_solve(const DevState& s, SolveVec v, hipStream_t st) {
-  k_root_solve<R><<<1, 256, 0, st>>>(s, v);
+void launch_root_solve(const DevState& s, int n_roots, SolveVec v, hipStream_t st) {
+  k_root_solve<R><<<n_roots, 256, 0, st>>>(s, v);
 }
 
 #define IEGPU_INSTANTIATE(R)                                                   \
@@ -2450,14 +2450,14 @@ void launch_root_solve(const DevState& s, SolveVec v, hipStream_t st) {
                                        const DevState&, int, int, int, double, \
                                        hipStream_t);                           \
   template void launch_root<R>(const DevGeom&, const DevTree&,                \
-                               const DevState&, bool, hipStream_t);            \
+                               const DevState&, int, bool, hipStream_t);       \
   template void launch_sweep_up<R>(const DevState&, int, int, SolveVec, bool, \
                                    bool, hipStream_t);                         \
   template void launch_diag<R>(const DevState&, int, int, SolveVec,           \
                                hipStream_t);                                   \
   template void launch_sweep_down<R>(const DevState&, int, int, SolveVec,     \
                                      hipStream_t);                             \
-  template void launch_root_solve<R>(const DevState&, SolveVec, hipStream_t);
+  template void launch_root_solve<R>(const DevState&, int, SolveVec, hipStream_t);
 
 IEGPU_INSTANTIATE(double)
 IEGPU_INSTANTIATE(float)

@@ -34,10 +34,11 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
                          const DevState& s, int level, int start, int count,
                          double tol, hipStream_t st);
 
-// Root: active list, dense skeleton block, LU.
+// Roots (slots 0 .. n_roots-1, one per problem): active list, dense
+// skeleton block, inverse.
 template <typename R>
 void launch_root(const DevGeom& g, const DevTree& t, const DevState& s,
-                 bool factor_lu, hipStream_t st);
+                 int n_roots, bool factor_lu, hipStream_t st);
 
 // Upward sweep of one level: x_r -= T^T x_s; x_s -= L x_r (or U^T x_r when
 // `use_Ut`); then x_r = X_rr^{-1} x_r when `diag`.
@@ -55,8 +56,8 @@ template <typename R>
 void launch_sweep_down(const DevState& s, int start, int count, SolveVec v,
                        hipStream_t st);
 
-// x(allskel) = A_root^{-1} x(allskel) with the root LU.
+// x(allskel) = A_root^{-1} x(allskel) for every root.
 template <typename R>
-void launch_root_solve(const DevState& s, SolveVec v, hipStream_t st);
+void launch_root_solve(const DevState& s, int n_roots, SolveVec v, hipStream_t st);
 
 }  // namespace iegpu

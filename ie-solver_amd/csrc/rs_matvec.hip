@@ -7,6 +7,10 @@
 // (broadcast).  The reciprocal of |r|^2 is v_rcp_f64 refined by one Newton
 // step.  For Stokes the w n_t n_s^T part of every entry sums to
 // n_t (sum_j w_j n_j . x_j), a global scalar computed first.
+//
+// A batch of independent problems has a block-diagonal operator: a target
+// sums only over its own problem's points, and a block of targets walks
+// only the source tiles of the problems it touches.
 
 #include <hip/hip_runtime.h>
 
@@ -44,6 +48,17 @@ __global__ __launch_bounds__(256) void k_wn_dot(DevGeom g, const double* x,
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// Problem owning point i: the largest p with prob_off[p] <= i.
+__device__ inline int problem_of(const DevGeom& g, int i) {
+  int lo = 0, hi = g.n_prob;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (g.prob_off[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
                                                  const double* wn_partial,
                                                  int wn_blocks, double* y) {
@@ -53,10 +68,21 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
   const bool live = i < g.n_points;
   const double xi = live ? g.pts[2 * i] : 0, yi = live ? g.pts[2 * i + 1] : 0;
   const bool stokes = g.pde == kPdeStokes;
+  // source range of this block (blo, bhi) and of this target (plo, phi)
+  int blo = 0, bhi = g.n_points, plo = 0, phi = g.n_points;
+  if (g.n_prob > 1) {
+    const int first = blockIdx.x * kTile;
+    const int last = min(first + kTile, g.n_points) - 1;
+    blo = g.prob_off[problem_of(g, first)];
+    bhi = g.prob_off[problem_of(g, last) + 1];
+    const int p = problem_of(g, live ? i : last);
+    plo = g.prob_off[p];
+    phi = g.prob_off[p + 1];
+  }
   double a0 = 0, a1 = 0;
-  for (int base = 0; base < g.n_points; base += kTile) {
+  for (int base = blo; base < bhi; base += kTile) {
     const int j = base + threadIdx.x;
-    if (j < g.n_points) {
+    if (j < bhi) {
       const double w = g.wts[j];
       sx[threadIdx.x] = g.pts[2 * j];
       sy[threadIdx.x] = g.pts[2 * j + 1];
@@ -72,9 +98,9 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
       }
     }
     __syncthreads();
-    const int cnt = min(kTile, g.n_points - base);
+    const int q0 = max(0, plo - base), q1 = min(min(kTile, bhi - base), phi - base);
     if (stokes) {
-      for (int q = 0; q < cnt; q++) {
+      for (int q = q0; q < q1; q++) {
         const double r0 = xi - sx[q], r1 = yi - sy[q];
         const double d2 = fma(r0, r0, r1 * r1);
         if (d2 == 0) continue;  // self term added below
@@ -85,7 +111,7 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
         a1 = fma(r1, proj, a1);
       }
     } else {
-      for (int q = 0; q < cnt; q++) {
+      for (int q = q0; q < q1; q++) {
         const double r0 = xi - sx[q], r1 = yi - sy[q];
         const double d2 = fma(r0, r0, r1 * r1);
         if (d2 == 0) continue;

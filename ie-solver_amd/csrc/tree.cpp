@@ -2,7 +2,9 @@
 #include "tree.hpp"
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <algorithm>
 #include <stdexcept>
 
 namespace iegpu {
@@ -179,6 +181,8 @@ void HostTree::build(const double* points, int np, int sd, int cap) {
   n_points = np;
   sol_dim = sd;
   leaf_cap = cap;
+  n_roots = 1;
+  point_off = {0, np};
   Builder B(points, sd, cap);
   B.run(np);
   n_boxes = (int)B.boxes.size();
@@ -248,6 +252,105 @@ void HostTree::build(const double* points, int np, int sd, int cap) {
           if (leaf[u]) src.push_back(u);
     }
     src_cnt[s] = (int)src.size() - src_off[s];
+  }
+}
+
+void HostTree::build_forest(const double* points, const int* poff, int np_, int sd,
+                            int cap) {
+  if (np_ < 1) throw std::invalid_argument("no problems");
+  if (np_ == 1) {
+    build(points, poff[1] - poff[0], sd, cap);
+    return;
+  }
+  std::vector<HostTree> sub(np_);
+  int nl = 0;
+  for (int p = 0; p < np_; p++) {
+    sub[p].build(points + 2 * (int64_t)poff[p], poff[p + 1] - poff[p], sd, cap);
+    nl = std::max(nl, sub[p].n_levels);
+  }
+  n_points = poff[np_];
+  sol_dim = sd;
+  leaf_cap = cap;
+  n_roots = np_;
+  point_off.assign(poff, poff + np_ + 1);
+  n_levels = nl;
+  // forest slot of every (problem, local slot); level-major, problem order
+  std::vector<std::vector<int>> fslot(np_);
+  for (int p = 0; p < np_; p++) fslot[p].assign(sub[p].n_boxes, -1);
+  level_start.assign(nl + 1, 0);
+  int ns = 0;
+  for (int l = 0; l < nl; l++) {
+    level_start[l] = ns;
+    for (int p = 0; p < np_; p++) {
+      const HostTree& S = sub[p];
+      if (l >= S.n_levels) continue;
+      for (int t = S.level_start[l]; t < S.level_start[l + 1]; t++) fslot[p][t] = ns++;
+    }
+  }
+  level_start[nl] = ns;
+  n_boxes = ns;
+  box_id.assign(ns, 0);
+  slot_of_id.assign(ns, 0);
+  level.assign(ns, 0);
+  parent.assign(ns, -1);
+  leaf.assign(ns, 0);
+  child.assign(4 * (size_t)ns, -1);
+  bl_x.assign(ns, 0);
+  bl_y.assign(ns, 0);
+  side.assign(ns, 0);
+  cx.assign(ns, 0);
+  cy.assign(ns, 0);
+  orig_off.assign(ns, 0);
+  orig_cnt.assign(ns, 0);
+  nbr_off.assign(ns, 0);
+  nbr_cnt.assign(ns, 0);
+  src_off.assign(ns, 0);
+  src_cnt.assign(ns, 0);
+  orig_dofs.clear();
+  nbr.clear();
+  src.clear();
+  int id0 = 0;
+  for (int p = 0; p < np_; p++) {
+    const HostTree& S = sub[p];
+    const std::vector<int>& f = fslot[p];
+    auto remap = [&f](int t) { return t >= 0 ? f[t] : -1; };
+    for (int t = 0; t < S.n_boxes; t++) {
+      const int s = f[t];
+      box_id[s] = id0 + S.box_id[t];
+      level[s] = S.level[t];
+      parent[s] = remap(S.parent[t]);
+      leaf[s] = S.leaf[t];
+      for (int q = 0; q < 4; q++) child[4 * (size_t)s + q] = remap(S.child[4 * (size_t)t + q]);
+      bl_x[s] = S.bl_x[t];
+      bl_y[s] = S.bl_y[t];
+      side[s] = S.side[t];
+      cx[s] = S.cx[t];
+      cy[s] = S.cy[t];
+    }
+    id0 += S.n_boxes;
+  }
+  for (int s = 0; s < ns; s++) slot_of_id[box_id[s]] = s;
+  // CSR lists in forest slot order
+  std::vector<int> prob_of(ns, 0), local(ns, 0);
+  for (int p = 0; p < np_; p++)
+    for (int t = 0; t < sub[p].n_boxes; t++) {
+      prob_of[fslot[p][t]] = p;
+      local[fslot[p][t]] = t;
+    }
+  for (int s = 0; s < ns; s++) {
+    const int p = prob_of[s], t = local[s];
+    const HostTree& S = sub[p];
+    const std::vector<int>& f = fslot[p];
+    const int dof0 = sd * poff[p];
+    orig_off[s] = (int)orig_dofs.size();
+    orig_cnt[s] = S.orig_cnt[t];
+    for (int u = 0; u < S.orig_cnt[t]; u++) orig_dofs.push_back(dof0 + S.orig_dofs[S.orig_off[t] + u]);
+    nbr_off[s] = (int)nbr.size();
+    nbr_cnt[s] = S.nbr_cnt[t];
+    for (int u = 0; u < S.nbr_cnt[t]; u++) nbr.push_back(f[S.nbr[S.nbr_off[t] + u]]);
+    src_off[s] = (int)src.size();
+    src_cnt[s] = S.src_cnt[t];
+    for (int u = 0; u < S.src_cnt[t]; u++) src.push_back(f[S.src[S.src_off[t] + u]]);
   }
 }
 

@@ -23,6 +23,10 @@ struct HostTree {
   int leaf_cap = 64;
   int n_boxes = 0;
   int n_levels = 0;
+  // Independent problems in this tree (a forest when > 1): problem p owns
+  // points [point_off[p], point_off[p+1]) and its root is slot p.
+  int n_roots = 1;
+  std::vector<int> point_off;
 
   // All per-box arrays are indexed by "slot": boxes sorted level-major, in
   // the reference's per-level order.  box_id[slot] is the creation index
@@ -46,6 +50,12 @@ struct HostTree {
   std::vector<int> src_off, src_cnt, src;
 
   void build(const double* points, int n_points, int sol_dim, int leaf_cap);
+  // One tree per problem, merged level by level: level l of the forest is
+  // level l of problem 0, then of problem 1, ...  DOF indices are global
+  // (offset by sol_dim * point_off[p]); creation ids are numbered problem
+  // after problem.
+  void build_forest(const double* points, const int* point_off, int n_problems,
+                    int sol_dim, int leaf_cap);
 };
 
 }  // namespace iegpu

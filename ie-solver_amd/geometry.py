@@ -147,6 +147,26 @@ def annulus(n):
                 holes=dict(centers=centers, radii=radii))
 
 
+def star_batch(n_problems, seed=0, nmin=2048, nmax=32768):
+    """Independent star boundaries for a batch: sizes log-uniform in
+    [nmin, nmax], k uniform in {3..9}, amplitude U[0.05, 0.25], seeded.
+    Returns the geometries and each one's (sizes, k, amp)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(nmin), math.log(nmax), n_problems))
+    sizes = np.clip(np.rint(sizes).astype(int), nmin, nmax)
+    ks = rng.integers(3, 10, n_problems)
+    amps = rng.uniform(0.05, 0.25, n_problems)
+    geoms = [star(int(n), amp=float(a), k=int(k)) for n, k, a in zip(sizes, ks, amps)]
+    return geoms, dict(sizes=sizes, k=ks, amp=amps)
+
+
+def batch_data(geoms, seed=0):
+    """Harmonic data per problem (4 seeded exterior sources each),
+    concatenated."""
+    return np.concatenate([harmonic_data(g["points"], *exterior_sources(seed + 1000 * i))
+                           for i, g in enumerate(geoms)])
+
+
 def exterior_sources(seed, n_src=4, center=(0.5, 0.5), rmin=0.7, rmax=1.0):
     """Seeded point sources outside the curve: radius U[rmin,rmax] from the
     centre, angle U[0,2pi), strengths N(0,1)."""

@@ -42,6 +42,7 @@ EXPORTS = [
     "iegpu_solve_refined", "iegpu_apply", "iegpu_set_id_method", "iegpu_id_stats",
     "iegpu_set_graphs", "iegpu_record_features", "iegpu_box_features", "iegpu_rank_model_init",
     "iegpu_rank_model_train", "iegpu_rank_model_grad", "iegpu_set_rank_model",
+    "iegpu_set_geometry_batch",
 ]
 RANK_FEATURES = 9
 RANK_MODEL_LEN = 1429
@@ -74,6 +75,8 @@ def load():
     L.iegpu_last_error.argtypes = [vp]
     L.iegpu_last_error.restype = ctypes.c_char_p
     L.iegpu_set_geometry.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.iegpu_set_geometry_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp,
+                                           _dp, _dp]
     L.iegpu_build_tree.argtypes = [vp, ctypes.c_int]
     L.iegpu_set_holes.argtypes = [vp, ctypes.c_int, _dp, _dp]
     L.iegpu_factor.argtypes = [vp, ctypes.c_double, ctypes.c_int]
@@ -157,6 +160,28 @@ class RSSolver:
         self.pde = pde
         self.ndof = n * (2 if pde == PDE_STOKES else 1)
         self.hole_dim = 0
+        self.dof_offsets = np.array([0, self.ndof], dtype=np.int64)
+
+    def set_geometry_batch(self, geoms, pde=PDE_LAPLACE):
+        """A batch of independent boundaries factored and solved together.
+        Right-hand sides and solutions are the problems' vectors concatenated;
+        `dof_offsets[p]:dof_offsets[p+1]` is problem p's slice."""
+        sd = 2 if pde == PDE_STOKES else 1
+        npts = [g["points"].shape[0] for g in geoms]
+        off = np.zeros(len(geoms) + 1, dtype=np.int32)
+        off[1:] = np.cumsum(npts)
+        self._arrays = [
+            _d(np.concatenate([g["points"] for g in geoms])).reshape(-1),
+            _d(np.concatenate([g["normals"] for g in geoms])).reshape(-1),
+            _d(np.concatenate([g["curvatures"] for g in geoms])),
+            _d(np.concatenate([g["weights"] for g in geoms]))]
+        self._poff = np.ascontiguousarray(off)
+        self._check(self.L.iegpu_set_geometry_batch(self.h, pde, len(geoms), _pi(self._poff),
+                                                    *[_pd(a) for a in self._arrays]))
+        self.pde = pde
+        self.ndof = int(off[-1]) * sd
+        self.hole_dim = 0
+        self.dof_offsets = off.astype(np.int64) * sd
 
     def build_tree(self, leaf_cap=64):
         self._check(self.L.iegpu_build_tree(self.h, leaf_cap))

@@ -58,6 +58,20 @@ int iegpu_set_geometry(iegpu_ctx* ctx, int pde, int n_points,
                        const double* points, const double* normals,
                        const double* curvatures, const double* weights);
 
+/* A batch of n_problems independent boundaries in one context: problem p
+ * owns nodes [point_offsets[p], point_offsets[p+1]) of the concatenated
+ * arrays (point_offsets[0] = 0, each problem non-empty).  build_tree makes
+ * one quadtree per problem, merged level by level, so every stage of
+ * factor and solve covers all problems in one launch per level; vectors
+ * are the problems' DOF vectors concatenated.  Holes, interior evaluation
+ * and the FP64 operator of a Stokes batch are not supported for
+ * n_problems > 1.  No counterpart in the reference (one boundary per
+ * QuadTree, ie_solver/quadtree/quadtree.h:95-139). */
+int iegpu_set_geometry_batch(iegpu_ctx* ctx, int pde, int n_problems,
+                             const int* point_offsets, const double* points,
+                             const double* normals, const double* curvatures,
+                             const double* weights);
+
 /* Build the quadtree (leaf boxes hold at most leaf_cap DOFs) and upload it.
  * Replaces QuadTree::initialize_tree, ie_solver/quadtree/quadtree.h:114-117
  * (quadtree.cpp:39-180; cap = MAX_LEAF_DOFS, quadtree.h:10). */

@@ -140,3 +140,52 @@ def test_holes_are_inside_and_disjoint():
         for j in range(i):
             assert math.hypot(*(c[i] - c[j])) > r[i] + r[j]
     assert g["points"].shape[0] == 20000
+
+
+def test_forest_tree_is_the_problems_trees_merged():
+    # A batch's quadtree is each problem's tree, merged level by level, with
+    # global DOF numbers and creation ids numbered problem after problem.
+    geoms, _ = G.star_batch(5, seed=3, nmin=300, nmax=3000)
+    s = iesolver.RSSolver(device=-1)
+    s.set_geometry_batch(geoms)
+    s.build_tree(64)
+    nl, nb, nd, _ = s.tree_info()
+    bi = s.box_info()
+    id0, dof0, levels = 0, 0, []
+    for g in geoms:
+        t = iesolver.RSSolver(device=-1)
+        t.set_geometry(g)
+        t.build_tree(64)
+        l1, b1, d1, _ = t.tree_info()
+        levels.append(l1)
+        ti = t.box_info()
+        sl = slice(id0, id0 + b1)
+        assert np.array_equal(bi["level"][sl], ti["level"])
+        assert np.array_equal(bi["is_leaf"][sl], ti["is_leaf"])
+        par = np.where(ti["parent"] >= 0, ti["parent"] + id0, -1)
+        assert np.array_equal(bi["parent"][sl], par)
+        ch = np.where(ti["children"] >= 0, ti["children"] + id0, -1)
+        assert np.array_equal(bi["children"][sl], ch)
+        for b in range(b1):
+            assert np.array_equal(s.box_list(id0 + b, 0), t.box_list(b, 0) + dof0)
+            assert np.array_equal(s.box_list(id0 + b, 1), t.box_list(b, 1) + id0)
+        id0 += b1
+        dof0 += d1
+    assert (nl, nb, nd) == (max(levels), id0, dof0)
+    # level 0 holds the roots, in problem order
+    roots = s.box_list(0, 5)
+    assert len(roots) == len(geoms) and (bi["level"][roots] == 0).all()
+
+
+def test_batch_arguments():
+    s = iesolver.RSSolver(device=-1)
+    g = G.circle(64)
+    L = s.L
+    off = np.array([0, 64], dtype=np.int32)
+    bad = np.array([0, 0, 64], dtype=np.int32)
+    a = [np.ascontiguousarray(g["points"]).reshape(-1), np.ascontiguousarray(g["normals"]).reshape(-1),
+         g["curvatures"], g["weights"]]
+    ptr = [x.ctypes.data_as(iesolver._dp) for x in a]
+    assert L.iegpu_set_geometry_batch(s.h, 0, 0, off.ctypes.data_as(iesolver._ip), *ptr) != 0
+    assert L.iegpu_set_geometry_batch(s.h, 0, 2, bad.ctypes.data_as(iesolver._ip), *ptr) != 0
+    assert L.iegpu_set_geometry_batch(s.h, 0, 1, off.ctypes.data_as(iesolver._ip), *ptr) == 0
