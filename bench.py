@@ -57,6 +57,12 @@ CONFIGS = {
     "star65k": dict(shape="star", n=65536, seed=0, tol=1e-10, leaf=64),
     # Laplace N=4096 circle r=0.5 (the reference's CPU-runnable case).
     "circle4k": dict(shape="circle", n=4096, seed=0, tol=1e-10, leaf=64),
+    # 256 independent Laplace problems, star boundaries with sizes
+    # log-uniform in [2048, 32768], k in {3..9}, amplitude U[0.05, 0.25],
+    # factored and solved together (one launch per stage per level); the
+    # FP32 factorization + FP64 refinement form is timed beside them.
+    "batch256": dict(shape="batch", n_problems=256, nmin=2048, nmax=32768, seed=0,
+                     tol=1e-10, leaf=64, fp32_tol=1e-6, cpu_problems=24),
 }
 
 
@@ -103,6 +109,116 @@ def stage_work(info, stage):
     return None, None
 
 
+def roofline_from(prof, info, id_stats, steps, traffic_table):
+    """Per-stage ms per step and the roofline of the stage with the most
+    device time (algorithmic work per launch / average launch duration)."""
+    roofline = None
+    stages_ms = {}
+    if prof:
+        for name, (ms, launches) in prof.items():
+            if launches:
+                stages_ms[name] = round(ms / steps, 4)
+        cands = []
+        for name in ("assemble", "rid", "qrcp", "decouple", "sweep_up", "sweep_down"):
+            ms, launches = prof[name]
+            if name == "rid":
+                # k_rid's work: the sketches on the matrix cores.
+                work, unit = (id_stats["sketch_gflop"] * 1e9, "flops") if id_stats else (None, None)
+            elif name == "qrcp" and id_stats is not None:
+                # What is left to the full QRCP: its boxes' flops are not
+                # separated from the sketched boxes', so it is not a candidate.
+                work, unit = None, None
+            else:
+                work, unit = stage_work(info, name)
+            if launches and work:
+                cands.append((ms, name, launches, work, unit))
+        defs = dict(assemble="8 m n bytes per box", decouple="6 s^2 r + 8 r^2 s + 2/3 r^3 flops",
+                    sweep_up="8 (3 s r + r^2) factor bytes", sweep_down="8 (3 s r + r^2) factor bytes",
+                    qrcp="2 m n^2 - 2/3 n^3 + k^2 r flops (full pivoted QR)",
+                    rid="2 l m n16 sketch flops on the matrix cores (k_rid)")
+        if cands:
+            ms, name, launches, work, unit = max(cands)
+            per_launch = work * steps / launches   # algorithmic work per launch
+            avg_s = ms / launches * 1e-3
+            traffic = traffic_table.get(name)
+            if unit == "bytes":
+                ach = per_launch / avg_s / 1e9
+                roofline = dict(kernel=name, bound="hbm", achieved=round(ach, 3),
+                                peak=PEAK_HBM_GBS, unit="GB/s", frac=round(ach / PEAK_HBM_GBS, 6),
+                                traffic=traffic, launches=launches,
+                                avg_launch_us=round(avg_s * 1e6, 3),
+                                work_per_launch=per_launch, work_def=defs[name])
+            else:
+                ach = per_launch / avg_s / 1e12
+                roofline = dict(kernel=name, bound="mfma", achieved=round(ach, 4),
+                                peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                                frac=round(ach / PEAK_FP64_TFLOPS, 6), traffic=traffic,
+                                launches=launches, avg_launch_us=round(avg_s * 1e6, 3),
+                                work_per_launch=per_launch, work_def=defs[name])
+    return stages_ms, roofline
+
+
+def time_steps(step, s, args, profile):
+    """ms per step of K steps after W warm-up steps, timed without HIP
+    events; with profile, K more steps with per-stage HIP events around
+    every launch give the per-stage device times (the events add gaps
+    between launches, so they stay out of the timed steps)."""
+    import torch
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = {}
+    if profile:
+        s.set_profiling(True)
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        prof = s.profile_table()
+        s.set_profiling(False)
+    return (t1 - t0) * 1e3 / args.steps, prof
+
+
+def tune(s, args, tol, precision):
+    """Autotune the launch caps for the current form (untimed); returns the
+    chosen caps."""
+    if args.no_autotune:
+        return None
+    caps, _ = s.autotune(tol, precision, reps=2)
+    return [int(c) for c in caps]
+
+
+def set_form(s, args):
+    """Select the ID of the timed form; returns the rank model's record."""
+    import iesolver
+    model_info = None
+    if args.form == "learned":
+        import rank_model
+        t0 = time.perf_counter()
+        if args.rank_model:
+            model = np.load(args.rank_model)
+            model_info = dict(loaded=args.rank_model, margin=3)
+        else:
+            X, y = rank_model.collect(rank_model.training_cases())
+            model, loss = rank_model.train(X, y)
+            model_info = dict(training_boxes=int(len(y)), final_loss=round(float(loss), 5),
+                              margin=3, seconds_untimed=round(time.perf_counter() - t0, 2),
+                              trained_on="stars k=3,5,9 N=16384, wavy seeds 1,2 N=65536, "
+                                         "Stokes holes N=16384 and star N=8192, "
+                                         "tol 1e-6/8/10")
+        s.set_rank_model(model, margin=3)
+        s.set_id_method(iesolver.ID_RANDOMIZED, 0)
+    elif args.form == "randomized":
+        s.set_id_method(iesolver.ID_RANDOMIZED, 32)
+    else:
+        s.set_id_method(iesolver.ID_QRCP)
+    return model_info
+
+
 def cpu_baseline(g, f, cfg, threads, reps):
     import oracle
     o = oracle.OracleProblem(g, leaf_cap=cfg["leaf"])
@@ -119,6 +235,143 @@ def cpu_baseline(g, f, cfg, threads, reps):
     return statistics.median(times), mu
 
 
+def run_batch(args, cfg):
+    """The batch workload: every problem's boxes share each stage launch."""
+    import torch
+    import iesolver
+    import oracle
+
+    geoms, meta = G.star_batch(cfg["n_problems"], seed=cfg["seed"], nmin=cfg["nmin"],
+                               nmax=cfg["nmax"])
+    f = G.batch_data(geoms, seed=cfg["seed"])
+    dev = torch.device("cuda:0")
+    s = iesolver.RSSolver(0)
+    s.set_geometry_batch(geoms)
+    s.build_tree(cfg["leaf"])
+    s.set_graphs(not args.no_graphs)
+    N = s.ndof
+    off = s.dof_offsets
+    f_dev = torch.from_numpy(f).to(dev)
+    mu_dev = torch.empty_like(f_dev)
+    torch.cuda.synchronize()
+
+    def step():
+        s.factor(cfg["tol"], iesolver.FP64)
+        s.solve_device(f_dev.data_ptr(), mu_dev.data_ptr(), None, None)
+
+    plain_ms, plain_prof, mu_plain, caps_plain = None, {}, None, None
+    if args.form == "plain" or not args.no_plain:
+        s.set_id_method(iesolver.ID_QRCP)
+        caps_plain = tune(s, args, cfg["tol"], iesolver.FP64)
+        plain_ms, plain_prof = time_steps(step, s, args, args.form == "plain" and not args.no_profile)
+        mu_plain = mu_dev.cpu().numpy()
+    model_info = None
+    if args.form == "plain":
+        ms_per_step, prof, caps = plain_ms, plain_prof, caps_plain
+    else:
+        model_info = set_form(s, args)
+        caps = tune(s, args, cfg["tol"], iesolver.FP64)
+        ms_per_step, prof = time_steps(step, s, args, not args.no_profile)
+    mu = mu_dev.cpu().numpy()
+    mu_exact, _, ref_its, ref_res = s.solve_refined(f, 1e-15, 4)
+
+    def err(x):
+        return float(np.linalg.norm(x - mu_exact) / np.linalg.norm(mu_exact))
+
+    id_stats = None
+    if args.form != "plain":
+        ns, nf, sf = s.id_stats(with_flops=True)
+        id_stats = dict(sketched=ns, fallback=nf, sketch_gflop=round(sf / 1e9, 3))
+    info = s.box_info()
+    nl, nb, nd, _ = s.tree_info()
+    stages_ms, roofline = roofline_from(prof, info, id_stats, args.steps, {})
+
+    # FP32 factorization as the preconditioner of FP64 refinement (host
+    # vectors in and out: the call uploads b and downloads x).
+    ms32, its32, res32, err32 = None, None, None, None
+    if not args.no_fp32:
+        s.set_id_method(iesolver.ID_QRCP)
+        for _ in range(args.warmup):
+            s.factor(cfg["fp32_tol"], iesolver.FP32)
+            s.solve_refined(f, 1e-11, 10)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s.factor(cfg["fp32_tol"], iesolver.FP32)
+            x32, _, its32, res32 = s.solve_refined(f, 1e-11, 10)
+        t1 = time.perf_counter()
+        ms32 = (t1 - t0) * 1e3 / args.steps
+        err32 = err(x32)
+
+    # Interior field of problem 0 vs its analytic harmonic function.
+    tg = G.interior_points(1, 100, radius=0.1)
+    ys, q = G.exterior_sources(cfg["seed"])
+    u = oracle.evaluate(geoms[0], tg, mu[off[0]:off[1]])
+    ue = G.harmonic_data(tg, ys, q)
+    interior_err = float(np.linalg.norm(u - ue) / np.linalg.norm(ue))
+
+    cpu, parity = None, None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        k = cfg["cpu_problems"]
+        times, parts = [], []
+        for p in range(k):
+            t_p, mu_p = cpu_baseline(geoms[p], f[off[p]:off[p + 1]], cfg, threads, 1)
+            times.append(t_p)
+            parts.append(np.linalg.norm(mu[off[p]:off[p + 1]] - mu_p) / np.linalg.norm(mu_p))
+        n_s = int(off[k])
+        cpu = dict(value=round(n_s / sum(times), 1), unit="unknowns/s", cores=threads, kind="port",
+                   sample=f"the batch's first {k} problems ({n_s} unknowns), each factored and "
+                          f"solved once after 1 warm-up, OpenMP over boxes, BLAS threads=1",
+                   seconds=round(sum(times), 4))
+        parity = float(max(parts))
+
+    out = {
+        "metric": METRIC,
+        "value": round(N / (ms_per_step * 1e-3), 1),
+        "unit": "unknowns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"laplace_{args.config}_{args.form}_fp64", "pde": "laplace",
+                   "n_problems": cfg["n_problems"], "n_unknowns": N,
+                   "problem_sizes": [int(meta["sizes"].min()), int(meta["sizes"].max())],
+                   "id_tol": cfg["tol"], "leaf_dofs": cfg["leaf"], "levels": nl, "boxes": nb,
+                   "form": args.form + "_fp64", "parallelism": "single_gpu"},
+        "plain_fp64_ms_per_step": round(plain_ms, 4) if plain_ms else None,
+        "speedup_vs_plain_fp64": round(plain_ms / ms_per_step, 4) if plain_ms else None,
+        "rel_err_vs_exact": err(mu),
+        "plain_fp64_rel_err_vs_exact": err(mu_plain) if mu_plain is not None else None,
+        "exact": f"FP64 refinement to relative residual {ref_res:.1e} ({ref_its} corrections)",
+        "fp32_refined": (dict(ms_per_step=round(ms32, 4), id_tol=cfg["fp32_tol"],
+                              corrections=its32, rel_residual=res32, rel_err_vs_exact=err32,
+                              unknowns_per_s=round(N / (ms32 * 1e-3), 1),
+                              note="FP32 factor + FP64 refinement to relative residual 1e-11; "
+                                   "b and x cross PCIe inside the step")
+                         if ms32 else None),
+        "hip_graph": not args.no_graphs,
+        "launch_caps": {"form": caps, "plain_fp64": caps_plain,
+                        "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp",
+                        "autotuned": not args.no_autotune},
+        "id_stats": id_stats,
+        "rank_model": model_info,
+        "rel_err_interior": interior_err,
+        "rel_err_vs_cpu_oracle": parity,
+        "stages_ms": stages_ms,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    s.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +386,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 comparator (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch caps instead of tuning them (untimed) per form")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="batch workload: skip the FP32 + refinement form")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-profile", action="store_true",
                     help="skip per-stage HIP events (roofline fields become null)")
@@ -143,6 +400,8 @@ def main():
               file=sys.stderr)
         return 2
     cfg = CONFIGS[args.config]
+    if cfg["shape"] == "batch":
+        return run_batch(args, cfg)
     g, f, ys, q = make_workload(cfg)
     N = g["points"].shape[0]
 
@@ -163,56 +422,21 @@ def main():
         s.solve_device(f_dev.data_ptr(), mu_dev.data_ptr(), None, None)
 
     def timed(profile):
-        """ms per step of K steps after W warm-up steps, timed without HIP
-        events; with profile, K more steps with per-stage HIP events around
-        every launch give the per-stage device times (the events add gaps
-        between launches, so they stay out of the timed steps)."""
-        for _ in range(args.warmup):
-            step()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        prof = {}
-        if profile:
-            s.set_profiling(True)
-            for _ in range(args.steps):
-                step()
-            torch.cuda.synchronize()
-            prof = s.profile_table()
-            s.set_profiling(False)
-        return (t1 - t0) * 1e3 / args.steps, prof
+        return time_steps(step, s, args, profile)
 
     # Comparator: the plain FP64 form on the same workload.
-    plain_ms, plain_prof, mu_plain = None, {}, None
+    plain_ms, plain_prof, mu_plain, caps_plain = None, {}, None, None
     if args.form == "plain" or not args.no_plain:
         s.set_id_method(iesolver.ID_QRCP)
+        caps_plain = tune(s, args, cfg["tol"], iesolver.FP64)
         plain_ms, plain_prof = timed(args.form == "plain" and not args.no_profile)
         mu_plain = mu_dev.cpu().numpy()
     model_info = None
     if args.form == "plain":
-        ms_per_step, prof = plain_ms, plain_prof
+        ms_per_step, prof, caps = plain_ms, plain_prof, caps_plain
     else:
-        if args.form == "learned":
-            import rank_model
-            t0 = time.perf_counter()
-            if args.rank_model:
-                model = np.load(args.rank_model)
-                model_info = dict(loaded=args.rank_model, margin=3)
-            else:
-                X, y = rank_model.collect(rank_model.training_cases())
-                model, loss = rank_model.train(X, y)
-                model_info = dict(training_boxes=int(len(y)), final_loss=round(float(loss), 5),
-                                  margin=3, seconds_untimed=round(time.perf_counter() - t0, 2),
-                                  trained_on="stars k=3,5,9 N=16384, wavy seeds 1,2 N=65536, "
-                                             "Stokes holes N=16384 and star N=8192, "
-                                             "tol 1e-6/8/10")
-            s.set_rank_model(model, margin=3)
-            s.set_id_method(iesolver.ID_RANDOMIZED, 0)
-        else:
-            s.set_id_method(iesolver.ID_RANDOMIZED, 32)
+        model_info = set_form(s, args)
+        caps = tune(s, args, cfg["tol"], iesolver.FP64)
         ms_per_step, prof = timed(not args.no_profile)
     mu = mu_dev.cpu().numpy()
     # Accuracy against the exact discrete solution (untimed): this form's
@@ -236,49 +460,8 @@ def main():
     ue = G.harmonic_data(tg, ys, q)
     interior_err = float(np.linalg.norm(u - ue) / np.linalg.norm(ue))
 
-    roofline = None
-    stages_ms = {}
-    if prof:
-        for name, (ms, launches) in prof.items():
-            if launches:
-                stages_ms[name] = round(ms / args.steps, 4)
-        cands = []
-        for name in ("assemble", "rid", "qrcp", "decouple", "sweep_up", "sweep_down"):
-            ms, launches = prof[name]
-            if name == "rid":
-                # k_rid's work: the sketches on the matrix cores.
-                work, unit = (id_stats["sketch_gflop"] * 1e9, "flops") if id_stats else (None, None)
-            elif name == "qrcp" and id_stats is not None:
-                # What is left to the full QRCP: its boxes' flops are not
-                # separated from the sketched boxes', so it is not a candidate.
-                work, unit = None, None
-            else:
-                work, unit = stage_work(info, name)
-            if launches and work:
-                cands.append((ms, name, launches, work, unit))
-        defs = dict(assemble="8 m n bytes per box", decouple="6 s^2 r + 8 r^2 s + 2/3 r^3 flops",
-                    sweep_up="8 (3 s r + r^2) factor bytes", sweep_down="8 (3 s r + r^2) factor bytes",
-                    qrcp="2 m n^2 - 2/3 n^3 + k^2 r flops (full pivoted QR)",
-                    rid="2 l m n16 sketch flops on the matrix cores (k_rid)")
-        if cands:
-            ms, name, launches, work, unit = max(cands)
-            per_launch = work * args.steps / launches   # algorithmic work per launch
-            avg_s = ms / launches * 1e-3
-            traffic = TRAFFIC.get(name) if args.form == "learned" else None
-            if unit == "bytes":
-                ach = per_launch / avg_s / 1e9
-                roofline = dict(kernel=name, bound="hbm", achieved=round(ach, 3),
-                                peak=PEAK_HBM_GBS, unit="GB/s", frac=round(ach / PEAK_HBM_GBS, 6),
-                                traffic=traffic, launches=launches,
-                                avg_launch_us=round(avg_s * 1e6, 3),
-                                work_per_launch=per_launch, work_def=defs[name])
-            else:
-                ach = per_launch / avg_s / 1e12
-                roofline = dict(kernel=name, bound="mfma", achieved=round(ach, 4),
-                                peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                                frac=round(ach / PEAK_FP64_TFLOPS, 6), traffic=traffic,
-                                launches=launches, avg_launch_us=round(avg_s * 1e6, 3),
-                                work_per_launch=per_launch, work_def=defs[name])
+    stages_ms, roofline = roofline_from(prof, info, id_stats, args.steps,
+                                        TRAFFIC if args.form == "learned" and args.config == "wavy262k" else {})
 
     cpu = None
     parity = None
@@ -316,6 +499,9 @@ def main():
         "rel_err_vs_plain_fp64": (float(np.linalg.norm(mu - mu_plain) / np.linalg.norm(mu_plain))
                                   if mu_plain is not None else None),
         "hip_graph": not args.no_graphs,
+        "launch_caps": {"form": caps, "plain_fp64": caps_plain,
+                        "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp",
+                        "autotuned": not args.no_autotune},
         "id_stats": id_stats,
         "rank_model": model_info,
         "rel_err_interior": interior_err,

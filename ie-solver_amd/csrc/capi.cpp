@@ -24,6 +24,7 @@ using namespace iegpu;
 static_assert(IEGPU_RANK_FEATURES == kFeat && IEGPU_RANK_MODEL_LEN == kMlpLen &&
                   IEGPU_RANK_PARAMS == kMlpParams,
               "rank model layout of iegpu.h");
+static_assert(IEGPU_NUM_LAUNCH_CAPS == kNumCaps, "launch caps of iegpu.h");
 
 namespace {
 
@@ -199,6 +200,7 @@ struct iegpu_ctx {
   bool use_graphs = false;
   hipGraphExec_t fgraph = nullptr;
   DevState fgraph_state{};
+  LaunchCaps caps, fgraph_caps;
   double fgraph_tol = 0;
   int fgraph_prec = -1;
   // rank model
@@ -420,7 +422,7 @@ void enqueue_factor(iegpu_ctx* c, double tol) {
     const int count = T.level_start[lv + 1] - start;
     for (int stage = 0; stage < kNumFactorStages; stage++)
       c->prof.run(stage, lv, st, [&] {
-        launch_factor_stage<R>(stage, g, t, s, lv, start, count, tol, st);
+        launch_factor_stage<R>(stage, g, t, s, lv, start, count, tol, c->caps, st);
       });
   }
   c->prof.run(kProfRoot, 0, st,
@@ -950,6 +952,7 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
         // Capture the launch sequence once per state; replay it after.
         const DevState cur = c->dstate();
         if (!c->fgraph || std::memcmp(&cur, &c->fgraph_state, sizeof(DevState)) != 0 ||
+            std::memcmp(&c->caps, &c->fgraph_caps, sizeof(LaunchCaps)) != 0 ||
             c->fgraph_tol != id_tol || c->fgraph_prec != precision) {
           if (c->fgraph) HIP_TRY(hipGraphExecDestroy(c->fgraph));
           c->fgraph = nullptr;
@@ -962,6 +965,7 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
           (void)hipGraphDestroy(gr);
           HIP_TRY(ie);
           c->fgraph_state = cur;
+          c->fgraph_caps = c->caps;
           c->fgraph_tol = id_tol;
           c->fgraph_prec = precision;
         }
@@ -1343,6 +1347,79 @@ int iegpu_evaluate(iegpu_ctx* c, int n_targets, const double* targets,
   } catch (const HipError& e) {
     return from_hip(c, e);
   }
+  return IEGPU_OK;
+}
+
+int iegpu_set_launch_caps(iegpu_ctx* c, const int* caps, int n) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!caps || n != kNumCaps) return fail(c, IEGPU_ERR_INVALID, "need IEGPU_NUM_LAUNCH_CAPS caps");
+  for (int i = 0; i < n; i++)
+    if (caps[i] < 1 || caps[i] > (1 << 20))
+      return fail(c, IEGPU_ERR_INVALID, "launch caps must be in [1, 2^20]");
+  LaunchCaps& L = c->caps;
+  L.assemble = caps[0];
+  L.qrcp_id = caps[1];
+  L.rowcomp4 = caps[2];
+  L.rowcomp8 = caps[3];
+  L.qrcp_id8 = caps[4];
+  L.qrcp = caps[5];
+  return IEGPU_OK;
+}
+
+int iegpu_get_launch_caps(const iegpu_ctx* c, int* caps, int n) {
+  if (!c || !caps || n != kNumCaps) return IEGPU_ERR_INVALID;
+  const LaunchCaps& L = c->caps;
+  const int v[kNumCaps] = {L.assemble, L.qrcp_id, L.rowcomp4, L.rowcomp8, L.qrcp_id8, L.qrcp};
+  for (int i = 0; i < n; i++) caps[i] = v[i];
+  return IEGPU_OK;
+}
+
+int iegpu_autotune(iegpu_ctx* c, double id_tol, int precision, int reps, int* caps_out,
+                   double* ms_out) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (reps < 1 || reps > 64) return fail(c, IEGPU_ERR_INVALID, "reps must be in [1, 64]");
+  // Coordinate descent over the caps: each candidate's time is the best of
+  // `reps` factorizations after one untimed run (which also re-captures
+  // the graph when graphs are on).  A candidate replaces the current value
+  // only if it is at least 1 % faster, so timing noise does not move it.
+  static const int kCand[] = {64, 256, 1024, 4096};
+  auto measure = [&](double* best) -> int {
+    *best = 1e300;
+    for (int r = 0; r <= reps; r++) {
+      const int st = iegpu_factor(c, id_tol, precision);
+      if (st != IEGPU_OK) return st;
+      if (r > 0) *best = std::min(*best, c->t_factor_ms);
+    }
+    return IEGPU_OK;
+  };
+  int cur[kNumCaps];
+  iegpu_get_launch_caps(c, cur, kNumCaps);
+  double best = 0;
+  int st = measure(&best);
+  if (st != IEGPU_OK) return st;
+  for (int k = 0; k < kNumCaps; k++) {
+    int pick = cur[k];
+    for (int v : kCand) {
+      if (v == cur[k]) continue;
+      int trial[kNumCaps];
+      std::memcpy(trial, cur, sizeof(trial));
+      trial[k] = v;
+      iegpu_set_launch_caps(c, trial, kNumCaps);
+      double t = 0;
+      st = measure(&t);
+      if (st != IEGPU_OK) return st;
+      if (t < 0.99 * best) {
+        best = t;
+        pick = v;
+      }
+    }
+    cur[k] = pick;
+    iegpu_set_launch_caps(c, cur, kNumCaps);
+  }
+  st = iegpu_factor(c, id_tol, precision);  // leave a factorization with the chosen caps
+  if (st != IEGPU_OK) return st;
+  if (caps_out) std::memcpy(caps_out, cur, sizeof(cur));
+  if (ms_out) *ms_out = best;
   return IEGPU_OK;
 }
 

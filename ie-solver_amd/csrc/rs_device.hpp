@@ -40,6 +40,7 @@ constexpr int kMaxChunks = 8;
 constexpr int kMaxLevels = 64;  // tree depth bound (per-level job counters)
 constexpr int kSmallBox = 64;     // decouple with every operand in LDS
 constexpr int kRootRegMax = 192;  // root blocks inverted in registers
+constexpr int kRootBlkMax = 512;  // root blocks inverted by the blocked kernel
 
 // Per-level lists of the boxes each QRCP kernel takes, built by prefix
 // sums (k_scan) so that a kernel's workgroups map onto exactly its boxes.

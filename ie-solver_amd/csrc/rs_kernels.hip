@@ -2040,10 +2040,10 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
     A[idx] = static_cast<R>(kernel_entry(g, act[i], act[j])) - upd(i, j);
   }
   __syncthreads();
-  if (factor_lu && n > kRootRegMax) {
+  if (factor_lu && n > kRootBlkMax) {
     // Gauss-Jordan inverse (the reference LU-factors the root block) for
-    // blocks too large for k_root_inv; the inverse, unpermuted, goes after
-    // the assembled matrix.
+    // blocks too large for k_root_inv and k_root_inv_blk; the inverse,
+    // unpermuted, goes after the assembled matrix.
     R* B = A + (int64_t)n * n;
     int* prow = root_pivots(s, b);
     const int info = gj_invert_into<R, NT>(A, B, n, prow);
@@ -2319,6 +2319,135 @@ __global__ __launch_bounds__(64) void k_sweep_down(DevState s, int start,
   }
 }
 
+// Blocked in-place Gauss-Jordan inverse of a root block with
+// kRootRegMax < n <= kRootBlkMax, one workgroup per root, thread t owning
+// row t.  Per panel of 16 columns J:
+//   1. the panel's 16 elimination steps run on the n x 16 slab in
+//      registers (pivot row: largest |value| among the rows not yet
+//      pivoted, as gj_invert; one barrier per step);
+//   2. the other columns take all 16 steps at once: with W the final slab
+//      and P the panel's pivot rows, the steps multiply every other column
+//      x by G = I + sum_k (W_k - e_{p_k}) e_{p_k}^T, so
+//      x_i += sum_k W_ik x_{p_k} (old values of the pivot rows), and a
+//      pivot row's own value is replaced rather than added to.
+// The result equals the column-by-column algorithm up to rounding order.
+// The inverse, unpermuted, goes after the matrix, as in k_root_inv.
+constexpr int kRootBlkB = 16;
+
+template <typename R>
+__global__ __launch_bounds__(kRootBlkMax) void k_root_inv_blk(DevState s, int factor_lu) {
+  constexpr int B = kRootBlkB, NW = kRootBlkMax / 64;
+  __shared__ R rowv[2][NW][B];
+  __shared__ double keys[2][NW];
+  __shared__ R Rl[kRootBlkMax * B];
+  __shared__ int prow[kRootBlkMax];
+  __shared__ int invrow[kRootBlkMax];
+  __shared__ int pan[B];
+  if (!factor_lu || !pools_ok(s)) return;
+  const int root = blockIdx.x;
+  const int n = s.act_cnt[root];
+  if (n <= kRootRegMax || n > kRootBlkMax) return;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i = tid;  // own row
+  const bool live = i < n;
+  R* A = root_block<R>(s, root);
+  bool used = !live;
+  int info = 0;
+  for (int j0 = 0; j0 < n; j0 += B) {
+    const int bw = min(B, n - j0);
+    R w[B];
+#pragma unroll
+    for (int k = 0; k < B; k++) w[k] = (live && k < bw) ? A[i + (int64_t)n * (j0 + k)] : R(0);
+    bool inpan = false;
+#pragma unroll
+    for (int k = 0; k < B; k++) {
+      if (k < bw) {
+        R v = fabs(w[k]);
+        if (!isfinite(v)) v = R(0);  // every unpivoted row stays a candidate
+        const double key = used ? -1.0 : pivot_key<R, 9>(v, i);
+        const double wbest = wave_max(key);
+        const int buf = k & 1;
+        if (!used && key == wbest) {
+#pragma unroll
+          for (int c = 0; c < B; c++) rowv[buf][wv][c] = w[c];
+        }
+        if (lane == 0) keys[buf][wv] = wbest;
+        __syncthreads();
+        double best = keys[buf][0];
+        int bwv = 0;
+#pragma unroll
+        for (int q = 1; q < NW; q++)
+          if (keys[buf][q] > best) {
+            best = keys[buf][q];
+            bwv = q;
+          }
+        const int p = pivot_key_pos<9>(best);
+        R rv[B];
+#pragma unroll
+        for (int c = 0; c < B; c++) rv[c] = rowv[buf][bwv][c];
+        const R piv = rv[k];
+        if (piv == R(0) && info == 0) info = j0 + k + 1;
+        const R inv = R(1) / piv;
+#pragma unroll
+        for (int c = 0; c < B; c++) rv[c] = (c == k) ? inv : rv[c] * inv;
+        const R cv = w[k];
+        w[k] = R(0);
+#pragma unroll
+        for (int c = 0; c < B; c++) w[c] -= cv * rv[c];
+        if (i == p) {
+#pragma unroll
+          for (int c = 0; c < B; c++) w[c] = rv[c];
+          used = true;
+          inpan = true;
+        }
+        if (tid == 0) {
+          prow[j0 + k] = p;
+          pan[k] = p;
+        }
+      }
+    }
+    __syncthreads();
+    // old values of the panel's pivot rows in every other column
+    for (int idx = tid; idx < n * B; idx += kRootBlkMax) {
+      const int c = idx / B, k = idx % B;
+      const bool skip = k >= bw || (c >= j0 && c < j0 + bw);
+      Rl[idx] = skip ? R(0) : A[pan[k] + (int64_t)n * c];
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < n; c0 += 8) {
+      R x[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int c = c0 + q;
+        const bool other = c < n && (c < j0 || c >= j0 + bw);
+        x[q] = (live && other && !inpan) ? A[i + (int64_t)n * c] : R(0);
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int c = c0 + q;
+        if (c < n && (c < j0 || c >= j0 + bw)) {
+          R acc = x[q];
+#pragma unroll
+          for (int k = 0; k < B; k++) acc += w[k] * Rl[c * B + k];
+          if (live) A[i + (int64_t)n * c] = acc;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < B; k++)
+      if (live && k < bw) A[i + (int64_t)n * (j0 + k)] = w[k];
+    __syncthreads();
+  }
+  if (tid < n) invrow[prow[tid]] = tid;
+  __syncthreads();
+  R* Bm = A + (int64_t)n * n;
+  for (int64_t idx = tid; idx < (int64_t)n * n; idx += kRootBlkMax) {
+    const int r = (int)(idx % n), c = (int)(idx / n);
+    Bm[invrow[r] + (int64_t)n * prow[c]] = A[idx];
+  }
+  if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
+}
+
 // x_root <- (root inverse) x_root: 256 threads, one row each per pass.
 template <typename R>
 __global__ __launch_bounds__(256) void k_root_solve(DevState s, SolveVec v) {
@@ -2360,7 +2489,7 @@ void launch_gaussian(double* out, int64_t n, uint64_t seed, hipStream_t st) {
 template <typename R>
 void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
                          const DevState& s, int level, int start, int count,
-                         double tol, hipStream_t st) {
+                         double tol, const LaunchCaps& caps, hipStream_t st) {
   if (count <= 0) return;
   switch (stage) {
     case kStageActive:
@@ -2373,7 +2502,7 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       k_scan<<<1, 1024, 0, st>>>(s, start, count, level);
       break;
     case kStageAssemble:
-      k_assemble<R><<<min(count, 64), 256, 0, st>>>(g, t, s, start, level);
+      k_assemble<R><<<min(count, caps.assemble), 256, 0, st>>>(g, t, s, start, level);
       break;
     case kStageRid:
       if (s.rid_mode != 0) {
@@ -2386,21 +2515,23 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
     case kStageQrcp:
       if (s.rid_mode != 0) {
         // Only the boxes without a successful sketch reach the full QRCP.
-        const int grid = min(count, 64);
+        const int grid = min(count, caps.qrcp_id);
         // (box, chunk) jobs of the listed boxes: up to kMaxChunks per box
-        k_rowcomp<R, 4><<<min(count * kMaxChunks, 256), kXThreads, 0, st>>>(g, t, s, start,
-                                                                           level);
+        k_rowcomp<R, 4><<<min(count * kMaxChunks, caps.rowcomp4), kXThreads, 0, st>>>(
+            g, t, s, start, level);
         k_qrcp_id<R, 4, 8, 8, true><<<grid, 512, 0, st>>>(g, t, s, start, level, tol);
         k_qrcp_id<R, 4, 8, 16, true><<<grid, kXThreads, 0, st>>>(g, t, s, start, level, tol);
       } else {
-        k_rowcomp<R, 4><<<min(2 * count, 256), kXThreads, 0, st>>>(g, t, s, start, level);
+        k_rowcomp<R, 4><<<min(2 * count, caps.rowcomp4), kXThreads, 0, st>>>(g, t, s, start,
+                                                                            level);
         k_qrcp_id<R, 4, 8, 8, false><<<count, 512, 0, st>>>(g, t, s, start, level, tol);
         k_qrcp_id<R, 4, 8, 16, false><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
       }
-      k_rowcomp<R, 8><<<min(2 * count, 64), kXThreads, 0, st>>>(g, t, s, start, level);
-      k_qrcp_id<R, 8, 4, 16, true><<<min(count, 64), kXThreads, 0, st>>>(g, t, s, start, level,
-                                                                         tol);
-      k_qrcp<R, 512><<<min(count, 64), 512, 0, st>>>(s, start, level, tol);
+      k_rowcomp<R, 8><<<min(2 * count, caps.rowcomp8), kXThreads, 0, st>>>(g, t, s, start,
+                                                                            level);
+      k_qrcp_id<R, 8, 4, 16, true><<<min(count, caps.qrcp_id8), kXThreads, 0, st>>>(
+          g, t, s, start, level, tol);
+      k_qrcp<R, 512><<<min(count, caps.qrcp), 512, 0, st>>>(s, start, level, tol);
       break;
     case kStageDecouple:
       k_decouple_small<R><<<count, 256, 0, st>>>(g, t, s, start);
@@ -2417,6 +2548,7 @@ void launch_root(const DevGeom& g, const DevTree& t, const DevState& s,
   k_active<<<n_roots, 64, 0, st>>>(t, s, 0);
   k_root<R, 1024><<<n_roots, 1024, 0, st>>>(g, t, s, factor_lu ? 1 : 0);
   k_root_inv<R><<<n_roots, kRootTR * kRootTC, 0, st>>>(s, factor_lu ? 1 : 0);
+  k_root_inv_blk<R><<<n_roots, kRootBlkMax, 0, st>>>(s, factor_lu ? 1 : 0);
 }
 
 template <typename R>
@@ -2448,7 +2580,7 @@ void launch_root_solve(const DevState& s, int n_roots, SolveVec v, hipStream_t s
 #define IEGPU_INSTANTIATE(R)                                                   \
   template void launch_factor_stage<R>(int, const DevGeom&, const DevTree&,   \
                                        const DevState&, int, int, int, double, \
-                                       hipStream_t);                           \
+                                       const LaunchCaps&, hipStream_t);        \
   template void launch_root<R>(const DevGeom&, const DevTree&,                \
                                const DevState&, int, bool, hipStream_t);       \
   template void launch_sweep_up<R>(const DevState&, int, int, SolveVec, bool, \

@@ -29,10 +29,25 @@ enum FactorStage : int {
   kNumFactorStages = 7
 };
 
+// Grid caps of the kernels that walk a device-side box list with a grid
+// stride (the host knows only the level's box count, not the list's
+// length).  Workgroups past the list's end exit at once, so a cap trades
+// launch width against nothing but idle workgroups; iegpu_autotune picks
+// them per workload.
+struct LaunchCaps {
+  int assemble = 1024;   // k_assemble (general-QRCP boxes' ID matrices)
+  int qrcp_id = 1024;    // k_qrcp_id register shapes, randomized-ID fallback
+  int rowcomp4 = 1024;   // k_rowcomp<4> (box, chunk) jobs
+  int rowcomp8 = 1024;   // k_rowcomp<8> (box, chunk) jobs
+  int qrcp_id8 = 1024;   // k_qrcp_id<8, 4, 16> (n <= 128 after row compression)
+  int qrcp = 1024;       // k_qrcp (general LDS/global QRCP)
+};
+constexpr int kNumCaps = 6;
+
 template <typename R>
 void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
                          const DevState& s, int level, int start, int count,
-                         double tol, hipStream_t st);
+                         double tol, const LaunchCaps& caps, hipStream_t st);
 
 // Roots (slots 0 .. n_roots-1, one per problem): active list, dense
 // skeleton block, inverse.

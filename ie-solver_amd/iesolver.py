@@ -42,8 +42,10 @@ EXPORTS = [
     "iegpu_solve_refined", "iegpu_apply", "iegpu_set_id_method", "iegpu_id_stats",
     "iegpu_set_graphs", "iegpu_record_features", "iegpu_box_features", "iegpu_rank_model_init",
     "iegpu_rank_model_train", "iegpu_rank_model_grad", "iegpu_set_rank_model",
-    "iegpu_set_geometry_batch",
+    "iegpu_set_geometry_batch", "iegpu_set_launch_caps", "iegpu_get_launch_caps",
+    "iegpu_autotune",
 ]
+NUM_LAUNCH_CAPS = 6
 RANK_FEATURES = 9
 RANK_MODEL_LEN = 1429
 RANK_PARAMS = 1409
@@ -103,6 +105,9 @@ def load():
                                          ctypes.c_float, ctypes.c_uint, fp]
     L.iegpu_rank_model_grad.argtypes = [vp, fp, fp, fp, ctypes.c_int, fp, fp]
     L.iegpu_set_rank_model.argtypes = [vp, fp, ctypes.c_int]
+    L.iegpu_set_launch_caps.argtypes = [vp, _ip, ctypes.c_int]
+    L.iegpu_get_launch_caps.argtypes = [vp, _ip, ctypes.c_int]
+    L.iegpu_autotune.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp]
     _lib = L
     return L
 
@@ -199,6 +204,23 @@ class RSSolver:
         """ID_QRCP (the reference's full pivoted QR) or ID_RANDOMIZED (Gaussian
         sketch of sketch_rows rows, full-QRCP fallback)."""
         self._check(self.L.iegpu_set_id_method(self.h, method, sketch_rows))
+
+    def set_launch_caps(self, caps):
+        c = np.ascontiguousarray(caps, dtype=np.int32)
+        self._check(self.L.iegpu_set_launch_caps(self.h, _pi(c), len(c)))
+
+    def launch_caps(self):
+        c = np.zeros(NUM_LAUNCH_CAPS, dtype=np.int32)
+        self._check(self.L.iegpu_get_launch_caps(self.h, _pi(c), len(c)))
+        return c
+
+    def autotune(self, id_tol, precision=FP64, reps=2):
+        """Pick the launch caps for this workload; returns (caps, best ms)."""
+        c = np.zeros(NUM_LAUNCH_CAPS, dtype=np.int32)
+        ms = ctypes.c_double(0)
+        self._check(self.L.iegpu_autotune(self.h, id_tol, precision, reps, _pi(c),
+                                          ctypes.byref(ms)))
+        return c, ms.value
 
     def set_graphs(self, on=True):
         """Replay the factorization as a captured HIP graph (iegpu_set_graphs)."""

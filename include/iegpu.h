@@ -173,6 +173,27 @@ int iegpu_profile(iegpu_ctx* ctx, int stage, int level, double* ms,
  * profiling is on or with holes.  No reference counterpart. */
 int iegpu_set_graphs(iegpu_ctx* ctx, int on);
 
+/* Launch widths of the factorization kernels that walk a device-side box
+ * list (the host knows a level's box count, not the list's length):
+ * caps[IEGPU_NUM_LAUNCH_CAPS] = workgroup caps of the general-QRCP
+ * assembly, the register-QRCP fallback of the randomized ID, the 4- and
+ * 8-wave row compressions, the 8-wave register QRCP and the general QRCP.
+ * Each in [1, 2^20]; default 1024.  The caps change launch widths only:
+ * every box's arithmetic is the same for any caps.  No reference
+ * counterpart. */
+#define IEGPU_NUM_LAUNCH_CAPS 6
+int iegpu_set_launch_caps(iegpu_ctx* ctx, const int* caps, int n);
+int iegpu_get_launch_caps(const iegpu_ctx* ctx, int* caps, int n);
+
+/* Autotuner: coordinate descent over the launch caps, each candidate
+ * (64, 256, 1024, 4096) timed as the best of `reps` factorizations of the
+ * current workload at (id_tol, precision) after one untimed run; a value
+ * changes only for a gain of at least 1 %.  Leaves the chosen caps set and
+ * the workload factored with them; writes the caps (may be NULL) and the
+ * best factorization time in ms (may be NULL).  No reference counterpart. */
+int iegpu_autotune(iegpu_ctx* ctx, double id_tol, int precision, int reps,
+                   int* caps, double* ms);
+
 /* Interpolative decomposition used by the next iegpu_factor.
  * IEGPU_ID_QRCP (default): the reference's full column-pivoted QR of each
  * box's ID matrix with the rank cut |R_ii| < tol |R_00|

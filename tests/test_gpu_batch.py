@@ -63,6 +63,27 @@ def test_batch_matches_problems_solved_alone():
         id0 += nb
 
 
+def test_batch_with_large_roots_matches_problems_alone():
+    # Two of the problems have root skeletons above 192 DOFs (blocked
+    # Gauss-Jordan root inverse), the others below (register inverse).
+    geoms = [G.star(4096, amp=0.25, k=9), G.star(1500, amp=0.1, k=4),
+             G.star(8192, amp=0.233, k=8), G.star(700, amp=0.05, k=3)]
+    f = G.batch_data(geoms, seed=7)
+    s = _batch(geoms)
+    mu, _ = s.solve(f)
+    off = s.dof_offsets
+    roots = s.box_info()["n_active"][s.box_list(0, 5)]
+    print("root sizes", roots.tolist(), flush=True)
+    assert (roots > 192).sum() == 2
+    for p, g in enumerate(geoms):
+        t = _single(g)
+        sl = slice(off[p], off[p + 1])
+        mp, _ = t.solve(f[sl])
+        err = _rel(mu[sl], mp)
+        print(f"problem {p} batch vs alone {err:.2e}", flush=True)
+        assert err <= 1e-12
+
+
 def test_batch_vs_oracle():
     geoms, _ = G.star_batch(3, seed=2, nmin=1024, nmax=3072)
     f = G.batch_data(geoms, seed=2)
@@ -127,3 +148,22 @@ def test_batch_graph_replays_are_exact():
         s.factor(TOL)
         m1, _ = s.solve(f)
         assert np.array_equal(m0, m1)
+
+
+def test_autotune_and_launch_caps_keep_results():
+    # The caps change launch widths only: the solution is bit-identical for
+    # any caps, including the autotuner's choice and one workgroup each.
+    geoms, _ = G.star_batch(6, seed=6, nmin=1024, nmax=8192)
+    f = G.batch_data(geoms, seed=6)
+    s = _batch(geoms)
+    m0, _ = s.solve(f)
+    caps, ms = s.autotune(TOL, reps=1)
+    print("autotuned caps", caps.tolist(), f"{ms:.3f} ms", flush=True)
+    assert set(caps.tolist()) <= {64, 256, 1024, 4096} and ms > 0
+    assert np.array_equal(s.launch_caps(), caps)
+    m1, _ = s.solve(f)
+    assert np.array_equal(m0, m1)
+    s.set_launch_caps([1] * iesolver.NUM_LAUNCH_CAPS)
+    s.factor(TOL)
+    m2, _ = s.solve(f)
+    assert np.array_equal(m0, m2)

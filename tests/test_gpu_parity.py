@@ -147,3 +147,29 @@ def test_fp32_factor_residual():
     err = _rel(mu, mo)
     print("fp32 factor rel err", err, flush=True)
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("precision", [iesolver.FP64, iesolver.FP32])
+def test_large_root_blocked_inverse(precision):
+    # A star with k=9, amplitude 0.25 has a root skeleton of 283 DOFs: the
+    # root block is inverted by the blocked Gauss-Jordan kernel (more than
+    # kRootRegMax = 192 rows).
+    g = G.star(4096, amp=0.25, k=9)
+    ys, q = G.exterior_sources(3)
+    f = G.harmonic_data(g["points"], ys, q)
+    K = oracle.dense_matrix(g)
+    md = np.linalg.solve(K, f)
+    if precision == iesolver.FP64:
+        s = _gpu(g)
+        bi = s.box_info()
+        root_n = int(bi["n_active"][bi["level"] == 0][0])
+        assert 192 < root_n <= 512, root_n
+        mu, _ = s.solve(f)
+        err = _rel(mu, md)
+        print(f"root n={root_n} rel err vs dense {err:.3e}", flush=True)
+        assert err <= 10 * TOL
+    else:
+        s = _gpu(g, tol=1e-6, precision=iesolver.FP32)
+        x, _, it, rel = s.solve_refined(f, rtol=1e-12, max_iter=8)
+        print(f"fp32 root: {it} corrections, rel residual {rel:.2e}", flush=True)
+        assert rel <= 1e-12 and _rel(x, md) <= 10 * TOL

@@ -189,3 +189,14 @@ def test_batch_arguments():
     assert L.iegpu_set_geometry_batch(s.h, 0, 0, off.ctypes.data_as(iesolver._ip), *ptr) != 0
     assert L.iegpu_set_geometry_batch(s.h, 0, 2, bad.ctypes.data_as(iesolver._ip), *ptr) != 0
     assert L.iegpu_set_geometry_batch(s.h, 0, 1, off.ctypes.data_as(iesolver._ip), *ptr) == 0
+
+
+def test_launch_caps_arguments():
+    s = iesolver.RSSolver(device=-1)
+    assert list(s.launch_caps()) == [1024] * iesolver.NUM_LAUNCH_CAPS
+    s.set_launch_caps([64, 256, 1024, 4096, 1, 2])
+    assert list(s.launch_caps()) == [64, 256, 1024, 4096, 1, 2]
+    with pytest.raises(iesolver.IeGpuError):
+        s.set_launch_caps([0] * iesolver.NUM_LAUNCH_CAPS)
+    with pytest.raises(iesolver.IeGpuError):
+        s.set_launch_caps([64] * 3)
