@@ -249,6 +249,7 @@ def run_batch(args, cfg):
     s.set_geometry_batch(geoms)
     s.build_tree(cfg["leaf"])
     s.set_graphs(not args.no_graphs)
+    s.set_overlap(not args.no_overlap)
     N = s.ndof
     off = s.dof_offsets
     f_dev = torch.from_numpy(f).to(dev)
@@ -356,6 +357,7 @@ def run_batch(args, cfg):
                                    "b and x cross PCIe inside the step")
                          if ms32 else None),
         "hip_graph": not args.no_graphs,
+        "overlap_id_decouple": not args.no_overlap,
         "launch_caps": {"form": caps, "plain_fp64": caps_plain,
                         "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp",
                         "autotuned": not args.no_autotune},
@@ -386,6 +388,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 comparator (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run each level's decouple on the main stream instead of "
+                         "overlapping it with the next level's ID")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch caps instead of tuning them (untimed) per form")
     ap.add_argument("--no-fp32", action="store_true",
@@ -413,6 +418,7 @@ def main():
     s.set_geometry(g, iesolver.PDE_LAPLACE)
     s.build_tree(cfg["leaf"])
     s.set_graphs(not args.no_graphs)  # both forms: factorization replayed from a HIP graph
+    s.set_overlap(not args.no_overlap)
     f_dev = torch.from_numpy(f).to(dev)
     mu_dev = torch.empty_like(f_dev)
     torch.cuda.synchronize()
@@ -499,6 +505,7 @@ def main():
         "rel_err_vs_plain_fp64": (float(np.linalg.norm(mu - mu_plain) / np.linalg.norm(mu_plain))
                                   if mu_plain is not None else None),
         "hip_graph": not args.no_graphs,
+        "overlap_id_decouple": not args.no_overlap,
         "launch_caps": {"form": caps, "plain_fp64": caps_plain,
                         "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp",
                         "autotuned": not args.no_autotune},

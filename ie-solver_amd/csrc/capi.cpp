@@ -149,6 +149,11 @@ struct Profiler {
 struct iegpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Decouple chain of the factorization (iegpu_set_overlap): level l's ID
+  // stages on `stream` overlap level l+1's decouple on `stream2`.
+  hipStream_t stream2 = nullptr;
+  bool overlap = true;
+  std::vector<hipEvent_t> evs;  // fork, one per level, join
   std::string err;
   Profiler prof;
 
@@ -201,6 +206,7 @@ struct iegpu_ctx {
   hipGraphExec_t fgraph = nullptr;
   DevState fgraph_state{};
   LaunchCaps caps, fgraph_caps;
+  bool fgraph_overlap = true;
   double fgraph_tol = 0;
   int fgraph_prec = -1;
   // rank model
@@ -404,26 +410,55 @@ void alloc_state(iegpu_ctx* c) {
   const size_t nr = c->tree.n_roots;
   c->s_root_piv.alloc(nr * (c->ncap + 64));
   c->s_root_mat.alloc(nr * 2 * c->ncap * c->ncap * 8);  // matrix + inverse per root
-  c->s_id_pool.alloc((size_t)c->id_cap * c->elem());
+  c->s_id_pool.alloc((size_t)2 * c->id_cap * c->elem());  // odd and even levels
   c->s_row_pool.alloc((size_t)c->row_cap);
   c->s_fac_pool.alloc((size_t)c->fac_cap * c->elem());
 }
 
+// The ID of level l needs only the skeletons of level l+1, not its Schur
+// complements, so with overlap on the decouple stage of every level runs
+// on a second stream, after its level's ID and the decouple of the level
+// below: level l's ID proceeds while level l+1 is decoupled.  The root
+// waits for both chains.  Per-stage profiling keeps one stream.
 template <typename R>
 void enqueue_factor(iegpu_ctx* c, double tol) {
   DevGeom g = c->geom();
   DevTree t = c->dtree();
   DevState s = c->dstate();
   hipStream_t st = c->stream;
-  launch_reset(s, st);
   const HostTree& T = c->tree;
+  const bool split = c->overlap && !c->prof.on && c->stream2 != nullptr;
+  hipStream_t st2 = split ? c->stream2 : st;
+  if (split && c->evs.size() < (size_t)T.n_levels + 2) {
+    for (size_t i = c->evs.size(); i < (size_t)T.n_levels + 2; i++) {
+      hipEvent_t e;
+      HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      c->evs.push_back(e);
+    }
+  }
+  launch_reset(s, st);
+  if (split) {
+    HIP_TRY(hipEventRecord(c->evs[0], st));
+    HIP_TRY(hipStreamWaitEvent(st2, c->evs[0], 0));
+  }
   for (int lv = T.n_levels - 1; lv >= 1; lv--) {
     const int start = T.level_start[lv];
     const int count = T.level_start[lv + 1] - start;
-    for (int stage = 0; stage < kNumFactorStages; stage++)
+    for (int stage = 0; stage < kStageDecouple; stage++)
       c->prof.run(stage, lv, st, [&] {
         launch_factor_stage<R>(stage, g, t, s, lv, start, count, tol, c->caps, st);
       });
+    if (split) {
+      HIP_TRY(hipEventRecord(c->evs[1 + lv], st));
+      HIP_TRY(hipStreamWaitEvent(st2, c->evs[1 + lv], 0));
+    }
+    c->prof.run(kStageDecouple, lv, st2, [&] {
+      launch_factor_stage<R>(kStageDecouple, g, t, s, lv, start, count, tol, c->caps, st2);
+    });
+  }
+  if (split) {
+    HIP_TRY(hipEventRecord(c->evs[T.n_levels + 1], st2));
+    HIP_TRY(hipStreamWaitEvent(st, c->evs[T.n_levels + 1], 0));
   }
   c->prof.run(kProfRoot, 0, st,
               [&] { launch_root<R>(g, t, s, T.n_roots, c->hole_dim == 0, st); });
@@ -519,6 +554,10 @@ int iegpu_create(iegpu_ctx** out, int device) {
   if (hipSetDevice(device) != hipSuccess) return IEGPU_ERR_HIP;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return IEGPU_ERR_HIP;
+  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    return IEGPU_ERR_HIP;
+  }
   *out = c.release();
   return IEGPU_OK;
 }
@@ -533,11 +572,14 @@ int iegpu_destroy(iegpu_ctx* c) {
   if (c->stream) {
     (void)hipStreamSynchronize(c->stream);
   }
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   c->prof.release();
   if (c->fgraph) (void)hipGraphExecDestroy(c->fgraph);
-  hipStream_t st = c->stream;
+  for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+  hipStream_t st = c->stream, st2 = c->stream2;
   delete c;
   if (st) (void)hipStreamDestroy(st);
+  if (st2) (void)hipStreamDestroy(st2);
   return IEGPU_OK;
 }
 
@@ -717,6 +759,12 @@ int iegpu_set_id_method(iegpu_ctx* c, int method, int sketch_rows) {
     return fail(c, IEGPU_ERR_INVALID, "sketch_rows must be 0, 16 or 32");
   c->rid_mode = 1;
   c->rid_width = sketch_rows;
+  return IEGPU_OK;
+}
+
+int iegpu_set_overlap(iegpu_ctx* c, int on) {
+  if (!c) return IEGPU_ERR_INVALID;
+  c->overlap = on != 0;
   return IEGPU_OK;
 }
 
@@ -953,6 +1001,7 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
         const DevState cur = c->dstate();
         if (!c->fgraph || std::memcmp(&cur, &c->fgraph_state, sizeof(DevState)) != 0 ||
             std::memcmp(&c->caps, &c->fgraph_caps, sizeof(LaunchCaps)) != 0 ||
+            c->overlap != c->fgraph_overlap ||
             c->fgraph_tol != id_tol || c->fgraph_prec != precision) {
           if (c->fgraph) HIP_TRY(hipGraphExecDestroy(c->fgraph));
           c->fgraph = nullptr;
@@ -966,6 +1015,7 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
           HIP_TRY(ie);
           c->fgraph_state = cur;
           c->fgraph_caps = c->caps;
+          c->fgraph_overlap = c->overlap;
           c->fgraph_tol = id_tol;
           c->fgraph_prec = precision;
         }

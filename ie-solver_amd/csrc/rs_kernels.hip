@@ -216,6 +216,10 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
     int64_t scratch = kind == kQrcpLds ? ld * n : 0;
     if (kind == kQrcpStack || kind == kQrcpStack8)
       scratch = (int64_t)id_chunks(m, kind == kQrcpStack ? kRegMaxRows : kRegMaxRows8) * n * n;
+    // k_decouple keeps the operands of a box with more than kSmallBox
+    // active DOFs (n^2 + r^2 + 2 r k <= 2 n^2 entries) in the same slot
+    // when they do not fit its LDS buffer.
+    if (n > kSmallBox) scratch = max(scratch, 2 * (int64_t)n * n);
     s.id_sz[b] = scratch;
     s.row_sz[b] = kind == kQrcpLds ? (m - proxy_rows(g, level)) / 2 : 0;
     s.fac_sz[b] = (n > 0) ? fac_slot_size(n) : 0;
@@ -326,7 +330,9 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
     const int kind = s.qkind[b];
-    s.id_off[b] = off[0];
+    // Scratch of odd and even levels in separate halves: a level's ID runs
+    // while the level below it is still being decoupled.
+    s.id_off[b] = (level & 1) * s.id_cap + off[0];
     s.row_off[b] = off[1];
     s.fac_off[b] = fbase + off[2];
     off[0] += s.id_sz[b];
@@ -2414,16 +2420,24 @@ __global__ __launch_bounds__(kRootBlkMax) void k_root_inv_blk(DevState s, int fa
       Rl[idx] = skip ? R(0) : A[pan[k] + (int64_t)n * c];
     }
     __syncthreads();
-    for (int c0 = 0; c0 < n; c0 += 8) {
-      R x[8];
+    // Own row, 16 columns per chunk; the next chunk's loads are issued
+    // before the current chunk's arithmetic so one load latency is exposed
+    // per panel rather than per chunk.
+    constexpr int CH = 16;
+    auto load_chunk = [&](int c0, R* x) {
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < CH; q++) {
         const int c = c0 + q;
         const bool other = c < n && (c < j0 || c >= j0 + bw);
         x[q] = (live && other && !inpan) ? A[i + (int64_t)n * c] : R(0);
       }
+    };
+    R x[CH], xn[CH];
+    load_chunk(0, x);
+    for (int c0 = 0; c0 < n; c0 += CH) {
+      if (c0 + CH < n) load_chunk(c0 + CH, xn);
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < CH; q++) {
         const int c = c0 + q;
         if (c < n && (c < j0 || c >= j0 + bw)) {
           R acc = x[q];
@@ -2432,6 +2446,8 @@ __global__ __launch_bounds__(kRootBlkMax) void k_root_inv_blk(DevState s, int fa
           if (live) A[i + (int64_t)n * c] = acc;
         }
       }
+#pragma unroll
+      for (int q = 0; q < CH; q++) x[q] = xn[q];
     }
 #pragma unroll
     for (int k = 0; k < B; k++)

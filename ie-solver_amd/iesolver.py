@@ -43,7 +43,7 @@ EXPORTS = [
     "iegpu_set_graphs", "iegpu_record_features", "iegpu_box_features", "iegpu_rank_model_init",
     "iegpu_rank_model_train", "iegpu_rank_model_grad", "iegpu_set_rank_model",
     "iegpu_set_geometry_batch", "iegpu_set_launch_caps", "iegpu_get_launch_caps",
-    "iegpu_autotune",
+    "iegpu_autotune", "iegpu_set_overlap",
 ]
 NUM_LAUNCH_CAPS = 6
 RANK_FEATURES = 9
@@ -105,6 +105,7 @@ def load():
                                          ctypes.c_float, ctypes.c_uint, fp]
     L.iegpu_rank_model_grad.argtypes = [vp, fp, fp, fp, ctypes.c_int, fp, fp]
     L.iegpu_set_rank_model.argtypes = [vp, fp, ctypes.c_int]
+    L.iegpu_set_overlap.argtypes = [vp, ctypes.c_int]
     L.iegpu_set_launch_caps.argtypes = [vp, _ip, ctypes.c_int]
     L.iegpu_get_launch_caps.argtypes = [vp, _ip, ctypes.c_int]
     L.iegpu_autotune.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp]
@@ -204,6 +205,11 @@ class RSSolver:
         """ID_QRCP (the reference's full pivoted QR) or ID_RANDOMIZED (Gaussian
         sketch of sketch_rows rows, full-QRCP fallback)."""
         self._check(self.L.iegpu_set_id_method(self.h, method, sketch_rows))
+
+    def set_overlap(self, on=True):
+        """Overlap each level's ID with the decouple of the level below
+        (second stream); results are identical either way."""
+        self._check(self.L.iegpu_set_overlap(self.h, 1 if on else 0))
 
     def set_launch_caps(self, caps):
         c = np.ascontiguousarray(caps, dtype=np.int32)

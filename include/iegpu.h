@@ -6,8 +6,10 @@
  * with no FFI; each entry point below names the reference interface it
  * replaces (paths relative to the reference root).  Plain pointers and
  * sizes only; host arrays are borrowed for the duration of the call;
- * device buffers are owned by the context.  One HIP stream per context; a
- * context is not thread-safe; separate contexts are independent.
+ * device buffers are owned by the context.  Each context owns two HIP
+ * streams (the second carries the factorization's decouple chain, see
+ * iegpu_set_overlap); a context is not thread-safe; separate contexts are
+ * independent.
  *
  * Every function returns an iegpu_status (0 = success).  On failure
  * iegpu_last_error() describes the cause.
@@ -172,6 +174,15 @@ int iegpu_profile(iegpu_ctx* ctx, int stage, int level, double* ms,
  * change, or a workspace growth, recaptures).  Not used while per-stage
  * profiling is on or with holes.  No reference counterpart. */
 int iegpu_set_graphs(iegpu_ctx* ctx, int on);
+
+/* on != 0 (default): iegpu_factor runs each level's decouple stage on a
+ * second stream of the context, so that the ID stages of level l overlap
+ * the decouple of level l+1 (the ID of a level needs only the skeletons
+ * below it, not their Schur complements).  Results are identical either
+ * way.  Per-stage profiling always uses one stream.  No reference
+ * counterpart (the reference runs each level's boxes in one OpenMP loop,
+ * skel_factorization.cpp:150-168). */
+int iegpu_set_overlap(iegpu_ctx* ctx, int on);
 
 /* Launch widths of the factorization kernels that walk a device-side box
  * list (the host knows a level's box count, not the list's length):

@@ -39,3 +39,40 @@ def test_graph_replays_match_eager(form):
     s.set_graphs(False)
     s.factor(1e-10)
     assert np.array_equal(s.solve(f)[0], mu0)
+
+
+@pytest.mark.parametrize("case", ["plain", "randomized", "stokes_holes", "fp32"])
+def test_overlap_matches_single_stream(case):
+    # Overlapping each level's ID with the decouple of the level below (a
+    # second stream) changes only the order in which independent kernels
+    # run: the solution is bit-identical to the single-stream run, eager and
+    # replayed from a graph.
+    pde, prec, tol = iesolver.PDE_LAPLACE, iesolver.FP64, 1e-10
+    if case == "stokes_holes":
+        g = G.circle_with_holes(8192, 8, seed=2)
+        f = G.tangent_data(g["normals"])
+        f[2 * g["counts"][0]:] = 0
+        pde = iesolver.PDE_STOKES
+    else:
+        g = G.wavy(32768, seed=4)
+        f = G.harmonic_data(g["points"], *G.exterior_sources(4))
+    if case == "fp32":
+        prec, tol = iesolver.FP32, 1e-6
+    s = iesolver.RSSolver(0)
+    s.set_geometry(g, pde)
+    s.build_tree(64)
+    if "holes" in g:
+        s.set_holes(g["holes"]["centers"], g["holes"]["radii"])
+    if case == "randomized":
+        s.set_id_method(iesolver.ID_RANDOMIZED, 32)
+    s.set_overlap(False)
+    s.factor(tol, prec)
+    m0, a0 = s.solve(f)
+    s.set_overlap(True)
+    for graphs in (False, True, True, True):
+        s.set_graphs(graphs)
+        s.factor(tol, prec)
+        m1, a1 = s.solve(f)
+        assert np.array_equal(m0, m1), (case, graphs)
+        if a0 is not None:
+            assert np.array_equal(a0, a1)
