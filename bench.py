@@ -359,7 +359,7 @@ def run_batch(args, cfg):
         "hip_graph": not args.no_graphs,
         "overlap_id_decouple": not args.no_overlap,
         "launch_caps": {"form": caps, "plain_fp64": caps_plain,
-                        "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp",
+                        "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp, decouple_split",
                         "autotuned": not args.no_autotune},
         "id_stats": id_stats,
         "rank_model": model_info,
@@ -507,7 +507,7 @@ def main():
         "hip_graph": not args.no_graphs,
         "overlap_id_decouple": not args.no_overlap,
         "launch_caps": {"form": caps, "plain_fp64": caps_plain,
-                        "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp",
+                        "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp, decouple_split",
                         "autotuned": not args.no_autotune},
         "id_stats": id_stats,
         "rank_model": model_info,

@@ -1413,13 +1413,15 @@ int iegpu_set_launch_caps(iegpu_ctx* c, const int* caps, int n) {
   L.rowcomp8 = caps[3];
   L.qrcp_id8 = caps[4];
   L.qrcp = caps[5];
+  L.dsplit = caps[6];
   return IEGPU_OK;
 }
 
 int iegpu_get_launch_caps(const iegpu_ctx* c, int* caps, int n) {
   if (!c || !caps || n != kNumCaps) return IEGPU_ERR_INVALID;
   const LaunchCaps& L = c->caps;
-  const int v[kNumCaps] = {L.assemble, L.qrcp_id, L.rowcomp4, L.rowcomp8, L.qrcp_id8, L.qrcp};
+  const int v[kNumCaps] = {L.assemble, L.qrcp_id, L.rowcomp4, L.rowcomp8, L.qrcp_id8, L.qrcp,
+                           L.dsplit};
   for (int i = 0; i < n; i++) caps[i] = v[i];
   return IEGPU_OK;
 }
@@ -1432,7 +1434,10 @@ int iegpu_autotune(iegpu_ctx* c, double id_tol, int precision, int reps, int* ca
   // `reps` factorizations after one untimed run (which also re-captures
   // the graph when graphs are on).  A candidate replaces the current value
   // only if it is at least 1 % faster, so timing noise does not move it.
-  static const int kCand[] = {64, 256, 1024, 4096};
+  static const int kCand[kNumCaps][4] = {{64, 256, 1024, 4096}, {64, 256, 1024, 4096},
+                                         {64, 256, 1024, 4096}, {64, 256, 1024, 4096},
+                                         {64, 256, 1024, 4096}, {64, 256, 1024, 4096},
+                                         {1, 2048, 8192, 1 << 20}};
   auto measure = [&](double* best) -> int {
     *best = 1e300;
     for (int r = 0; r <= reps; r++) {
@@ -1449,7 +1454,7 @@ int iegpu_autotune(iegpu_ctx* c, double id_tol, int precision, int reps, int* ca
   if (st != IEGPU_OK) return st;
   for (int k = 0; k < kNumCaps; k++) {
     int pick = cur[k];
-    for (int v : kCand) {
+    for (int v : kCand[k]) {
       if (v == cur[k]) continue;
       int trial[kNumCaps];
       std::memcpy(trial, cur, sizeof(trial));

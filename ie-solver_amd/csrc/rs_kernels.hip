@@ -1559,16 +1559,23 @@ __device__ int gj_invert(R* a, R* b, int n, int* prow) {
 // Gauss-Jordan inverse of X (r x r, r <= 16 U, in LDS or global memory,
 // ld r) with the matrix in the registers of 256 threads on a 16 x 16 grid,
 // thread (tr, tc) holding rows tr + 16 u and columns tc + 16 v (U x U
-// entries).  Per step the pivot column (kept in LDS by its owners) gives
-// the pivot row by one wave maximum over packed keys (same rule as
-// gj_invert), the pivot row's owners publish it, and every thread updates
-// its entries with one FMA each.  M (as gj_invert) is written back to X.
-// colbuf holds 2 x 16 U and rowbuf 16 U entries.  Returns 0, or j + 1 for
-// the first zero pivot.
+// entries).  Per step:
+//   * every thread reads its rows of the pivot column (kept in LDS by its
+//     owners) and the pivot row is the largest packed key (same rule as
+//     gj_invert) over the 16 lanes of its DPP row, which together hold all
+//     16 U rows, so every lane finds it without a block barrier;
+//   * the pivot and the pivot row's entries that a thread needs (columns
+//     tc + 16 v) are held by lane tr = p % 16 of the same 16-lane DPP row,
+//     which broadcasts them (row_newbcast);
+//   * each entry takes one FMA; column j restarts from zero and the pivot
+//     row becomes the scaled pivot row;
+//   * the owners of column j + 1 publish it, then one block barrier.
+// M (as gj_invert) is written back to X.  colbuf holds 2 x 16 U entries.
+// Returns 0, or j + 1 for the first zero pivot.
 template <typename R, int U = 4>
-__device__ int gj_invert_regs(R* X, int r, int* prow, R* colbuf, R* rowbuf) {
-  constexpr int RM = 16 * U, KH = (RM + 63) / 64;
-  const int tid = threadIdx.x, lane = tid & 63;
+__device__ __forceinline__ int gj_invert_regs(R* X, int r, int* prow, R* colbuf) {
+  constexpr int RM = 16 * U;
+  const int tid = threadIdx.x;
   const int tr = tid & 15, tc = tid >> 4;
   R a[U][U];
 #pragma unroll
@@ -1584,78 +1591,75 @@ __device__ int gj_invert_regs(R* X, int r, int* prow, R* colbuf, R* rowbuf) {
       if (tr + 16 * u < r) colbuf[tr + 16 * u] = a[u][0];
   }
   __syncthreads();
-  bool used[KH];  // row lane + 64 h already pivoted (every wave keeps a copy)
-#pragma unroll
-  for (int h = 0; h < KH; h++) used[h] = false;
+  unsigned used = 0;  // bit u: row tr + 16 u already pivoted
   int info = 0;
   for (int j = 0; j < r; j++) {
     const R* col = colbuf + RM * (j & 1);
     R* coln = colbuf + RM * ((j + 1) & 1);
+    R cv[U];
     double key = -1.0;
-    int first = -1;  // first candidate row (a NaN column has no key)
+    unsigned cand = 0;
 #pragma unroll
-    for (int h = KH - 1; h >= 0; h--) {
-      const int row = lane + 64 * h;
-      const bool cand = row < r && !used[h];
-      const R mag = cand ? fabs(col[row]) : R(0);
-      key = fmax(key, (cand && mag == mag) ? pivot_key(mag, row) : -1.0);
-      const uint64_t cb = __ballot(cand);
-      if (cb) first = 64 * h + __ffsll((unsigned long long)cb) - 1;
+    for (int u = 0; u < U; u++) {
+      const int row = tr + 16 * u;
+      cv[u] = row < r ? col[row] : R(0);
+      const bool c = row < r && !((used >> u) & 1);
+      cand |= c ? (1u << u) : 0u;
+      const R mag = fabs(cv[u]);
+      key = fmax(key, (c && mag == mag) ? pivot_key(mag, row) : -1.0);
     }
-    const double best = wave_max(key);
-    const int p = best >= 0 ? pivot_key_pos(best) : first;
+    key = fmax(key, dpp_f64<dpp::kQuadSwap1, 0xF>(key));
+    key = fmax(key, dpp_f64<dpp::kQuadSwap2, 0xF>(key));
+    key = fmax(key, dpp_f64<dpp::kRowHalfMirror, 0xF>(key));
+    key = fmax(key, dpp_f64<dpp::kRowMirror, 0xF>(key));
+    int p;
+    if (key >= 0) {
+      p = pivot_key_pos(key);
+    } else {  // NaN column: the first row not yet pivoted
+      int f = cand ? tr + 16 * (__ffs(cand) - 1) : 0x7fffffff;
 #pragma unroll
-    for (int h = 0; h < KH; h++)
-      if (lane + 64 * h == p) used[h] = true;
+      for (int o = 1; o < 16; o <<= 1) f = min(f, __shfl_xor(f, o, 16));
+      p = f;
+    }
+    p = __builtin_amdgcn_readfirstlane(p);  // the same in every lane
+    const int up = p >> 4, pr = p & 15;
+    if (tr == pr) used |= 1u << up;
     if (tid == 0) prow[j] = p;
-    const R piv = col[p];
+    // Lane pr of this 16-lane row holds row p's entries of the row's
+    // columns, and cv[up] = the pivot.
+    R pc = cv[0];
+#pragma unroll
+    for (int u = 1; u < U; u++) pc = (u == up) ? cv[u] : pc;
+    const R piv = row_lane(pc, pr);
     if (piv == R(0) && info == 0) info = j + 1;
     const R inv = R(1) / piv;
-    const int up = p >> 4;
-    if (tr == (p & 15)) {
+    R rv[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (u != up) continue;
+    for (int v = 0; v < U; v++) {
+      const int c = tc + 16 * v;
+      R pv = a[0][v];
 #pragma unroll
-        for (int v = 0; v < U; v++) {
-          const int c = tc + 16 * v;
-          if (c < r) rowbuf[c] = (c == j) ? R(1) : a[u][v];
-        }
-      }
-    }
-    __syncthreads();
-    R cv[U], rv[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) cv[u] = (tr + 16 * u < r) ? col[tr + 16 * u] : R(0);
-#pragma unroll
-    for (int v = 0; v < U; v++) rv[v] = (tc + 16 * v < r) ? rowbuf[tc + 16 * v] * inv : R(0);
-    if (tc == (j & 15)) {  // column j restarts from zero
-#pragma unroll
-      for (int v = 0; v < U; v++) {
-        if (v != (j >> 4)) continue;
-#pragma unroll
-        for (int u = 0; u < U; u++) a[u][v] = R(0);
-      }
+      for (int u = 1; u < U; u++) pv = (u == up) ? a[u][v] : pv;
+      pv = row_lane(pv, pr);
+      rv[v] = c < r ? ((c == j) ? R(1) : pv) * inv : R(0);
     }
 #pragma unroll
     for (int u = 0; u < U; u++)
 #pragma unroll
-      for (int v = 0; v < U; v++) a[u][v] -= cv[u] * rv[v];
-    if (tr == (p & 15)) {  // the pivot row becomes the scaled pivot row
+      for (int v = 0; v < U; v++) {
+        const bool colj = tc + 16 * v == j, rowp = tr + 16 * u == p;
+        const R base = colj ? R(0) : a[u][v];
+        const R val = base - cv[u] * rv[v];
+        a[u][v] = rowp ? rv[v] : val;
+      }
+    if (j + 1 < r && tc == ((j + 1) & 15)) {
+      const int vn = (j + 1) >> 4;
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        if (u != up) continue;
+        R x = a[u][0];
 #pragma unroll
-        for (int v = 0; v < U; v++) a[u][v] = rv[v];
-      }
-    }
-    if (j + 1 < r && tc == ((j + 1) & 15)) {
-#pragma unroll
-      for (int v = 0; v < U; v++) {
-        if (v != ((j + 1) >> 4)) continue;
-#pragma unroll
-        for (int u = 0; u < U; u++)
-          if (tr + 16 * u < r) coln[tr + 16 * u] = a[u][v];
+        for (int v = 1; v < U; v++) x = (v == vn) ? a[u][v] : x;
+        if (tr + 16 * u < r) coln[tr + 16 * u] = x;
       }
     }
     __syncthreads();
@@ -1841,21 +1845,25 @@ __device__ void decouple_tail(const R* M, const int* prow, const R* Krs, const R
   }
 }
 
-// Compressed boxes with |active| <= 64, every operand in LDS.  Only the
-// skeleton rows of K_BN are stored (K(s,s), K(s,r)); the redundant rows
-// are evaluated where K_rs and X_rr need them, which keeps the buffer at
-// n^2 + 2 k r <= 6144 entries and three boxes per CU.
-template <typename R>
+// Compressed boxes with NMIN < |active| <= NMAX (<= 64), every operand in
+// LDS.  Only the skeleton rows of K_BN are stored (K(s,s), K(s,r)); the
+// redundant rows are evaluated where K_rs and X_rr need them, which keeps
+// the buffer at n^2 + 2 k r <= 1.5 NMAX^2 entries: three boxes per CU for
+// NMAX = 64, five for NMAX = 44.
+template <typename R, int NMIN, int NMAX>
 __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
                                                         DevState s, int start) {
   constexpr int NT = 256;
-  constexpr int kBuf = 6144;  // >= n^2 + 2 k r for n <= 64
+  // >= n^2 + 2 k r (k r <= n^2 / 4); the two size classes let the boxes
+  // with n <= kSmallBox1 run at a higher occupancy
+  constexpr int kBuf = NMAX * NMAX + NMAX * NMAX / 2;
+  static_assert(NMAX * sizeof(DofGeo) >= 2 * 64 * sizeof(R), "pivot columns fit in geo");
   __shared__ R buf[kBuf];
-  __shared__ DofGeo geo[kSmallBox];
-  __shared__ signed char pch[kSmallBox];
-  __shared__ short ploc[kSmallBox];
-  __shared__ short sperm[kSmallBox];
-  __shared__ int prow[kSmallBox];
+  __shared__ DofGeo geo[NMAX];
+  __shared__ signed char pch[NMAX];
+  __shared__ short ploc[NMAX];
+  __shared__ short sperm[NMAX];
+  __shared__ int prow[NMAX];
   __shared__ const R* chS[4];
   __shared__ int chN[4];
 
@@ -1863,7 +1871,7 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
   const int tid = threadIdx.x;
   if (!pools_ok(s)) return;
   const int n = s.act_cnt[b], k = s.rank[b];
-  if (n == 0 || n > kSmallBox || k == 0) return;  // k_decouple's box
+  if (n <= NMIN || n > NMAX || k == 0) return;  // the other class's or k_decouple's box
   load_child_blocks<R>(t, s, b, n, chS, chN, pch, ploc, NT);
   const int* act = s.act_list + (int64_t)b * s.ncap;
   for (int i = tid; i < n; i += NT) {
@@ -1911,9 +1919,9 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
              dot2(Krs + i, r, T + k * j, 1, k);
   }
   __syncthreads();
-  // The DOF geometry is dead: its space holds the pivot column and row.
+  // The DOF geometry is dead: its space holds the pivot columns.
   R* colbuf = reinterpret_cast<R*>(geo);
-  const int info = gj_invert_regs<R>(X, r, prow, colbuf, colbuf + 128);
+  const int info = gj_invert_regs<R>(X, r, prow, colbuf);
   if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
   decouple_tail<R, NT>(X, prow, Krs, Ksr, buf, r, k, F, fl, upd, sperm);
   if (tid == 0) s.upd_kind[b] = 1;
@@ -1983,9 +1991,9 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   // X_rr^{-1}: in registers for r <= 128 (the LDS buffer is free: the
   // blocks of these boxes live in the scratch slot), else step by step.
   int info;
-  if (NT == 256 && r <= 64 && P != sbuf) info = gj_invert_regs<R, 4>(X, r, prow, sbuf, sbuf + 128);
+  if (NT == 256 && r <= 64 && P != sbuf) info = gj_invert_regs<R, 4>(X, r, prow, sbuf);
   else if (NT == 256 && r <= 128 && P != sbuf)
-    info = gj_invert_regs<R, 8>(X, r, prow, sbuf, sbuf + 256);
+    info = gj_invert_regs<R, 8>(X, r, prow, sbuf);
   else info = gj_invert_into<R, NT>(X, P, r, prow);
   if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
   decouple_tail<R, NT>(X, prow, Krs, Ksr, P, r, k, F, fl, upd, sperm);
@@ -2550,7 +2558,12 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       k_qrcp<R, 512><<<min(count, caps.qrcp), 512, 0, st>>>(s, start, level, tol);
       break;
     case kStageDecouple:
-      k_decouple_small<R><<<count, 256, 0, st>>>(g, t, s, start);
+      if (count >= caps.dsplit) {
+        k_decouple_small<R, 0, kSmallBox1><<<count, 256, 0, st>>>(g, t, s, start);
+        k_decouple_small<R, kSmallBox1, kSmallBox><<<count, 256, 0, st>>>(g, t, s, start);
+      } else {
+        k_decouple_small<R, 0, kSmallBox><<<count, 256, 0, st>>>(g, t, s, start);
+      }
       k_decouple<R, 256><<<count, 256, 0, st>>>(g, t, s, start);
       break;
     default:

@@ -29,11 +29,10 @@ enum FactorStage : int {
   kNumFactorStages = 7
 };
 
-// Grid caps of the kernels that walk a device-side box list with a grid
+// Launch shapes chosen per workload by iegpu_autotune.  The caps are the
+// grids of the kernels that walk a device-side box list with a grid
 // stride (the host knows only the level's box count, not the list's
-// length).  Workgroups past the list's end exit at once, so a cap trades
-// launch width against nothing but idle workgroups; iegpu_autotune picks
-// them per workload.
+// length); workgroups past the list's end exit at once.
 struct LaunchCaps {
   int assemble = 1024;   // k_assemble (general-QRCP boxes' ID matrices)
   int qrcp_id = 1024;    // k_qrcp_id register shapes, randomized-ID fallback
@@ -41,8 +40,13 @@ struct LaunchCaps {
   int rowcomp8 = 1024;   // k_rowcomp<8> (box, chunk) jobs
   int qrcp_id8 = 1024;   // k_qrcp_id<8, 4, 16> (n <= 128 after row compression)
   int qrcp = 1024;       // k_qrcp (general LDS/global QRCP)
+  // Levels with at least this many boxes decouple their n <= 64 boxes in
+  // two LDS size classes (n <= 44 at five boxes per CU, the rest at
+  // three), in two launches; smaller levels use one launch of the larger
+  // class, whose single box latency is all they pay.
+  int dsplit = 8192;
 };
-constexpr int kNumCaps = 6;
+constexpr int kNumCaps = 7;
 
 template <typename R>
 void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,

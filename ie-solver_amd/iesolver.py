@@ -45,7 +45,7 @@ EXPORTS = [
     "iegpu_set_geometry_batch", "iegpu_set_launch_caps", "iegpu_get_launch_caps",
     "iegpu_autotune", "iegpu_set_overlap",
 ]
-NUM_LAUNCH_CAPS = 6
+NUM_LAUNCH_CAPS = 7
 RANK_FEATURES = 9
 RANK_MODEL_LEN = 1429
 RANK_PARAMS = 1409

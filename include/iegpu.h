@@ -184,20 +184,23 @@ int iegpu_set_graphs(iegpu_ctx* ctx, int on);
  * skel_factorization.cpp:150-168). */
 int iegpu_set_overlap(iegpu_ctx* ctx, int on);
 
-/* Launch widths of the factorization kernels that walk a device-side box
- * list (the host knows a level's box count, not the list's length):
- * caps[IEGPU_NUM_LAUNCH_CAPS] = workgroup caps of the general-QRCP
- * assembly, the register-QRCP fallback of the randomized ID, the 4- and
- * 8-wave row compressions, the 8-wave register QRCP and the general QRCP.
- * Each in [1, 2^20]; default 1024.  The caps change launch widths only:
- * every box's arithmetic is the same for any caps.  No reference
- * counterpart. */
-#define IEGPU_NUM_LAUNCH_CAPS 6
+/* Launch shapes of the factorization, caps[IEGPU_NUM_LAUNCH_CAPS]:
+ * [0..5] workgroup caps of the kernels that walk a device-side box list
+ * (the host knows a level's box count, not the list's length): the
+ * general-QRCP assembly, the register-QRCP fallback of the randomized ID,
+ * the 4- and 8-wave row compressions, the 8-wave register QRCP and the
+ * general QRCP (default 1024 each); [6] the box count from which a level
+ * decouples its small boxes in two LDS size classes (two launches, higher
+ * occupancy for n <= 44) instead of one (default 8192).  Each in
+ * [1, 2^20].  Launch shapes only: every box's arithmetic is the same for
+ * any values.  No reference counterpart. */
+#define IEGPU_NUM_LAUNCH_CAPS 7
 int iegpu_set_launch_caps(iegpu_ctx* ctx, const int* caps, int n);
 int iegpu_get_launch_caps(const iegpu_ctx* ctx, int* caps, int n);
 
 /* Autotuner: coordinate descent over the launch caps, each candidate
- * (64, 256, 1024, 4096) timed as the best of `reps` factorizations of the
+ * (64, 256, 1024, 4096 for the grid caps; 1, 2048, 8192, 2^20 for the
+ * decouple split) timed as the best of `reps` factorizations of the
  * current workload at (id_tol, precision) after one untimed run; a value
  * changes only for a gain of at least 1 %.  Leaves the chosen caps set and
  * the workload factored with them; writes the caps (may be NULL) and the
