@@ -184,12 +184,11 @@ def time_steps(step, s, args, profile):
 
 
 def tune(s, args, tol, precision):
-    """Autotune the launch caps for the current form (untimed); returns the
-    chosen caps."""
-    if args.no_autotune:
-        return None
-    caps, _ = s.autotune(tol, precision, reps=2)
-    return [int(c) for c in caps]
+    """With --autotune, tune the launch caps for the current form (untimed);
+    returns the caps in use."""
+    if args.autotune:
+        s.autotune(tol, precision, reps=2)
+    return [int(c) for c in s.launch_caps()]
 
 
 def set_form(s, args):
@@ -248,7 +247,7 @@ def run_batch(args, cfg):
     s = iesolver.RSSolver(0)
     s.set_geometry_batch(geoms)
     s.build_tree(cfg["leaf"])
-    s.set_graphs(not args.no_graphs)
+    s.set_graphs(args.graphs)
     s.set_overlap(not args.no_overlap)
     N = s.ndof
     off = s.dof_offsets
@@ -356,11 +355,11 @@ def run_batch(args, cfg):
                               note="FP32 factor + FP64 refinement to relative residual 1e-11; "
                                    "b and x cross PCIe inside the step")
                          if ms32 else None),
-        "hip_graph": not args.no_graphs,
+        "hip_graph": args.graphs,
         "overlap_id_decouple": not args.no_overlap,
         "launch_caps": {"form": caps, "plain_fp64": caps_plain,
                         "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp, decouple_split",
-                        "autotuned": not args.no_autotune},
+                        "autotuned": args.autotune},
         "id_stats": id_stats,
         "rank_model": model_info,
         "rel_err_interior": interior_err,
@@ -383,16 +382,18 @@ def main():
     ap.add_argument("--form", default="learned", choices=["learned", "randomized", "plain"])
     ap.add_argument("--rank-model", default=None,
                     help="load a trained model (.npy from rank_model.py) instead of training")
-    ap.add_argument("--no-graphs", action="store_true",
-                    help="launch the factorization kernel by kernel instead of from a HIP graph")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay the factorization from a captured HIP graph (the replay "
+                         "runs the two overlapped chains on shared queues, which measured "
+                         "slower than launching the two streams directly)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 comparator (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-overlap", action="store_true",
                     help="run each level's decouple on the main stream instead of "
                          "overlapping it with the next level's ID")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default launch caps instead of tuning them (untimed) per form")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the launch caps (untimed) per form before timing it")
     ap.add_argument("--no-fp32", action="store_true",
                     help="batch workload: skip the FP32 + refinement form")
     ap.add_argument("--cpu-reps", type=int, default=5)
@@ -417,7 +418,7 @@ def main():
     s = iesolver.RSSolver(0)
     s.set_geometry(g, iesolver.PDE_LAPLACE)
     s.build_tree(cfg["leaf"])
-    s.set_graphs(not args.no_graphs)  # both forms: factorization replayed from a HIP graph
+    s.set_graphs(args.graphs)
     s.set_overlap(not args.no_overlap)
     f_dev = torch.from_numpy(f).to(dev)
     mu_dev = torch.empty_like(f_dev)
@@ -504,11 +505,11 @@ def main():
         "exact": f"FP64 refinement to relative residual {ref_res:.1e} ({ref_its} corrections)",
         "rel_err_vs_plain_fp64": (float(np.linalg.norm(mu - mu_plain) / np.linalg.norm(mu_plain))
                                   if mu_plain is not None else None),
-        "hip_graph": not args.no_graphs,
+        "hip_graph": args.graphs,
         "overlap_id_decouple": not args.no_overlap,
         "launch_caps": {"form": caps, "plain_fp64": caps_plain,
                         "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp, decouple_split",
-                        "autotuned": not args.no_autotune},
+                        "autotuned": args.autotune},
         "id_stats": id_stats,
         "rank_model": model_info,
         "rel_err_interior": interior_err,

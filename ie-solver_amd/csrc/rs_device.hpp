@@ -266,35 +266,6 @@ __device__ inline int dpp_i32(int v) {
   return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
 }
 
-// Value of lane l (0..15, wave-uniform) of each 16-lane DPP row, in every
-// lane of that row (DPP row_newbcast).
-template <typename R, int L>
-__device__ inline R row_lane_c(R v) {
-  if constexpr (sizeof(R) == 8) return dpp_f64<0x150 + L, 0xF>(v);
-  else return dpp_f32<0x150 + L, 0xF>(v);
-}
-template <typename R>
-__device__ inline R row_lane(R v, int l) {
-  switch (l) {
-    case 0: return row_lane_c<R, 0>(v);
-    case 1: return row_lane_c<R, 1>(v);
-    case 2: return row_lane_c<R, 2>(v);
-    case 3: return row_lane_c<R, 3>(v);
-    case 4: return row_lane_c<R, 4>(v);
-    case 5: return row_lane_c<R, 5>(v);
-    case 6: return row_lane_c<R, 6>(v);
-    case 7: return row_lane_c<R, 7>(v);
-    case 8: return row_lane_c<R, 8>(v);
-    case 9: return row_lane_c<R, 9>(v);
-    case 10: return row_lane_c<R, 10>(v);
-    case 11: return row_lane_c<R, 11>(v);
-    case 12: return row_lane_c<R, 12>(v);
-    case 13: return row_lane_c<R, 13>(v);
-    case 14: return row_lane_c<R, 14>(v);
-    default: return row_lane_c<R, 15>(v);
-  }
-}
-
 // Index of the calling wave in its workgroup, as a wave-uniform (scalar)
 // value: the compiler cannot tell that threadIdx.x / 64 is uniform.
 __device__ inline int wave_id() {

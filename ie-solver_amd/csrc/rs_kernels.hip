@@ -1565,8 +1565,8 @@ __device__ int gj_invert(R* a, R* b, int n, int* prow) {
 //     gj_invert) over the 16 lanes of its DPP row, which together hold all
 //     16 U rows, so every lane finds it without a block barrier;
 //   * the pivot and the pivot row's entries that a thread needs (columns
-//     tc + 16 v) are held by lane tr = p % 16 of the same 16-lane DPP row,
-//     which broadcasts them (row_newbcast);
+//     tc + 16 v) are held by lane tr = p % 16 of the same 16-lane group,
+//     read with a lane permute (no LDS store, no barrier);
 //   * each entry takes one FMA; column j restarts from zero and the pivot
 //     row becomes the scaled pivot row;
 //   * the owners of column j + 1 publish it, then one block barrier.
@@ -1630,7 +1630,8 @@ __device__ __forceinline__ int gj_invert_regs(R* X, int r, int* prow, R* colbuf)
     R pc = cv[0];
 #pragma unroll
     for (int u = 1; u < U; u++) pc = (u == up) ? cv[u] : pc;
-    const R piv = row_lane(pc, pr);
+    const int src = (threadIdx.x & 48) | pr;
+    const R piv = __shfl(pc, src, 64);
     if (piv == R(0) && info == 0) info = j + 1;
     const R inv = R(1) / piv;
     R rv[U];
@@ -1640,7 +1641,7 @@ __device__ __forceinline__ int gj_invert_regs(R* X, int r, int* prow, R* colbuf)
       R pv = a[0][v];
 #pragma unroll
       for (int u = 1; u < U; u++) pv = (u == up) ? a[u][v] : pv;
-      pv = row_lane(pv, pr);
+      pv = __shfl(pv, src, 64);
       rv[v] = c < r ? ((c == j) ? R(1) : pv) * inv : R(0);
     }
 #pragma unroll
