@@ -320,6 +320,14 @@ void build_border(iegpu_ctx* c, const double* centers, const double* radii,
                   std::vector<double>* U, std::vector<double>* PsiT) {
   const int np = c->n_points, h = c->n_holes;
   const int64_t nd = c->ndof();
+  if (c->pde == kPdeLaplaceNeumann) {
+    // linear_solve.cpp:61-64,106-109: no border for Laplace Neumann; the
+    // multiply connected problem is solved like a simply connected one.
+    c->hole_dim = 0;
+    U->clear();
+    PsiT->clear();
+    return;
+  }
   if (c->pde == kPdeLaplace) {
     c->hole_dim = h;
     U->assign(nd * h, 0.0);
@@ -592,8 +600,8 @@ int set_geometry(iegpu_ctx* c, int pde, int n_problems, const int* point_off,
                  const double* points, const double* normals,
                  const double* curvatures, const double* weights) {
   const int n_points = point_off[n_problems];
-  if (pde != kPdeLaplace && pde != kPdeStokes)
-    return fail(c, IEGPU_ERR_INVALID, "pde must be LAPLACE (0) or STOKES (2)");
+  if (pde != kPdeLaplace && pde != kPdeLaplaceNeumann && pde != kPdeStokes)
+    return fail(c, IEGPU_ERR_INVALID, "pde must be LAPLACE (0), LAPLACE_NEUMANN (1) or STOKES (2)");
   if (n_points <= 0 || !points || !normals || !curvatures || !weights)
     return fail(c, IEGPU_ERR_INVALID, "empty boundary or null array");
   for (int i = 0; i < 2 * n_points; i++)
@@ -1253,8 +1261,9 @@ int iegpu_solve_refined(iegpu_ctx* c, const double* b, double* x, double* alpha,
     return fail(c, IEGPU_ERR_INVALID, "bad refinement arguments");
   const int h = c->hole_dim;
   if (h > 0 && !alpha) return fail(c, IEGPU_ERR_INVALID, "alpha required with holes");
-  if (c->n_prob > 1 && c->pde == kPdeStokes)
-    return fail(c, IEGPU_ERR_INVALID, "the FP64 operator of a Stokes batch is not supported");
+  if (c->n_prob > 1 && c->pde != kPdeLaplace)
+    return fail(c, IEGPU_ERR_INVALID,
+                "the FP64 operator of a Stokes or Laplace Neumann batch is not supported");
   try {
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t st = c->stream;
@@ -1330,8 +1339,9 @@ int iegpu_apply(iegpu_ctx* c, const double* x, const double* alpha, double* y,
   if (c->n_points == 0) return fail(c, IEGPU_ERR_INVALID, "set_geometry first");
   if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
   if (!x || !y) return fail(c, IEGPU_ERR_INVALID, "null vector");
-  if (c->n_prob > 1 && c->pde == kPdeStokes)
-    return fail(c, IEGPU_ERR_INVALID, "the FP64 operator of a Stokes batch is not supported");
+  if (c->n_prob > 1 && c->pde != kPdeLaplace)
+    return fail(c, IEGPU_ERR_INVALID,
+                "the FP64 operator of a Stokes or Laplace Neumann batch is not supported");
   const int h = c->hole_dim;
   try {
     HIP_TRY(hipSetDevice(c->device));
@@ -1380,7 +1390,7 @@ int iegpu_evaluate(iegpu_ctx* c, int n_targets, const double* targets,
       dmask = c->e_mask.p;
     }
     const double* dalpha = nullptr;
-    const bool holes = c->n_holes > 0 && alpha;
+    const bool holes = c->hole_dim > 0 && alpha;
     if (holes) {
       c->e_alpha.upload(alpha, c->hole_dim, st);
       dalpha = c->e_alpha.p;

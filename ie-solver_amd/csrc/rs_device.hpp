@@ -14,7 +14,7 @@ constexpr int kMinCompress = 16;        // skel_factorization.h:11
 constexpr int kMaxBoxDofs = 1024;       // hard cap on |active| of one box
 constexpr int kWave = 64;
 
-enum : int { kPdeLaplace = 0, kPdeStokes = 2 };
+enum : int { kPdeLaplace = 0, kPdeLaplaceNeumann = 1, kPdeStokes = 2 };
 
 // Error bits raised by device code (read back once after a factorization).
 enum : int {
@@ -554,9 +554,21 @@ __device__ inline double stokes_entry(const DofGeo& t, const DofGeo& s, bool dia
   return pot * ra * rb + s.w * tna * snb;
 }
 
+// Laplace Neumann (adjoint double layer with the target's normal, plus
+// the rank-one term w_s that fixes the constant), ie_solver/kernel/
+// kernel.cpp:80-92 and :168-209 (IMPROVE_CONDITION = 0).
+__device__ inline double laplace_neumann_entry(const DofGeo& t, const DofGeo& s, bool diag_ok) {
+  const double scale = 1.0 / (2 * M_PI);
+  if (diag_ok && t.x == s.x && t.y == s.y) return s.w - 0.5 + 0.5 * s.k * s.w * scale;
+  const double r0 = t.x - s.x, r1 = t.y - s.y;
+  return s.w + s.w * scale * (r0 * t.n1 + r1 * t.n2) * recip(r0 * r0 + r1 * r1);
+}
+
 __device__ inline double kernel_entry(int pde, const DofGeo& t, const DofGeo& s,
                                       bool diag_ok = true) {
-  return pde == kPdeStokes ? stokes_entry(t, s, diag_ok) : laplace_entry(t, s, diag_ok);
+  if (pde == kPdeStokes) return stokes_entry(t, s, diag_ok);
+  if (pde == kPdeLaplaceNeumann) return laplace_neumann_entry(t, s, diag_ok);
+  return laplace_entry(t, s, diag_ok);
 }
 __device__ inline double kernel_entry(const DevGeom& g, int t, int s) {
   return kernel_entry(g.pde, dof_geo(g, t), dof_geo(g, s));

@@ -59,6 +59,12 @@ __global__ __launch_bounds__(kTile) void k_eval_partial(DevGeom g, int nt,
         a0 += pot * r0 * proj;
         a1 += pot * r1 * proj;
       }
+    } else if (g.pde == kPdeLaplaceNeumann) {
+      // single layer w (1/2 pi) log|r| (kernel.cpp:95-102)
+      for (int q = 0; q < cnt; q++) {
+        const double r0 = x0 - sx[q], r1 = x1 - sy[q];
+        a0 += sw[q] * (1.0 / (2 * M_PI)) * log(sqrt(r0 * r0 + r1 * r1)) * sm0[q];
+      }
     } else {
       for (int q = 0; q < cnt; q++) {
         const double r0 = x0 - sx[q], r1 = x1 - sy[q];

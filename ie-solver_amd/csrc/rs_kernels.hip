@@ -2533,6 +2533,8 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       if (s.rid_mode != 0) {
         if (g.pde == kPdeStokes)
           k_rid<R, kPdeStokes><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
+        else if (g.pde == kPdeLaplaceNeumann)
+          k_rid<R, kPdeLaplaceNeumann><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
         else
           k_rid<R, kPdeLaplace><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
       }

@@ -1,6 +1,7 @@
 // FP64 matrix-free apply of the second-kind boundary operator, y = K x,
-// with the entries of ie_solver/kernel/kernel.cpp:123-165 (Laplace) and
-// :212-290 (Stokes), used for the residual of iterative refinement.
+// with the entries of ie_solver/kernel/kernel.cpp:123-165 (Laplace),
+// :168-209 (Laplace Neumann) and :212-290 (Stokes), used for the residual
+// of iterative refinement.
 //
 // One thread per target point; sources staged through LDS in tiles of 256
 // (pre-scaled by the weights), every wave reading the same source
@@ -31,14 +32,17 @@ __device__ inline double rcp_f64(double d) {
   return fma(y, e, y);
 }
 
-// c = sum_j w_j (n_j . x_j) over Stokes DOF pairs (block partials).
+// c = sum_j w_j (n_j . x_j) over Stokes DOF pairs, or sum_j w_j x_j for
+// Laplace Neumann (its rank-one term), as block partials.
 __global__ __launch_bounds__(256) void k_wn_dot(DevGeom g, const double* x,
                                                 double* partial) {
   __shared__ double red[256];
+  const bool stokes = g.pde == kPdeStokes;
   double acc = 0;
   for (int64_t j = blockIdx.x * 256 + threadIdx.x; j < g.n_points;
        j += (int64_t)gridDim.x * 256)
-    acc += g.wts[j] * (g.nrm[2 * j] * x[2 * j] + g.nrm[2 * j + 1] * x[2 * j + 1]);
+    acc += stokes ? g.wts[j] * (g.nrm[2 * j] * x[2 * j] + g.nrm[2 * j + 1] * x[2 * j + 1])
+                  : g.wts[j] * x[j];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -68,6 +72,8 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
   const bool live = i < g.n_points;
   const double xi = live ? g.pts[2 * i] : 0, yi = live ? g.pts[2 * i + 1] : 0;
   const bool stokes = g.pde == kPdeStokes;
+  const bool neumann = g.pde == kPdeLaplaceNeumann;
+  const double tn0 = live ? g.nrm[2 * i] : 0, tn1 = live ? g.nrm[2 * i + 1] : 0;
   // source range of this block (blo, bhi) and of this target (plo, phi)
   int blo = 0, bhi = g.n_points, plo = 0, phi = g.n_points;
   if (g.n_prob > 1) {
@@ -91,6 +97,8 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
         sb[threadIdx.x] = w * (1.0 / M_PI) * g.nrm[2 * j + 1];
         sm0[threadIdx.x] = x[2 * j];
         sm1[threadIdx.x] = x[2 * j + 1];
+      } else if (neumann) {
+        sa[threadIdx.x] = w * (1.0 / (2 * M_PI)) * x[j];
       } else {
         const double s = -w * (1.0 / (2 * M_PI)) * x[j];
         sa[threadIdx.x] = s * g.nrm[2 * j];
@@ -109,6 +117,13 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
         const double proj = fma(r0, sm0[q], r1 * sm1[q]) * pot;
         a0 = fma(r0, proj, a0);
         a1 = fma(r1, proj, a1);
+      }
+    } else if (neumann) {
+      for (int q = q0; q < q1; q++) {
+        const double r0 = xi - sx[q], r1 = yi - sy[q];
+        const double d2 = fma(r0, r0, r1 * r1);
+        if (d2 == 0) continue;
+        a0 = fma(fma(r0, tn0, r1 * tn1) * sa[q], rcp_f64(d2), a0);
       }
     } else {
       for (int q = q0; q < q1; q++) {
@@ -133,6 +148,11 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
     const double tx = t0 * x0 + t1 * x1;
     y[2 * i] = a0 - 0.5 * x0 + pot * t0 * tx + n0 * c;
     y[2 * i + 1] = a1 - 0.5 * x1 + pot * t1 * tx + n1 * c;
+  } else if (neumann) {
+    // sum_j w_j x_j (every entry's rank-one part, the diagonal's too)
+    double c = 0;
+    for (int b = 0; b < wn_blocks; b++) c += wn_partial[b];
+    y[i] = a0 + c + (-0.5 + 0.5 * kap * w * (1.0 / (2 * M_PI))) * x[i];
   } else {
     y[i] = a0 + (0.5 + 0.5 * kap * w * (1.0 / (2 * M_PI))) * x[i];
   }
@@ -221,9 +241,9 @@ int matvec_partials(const DevGeom& g) {
 
 void launch_matvec(const DevGeom& g, const double* x, double* y, double* partial,
                    hipStream_t st) {
-  if (g.pde == kPdeStokes) k_wn_dot<<<256, 256, 0, st>>>(g, x, partial);
-  k_apply<<<(g.n_points + kTile - 1) / kTile, kTile, 0, st>>>(
-      g, x, partial, g.pde == kPdeStokes ? 256 : 0, y);
+  const bool wn = g.pde == kPdeStokes || g.pde == kPdeLaplaceNeumann;
+  if (wn) k_wn_dot<<<256, 256, 0, st>>>(g, x, partial);
+  k_apply<<<(g.n_points + kTile - 1) / kTile, kTile, 0, st>>>(g, x, partial, wn ? 256 : 0, y);
 }
 
 void launch_border_apply(const double* U, const double* PsiT, const double* x,

@@ -147,6 +147,25 @@ def annulus(n):
                 holes=dict(centers=centers, radii=radii))
 
 
+def donut(n):
+    """The reference's Donut (ie_solver/boundaries/donut.cpp:9-52): circle of
+    radius 1 about (0.5,0.5) (n nodes) around a hole of radius 0.5 at the
+    same centre (n/5 nodes, inward normals, curvature -2)."""
+    i = np.arange(n, dtype=np.float64)
+    ang = i * 2.0 * math.pi / n
+    hn = n // 5
+    a2 = np.arange(hn, dtype=np.float64) * 2.0 * math.pi / hn
+    pts = np.concatenate([np.stack([0.5 + np.cos(ang), 0.5 + np.sin(ang)], 1),
+                          np.stack([0.5 + 0.5 * np.cos(a2), 0.5 + 0.5 * np.sin(a2)], 1)])
+    nrm = np.concatenate([np.stack([np.cos(ang), np.sin(ang)], 1),
+                          np.stack([-np.cos(a2), -np.sin(a2)], 1)])
+    return dict(points=pts, normals=nrm,
+                curvatures=np.concatenate([np.ones(n), np.full(hn, -2.0)]),
+                weights=np.concatenate([np.full(n, 2 * math.pi / n), np.full(hn, math.pi / hn)]),
+                holes=dict(centers=np.array([[0.5, 0.5]]), radii=np.array([0.5])),
+                counts=np.array([n, hn]))
+
+
 def star_batch(n_problems, seed=0, nmin=2048, nmax=32768):
     """Independent star boundaries for a batch: sizes log-uniform in
     [nmin, nmax], k uniform in {3..9}, amplitude U[0.05, 0.25], seeded.

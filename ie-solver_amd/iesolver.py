@@ -23,6 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libiesolver_amd.so")
 
 PDE_LAPLACE = 0
+PDE_LAPLACE_NEUMANN = 1
 PDE_STOKES = 2
 FP64 = 0
 FP32 = 1

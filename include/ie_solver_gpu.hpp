@@ -449,7 +449,8 @@ struct Kernel {
       : solution_dimension(sd), domain_dimension(dd), pde(p), boundary(b),
         domain_points(std::move(dom)) {}
 
-  // kernel.cpp:123-165 (Laplace) and :212-290 (Stokes).
+  // kernel.cpp:123-165 (Laplace), :168-209 (Laplace Neumann) and :212-290
+  // (Stokes).
   ie_Mat operator()(const std::vector<int>& I, const std::vector<int>& J) const {
     ie_Mat K((int)I.size(), (int)J.size());
     const auto& P = boundary->points;
@@ -472,6 +473,11 @@ struct Kernel {
             const double pot = W[sp] / M_PI * (r0 * sn[0] + r1 * sn[1]) / (d2 * d2);
             v = pot * r[a] * r[b] + W[sp] * tn[a] * sn[b];
           }
+        } else if (pde == ie_solver_config::LAPLACE_NEUMANN) {  // kernel.cpp:168-209
+          const int t = I[i], s = J[j];
+          const double r0 = P[2 * t] - P[2 * s], r1 = P[2 * t + 1] - P[2 * s + 1];
+          if (r0 == 0 && r1 == 0) v = W[s] - 0.5 + 0.5 * C[s] * W[s] / (2 * M_PI);
+          else v = W[s] + W[s] / (2 * M_PI) * (r0 * N[2 * t] + r1 * N[2 * t + 1]) / (r0 * r0 + r1 * r1);
         } else {
           const int t = I[i], s = J[j];
           const double r0 = P[2 * t] - P[2 * s], r1 = P[2 * t + 1] - P[2 * s + 1];
@@ -486,8 +492,8 @@ struct Kernel {
 
 inline int gpu_pde(ie_solver_config::Pde p) {
   if (p == ie_solver_config::LAPLACE) return IEGPU_PDE_LAPLACE;
-  if (p == ie_solver_config::STOKES) return IEGPU_PDE_STOKES;
-  throw GpuError(IEGPU_ERR_INVALID, "Laplace-Neumann is not implemented on the GPU path");
+  if (p == ie_solver_config::LAPLACE_NEUMANN) return IEGPU_PDE_LAPLACE_NEUMANN;
+  return IEGPU_PDE_STOKES;
 }
 
 // skel_factorization.h:19-95.

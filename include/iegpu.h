@@ -37,7 +37,7 @@ typedef enum {
 } iegpu_status;
 
 /* ie_solver_config::Pde (ie_solver/io/io.h:19-23). */
-enum { IEGPU_PDE_LAPLACE = 0, IEGPU_PDE_STOKES = 2 };
+enum { IEGPU_PDE_LAPLACE = 0, IEGPU_PDE_LAPLACE_NEUMANN = 1, IEGPU_PDE_STOKES = 2 };
 
 /* Arithmetic of the stored factorization. */
 enum { IEGPU_FP64 = 0, IEGPU_FP32 = 1 };

@@ -15,6 +15,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle_rs.so")
 _lib = None
 
 PDE_LAPLACE = 0
+PDE_LAPLACE_NEUMANN = 1
 PDE_STOKES = 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -207,7 +208,7 @@ def proxy_block(geom, cx, cy, r, box, pde=PDE_LAPLACE):
     pts = [_d(geom["points"]).reshape(-1), _d(geom["normals"]).reshape(-1),
            _d(geom["curvatures"]), _d(geom["weights"])]
     box = _i(box)
-    rows = 256 if pde == PDE_LAPLACE else 512
+    rows = 512 if pde == PDE_STOKES else 256
     out = np.zeros(rows * len(box))
     lib().rso_proxy_block(pde, geom["points"].shape[0], *[_pd(a) for a in pts],
                           cx, cy, r, len(box), _pi(box), _pd(out))

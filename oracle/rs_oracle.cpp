@@ -15,8 +15,8 @@
 // What follows which reference function (paths relative to the reference
 // root):
 //   kernel entries        ie_solver/kernel/kernel.cpp:123-165 (Laplace),
-//                         :212-290 (Stokes)
-//   proxy blocks          ie_solver/kernel/kernel.cpp:426-591
+//                         :168-209 (Laplace Neumann), :212-290 (Stokes)
+//   proxy blocks          ie_solver/kernel/kernel.cpp:426-591, :594-646
 //   ID-matrix rows        ie_solver/kernel/kernel.cpp:293-423
 //   interpolative decomp. ie_solver/ie_mat.cpp:548-588
 //   tree build            ie_solver/quadtree/quadtree.cpp:39-432
@@ -66,6 +66,7 @@ namespace {
 constexpr int kColMajor = 102;
 constexpr int kNoTrans = 111;
 constexpr int kPdeLaplace = 0;
+constexpr int kPdeLaplaceNeumann = 1;
 constexpr int kPdeStokes = 2;
 constexpr int kNumProxy = 128;        // kernel.h:11
 constexpr double kRadiusRatio = 1.5;  // kernel.h:12
@@ -188,11 +189,31 @@ struct Geometry {
   std::vector<double> pts, nrm, curv, wts;
 };
 
-// Double-layer kernel block K(I,J) over DOF indices
-// (kernel.cpp:123-165 Laplace, :212-290 Stokes).
+// Kernel block K(I,J) over DOF indices (kernel.cpp:123-165 Laplace double
+// layer, :168-209 Laplace Neumann, :212-290 Stokes).
 Mat kernel_block(const Geometry& g, const std::vector<int>& I,
                  const std::vector<int>& J) {
   Mat K((int)I.size(), (int)J.size());
+  if (g.pde == kPdeLaplaceNeumann) {
+    const double scale = 1.0 / (2 * M_PI);
+    for (int j = 0; j < (int)J.size(); j++) {
+      const int s = J[j];
+      const double sp1 = g.pts[2 * s], sp2 = g.pts[2 * s + 1];
+      const double sw = g.wts[s], sc = g.curv[s];
+      for (int i = 0; i < (int)I.size(); i++) {
+        const int t = I[i];
+        const double tp1 = g.pts[2 * t], tp2 = g.pts[2 * t + 1];
+        const double tn1 = g.nrm[2 * t], tn2 = g.nrm[2 * t + 1];
+        if (tp1 == sp1 && tp2 == sp2) {
+          K(i, j) = sw - 0.5 + 0.5 * sc * sw * scale;
+        } else {
+          const double r0 = tp1 - sp1, r1 = tp2 - sp2;
+          K(i, j) = sw + sw * scale * (r0 * tn1 + r1 * tn2) / (r0 * r0 + r1 * r1);
+        }
+      }
+    }
+    return K;
+  }
   if (g.pde == kPdeLaplace) {
     const double scale = 1.0 / (2 * M_PI);
     for (int j = 0; j < (int)J.size(); j++) {
@@ -261,6 +282,27 @@ Mat proxy_block(const Geometry& g, double cx, double cy, double r,
     pn2[i] = std::sin(ang);
   }
   const int nb = (int)box.size();
+  if (g.pde == kPdeLaplaceNeumann) {  // kernel.cpp:594-646
+    const double scale = 1.0 / (2 * M_PI);
+    Mat M(2 * kNumProxy, nb);
+    for (int j = 0; j < nb; j++) {  // box (source) -> proxy (target, radial normal)
+      const int s = box[j];
+      const double sp1 = g.pts[2 * s], sp2 = g.pts[2 * s + 1], sw = g.wts[s];
+      for (int p = 0; p < kNumProxy; p++) {
+        const double r0 = px[p] - sp1, r1 = py[p] - sp2;
+        M(p, j) = sw + sw * scale * (r0 * pn1[p] + r1 * pn2[p]) / (r0 * r0 + r1 * r1);
+      }
+    }
+    for (int p = 0; p < kNumProxy; p++) {  // proxy (source, weight pw) -> box (target)
+      for (int i = 0; i < nb; i++) {
+        const int t = box[i];
+        const double r0 = g.pts[2 * t] - px[p], r1 = g.pts[2 * t + 1] - py[p];
+        M(kNumProxy + p, i) = pw + pw * scale * (r0 * g.nrm[2 * t] + r1 * g.nrm[2 * t + 1]) /
+                                       (r0 * r0 + r1 * r1);
+      }
+    }
+    return M;
+  }
   if (g.pde == kPdeLaplace) {
     const double scale = 1.0 / (2 * M_PI);
     Mat M(2 * kNumProxy, nb);
@@ -922,7 +964,8 @@ extern "C" {
 void* rso_create(int pde, int n_points, const double* points,
                  const double* normals, const double* curvatures,
                  const double* weights, int leaf_cap) {
-  if (n_points <= 0 || (pde != kPdeLaplace && pde != kPdeStokes)) return nullptr;
+  if (n_points <= 0 || (pde != kPdeLaplace && pde != kPdeLaplaceNeumann && pde != kPdeStokes))
+    return nullptr;
   Problem* p = new Problem();
   p->g.pde = pde;
   p->g.sol_dim = (pde == kPdeStokes) ? 2 : 1;
@@ -945,6 +988,9 @@ void rso_record_id_mats(void* h, int on) {
 int rso_set_holes(void* h, int n_holes, const double* centers,
                   const double* radii) {
   Problem* p = static_cast<Problem*>(h);
+  // linear_solve.cpp:61-64,106-109: Laplace Neumann has no border; its
+  // multiply connected problem is solved like a simply connected one.
+  if (p->g.pde == kPdeLaplaceNeumann) n_holes = 0;
   p->n_holes = n_holes;
   p->hole_cx.clear();
   p->hole_cy.clear();
@@ -1114,14 +1160,23 @@ int rso_id(int m, int n, const double* A, double tol, int* perm, double* T) {
 }
 
 // Interior evaluation u = K_domain * mu (initialization.cpp:11-135): the
-// Laplace kernel with an off-surface target, or the Stokes double layer
-// without the normal term.  Targets are assumed inside the domain.
+// Laplace kernel with an off-surface target, the Laplace Neumann single
+// layer (initialization.cpp:66-97), or the Stokes double layer without the
+// normal term.  Targets are assumed inside the domain.
 int rso_eval(int pde, int n_points, const double* points, const double* normals,
              const double* weights, int n_targets, const double* targets,
              const double* mu, double* u) {
   for (int t = 0; t < n_targets; t++) {
     const double x0 = targets[2 * t], x1 = targets[2 * t + 1];
-    if (pde == kPdeLaplace) {
+    if (pde == kPdeLaplaceNeumann) {  // single layer, kernel.cpp:95-102
+      const double scale = 1.0 / (2 * M_PI);
+      double acc = 0;
+      for (int j = 0; j < n_points; j++) {
+        const double r0 = x0 - points[2 * j], r1 = x1 - points[2 * j + 1];
+        acc += weights[j] * scale * std::log(std::sqrt(r0 * r0 + r1 * r1)) * mu[j];
+      }
+      u[t] = acc;
+    } else if (pde == kPdeLaplace) {
       const double scale = 1.0 / (2 * M_PI);
       double acc = 0;
       for (int j = 0; j < n_points; j++) {

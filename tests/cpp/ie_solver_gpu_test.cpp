@@ -1,9 +1,9 @@
 // The reference's end-to-end tests (ie_solver/ie_solver_test.cpp), run
 // through the C++ facade on the GPU path.  Same cases and tolerances:
 // backward error <= 50 id_tol (:29-96) and analytic agreement <= 10 id_tol
-// (:378-562).  Laplace-Neumann and the perturbation/update tests are out of
-// scope for the GPU path.  Prints one line per case; exit status 0 iff all
-// pass.
+// (:378-562), Laplace Neumann included.  The perturbation/update tests are
+// out of scope for the GPU path.  Prints one line per case; exit status 0
+// iff all pass.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -35,7 +35,7 @@ static double solve_err(const ie_solver_config& config, Boundary* boundary) {
   std::vector<int> all;
   for (int i = 0; i < config.solution_dimension * (int)boundary->weights.size(); i++)
     all.push_back(i);
-  if (!boundary->holes.empty()) {
+  if (!boundary->holes.empty() && config.pde != ie_solver_config::LAPLACE_NEUMANN) {
     ie_Mat U = initialize_U_mat(config.pde, boundary->holes, boundary->points);
     ie_Mat Psi = initialize_Psi_mat(config.pde, boundary->holes, boundary);
     quadtree.U = U;
@@ -90,6 +90,25 @@ static double laplace_error(const ie_Mat& sol, const std::vector<double>& dom, B
   return std::sqrt(dn) / std::sqrt(nt);
 }
 
+// ie_solver_test.cpp:433-471: the field minus log r, less its mean.
+static double laplace_neumann_error(const ie_Mat& sol, const std::vector<double>& dom,
+                                    Boundary* b) {
+  std::vector<double> d;
+  for (size_t i = 0; i < dom.size(); i += 2) {
+    const Vec2 x(dom[i], dom[i + 1]);
+    if (!b->is_in_domain(x) || std::isnan(sol.get(i / 2, 0))) continue;
+    const double r = std::sqrt(std::pow(x.a[0] - 0.5, 2) + std::pow(x.a[1] - 0.5, 2));
+    const double v = sol.get(i / 2, 0) - std::log(r);
+    if (v != 0) d.push_back(v);
+  }
+  double avg = 0;
+  for (double v : d) avg += v;
+  avg /= d.size();
+  double mx = 0;
+  for (double v : d) mx = std::fmax(mx, v - avg);
+  return mx;
+}
+
 // ie_solver_test.cpp:474-507
 static double stokes_error(const ie_Mat& sol, const std::vector<double>& dom, Boundary* b) {
   double mx = 0;
@@ -129,6 +148,9 @@ int main() {
   const double atol = 10 * std::fmax(tol, 1e-6);
   ie_solver_config lap;
   lap.id_tol = tol;
+  ie_solver_config neu = lap;
+  neu.pde = ie_solver_config::LAPLACE_NEUMANN;
+  neu.boundary_condition = ALL_ONES;
   ie_solver_config sto = lap;
   sto.solution_dimension = 2;
   sto.pde = ie_solver_config::STOKES;
@@ -137,13 +159,18 @@ int main() {
   // ie_solver_test.cpp:112-236 (N=512) and :240-372 (N=4096)
   for (int big = 0; big < 2; big++) {
     const std::string p = big ? "Big" : "";
-    ie_solver_config l = lap, s = sto;
+    ie_solver_config l = lap, s = sto, nm = neu;
     if (big) {
       l.num_boundary_points = 4096;
       s.num_boundary_points = 4096;
+      nm.num_boundary_points = 4096;
     }
     backward_case<Circle>(p + "LaplaceCircleBackwardError", l);
+    backward_case<Circle>(p + "LaplaceNeumannCircleBackwardError", nm);
     backward_case<Circle>(p + "StokesCircleBackwardError", s);
+    ie_solver_config na = nm;
+    na.boundary_shape = Boundary::ANNULUS;
+    backward_case<Annulus>(p + "LaplaceNeumannAnnulusBackwardError", na);
     ie_solver_config la = l;
     la.boundary_shape = Boundary::ANNULUS;
     backward_case<Annulus>(p + "LaplaceAnnulusBackwardError", la);
@@ -165,6 +192,16 @@ int main() {
     std::vector<double> dom;
     ie_Mat sol = get_dom_sol(lap, &dom, &b);
     expect_le("LaplaceAnnulusAnalyticAgreementElectron", laplace_error(sol, dom, &b), atol);
+  }
+  {  // ie_solver_test.cpp:510-525
+    ie_solver_config c = neu;
+    c.boundary_condition = DEFAULT;
+    c.boundary_shape = Boundary::DONUT;
+    Donut b;
+    b.initialize(c.num_boundary_points, c.boundary_condition);
+    std::vector<double> dom;
+    ie_Mat sol = get_dom_sol(c, &dom, &b);
+    expect_le("LaplaceNeumannDonutAnalyticAgreement", laplace_neumann_error(sol, dom, &b), atol);
   }
   {  // ie_solver_test.cpp:528-543
     Circle b;

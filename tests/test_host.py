@@ -59,8 +59,9 @@ def test_error_codes():
     s = iesolver.RSSolver(device=-1)
     g = G.circle(64)
     with pytest.raises(iesolver.IeGpuError) as e:
-        s.set_geometry(g, pde=1)
+        s.set_geometry(g, pde=3)  # 0 Laplace, 1 Laplace Neumann, 2 Stokes
     assert e.value.code == 1
+    iesolver.RSSolver(device=-1).set_geometry(g, pde=iesolver.PDE_LAPLACE_NEUMANN)
     bad = dict(g)
     bad["points"] = g["points"].copy()
     bad["points"][3, 0] = np.nan

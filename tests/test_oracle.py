@@ -318,3 +318,89 @@ def test_id_matrix_stacking():
     assert np.array_equal(M[:nr], K[np.ix_(ic, act)])
     assert np.array_equal(M[nr:2 * nr], K[np.ix_(act, ic)].T)
     assert np.allclose(M[2 * nr:], oracle.proxy_block(g, c[0], c[1], 1.5 * side, act), rtol=0, atol=1e-14)
+
+
+# --------------------------------------------------------------------------
+# Laplace Neumann (kernel.cpp:80-102, 168-209, 594-646)
+# --------------------------------------------------------------------------
+
+NEU = oracle.PDE_LAPLACE_NEUMANN
+
+
+def test_neumann_circle_closed_form():
+    # On the circle r = 0.5, (t - s) . n_t / |t - s|^2 = 1/(2R) for any two
+    # points, so K = (w + 1/(2N)) 11^T - I/2 with w = pi/N.
+    n = 256
+    g = G.circle(n)
+    K = oracle.dense_matrix(g, NEU)
+    want = np.full((n, n), math.pi / n + 1 / (2 * n)) - 0.5 * np.eye(n)
+    assert np.abs(K - want).max() <= 1e-12
+
+
+def test_neumann_is_adjoint_double_layer_plus_weights():
+    # Off the diagonal K_N(i,j) - w_j = (w_j / w_i) K_L(j,i): the target's
+    # normal replaces the source's.  On it K_N(i,i) = K_L(i,i) + w_i - 1.
+    g = G.star(300, amp=0.2, k=5)
+    KN = oracle.dense_matrix(g, NEU)
+    KL = oracle.dense_matrix(g, oracle.PDE_LAPLACE)
+    w = g["weights"]
+    off = ~np.eye(300, dtype=bool)
+    lhs = (KN - w[None, :])[off]
+    rhs = ((w[None, :] / w[:, None]) * KL.T)[off]
+    assert np.abs(lhs - rhs).max() <= 1e-12 * np.abs(rhs).max()
+    assert np.abs(np.diag(KN) - (np.diag(KL) + w - 1)).max() <= 1e-14
+
+
+def test_neumann_proxy_block_formula():
+    g = G.star(512)
+    box = np.arange(40, 70)
+    cx, cy, r = 0.6, 0.5, 0.07
+    M = oracle.proxy_block(g, cx, cy, r, box, NEU)
+    ang = 2 * math.pi * np.arange(128) / 128
+    pp = np.stack([cx + r * np.cos(ang), cy + r * np.sin(ang)], 1)
+    pn = np.stack([np.cos(ang), np.sin(ang)], 1)
+    pw = 2 * math.pi * r / 128
+    p, nm, w = g["points"], g["normals"], g["weights"]
+    d = pp[:, None, :] - p[None, box, :]  # proxy target <- box source
+    top = w[box] + w[box] / (2 * math.pi) * np.einsum("pbk,pk->pb", d, pn) / np.einsum("pbk,pbk->pb", d, d)
+    e = p[None, box, :] - pp[:, None, :]  # box target <- proxy source
+    bot = pw + pw / (2 * math.pi) * np.einsum("pbk,bk->pb", e, nm[box]) / np.einsum("pbk,pbk->pb", e, e)
+    assert np.abs(M[:128] - top).max() < 1e-12
+    assert np.abs(M[128:] - bot).max() < 1e-12
+
+
+@pytest.mark.parametrize("shape", ["circle", "star", "annulus"])
+def test_reference_neumann_backward_error(shape):
+    # ie_solver_test.cpp:124-135, 165-176, 209-220 through check_solve_err
+    # (:29-96): all-ones data, N = 512, tol 1e-6, |K mu - f| <= 50 tol; the
+    # annulus's holes take no border for Laplace Neumann.
+    tol = 1e-6
+    g = {"circle": G.circle(512), "star": G.star(512), "annulus": G.annulus(512)}[shape]
+    n = g["points"].shape[0]
+    f = np.ones(n)
+    o = oracle.OracleProblem(g, pde=NEU)
+    assert o.hole_dim == 0
+    o.factor(tol, 4)
+    mu, alpha = o.solve(f, 4)
+    assert alpha is None or len(alpha) == 0
+    K = oracle.dense_matrix(g, NEU)
+    assert np.abs(K @ mu - f).max() <= 50 * tol
+
+
+def test_reference_neumann_donut_analytic():
+    # ie_solver_test.cpp:433-471, 510-525: donut, data 1 on the outer
+    # circle and -1/r_hole on the hole; the single-layer field is log r plus
+    # a constant.
+    tol = 1e-6
+    g = G.donut(512)
+    n, hn = g["counts"]
+    f = np.concatenate([np.ones(n), np.full(hn, -2.0)])
+    o = oracle.OracleProblem(g, pde=NEU)
+    o.factor(tol, 4)
+    mu, _ = o.solve(f, 4)
+    rng = np.random.default_rng(2)
+    rr = rng.uniform(0.6, 0.9, 100)
+    th = rng.uniform(0, 2 * math.pi, 100)
+    tg = np.stack([0.5 + rr * np.cos(th), 0.5 + rr * np.sin(th)], 1)
+    d = oracle.evaluate(g, tg, mu, NEU) - np.log(rr)
+    assert np.abs(d - d.mean()).max() <= 10 * tol
