@@ -119,3 +119,36 @@ def test_fp64_refinement_is_noop_when_converged():
     mu, _, iters, rr = s.solve_refined(f, rtol=1e-9, max_iter=5)
     assert iters == 0 and rr <= 1e-9
     assert np.array_equal(mu, mu0)
+
+
+def test_stokes_holes_full_size_fp32_refined():
+    # The Stokes workload at its full size: outer circle r = 0.5 plus 8
+    # seeded holes (radii U[0.02, 0.05]), N = 131072 points (262,144 + 24
+    # unknowns).  The FP32 factorization preconditions FP64 refinement on
+    # the bordered operator down to a relative residual of 1e-10; its
+    # interior field is checked against the exact discrete solution (the
+    # plain FP64 factorization refined to a relative residual of 1e-15).
+    g = G.circle_with_holes(131072, 8, seed=0)
+    f = G.tangent_data(g["normals"])
+    f[2 * g["counts"][0]:] = 0
+    m = _solver(g, iesolver.PDE_STOKES, tol=1e-6, precision=iesolver.FP32)
+    mu, alpha, iters, rr = m.solve_refined(f, rtol=1e-10, max_iter=10)
+    print(f"stokes N=131072 holes: {iters} corrections, rel residual {rr:.2e}", flush=True)
+    assert rr <= 1e-10 and iters <= 6
+    p = _solver(g, iesolver.PDE_STOKES, tol=1e-10)
+    mp, ap = p.solve(f)
+    mx, ax, _, rx = p.solve_refined(f, rtol=1e-15, max_iter=6)
+    assert rx <= 1e-12  # the FP64 operator's floor on this bordered system is ~5e-13
+    rng = np.random.default_rng(3)
+    tg = 0.5 + np.stack([rng.uniform(-0.4, 0.4, 400), rng.uniform(-0.4, 0.4, 400)], 1)
+    c, r = g["holes"]["centers"], g["holes"]["radii"]
+    inside = np.hypot(tg[:, 0] - 0.5, tg[:, 1] - 0.5) < 0.45
+    for k in range(len(r)):
+        inside &= np.hypot(tg[:, 0] - c[k, 0], tg[:, 1] - c[k, 1]) > r[k] + 0.01
+    tg = tg[inside]
+    ux = p.evaluate(tg, mx, ax)
+    err = _rel(p.evaluate(tg, mu, alpha), ux)
+    err_plain = _rel(p.evaluate(tg, mp, ap), ux)
+    print(f"interior field vs exact over {len(tg)} points: FP32-refined {err:.2e}, "
+          f"plain FP64 {err_plain:.2e}", flush=True)
+    assert err <= 1e-9
