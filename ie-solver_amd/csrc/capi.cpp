@@ -197,6 +197,9 @@ struct iegpu_ctx {
   DevBuf<int64_t> s_id_sz, s_row_sz, s_fac_sz, s_id_off, s_row_off, s_fac_off,
       s_fac_top;
   DevBuf<unsigned char> s_id_pool, s_fac_pool, s_root_mat;
+  // X_rr and root blocks kept for the compressed forward apply
+  bool keep_operator = false;
+  DevBuf<unsigned char> s_fwd_pool, s_root_fwd;
   int64_t id_cap = 0, row_cap = 0, fac_cap = 0;
   // randomized ID
   int rid_mode = 0, rid_width = 0;
@@ -290,6 +293,8 @@ struct iegpu_ctx {
     s.err = s_err.p;
     s.root_mat = s_root_mat.p;
     s.root_piv = s_root_piv.p;
+    s.fwd_pool = keep_operator ? s_fwd_pool.p : nullptr;
+    s.root_fwd = keep_operator ? s_root_fwd.p : nullptr;
     s.rid_mode = rid_mode;
     s.rid_width = rid_width;
     s.sk_w = s_sk_w.p;
@@ -421,6 +426,10 @@ void alloc_state(iegpu_ctx* c) {
   c->s_id_pool.alloc((size_t)2 * c->id_cap * c->elem());  // odd and even levels
   c->s_row_pool.alloc((size_t)c->row_cap);
   c->s_fac_pool.alloc((size_t)c->fac_cap * c->elem());
+  if (c->keep_operator) {
+    c->s_fwd_pool.alloc((size_t)c->fac_cap * c->elem());
+    c->s_root_fwd.alloc(nr * c->ncap * c->ncap * 8);
+  }
 }
 
 // The ID of level l needs only the skeletons of level l+1, not its Schur
@@ -767,6 +776,50 @@ int iegpu_set_id_method(iegpu_ctx* c, int method, int sketch_rows) {
     return fail(c, IEGPU_ERR_INVALID, "sketch_rows must be 0, 16 or 32");
   c->rid_mode = 1;
   c->rid_width = sketch_rows;
+  return IEGPU_OK;
+}
+
+int iegpu_set_keep_operator(iegpu_ctx* c, int on) {
+  if (!c) return IEGPU_ERR_INVALID;
+  c->keep_operator = on != 0;
+  c->factored = false;
+  return IEGPU_OK;
+}
+
+int iegpu_sparse_matvec(iegpu_ctx* c, const double* x, double* b) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!c->factored) return fail(c, IEGPU_ERR_INVALID, "factor first");
+  if (!c->keep_operator)
+    return fail(c, IEGPU_ERR_INVALID, "factor with iegpu_set_keep_operator(ctx, 1) first");
+  if (!x || !b) return fail(c, IEGPU_ERR_INVALID, "null vector");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int64_t nd = c->ndof();
+    c->d_work.alloc(nd);
+    HIP_TRY(hipMemcpyAsync(c->d_work.p, x, nd * sizeof(double), hipMemcpyHostToDevice, st));
+    const DevState s = c->dstate();
+    const HostTree& T = c->tree;
+    double* v = c->d_work.p;
+    const bool f32 = c->precision == IEGPU_FP32;
+    for (int lv = T.n_levels - 1; lv >= 1; lv--) {
+      const int start = T.level_start[lv], count = T.level_start[lv + 1] - start;
+      if (f32) launch_fwd_up<float>(s, start, count, v, st);
+      else launch_fwd_up<double>(s, start, count, v, st);
+    }
+    if (f32) launch_fwd_root<float>(s, T.n_roots, v, st);
+    else launch_fwd_root<double>(s, T.n_roots, v, st);
+    for (int lv = 1; lv < T.n_levels; lv++) {
+      const int start = T.level_start[lv], count = T.level_start[lv + 1] - start;
+      if (f32) launch_fwd_down<float>(s, start, count, v, st);
+      else launch_fwd_down<double>(s, start, count, v, st);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(b, v, nd * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
   return IEGPU_OK;
 }
 

@@ -159,6 +159,11 @@ struct DevState {
                        // pool size a level needed on overflow (1024 elements)
   void* root_mat;      // per root: 2 ncap^2 elements (matrix + inverse)
   int* root_piv;       // per root: ncap + 64
+  // Kept for the compressed forward apply when requested (else null): X_rr
+  // of every compressed box at its factor-slot offset, and each root's
+  // block (ncap^2 elements per root), as formed before their inversion.
+  void* fwd_pool;
+  void* root_fwd;
   // Randomized ID (rid_mode 0: off; 1: sketch width rid_width for every
   // eligible box; 2: width from the rank model).
   int rid_mode;

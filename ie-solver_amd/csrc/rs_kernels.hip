@@ -1920,6 +1920,10 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
              dot2(Krs + i, r, T + k * j, 1, k);
   }
   __syncthreads();
+  if (s.fwd_pool) {
+    R* Xf = reinterpret_cast<R*>(s.fwd_pool) + s.fac_off[b];
+    for (int idx = tid; idx < r * r; idx += NT) Xf[idx] = X[idx];
+  }
   // The DOF geometry is dead: its space holds the pivot columns.
   R* colbuf = reinterpret_cast<R*>(geo);
   const int info = gj_invert_regs<R>(X, r, prow, colbuf);
@@ -1989,6 +1993,11 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   }
   __syncthreads();
   decouple_blocks<R, NT>(P, F + fl.T, n, k, Krs, Ksr, X);
+  if (s.fwd_pool) {
+    R* Xf = reinterpret_cast<R*>(s.fwd_pool) + s.fac_off[b];
+    for (int64_t idx = tid; idx < (int64_t)r * r; idx += NT) Xf[idx] = X[idx];
+    __syncthreads();
+  }
   // X_rr^{-1}: in registers for r <= 128 (the LDS buffer is free: the
   // blocks of these boxes live in the scratch slot), else step by step.
   int info;
@@ -2055,6 +2064,11 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
     A[idx] = static_cast<R>(kernel_entry(g, act[i], act[j])) - upd(i, j);
   }
   __syncthreads();
+  if (s.root_fwd) {
+    R* Af = reinterpret_cast<R*>(s.root_fwd) + (int64_t)s.ncap * s.ncap * b;
+    for (int64_t idx = tid; idx < (int64_t)n * n; idx += NT) Af[idx] = A[idx];
+    __syncthreads();
+  }
   if (factor_lu && n > kRootBlkMax) {
     // Gauss-Jordan inverse (the reference LU-factors the root block) for
     // blocks too large for k_root_inv and k_root_inv_blk; the inverse,
@@ -2501,6 +2515,98 @@ __global__ __launch_bounds__(256) void k_root_solve(DevState s, SolveVec v) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Compressed forward apply b = A x (SkelFactorization::sparse_matvec,
+// skel_factorization.cpp:429-504): the solve's steps undone in reverse
+// order.  Per compressed box, levels leaves first:
+//   b_s += T b_r;  b_r += U b_s;  b_r = X_rr b_r
+// (X_rr may follow each box at once: no later box touches its redundant
+// entries); then every root b = A_root b; then, root side first,
+//   b_s += L b_r;  b_r += T^T b_s.
+// One wave per box.
+// ---------------------------------------------------------------------------
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_fwd_up(DevState s, int start, double* x) {
+  __shared__ double xs[kMaxBoxDofs];
+  __shared__ double xr[kMaxBoxDofs];
+  const int b = start + blockIdx.x;
+  const int k = s.rank[b];
+  if (k == 0) return;
+  const int lane = threadIdx.x;
+  const int r = s.act_cnt[b] - k;
+  const FacLayout fl = FacLayout::make(k, r);
+  const R* F = reinterpret_cast<const R*>(s.fac_pool) + s.fac_off[b];
+  const R* Xf = reinterpret_cast<const R*>(s.fwd_pool) + s.fac_off[b];
+  const int* sk = s.out_list + (int64_t)b * s.ncap;
+  const int* rd = s.red_list + (int64_t)b * s.ncap;
+  for (int a = lane; a < k; a += 64) xs[a] = x[sk[a]];
+  for (int i = lane; i < r; i += 64) xr[i] = x[rd[i]];
+  wave_sync();
+  for (int a = lane; a < k; a += 64) xs[a] += dot_ld(F + fl.T + a, k, xr, r);
+  wave_sync();
+  for (int i = lane; i < r; i += 64) xr[i] += dot_ld(F + fl.U + i, r, xs, k);
+  wave_sync();
+  double y[kMaxBoxDofs / 64];
+#pragma unroll
+  for (int q = 0; q < kMaxBoxDofs / 64; q++) {
+    const int i = lane + 64 * q;
+    y[q] = i < r ? dot_ld(Xf + i, r, xr, r) : 0.0;
+  }
+  for (int a = lane; a < k; a += 64) x[sk[a]] = xs[a];
+#pragma unroll
+  for (int q = 0; q < kMaxBoxDofs / 64; q++) {
+    const int i = lane + 64 * q;
+    if (i < r) x[rd[i]] = y[q];
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_fwd_root(DevState s, double* x) {
+  __shared__ double xa[kMaxBoxDofs];
+  const int tid = threadIdx.x;
+  const int n = s.act_cnt[blockIdx.x];
+  const int* act = s.act_list + (int64_t)blockIdx.x * s.ncap;
+  const R* A = reinterpret_cast<const R*>(s.root_fwd) + (int64_t)s.ncap * s.ncap * blockIdx.x;
+  for (int i = tid; i < n; i += 256) xa[i] = x[act[i]];
+  __syncthreads();
+  double y[kMaxBoxDofs / 256];
+#pragma unroll
+  for (int q = 0; q < kMaxBoxDofs / 256; q++) {
+    const int i = tid + 256 * q;
+    y[q] = i < n ? dot_ld(A + i, n, xa, n) : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < kMaxBoxDofs / 256; q++) {
+    const int i = tid + 256 * q;
+    if (i < n) x[act[i]] = y[q];
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_fwd_down(DevState s, int start, double* x) {
+  __shared__ double xs[kMaxBoxDofs];
+  __shared__ double xr[kMaxBoxDofs];
+  const int b = start + blockIdx.x;
+  const int k = s.rank[b];
+  if (k == 0) return;
+  const int lane = threadIdx.x;
+  const int r = s.act_cnt[b] - k;
+  const FacLayout fl = FacLayout::make(k, r);
+  const R* F = reinterpret_cast<const R*>(s.fac_pool) + s.fac_off[b];
+  const int* sk = s.out_list + (int64_t)b * s.ncap;
+  const int* rd = s.red_list + (int64_t)b * s.ncap;
+  for (int a = lane; a < k; a += 64) xs[a] = x[sk[a]];
+  for (int i = lane; i < r; i += 64) xr[i] = x[rd[i]];
+  wave_sync();
+  for (int a = lane; a < k; a += 64) xs[a] += dot_ld(F + fl.L + a, k, xr, r);
+  wave_sync();
+  for (int i = lane; i < r; i += 64) xr[i] += dot_ld(F + fl.T + (int64_t)k * i, 1, xs, k);
+  wave_sync();
+  for (int a = lane; a < k; a += 64) x[sk[a]] = xs[a];
+  for (int i = lane; i < r; i += 64) x[rd[i]] = xr[i];
+}
+
 }  // namespace
 
 void launch_reset(const DevState& s, hipStream_t st) {
@@ -2609,6 +2715,19 @@ void launch_root_solve(const DevState& s, int n_roots, SolveVec v, hipStream_t s
   k_root_solve<R><<<n_roots, 256, 0, st>>>(s, v);
 }
 
+template <typename R>
+void launch_fwd_up(const DevState& s, int start, int count, double* x, hipStream_t st) {
+  if (count > 0) k_fwd_up<R><<<count, 64, 0, st>>>(s, start, x);
+}
+template <typename R>
+void launch_fwd_root(const DevState& s, int n_roots, double* x, hipStream_t st) {
+  k_fwd_root<R><<<n_roots, 256, 0, st>>>(s, x);
+}
+template <typename R>
+void launch_fwd_down(const DevState& s, int start, int count, double* x, hipStream_t st) {
+  if (count > 0) k_fwd_down<R><<<count, 64, 0, st>>>(s, start, x);
+}
+
 #define IEGPU_INSTANTIATE(R)                                                   \
   template void launch_factor_stage<R>(int, const DevGeom&, const DevTree&,   \
                                        const DevState&, int, int, int, double, \
@@ -2621,7 +2740,10 @@ void launch_root_solve(const DevState& s, int n_roots, SolveVec v, hipStream_t s
                                hipStream_t);                                   \
   template void launch_sweep_down<R>(const DevState&, int, int, SolveVec,     \
                                      hipStream_t);                             \
-  template void launch_root_solve<R>(const DevState&, int, SolveVec, hipStream_t);
+  template void launch_root_solve<R>(const DevState&, int, SolveVec, hipStream_t); \
+  template void launch_fwd_up<R>(const DevState&, int, int, double*, hipStream_t); \
+  template void launch_fwd_root<R>(const DevState&, int, double*, hipStream_t);    \
+  template void launch_fwd_down<R>(const DevState&, int, int, double*, hipStream_t);
 
 IEGPU_INSTANTIATE(double)
 IEGPU_INSTANTIATE(float)

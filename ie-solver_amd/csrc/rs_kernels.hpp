@@ -79,4 +79,15 @@ void launch_sweep_down(const DevState& s, int start, int count, SolveVec v,
 template <typename R>
 void launch_root_solve(const DevState& s, int n_roots, SolveVec v, hipStream_t st);
 
+// Compressed forward apply b = A x in place on x (ndof), from the factors
+// and the X_rr / root blocks kept in fwd_pool / root_fwd: per level,
+// leaves first, launch_fwd_up; then launch_fwd_root; then per level, root
+// side first, launch_fwd_down.
+template <typename R>
+void launch_fwd_up(const DevState& s, int start, int count, double* x, hipStream_t st);
+template <typename R>
+void launch_fwd_root(const DevState& s, int n_roots, double* x, hipStream_t st);
+template <typename R>
+void launch_fwd_down(const DevState& s, int start, int count, double* x, hipStream_t st);
+
 }  // namespace iegpu

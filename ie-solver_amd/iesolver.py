@@ -44,7 +44,7 @@ EXPORTS = [
     "iegpu_set_graphs", "iegpu_record_features", "iegpu_box_features", "iegpu_rank_model_init",
     "iegpu_rank_model_train", "iegpu_rank_model_grad", "iegpu_set_rank_model",
     "iegpu_set_geometry_batch", "iegpu_set_launch_caps", "iegpu_get_launch_caps",
-    "iegpu_autotune", "iegpu_set_overlap",
+    "iegpu_autotune", "iegpu_set_overlap", "iegpu_set_keep_operator", "iegpu_sparse_matvec",
 ]
 NUM_LAUNCH_CAPS = 7
 RANK_FEATURES = 9
@@ -107,6 +107,8 @@ def load():
     L.iegpu_rank_model_grad.argtypes = [vp, fp, fp, fp, ctypes.c_int, fp, fp]
     L.iegpu_set_rank_model.argtypes = [vp, fp, ctypes.c_int]
     L.iegpu_set_overlap.argtypes = [vp, ctypes.c_int]
+    L.iegpu_set_keep_operator.argtypes = [vp, ctypes.c_int]
+    L.iegpu_sparse_matvec.argtypes = [vp, _dp, _dp]
     L.iegpu_set_launch_caps.argtypes = [vp, _ip, ctypes.c_int]
     L.iegpu_get_launch_caps.argtypes = [vp, _ip, ctypes.c_int]
     L.iegpu_autotune.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp]
@@ -206,6 +208,17 @@ class RSSolver:
         """ID_QRCP (the reference's full pivoted QR) or ID_RANDOMIZED (Gaussian
         sketch of sketch_rows rows, full-QRCP fallback)."""
         self._check(self.L.iegpu_set_id_method(self.h, method, sketch_rows))
+
+    def set_keep_operator(self, on=True):
+        """Keep X_rr and the root blocks at the next factor (for sparse_matvec)."""
+        self._check(self.L.iegpu_set_keep_operator(self.h, 1 if on else 0))
+
+    def sparse_matvec(self, x):
+        """b = A x with the compressed operator (SkelFactorization::sparse_matvec)."""
+        x = _d(x)
+        b = np.zeros(self.ndof)
+        self._check(self.L.iegpu_sparse_matvec(self.h, _pd(x), _pd(b)))
+        return b
 
     def set_overlap(self, on=True):
         """Overlap each level's ID with the decouple of the level below

@@ -175,6 +175,20 @@ int iegpu_profile(iegpu_ctx* ctx, int stage, int level, double* ms,
  * profiling is on or with holes.  No reference counterpart. */
 int iegpu_set_graphs(iegpu_ctx* ctx, int on);
 
+/* on != 0: the next iegpu_factor also keeps X_rr of every compressed box
+ * and each root block as formed before their inversion (one more factor
+ * pool and ncap^2 doubles per root), so that iegpu_sparse_matvec can apply
+ * the compressed operator.  Default off.  No reference counterpart (the
+ * reference's tree keeps X_rr and allskel_mat, quadtree.h:62-66,104). */
+int iegpu_set_keep_operator(iegpu_ctx* ctx, int on);
+
+/* b = A x with the compressed operator of the stored factorization (the
+ * skeletonization applied forward; no border with holes), host pointers of
+ * ndof doubles.  Replaces SkelFactorization::sparse_matvec,
+ * skel_factorization.h:89 (skel_factorization.cpp:429-504).  Needs a
+ * factorization made with iegpu_set_keep_operator(ctx, 1). */
+int iegpu_sparse_matvec(iegpu_ctx* ctx, const double* x, double* b);
+
 /* on != 0 (default): iegpu_factor runs each level's decouple stage on a
  * second stream of the context, so that the ID stages of level l overlap
  * the decouple of level l+1 (the ID of a level needs only the skeletons

@@ -45,6 +45,7 @@ def lib():
         L.rso_get_U_Psi.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.rso_factor.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
         L.rso_solve.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int]
+        L.rso_sparse_matvec.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.rso_num_levels.argtypes = [ctypes.c_void_p]
         L.rso_num_boxes.argtypes = [ctypes.c_void_p]
         L.rso_box_info.argtypes = [ctypes.c_void_p, _ip, _ip, _ip, _ip, _dp,
@@ -145,6 +146,14 @@ class OracleProblem:
         U = Ub.reshape(self.hole_dim, self.ndof).T
         Psi = Pb.reshape(self.ndof, self.hole_dim).T
         return U, Psi
+
+    def sparse_matvec(self, x):
+        """b = A x with the compressed operator (skel_factorization.cpp:429-504)."""
+        x = _d(x)
+        b = np.zeros(self.ndof)
+        if lib().rso_sparse_matvec(self.h, _pd(x), _pd(b)) != 0:
+            raise RuntimeError("factor first")
+        return b
 
     def num_levels(self):
         return lib().rso_num_levels(self.h)

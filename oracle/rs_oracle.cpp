@@ -25,6 +25,7 @@
 //   decouple              .../skel_factorization.cpp:77-128
 //   Schur updates         .../skel_factorization.cpp:323-381
 //   solve                 .../skel_factorization.cpp:507-592
+//   sparse_matvec         .../skel_factorization.cpp:429-504
 //   multiply-conn. solve  .../skel_factorization.cpp:596-777
 //   U / Psi matrices      ie_solver/linear_solve.cpp:20-112
 //   interior evaluation   ie_solver/initialization.cpp:11-135,
@@ -378,7 +379,7 @@ struct Node {
   std::vector<Node*> neighbors;
   std::vector<int> original, active, skel, redundant, near, perm;
   int m_rows = 0;  // ID matrix height at compression time
-  Mat T, L, U, Xlu, schur, id_mat;
+  Mat T, L, U, X, Xlu, schur, id_mat;
   std::vector<int> piv;
 };
 
@@ -687,6 +688,7 @@ struct Problem {
                 mul(Krs, T));
     Mat Ksr = sub(gather(KBN, s, rr), mul(gather(KBN, s, s), T));
     int info = 0;
+    node->X = X;  // X_rr, kept for sparse_matvec (quadtree.h:62)
     lu_factor(X, &node->Xlu, &node->piv, &info);
     node->L = lu_right_solve(node->Xlu, node->piv, Ksr);
     node->U = lu_left_solve(node->Xlu, node->piv, Krs);
@@ -907,6 +909,40 @@ struct Problem {
     return status;
   }
 
+  // vec(rows) = M vec(rows) (skel_factorization.cpp:408-416).
+  static void diag_mul(const Mat& M, Mat* X, const std::vector<int>& rows) {
+    if (rows.empty()) return;
+    Mat y = mul(M, gather_rows(*X, rows));
+    for (int c = 0; c < X->w; c++)
+      for (int i = 0; i < (int)rows.size(); i++) (*X)(rows[i], c) = y(i, c);
+  }
+
+  // skel_factorization.cpp:429-504 (sparse_matvec): the skeletonization
+  // applied forward; skelnear = skel under weak admissibility.
+  void sparse_matvec(const double* x, double* b) const {
+    Mat B(ndof(), 1);
+    std::memcpy(B.data(), x, sizeof(double) * ndof());
+    const int lv = (int)levels.size();
+    for (int l = lv - 1; l >= 0; l--)
+      for (Node* n : levels[l]) {
+        if (!n->compressed) continue;
+        sweep(n->T, &B, n->redundant, n->skel, false, 1.0);
+        sweep(n->U, &B, n->skel, n->redundant, false, 1.0);
+      }
+    for (int l = lv - 1; l >= 0; l--)
+      for (Node* n : levels[l])
+        if (n->compressed) diag_mul(n->X, &B, n->redundant);
+    if (!allskel.empty()) diag_mul(allskel_mat, &B, allskel);
+    for (int l = 0; l < lv; l++)
+      for (int i = (int)levels[l].size() - 1; i >= 0; i--) {
+        Node* n = levels[l][i];
+        if (!n->compressed) continue;
+        sweep(n->L, &B, n->redundant, n->skel, false, 1.0);
+        sweep(n->T, &B, n->skel, n->redundant, true, 1.0);
+      }
+    std::memcpy(b, B.data(), sizeof(double) * ndof());
+  }
+
   // skel_factorization.cpp:507-592 (solve).
   void solve(const double* b, double* x, int nt) const {
     Mat X(ndof(), 1);
@@ -1034,6 +1070,13 @@ int rso_solve(void* h, const double* b, double* mu, double* alpha,
   } else {
     p->solve(b, mu, num_threads);
   }
+  return 0;
+}
+
+int rso_sparse_matvec(void* h, const double* x, double* b) {
+  Problem* p = static_cast<Problem*>(h);
+  if (!p->factored) return 1;
+  p->sparse_matvec(x, b);
   return 0;
 }
 

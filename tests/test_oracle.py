@@ -404,3 +404,23 @@ def test_reference_neumann_donut_analytic():
     tg = np.stack([0.5 + rr * np.cos(th), 0.5 + rr * np.sin(th)], 1)
     d = oracle.evaluate(g, tg, mu, NEU) - np.log(rr)
     assert np.abs(d - d.mean()).max() <= 10 * tol
+
+
+# --------------------------------------------------------------------------
+# Compressed forward apply (skel_factorization.cpp:429-504)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("pde", [oracle.PDE_LAPLACE, oracle.PDE_LAPLACE_NEUMANN, oracle.PDE_STOKES])
+def test_sparse_matvec_inverts_solve_and_approximates_K(pde):
+    # The forward apply undoes the solve step by step, so solve(A x) = x to
+    # rounding; and A equals the dense operator up to the ID error.
+    tol = 1e-10
+    g = G.star(2048) if pde != oracle.PDE_STOKES else G.circle(1024)
+    o = oracle.OracleProblem(g, pde=pde)
+    o.factor(tol, 4)
+    x = np.random.default_rng(7).standard_normal(o.ndof)
+    b = o.sparse_matvec(x)
+    y, _ = o.solve(b, 4)
+    assert rel(y, x) <= 1e-12
+    K = oracle.dense_matrix(g, pde)
+    assert rel(b, K @ x) <= 50 * tol
