@@ -1314,9 +1314,6 @@ int iegpu_solve_refined(iegpu_ctx* c, const double* b, double* x, double* alpha,
     return fail(c, IEGPU_ERR_INVALID, "bad refinement arguments");
   const int h = c->hole_dim;
   if (h > 0 && !alpha) return fail(c, IEGPU_ERR_INVALID, "alpha required with holes");
-  if (c->n_prob > 1 && c->pde != kPdeLaplace)
-    return fail(c, IEGPU_ERR_INVALID,
-                "the FP64 operator of a Stokes or Laplace Neumann batch is not supported");
   try {
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t st = c->stream;
@@ -1392,9 +1389,6 @@ int iegpu_apply(iegpu_ctx* c, const double* x, const double* alpha, double* y,
   if (c->n_points == 0) return fail(c, IEGPU_ERR_INVALID, "set_geometry first");
   if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
   if (!x || !y) return fail(c, IEGPU_ERR_INVALID, "null vector");
-  if (c->n_prob > 1 && c->pde != kPdeLaplace)
-    return fail(c, IEGPU_ERR_INVALID,
-                "the FP64 operator of a Stokes or Laplace Neumann batch is not supported");
   const int h = c->hole_dim;
   try {
     HIP_TRY(hipSetDevice(c->device));

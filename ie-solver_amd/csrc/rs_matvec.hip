@@ -32,15 +32,17 @@ __device__ inline double rcp_f64(double d) {
   return fma(y, e, y);
 }
 
-// c = sum_j w_j (n_j . x_j) over Stokes DOF pairs, or sum_j w_j x_j for
-// Laplace Neumann (its rank-one term), as block partials.
+// c_p = sum_j w_j (n_j . x_j) over Stokes DOF pairs, or sum_j w_j x_j for
+// Laplace Neumann (its rank-one term), over problem p's points (one
+// workgroup per problem; all points when n_prob = 1).
 __global__ __launch_bounds__(256) void k_wn_dot(DevGeom g, const double* x,
                                                 double* partial) {
   __shared__ double red[256];
   const bool stokes = g.pde == kPdeStokes;
+  const int lo = g.n_prob > 1 ? g.prob_off[blockIdx.x] : 0;
+  const int hi = g.n_prob > 1 ? g.prob_off[blockIdx.x + 1] : g.n_points;
   double acc = 0;
-  for (int64_t j = blockIdx.x * 256 + threadIdx.x; j < g.n_points;
-       j += (int64_t)gridDim.x * 256)
+  for (int64_t j = lo + threadIdx.x; j < hi; j += 256)
     acc += stokes ? g.wts[j] * (g.nrm[2 * j] * x[2 * j] + g.nrm[2 * j + 1] * x[2 * j + 1])
                   : g.wts[j] * x[j];
   red[threadIdx.x] = acc;
@@ -64,8 +66,7 @@ __device__ inline int problem_of(const DevGeom& g, int i) {
 }
 
 __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
-                                                 const double* wn_partial,
-                                                 int wn_blocks, double* y) {
+                                                 const double* wn, double* y) {
   __shared__ double sx[kTile], sy[kTile], sa[kTile], sb[kTile], sm0[kTile],
       sm1[kTile];
   const int i = blockIdx.x * kTile + threadIdx.x;
@@ -75,15 +76,15 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
   const bool neumann = g.pde == kPdeLaplaceNeumann;
   const double tn0 = live ? g.nrm[2 * i] : 0, tn1 = live ? g.nrm[2 * i + 1] : 0;
   // source range of this block (blo, bhi) and of this target (plo, phi)
-  int blo = 0, bhi = g.n_points, plo = 0, phi = g.n_points;
+  int blo = 0, bhi = g.n_points, plo = 0, phi = g.n_points, prob = 0;
   if (g.n_prob > 1) {
     const int first = blockIdx.x * kTile;
     const int last = min(first + kTile, g.n_points) - 1;
     blo = g.prob_off[problem_of(g, first)];
     bhi = g.prob_off[problem_of(g, last) + 1];
-    const int p = problem_of(g, live ? i : last);
-    plo = g.prob_off[p];
-    phi = g.prob_off[p + 1];
+    prob = problem_of(g, live ? i : last);
+    plo = g.prob_off[prob];
+    phi = g.prob_off[prob + 1];
   }
   double a0 = 0, a1 = 0;
   for (int base = blo; base < bhi; base += kTile) {
@@ -139,8 +140,7 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
   const double w = g.wts[i], kap = g.curv[i];
   const double n0 = g.nrm[2 * i], n1 = g.nrm[2 * i + 1];
   if (stokes) {
-    double c = 0;
-    for (int b = 0; b < wn_blocks; b++) c += wn_partial[b];
+    const double c = wn[prob];
     // self 2x2 block without its n n^T part (kernel.cpp:245-263)
     const double pot = -0.5 * kap * w * (1.0 / M_PI);
     const double t0 = -n1, t1 = n0;
@@ -150,9 +150,7 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
     y[2 * i + 1] = a1 - 0.5 * x1 + pot * t1 * tx + n1 * c;
   } else if (neumann) {
     // sum_j w_j x_j (every entry's rank-one part, the diagonal's too)
-    double c = 0;
-    for (int b = 0; b < wn_blocks; b++) c += wn_partial[b];
-    y[i] = a0 + c + (-0.5 + 0.5 * kap * w * (1.0 / (2 * M_PI))) * x[i];
+    y[i] = a0 + wn[prob] + (-0.5 + 0.5 * kap * w * (1.0 / (2 * M_PI))) * x[i];
   } else {
     y[i] = a0 + (0.5 + 0.5 * kap * w * (1.0 / (2 * M_PI))) * x[i];
   }
@@ -235,15 +233,14 @@ void launch_axpy(double* y, const double* x, int64_t n, hipStream_t st) {
 }
 
 int matvec_partials(const DevGeom& g) {
-  (void)g;
-  return 256;
+  return g.n_prob > 256 ? g.n_prob : 256;
 }
 
 void launch_matvec(const DevGeom& g, const double* x, double* y, double* partial,
                    hipStream_t st) {
-  const bool wn = g.pde == kPdeStokes || g.pde == kPdeLaplaceNeumann;
-  if (wn) k_wn_dot<<<256, 256, 0, st>>>(g, x, partial);
-  k_apply<<<(g.n_points + kTile - 1) / kTile, kTile, 0, st>>>(g, x, partial, wn ? 256 : 0, y);
+  if (g.pde == kPdeStokes || g.pde == kPdeLaplaceNeumann)
+    k_wn_dot<<<g.n_prob, 256, 0, st>>>(g, x, partial);
+  k_apply<<<(g.n_points + kTile - 1) / kTile, kTile, 0, st>>>(g, x, partial, y);
 }
 
 void launch_border_apply(const double* U, const double* PsiT, const double* x,

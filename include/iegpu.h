@@ -65,9 +65,9 @@ int iegpu_set_geometry(iegpu_ctx* ctx, int pde, int n_points,
  * arrays (point_offsets[0] = 0, each problem non-empty).  build_tree makes
  * one quadtree per problem, merged level by level, so every stage of
  * factor and solve covers all problems in one launch per level; vectors
- * are the problems' DOF vectors concatenated.  Holes, interior evaluation
- * and the FP64 operator of a Stokes batch are not supported for
- * n_problems > 1.  No counterpart in the reference (one boundary per
+ * are the problems' DOF vectors concatenated.  Holes and interior
+ * evaluation are not supported for n_problems > 1.  No counterpart in the
+ * reference (one boundary per
  * QuadTree, ie_solver/quadtree/quadtree.h:95-139). */
 int iegpu_set_geometry_batch(iegpu_ctx* ctx, int pde, int n_problems,
                              const int* point_offsets, const double* points,

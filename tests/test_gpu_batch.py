@@ -168,3 +168,26 @@ def test_autotune_and_launch_caps_keep_results():
     s.factor(TOL)
     m2, _ = s.solve(f)
     assert np.array_equal(m0, m2)
+
+
+@pytest.mark.parametrize("pde", [iesolver.PDE_STOKES, iesolver.PDE_LAPLACE_NEUMANN])
+def test_batch_operator_with_rank_one_terms(pde):
+    # Stokes (its n n^T part) and Laplace Neumann (its w_j part) sum a
+    # global term per problem: the batch's operator equals each problem's
+    # own, and FP32 + refinement converges on the batch.
+    geoms, _ = G.star_batch(3, seed=10, nmin=400, nmax=1500)
+    s = iesolver.RSSolver(0)
+    s.set_geometry_batch(geoms, pde)
+    x = np.random.default_rng(10).standard_normal(s.ndof)
+    y, _ = s.apply(x)
+    off = s.dof_offsets
+    for p, g in enumerate(geoms):
+        t = iesolver.RSSolver(0)
+        t.set_geometry(g, pde)
+        sl = slice(off[p], off[p + 1])
+        assert np.array_equal(y[sl], t.apply(x[sl])[0]), p
+    s.build_tree(64)
+    s.factor(1e-6, iesolver.FP32)
+    f = s.apply(np.ones(s.ndof))[0]
+    xr, _, it, rel = s.solve_refined(f, rtol=1e-12, max_iter=10)
+    assert rel <= 1e-12 and _rel(xr, np.ones(s.ndof)) <= 1e-9
