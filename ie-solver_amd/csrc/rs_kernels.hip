@@ -983,7 +983,7 @@ struct QrcpShape {
 // would keep more values live across the step loop than the registers
 // hold); the few n <= 128 boxes share a small strided grid, and so do the
 // randomized ID's few fallbacks (STRIDE: the loop's spills stay there).
-template <typename R, int CPW, int RPL, int NW, bool STRIDE>
+template <typename R, int CPW, int RPL, int NW, bool STRIDE, bool LOOP = true>
 __global__ __launch_bounds__(64 * NW, (QrcpShape<CPW, RPL, NW>::kWavesPerSimd)) void k_qrcp_id(
     DevGeom g, DevTree t, DevState s, int start, int level, double tol_d) {
   constexpr int L = QrcpShape<CPW, RPL, NW>::kList;
@@ -997,11 +997,18 @@ __global__ __launch_bounds__(64 * NW, (QrcpShape<CPW, RPL, NW>::kWavesPerSimd)) 
     const bool fb = L != kLstReg8 && s.rid_mode != 0;
     const int LL = fb ? kLstFallback : L;
     const int cnt = list_count(s, level, LL);
-    for (int li = blockIdx.x; li < cnt; li += gridDim.x) {
-      const int b = list_at(s, start, LL, li);
-      if (fb && (L == kLstRegHalf) != (s.act_cnt[b] <= kHalfCols)) continue;
+    if constexpr (LOOP) {
+      for (int li = blockIdx.x; li < cnt; li += gridDim.x) {
+        const int b = list_at(s, start, LL, li);
+        if (fb && (L == kLstRegHalf) != (s.act_cnt[b] <= kHalfCols)) continue;
+        qrcp_id_box<R, CPW, RPL, NW>(g, t, s, start, level, b, tol_d);
+        __syncthreads();
+      }
+    } else {  // the grid covers the list: one box per workgroup
+      if ((int)blockIdx.x >= cnt) return;
+      const int b = list_at(s, start, LL, blockIdx.x);
+      if (fb && (L == kLstRegHalf) != (s.act_cnt[b] <= kHalfCols)) return;
       qrcp_id_box<R, CPW, RPL, NW>(g, t, s, start, level, b, tol_d);
-      __syncthreads();
     }
   }
 }
@@ -2645,15 +2652,24 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
           k_rid<R, kPdeLaplace><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
       }
       break;
-    case kStageQrcp:
+    case kStageQrcp: {
+      // A register-QRCP kernel whose grid covers its whole list (known from
+      // the level's box count) runs one box per workgroup, without the
+      // grid-stride loop and the registers it keeps live (no spills).
+      const int jobs_max = count * kMaxChunks;
       if (s.rid_mode != 0) {
         // Only the boxes without a successful sketch reach the full QRCP.
-        const int grid = min(count, caps.qrcp_id);
-        // (box, chunk) jobs of the listed boxes: up to kMaxChunks per box
-        k_rowcomp<R, 4><<<min(count * kMaxChunks, caps.rowcomp4), kXThreads, 0, st>>>(
-            g, t, s, start, level);
-        k_qrcp_id<R, 4, 8, 8, true><<<grid, 512, 0, st>>>(g, t, s, start, level, tol);
-        k_qrcp_id<R, 4, 8, 16, true><<<grid, kXThreads, 0, st>>>(g, t, s, start, level, tol);
+        k_rowcomp<R, 4><<<min(jobs_max, caps.rowcomp4), kXThreads, 0, st>>>(g, t, s, start,
+                                                                            level);
+        if (count <= caps.qrcp_id) {
+          k_qrcp_id<R, 4, 8, 8, true, false><<<count, 512, 0, st>>>(g, t, s, start, level, tol);
+          k_qrcp_id<R, 4, 8, 16, true, false><<<count, kXThreads, 0, st>>>(g, t, s, start,
+                                                                           level, tol);
+        } else {
+          const int grid = caps.qrcp_id;
+          k_qrcp_id<R, 4, 8, 8, true><<<grid, 512, 0, st>>>(g, t, s, start, level, tol);
+          k_qrcp_id<R, 4, 8, 16, true><<<grid, kXThreads, 0, st>>>(g, t, s, start, level, tol);
+        }
       } else {
         k_rowcomp<R, 4><<<min(2 * count, caps.rowcomp4), kXThreads, 0, st>>>(g, t, s, start,
                                                                             level);
@@ -2661,11 +2677,16 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
         k_qrcp_id<R, 4, 8, 16, false><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
       }
       k_rowcomp<R, 8><<<min(2 * count, caps.rowcomp8), kXThreads, 0, st>>>(g, t, s, start,
-                                                                            level);
-      k_qrcp_id<R, 8, 4, 16, true><<<min(count, caps.qrcp_id8), kXThreads, 0, st>>>(
-          g, t, s, start, level, tol);
+                                                                          level);
+      if (count <= caps.qrcp_id8)
+        k_qrcp_id<R, 8, 4, 16, true, false><<<count, kXThreads, 0, st>>>(g, t, s, start, level,
+                                                                         tol);
+      else
+        k_qrcp_id<R, 8, 4, 16, true><<<caps.qrcp_id8, kXThreads, 0, st>>>(g, t, s, start, level,
+                                                                          tol);
       k_qrcp<R, 512><<<min(count, caps.qrcp), 512, 0, st>>>(s, start, level, tol);
       break;
+    }
     case kStageDecouple:
       if (count >= caps.dsplit) {
         k_decouple_small<R, 0, kSmallBox1><<<count, 256, 0, st>>>(g, t, s, start);
