@@ -256,8 +256,11 @@ def run_batch(args, cfg):
     torch.cuda.synchronize()
 
     def step():
-        s.factor(cfg["tol"], iesolver.FP64)
-        s.solve_device(f_dev.data_ptr(), mu_dev.data_ptr(), None, None)
+        if args.no_fused:
+            s.factor(cfg["tol"], iesolver.FP64)
+            s.solve_device(f_dev.data_ptr(), mu_dev.data_ptr(), None, None)
+        else:  # the solve's upward sweeps beside the factorization
+            s.factor_solve_device(cfg["tol"], iesolver.FP64, f_dev.data_ptr(), mu_dev.data_ptr())
 
     plain_ms, plain_prof, mu_plain, caps_plain = None, {}, None, None
     if args.form == "plain" or not args.no_plain:
@@ -357,6 +360,7 @@ def run_batch(args, cfg):
                          if ms32 else None),
         "hip_graph": args.graphs,
         "overlap_id_decouple": not args.no_overlap,
+        "fused_factor_solve": not args.no_fused,
         "launch_caps": {"form": caps, "plain_fp64": caps_plain,
                         "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp, decouple_split",
                         "autotuned": args.autotune},
@@ -389,6 +393,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 comparator (profiling runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="iegpu_factor then iegpu_solve_device instead of iegpu_factor_solve_device")
     ap.add_argument("--no-overlap", action="store_true",
                     help="run each level's decouple on the main stream instead of "
                          "overlapping it with the next level's ID")
@@ -425,8 +431,11 @@ def main():
     torch.cuda.synchronize()
 
     def step():
-        s.factor(cfg["tol"], iesolver.FP64)
-        s.solve_device(f_dev.data_ptr(), mu_dev.data_ptr(), None, None)
+        if args.no_fused:
+            s.factor(cfg["tol"], iesolver.FP64)
+            s.solve_device(f_dev.data_ptr(), mu_dev.data_ptr(), None, None)
+        else:  # the solve's upward sweeps beside the factorization
+            s.factor_solve_device(cfg["tol"], iesolver.FP64, f_dev.data_ptr(), mu_dev.data_ptr())
 
     def timed(profile):
         return time_steps(step, s, args, profile)
@@ -507,6 +516,7 @@ def main():
                                   if mu_plain is not None else None),
         "hip_graph": args.graphs,
         "overlap_id_decouple": not args.no_overlap,
+        "fused_factor_solve": not args.no_fused,
         "launch_caps": {"form": caps, "plain_fp64": caps_plain,
                         "order": "assemble, qrcp_id, rowcomp4, rowcomp8, qrcp_id8, qrcp, decouple_split",
                         "autotuned": args.autotune},

@@ -210,6 +210,8 @@ struct iegpu_ctx {
   DevState fgraph_state{};
   LaunchCaps caps, fgraph_caps;
   bool fgraph_overlap = true;
+  const double* fgraph_fb = nullptr;
+  double* fgraph_fx = nullptr;
   double fgraph_tol = 0;
   int fgraph_prec = -1;
   // rank model
@@ -437,13 +439,19 @@ void alloc_state(iegpu_ctx* c) {
 // on a second stream, after its level's ID and the decouple of the level
 // below: level l's ID proceeds while level l+1 is decoupled.  The root
 // waits for both chains.  Per-stage profiling keeps one stream.
+// With fb / fx (device, ndof) the solve x = A^{-1} b is fused in: each
+// level's upward sweep follows that level's decouple on the decouple chain,
+// and the root solve and the downward sweeps follow the root.
 template <typename R>
-void enqueue_factor(iegpu_ctx* c, double tol) {
+void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
+                    double* fx = nullptr) {
   DevGeom g = c->geom();
   DevTree t = c->dtree();
   DevState s = c->dstate();
   hipStream_t st = c->stream;
   const HostTree& T = c->tree;
+  const SolveVec fv{fx, (int64_t)c->ndof(), 1};
+  if (fx && fb != fx) launch_copy(fx, fb, c->ndof(), st);
   const bool split = c->overlap && !c->prof.on && c->stream2 != nullptr;
   hipStream_t st2 = split ? c->stream2 : st;
   if (split && c->evs.size() < (size_t)T.n_levels + 2) {
@@ -472,6 +480,9 @@ void enqueue_factor(iegpu_ctx* c, double tol) {
     c->prof.run(kStageDecouple, lv, st2, [&] {
       launch_factor_stage<R>(kStageDecouple, g, t, s, lv, start, count, tol, c->caps, st2);
     });
+    if (fx)
+      c->prof.run(kProfSweepUp, lv, st2,
+                  [&] { launch_sweep_up<R>(s, start, count, fv, false, true, st2); });
   }
   if (split) {
     HIP_TRY(hipEventRecord(c->evs[T.n_levels + 1], st2));
@@ -479,6 +490,15 @@ void enqueue_factor(iegpu_ctx* c, double tol) {
   }
   c->prof.run(kProfRoot, 0, st,
               [&] { launch_root<R>(g, t, s, T.n_roots, c->hole_dim == 0, st); });
+  if (fx) {
+    c->prof.run(kProfRootSolve, 0, st, [&] { launch_root_solve<R>(s, T.n_roots, fv, st); });
+    for (int lv = 1; lv < T.n_levels; lv++) {
+      const int start = T.level_start[lv];
+      const int count = T.level_start[lv + 1] - start;
+      c->prof.run(kProfSweepDown, lv, st,
+                  [&] { launch_sweep_down<R>(s, start, count, fv, st); });
+    }
+  }
 }
 
 template <typename R>
@@ -1016,7 +1036,27 @@ int iegpu_id_stats(iegpu_ctx* c, int* n_sketched, int* n_fallback, double* sketc
   return IEGPU_OK;
 }
 
+namespace {
+int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, double* fx);
+}
+
 int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
+  return factor_impl(c, id_tol, precision, nullptr, nullptr);
+}
+
+int iegpu_factor_solve_device(iegpu_ctx* c, double id_tol, int precision, const double* d_b,
+                              double* d_x) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!d_b || !d_x) return fail(c, IEGPU_ERR_INVALID, "null vector");
+  if (d_b == d_x)  // a workspace retry restarts from b
+    return fail(c, IEGPU_ERR_INVALID, "b and x must be distinct buffers");
+  if (c->hole_dim > 0)
+    return fail(c, IEGPU_ERR_INVALID, "with holes, factor and solve separately");
+  return factor_impl(c, id_tol, precision, d_b, d_x);
+}
+
+namespace {
+int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, double* fx) {
   if (!c) return IEGPU_ERR_INVALID;
   if (!c->have_tree) return fail(c, IEGPU_ERR_INVALID, "build_tree first");
   if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
@@ -1063,12 +1103,13 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
         if (!c->fgraph || std::memcmp(&cur, &c->fgraph_state, sizeof(DevState)) != 0 ||
             std::memcmp(&c->caps, &c->fgraph_caps, sizeof(LaunchCaps)) != 0 ||
             c->overlap != c->fgraph_overlap ||
-            c->fgraph_tol != id_tol || c->fgraph_prec != precision) {
+            c->fgraph_tol != id_tol || c->fgraph_prec != precision || c->fgraph_fb != fb ||
+            c->fgraph_fx != fx) {
           if (c->fgraph) HIP_TRY(hipGraphExecDestroy(c->fgraph));
           c->fgraph = nullptr;
           HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-          if (precision == IEGPU_FP32) enqueue_factor<float>(c, id_tol);
-          else enqueue_factor<double>(c, id_tol);
+          if (precision == IEGPU_FP32) enqueue_factor<float>(c, id_tol, fb, fx);
+          else enqueue_factor<double>(c, id_tol, fb, fx);
           hipGraph_t gr = nullptr;
           HIP_TRY(hipStreamEndCapture(c->stream, &gr));
           const hipError_t ie = hipGraphInstantiate(&c->fgraph, gr, nullptr, nullptr, 0);
@@ -1076,15 +1117,17 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
           HIP_TRY(ie);
           c->fgraph_state = cur;
           c->fgraph_caps = c->caps;
+          c->fgraph_fb = fb;
+          c->fgraph_fx = fx;
           c->fgraph_overlap = c->overlap;
           c->fgraph_tol = id_tol;
           c->fgraph_prec = precision;
         }
         HIP_TRY(hipGraphLaunch(c->fgraph, c->stream));
       } else if (precision == IEGPU_FP32) {
-        enqueue_factor<float>(c, id_tol);
+        enqueue_factor<float>(c, id_tol, fb, fx);
       } else {
-        enqueue_factor<double>(c, id_tol);
+        enqueue_factor<double>(c, id_tol, fb, fx);
       }
       HIP_TRY(hipGetLastError());
       int h_err[4] = {0, 0, 0, 0};
@@ -1152,6 +1195,8 @@ int iegpu_factor(iegpu_ctx* c, double id_tol, int precision) {
   }
   return IEGPU_OK;
 }
+
+}  // namespace
 
 int iegpu_solve_device(iegpu_ctx* c, const double* d_b, double* d_mu,
                        double* d_alpha, void* stream) {

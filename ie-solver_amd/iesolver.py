@@ -85,6 +85,7 @@ def load():
     L.iegpu_factor.argtypes = [vp, ctypes.c_double, ctypes.c_int]
     L.iegpu_solve.argtypes = [vp, _dp, _dp, _dp]
     L.iegpu_solve_device.argtypes = [vp, vp, vp, vp, vp]
+    L.iegpu_factor_solve_device.argtypes = [vp, ctypes.c_double, ctypes.c_int, vp, vp]
     L.iegpu_tree_info.argtypes = [vp, _ip, _ip, _ip, _ip]
     L.iegpu_box_info.argtypes = [vp, _ip, _ip, _ip, _ip, _ip, _ip, _ip]
     L.iegpu_box_list.argtypes = [vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int]
@@ -340,6 +341,11 @@ class RSSolver:
         self._check(self.L.iegpu_evaluate(self.h, nt, _pd(tg), None if mask is None else _pi(mask),
                                           _pd(mu), None if al is None else _pd(al), _pd(out)))
         return out
+
+    def factor_solve_device(self, id_tol, precision, b_ptr, x_ptr):
+        """Factor and solve one right-hand side (device pointers, distinct);
+        each level's upward sweep runs beside the factorization."""
+        self._check(self.L.iegpu_factor_solve_device(self.h, id_tol, precision, b_ptr, x_ptr))
 
     def solve_device(self, b_ptr, mu_ptr, alpha_ptr=None, stream=None):
         self._check(self.L.iegpu_solve_device(self.h, b_ptr, mu_ptr, alpha_ptr, stream))

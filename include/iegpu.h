@@ -93,6 +93,17 @@ int iegpu_set_holes(iegpu_ctx* ctx, int n_holes, const double* centers,
  * (skel_factorization.cpp:140-310). */
 int iegpu_factor(iegpu_ctx* ctx, double id_tol, int precision);
 
+/* iegpu_factor followed by the solve x = A^{-1} b on device buffers (ndof
+ * doubles each, distinct), ordered on the context's stream: each level's
+ * upward sweep runs right after that level's decouple, beside the next
+ * levels' work, and the root solve and downward sweeps after the root.
+ * The stored factorization is the same as iegpu_factor's; x equals
+ * iegpu_factor + iegpu_solve_device to rounding.  Not with holes.  No
+ * reference counterpart (bie_time_trial times skeletonize + solve,
+ * ie_solver/linear_solve.cpp:156-185). */
+int iegpu_factor_solve_device(iegpu_ctx* ctx, double id_tol, int precision,
+                              const double* d_b, double* d_x);
+
 /* Solve with the stored factorization: mu (ndof) from b (ndof); with holes
  * alpha (hole_dim) is also returned (must be non-NULL).  Host pointers.
  * Replaces linear_solve, ie_solver/linear_solve.h:33-35 (solve /

@@ -4,6 +4,9 @@ program's own eager work (solves, operator applies, synchronizations)
 between the replays, for the plain and the randomized forms."""
 import numpy as np
 import pytest
+# torch's HIP runtime has to initialize before this library loads the
+# system one (as in bench.py), so it is imported when the module is.
+import torch
 
 import geometry as G
 import iesolver
@@ -76,3 +79,48 @@ def test_overlap_matches_single_stream(case):
         assert np.array_equal(m0, m1), (case, graphs)
         if a0 is not None:
             assert np.array_equal(a0, a1)
+
+
+@pytest.mark.parametrize("case", ["plain", "randomized", "fp32", "batch"])
+def test_fused_factor_solve_matches_separate(case):
+    # iegpu_factor_solve_device runs each level's upward sweep beside the
+    # factorization: the same kernels on the same data, so x is
+    # bit-identical to iegpu_factor + iegpu_solve_device, eager and replayed.
+    prec, tol = iesolver.FP64, 1e-10
+    s = iesolver.RSSolver(0)
+    if case == "batch":
+        geoms, _ = G.star_batch(4, seed=11, nmin=1024, nmax=8192)
+        s.set_geometry_batch(geoms)
+        f = G.batch_data(geoms, seed=11)
+    else:
+        g = G.wavy(32768, seed=12)
+        s.set_geometry(g)
+        f = G.harmonic_data(g["points"], *G.exterior_sources(12))
+    s.build_tree(64)
+    if case == "randomized":
+        s.set_id_method(iesolver.ID_RANDOMIZED, 32)
+    if case == "fp32":
+        prec, tol = iesolver.FP32, 1e-6
+    dev = torch.device("cuda:0")
+    fd = torch.from_numpy(f).to(dev)
+    x0 = torch.empty_like(fd)
+    # the library works on its own non-blocking stream: torch's fills must
+    # be finished before a buffer is handed over
+    torch.cuda.synchronize()
+    s.factor(tol, prec)
+    s.solve_device(fd.data_ptr(), x0.data_ptr())
+    torch.cuda.synchronize()
+    ref = x0.cpu().numpy()
+    for graphs in (False, True, True):
+        s.set_graphs(graphs)
+        x1 = torch.zeros_like(fd)
+        torch.cuda.synchronize()
+        s.factor_solve_device(tol, prec, fd.data_ptr(), x1.data_ptr())
+        torch.cuda.synchronize()
+        assert np.array_equal(x1.cpu().numpy(), ref), (case, graphs)
+        # the stored factorization is the usual one
+        x2 = torch.zeros_like(fd)
+        torch.cuda.synchronize()
+        s.solve_device(fd.data_ptr(), x2.data_ptr())
+        torch.cuda.synchronize()
+        assert np.array_equal(x2.cpu().numpy(), ref)
