@@ -178,6 +178,7 @@ struct iegpu_ctx {
   // holes
   int n_holes = 0;
   int hole_dim = 0;
+  std::vector<double> h_hole_c, h_hole_r;
   DevBuf<double> d_U, d_PsiT;  // ndof x hole_dim each
   DevBuf<double> d_mU, d_mPsiT, d_DinvC, d_gram, d_partial, d_Smat, d_tail,
       d_ybuf, d_alpha;
@@ -225,6 +226,18 @@ struct iegpu_ctx {
 
   double t_factor_ms = 0, t_solve_ms = 0, t_root_ms = 0;
   double fac_bytes = 0;
+
+  // geometry update (iegpu_update_geometry): boxes to re-factor per slot,
+  // and whether the next factorization may keep the other boxes' factors
+  std::vector<int> h_dirty;
+  DevBuf<int> s_dirty;
+  bool update_pending = false;  // dirty flags valid for the next factor
+  bool inc_run = false;         // the factorization being enqueued is incremental
+  bool last_incremental = false;
+  int n_dirty = 0;
+  double last_tol = 0;  // id_tol and precision of the stored factorization
+  int last_prec = -1;
+  int64_t kept_top = 0;  // factor-pool elements in use after the last factor
 
   int ndof() const { return n_points * sol_dim; }
   size_t elem() const { return precision == IEGPU_FP32 ? 4 : 8; }
@@ -305,6 +318,7 @@ struct iegpu_ctx {
     s.rank_model = d_model.p;
     s.rid_margin = rid_margin;
     s.feat = record_features ? d_feat.p : nullptr;
+    s.dirty = inc_run ? s_dirty.p : nullptr;
     return s;
   }
 };
@@ -388,6 +402,24 @@ void build_border(iegpu_ctx* c, const double* centers, const double* radii,
   }
 }
 
+// Grows a device buffer to `bytes`, keeping its contents.
+void grow_keep(DevBuf<unsigned char>& b, size_t bytes, hipStream_t st) {
+  if (bytes <= b.n) return;
+  unsigned char* p = nullptr;
+  HIP_TRY(hipMalloc(&p, bytes));
+  if (b.p && b.n) {
+    const hipError_t e = hipMemcpyAsync(p, b.p, b.n, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) HIP_TRY(hipStreamSynchronize(st));
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      throw HipError{e, "hipMemcpyAsync"};
+    }
+  }
+  b.release();
+  b.p = p;
+  b.n = bytes;
+}
+
 void alloc_state(iegpu_ctx* c) {
   const int nb = c->tree.n_boxes;
   const int64_t nl = (int64_t)nb * c->ncap;
@@ -454,8 +486,11 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
   if (fx && fb != fx) launch_copy(fx, fb, c->ndof(), st);
   const bool split = c->overlap && !c->prof.on && c->stream2 != nullptr;
   hipStream_t st2 = split ? c->stream2 : st;
-  if (split && c->evs.size() < (size_t)T.n_levels + 2) {
-    for (size_t i = c->evs.size(); i < (size_t)T.n_levels + 2; i++) {
+  // evs[0]: start; evs[1 + lv]: level lv's ID done; evs[n_levels + 1]: the
+  // decouple chain done; evs[n_levels + 2 + lv]: level lv's decouple done.
+  const size_t n_ev = 2 * (size_t)T.n_levels + 2;
+  if (split && c->evs.size() < n_ev) {
+    for (size_t i = c->evs.size(); i < n_ev; i++) {
       hipEvent_t e;
       HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       c->evs.push_back(e);
@@ -469,6 +504,11 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
   for (int lv = T.n_levels - 1; lv >= 1; lv--) {
     const int start = T.level_start[lv];
     const int count = T.level_start[lv + 1] - start;
+    // The ID scratch pool has one half per level parity, and the decouple
+    // of a box above kSmallBox DOFs may keep its operands in its ID slot:
+    // level lv's ID waits until level lv + 2 (same half) is decoupled.
+    if (split && lv + 2 < T.n_levels)
+      HIP_TRY(hipStreamWaitEvent(st, c->evs[T.n_levels + 2 + lv + 2], 0));
     for (int stage = 0; stage < kStageDecouple; stage++)
       c->prof.run(stage, lv, st, [&] {
         launch_factor_stage<R>(stage, g, t, s, lv, start, count, tol, c->caps, st);
@@ -480,6 +520,7 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
     c->prof.run(kStageDecouple, lv, st2, [&] {
       launch_factor_stage<R>(kStageDecouple, g, t, s, lv, start, count, tol, c->caps, st2);
     });
+    if (split) HIP_TRY(hipEventRecord(c->evs[T.n_levels + 2 + lv], st2));
     if (fx)
       c->prof.run(kProfSweepUp, lv, st2,
                   [&] { launch_sweep_up<R>(s, start, count, fv, false, true, st2); });
@@ -660,6 +701,7 @@ int set_geometry(iegpu_ctx* c, int pde, int n_problems, const int* point_off,
     }
     c->have_tree = false;
     c->factored = false;
+    c->update_pending = false;
     c->n_holes = 0;
     c->hole_dim = 0;
   } catch (const HipError& e) {
@@ -692,6 +734,103 @@ int iegpu_set_geometry_batch(iegpu_ctx* c, int pde, int n_problems,
                       weights);
 }
 
+namespace {
+void upload_tree(iegpu_ctx* c);
+void upload_border(iegpu_ctx* c);
+std::vector<int> mark_update(const HostTree& T, const std::vector<int>& leaves);
+}  // namespace
+
+int iegpu_update_geometry(iegpu_ctx* c, int n_points, const double* points,
+                          const double* normals, const double* curvatures,
+                          const double* weights, int* tree_kept) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (tree_kept) *tree_kept = 0;
+  if (!c->have_tree) return fail(c, IEGPU_ERR_INVALID, "build_tree first");
+  if (n_points != c->n_points)
+    return fail(c, IEGPU_ERR_INVALID, "an update keeps the number of points");
+  if (!points || !normals || !curvatures || !weights)
+    return fail(c, IEGPU_ERR_INVALID, "null array");
+  for (int i = 0; i < 2 * n_points; i++)
+    if (!std::isfinite(points[i]) || !std::isfinite(normals[i]))
+      return fail(c, IEGPU_ERR_INVALID, "non-finite point or normal");
+  for (int i = 0; i < n_points; i++)
+    if (!std::isfinite(curvatures[i]) || !std::isfinite(weights[i]))
+      return fail(c, IEGPU_ERR_INVALID, "non-finite curvature or weight");
+  try {
+    std::vector<int> changed;
+    for (int i = 0; i < n_points; i++)
+      if (points[2 * i] != c->h_pts[2 * i] || points[2 * i + 1] != c->h_pts[2 * i + 1] ||
+          normals[2 * i] != c->h_nrm[2 * i] || normals[2 * i + 1] != c->h_nrm[2 * i + 1] ||
+          curvatures[i] != c->h_curv[i] || weights[i] != c->h_wts[i])
+        changed.push_back(i);
+    // The moved points go into the kept cells; if that fails the tree is
+    // rebuilt for the new nodes.
+    HostTree nt = c->tree;
+    std::vector<int> was, now;
+    const bool keep = nt.reinsert(points, &was, &now);
+    if (!keep) {
+      nt = HostTree();
+      nt.build_forest(points, c->h_poff.data(), c->n_prob, c->sol_dim, c->tree.leaf_cap);
+      if (nt.n_levels > kMaxLevels)
+        return fail(c, IEGPU_ERR_INVALID, "quadtree deeper than 64 levels");
+    }
+    const bool had_factor = c->factored;
+    c->h_pts.assign(points, points + 2 * n_points);
+    c->h_nrm.assign(normals, normals + 2 * n_points);
+    c->h_curv.assign(curvatures, curvatures + n_points);
+    c->h_wts.assign(weights, weights + n_points);
+    c->factored = false;
+    c->update_pending = false;
+    c->tree = std::move(nt);
+    if (keep) {
+      std::vector<int> leaves;
+      for (int p : changed) {
+        leaves.push_back(was[p]);
+        leaves.push_back(now[p]);
+      }
+      c->h_dirty = mark_update(c->tree, leaves);
+    } else {
+      c->h_dirty.assign(c->tree.n_boxes, 1);
+    }
+    c->n_dirty = 0;
+    for (int v : c->h_dirty) c->n_dirty += v;
+    if (c->device >= 0) {
+      HIP_TRY(hipSetDevice(c->device));
+      if (c->fgraph) {
+        HIP_TRY(hipGraphExecDestroy(c->fgraph));
+        c->fgraph = nullptr;
+      }
+      c->d_pts.upload(c->h_pts, c->stream);
+      c->d_nrm.upload(c->h_nrm, c->stream);
+      c->d_curv.upload(c->h_curv, c->stream);
+      c->d_wts.upload(c->h_wts, c->stream);
+      if (keep) c->s_dirty.upload(c->h_dirty, c->stream);
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      upload_tree(c);
+      if (c->n_holes > 0) upload_border(c);
+      c->update_pending = keep && had_factor;
+    }
+    if (tree_kept) *tree_kept = keep ? 1 : 0;
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  } catch (const std::exception& e) {
+    return fail(c, IEGPU_ERR_INVALID, e.what());
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_update_info(const iegpu_ctx* c, int* n_dirty, int* last_incremental, int* dirty) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (n_dirty) *n_dirty = c->n_dirty;
+  if (last_incremental) *last_incremental = c->last_incremental ? 1 : 0;
+  if (dirty && c->have_tree) {
+    const HostTree& T = c->tree;
+    for (int id = 0; id < T.n_boxes; id++)
+      dirty[id] = (int)c->h_dirty.size() == T.n_boxes ? c->h_dirty[T.slot_of_id[id]] : 1;
+  }
+  return IEGPU_OK;
+}
+
 int iegpu_build_tree(iegpu_ctx* c, int leaf_cap) {
   if (!c) return IEGPU_ERR_INVALID;
   if (c->n_points == 0) return fail(c, IEGPU_ERR_INVALID, "set_geometry first");
@@ -704,7 +843,19 @@ int iegpu_build_tree(iegpu_ctx* c, int leaf_cap) {
       return fail(c, IEGPU_ERR_INVALID, "quadtree deeper than 64 levels");
     c->have_tree = true;
     c->factored = false;
+    c->update_pending = false;
     if (c->device < 0) return IEGPU_OK;
+    upload_tree(c);
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  } catch (const std::exception& e) {
+    return fail(c, IEGPU_ERR_INVALID, e.what());
+  }
+  return IEGPU_OK;
+}
+
+namespace {
+void upload_tree(iegpu_ctx* c) {
     HIP_TRY(hipSetDevice(c->device));
     const HostTree& T = c->tree;
     hipStream_t st = c->stream;
@@ -726,15 +877,36 @@ int iegpu_build_tree(iegpu_ctx* c, int leaf_cap) {
       c->t_src.upload(T.src, st);
     }
     HIP_TRY(hipStreamSynchronize(st));
-    c->have_tree = true;
-    c->factored = false;
-  } catch (const HipError& e) {
-    return from_hip(c, e);
-  } catch (const std::exception& e) {
-    return fail(c, IEGPU_ERR_INVALID, e.what());
-  }
-  return IEGPU_OK;
 }
+
+// Boxes to re-factor after a geometry update, given the leaves that lost,
+// gained or hold a changed point, following QuadTree::perturb
+// (quadtree.cpp:680-697: the leaf of every added point, and for deleted
+// points the leaf they leave, is passed to mark_neighbors_and_parents) and
+// mark_neighbors_and_parents (quadtree.cpp:435-461: the box, its
+// neighbours, then the same for its parent, up to the root).  Then every
+// marked box's parent is marked too (a box re-factored under a kept parent
+// would leave the parent's lists stale; the reference's rule gives this by
+// itself on the tested boundaries), and every root is re-factored.
+std::vector<int> mark_update(const HostTree& T, const std::vector<int>& leaves) {
+  std::vector<int> dirty(T.n_boxes, 0);
+  std::vector<char> seen(T.n_boxes, 0);
+  for (int s : leaves) {
+    // one walk per leaf: the walk from a leaf depends on the leaf alone
+    if (s < 0 || seen[s]) continue;
+    seen[s] = 1;
+    for (; s >= 0; s = T.parent[s]) {
+      dirty[s] = 1;
+      for (int t = 0; t < T.nbr_cnt[s]; t++) dirty[T.nbr[T.nbr_off[s] + t]] = 1;
+    }
+  }
+  for (int lv = T.n_levels - 1; lv >= 1; lv--)
+    for (int s = T.level_start[lv]; s < T.level_start[lv + 1]; s++)
+      if (dirty[s]) dirty[T.parent[s]] = 1;
+  for (int r = 0; r < T.n_roots; r++) dirty[r] = 1;
+  return dirty;
+}
+}  // namespace
 
 int iegpu_set_holes(iegpu_ctx* c, int n_holes, const double* centers,
                     const double* radii) {
@@ -750,10 +922,27 @@ int iegpu_set_holes(iegpu_ctx* c, int n_holes, const double* centers,
     HIP_TRY(hipSetDevice(c->device));
     c->n_holes = n_holes;
     c->factored = false;
+    c->update_pending = false;
     if (n_holes == 0) {
       c->hole_dim = 0;
       return IEGPU_OK;
     }
+    c->h_hole_c.assign(centers, centers + 2 * n_holes);
+    c->h_hole_r.assign(radii, radii + n_holes);
+    upload_border(c);
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+namespace {
+// Border blocks of the holes' extra unknowns (they depend on the boundary
+// geometry, so a geometry update rebuilds them).
+void upload_border(iegpu_ctx* c) {
+    const double* centers = c->h_hole_c.data();
+    const double* radii = c->h_hole_r.data();
+    const int n_holes = c->n_holes;
     std::vector<double> U, PsiT;
     build_border(c, centers, radii, &U, &PsiT);
     const int64_t nd = c->ndof();
@@ -771,14 +960,12 @@ int iegpu_set_holes(iegpu_ctx* c, int n_holes, const double* centers,
     c->d_mask.alloc(nd);
     c->d_hole_c.upload(centers, 2 * (size_t)n_holes, st);
     HIP_TRY(hipStreamSynchronize(st));
-  } catch (const HipError& e) {
-    return from_hip(c, e);
-  }
-  return IEGPU_OK;
 }
+}  // namespace
 
 int iegpu_set_id_method(iegpu_ctx* c, int method, int sketch_rows) {
   if (!c) return IEGPU_ERR_INVALID;
+  c->update_pending = false;  // the kept factors would mix two ID methods
   if (method == IEGPU_ID_QRCP) {
     c->rid_mode = 0;
     c->rid_width = 0;
@@ -801,6 +988,7 @@ int iegpu_set_id_method(iegpu_ctx* c, int method, int sketch_rows) {
 
 int iegpu_set_keep_operator(iegpu_ctx* c, int on) {
   if (!c) return IEGPU_ERR_INVALID;
+  c->update_pending = false;  // kept boxes would lack their forward-apply blocks
   c->keep_operator = on != 0;
   c->factored = false;
   return IEGPU_OK;
@@ -988,6 +1176,7 @@ int iegpu_rank_model_grad(iegpu_ctx* c, const float* model, const float* X, cons
 
 int iegpu_set_rank_model(iegpu_ctx* c, const float* model, int margin) {
   if (!c) return IEGPU_ERR_INVALID;
+  c->update_pending = false;  // the kept factors would mix two ID methods
   if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
   if (!model || margin < 0 || margin > 16) return fail(c, IEGPU_ERR_INVALID, "bad rank model");
   for (int i = 0; i < kMlpLen; i++)
@@ -1084,6 +1273,10 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
       c->d_omega.alloc(no);
       launch_gaussian(c->d_omega.p, no, 0x5EEDull, c->stream);
     }
+    // After iegpu_update_geometry on the same tree, only the marked boxes
+    // are re-factored (same tolerance and precision as the kept factors).
+    c->inc_run = c->update_pending && c->last_tol == id_tol && c->last_prec == precision;
+    c->update_pending = false;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1096,8 +1289,11 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
         c->d_Spiv.alloc(ns_max);
         c->d_ybuf.alloc(ns_max);
       }
+      if (c->inc_run)  // new slots go past the kept ones (a failed attempt moved the top)
+        HIP_TRY(hipMemcpyAsync(c->s_fac_top.p, &c->kept_top, sizeof(int64_t),
+                               hipMemcpyHostToDevice, c->stream));
       HIP_TRY(hipEventRecord(e0, c->stream));
-      if (c->use_graphs && !c->prof.on && c->hole_dim == 0) {
+      if (c->use_graphs && !c->prof.on && c->hole_dim == 0 && !c->inc_run) {
         // Capture the launch sequence once per state; replay it after.
         const DevState cur = c->dstate();
         if (!c->fgraph || std::memcmp(&cur, &c->fgraph_state, sizeof(DevState)) != 0 ||
@@ -1147,7 +1343,16 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
         errbits = h_err[0];
         break;
       }
-      if (errbits & kErrBoxCap) {
+      // A larger box capacity changes every per-box list's stride, so the
+      // kept boxes are lost; the ID and row pools are scratch, and the
+      // factor pool grows with its contents.  A pool overflow leaves the
+      // level's boxes uncompressed, which can overflow their parents' box
+      // capacity as well: in an incremental run that bit counts only when
+      // it comes alone.
+      bool box_cap = (errbits & kErrBoxCap) != 0;
+      if (c->inc_run && (errbits & (kErrIdPool | kErrRowPool | kErrFacPool))) box_cap = false;
+      if (box_cap) c->inc_run = false;
+      if (box_cap) {
         if (c->ncap >= kMaxBoxDofs) {
           (void)hipEventDestroy(e0);
           (void)hipEventDestroy(e1);
@@ -1160,19 +1365,29 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
         c->id_cap = std::max<int64_t>(2 * c->id_cap, ((int64_t)h_err[2] << 10) * 5 / 4);
       if (errbits & kErrRowPool)
         c->row_cap = std::max<int64_t>(2 * c->row_cap, ((int64_t)h_err[3] << 10) * 5 / 4);
-      if (errbits & kErrFacPool) c->fac_cap *= 2;
+      if (errbits & kErrFacPool) {
+        c->fac_cap *= 2;
+        if (c->inc_run) {
+          grow_keep(c->s_fac_pool, (size_t)c->fac_cap * c->elem(), c->stream);
+          if (c->keep_operator)
+            grow_keep(c->s_fwd_pool, (size_t)c->fac_cap * c->elem(), c->stream);
+        }
+      }
       if (attempt == 7) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         return fail(c, IEGPU_ERR_CAPACITY, "workspace growth did not converge");
       }
     }
+    c->last_incremental = c->inc_run;
+    c->inc_run = false;
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     c->t_factor_ms = ms;
     int64_t top = 0;
     HIP_TRY(hipMemcpy(&top, c->s_fac_top.p, sizeof(top), hipMemcpyDeviceToHost));
     c->fac_bytes = (double)top * c->elem();
+    c->kept_top = top;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (errbits & kErrSingular) {
@@ -1189,8 +1404,11 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
       }
     }
     c->factored = true;
+    c->last_tol = id_tol;
+    c->last_prec = precision;
   } catch (const HipError& e) {
     c->factored = false;
+    c->inc_run = false;
     return from_hip(c, e);
   }
   return IEGPU_OK;

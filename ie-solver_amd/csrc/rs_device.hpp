@@ -174,7 +174,16 @@ struct DevState {
   const float* rank_model;  // kMlpLen floats (rid_mode 2)
   int rid_margin;           // ranks added to the prediction before sizing
   float* feat;              // kFeat per box, written by k_rows when non-null
+  // Incremental re-factorization after a geometry update (null: every box).
+  // A box with dirty[b] == 0 keeps its lists, rank and factor slot from the
+  // previous factorization; the per-level kernels skip it and k_scan gives
+  // it no scratch, no list entry and no new factor slot.
+  const int* dirty;
 };
+
+__device__ inline bool box_clean(const DevState& s, int b) {
+  return s.dirty != nullptr && s.dirty[b] == 0;
+}
 
 __device__ inline int* list_slot(const DevState& s, int level, int L) {
   return s.lcnt + 2 * (level * kNumLists + L);

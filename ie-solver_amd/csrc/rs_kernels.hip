@@ -54,7 +54,8 @@ __global__ void k_reset(DevState s) {
     s.err[1] = 0;
     s.err[2] = 0;
     s.err[3] = 0;
-    *s.fac_top = 0;
+    // an incremental run appends new factor slots past the kept ones
+    if (!s.dirty) *s.fac_top = 0;
   }
   if (threadIdx.x < kMaxLevels) s.njobs[threadIdx.x] = 0;
   for (int i = threadIdx.x; i < 2 * kMaxLevels * kNumLists; i += blockDim.x) s.lcnt[i] = 0;
@@ -66,6 +67,7 @@ __global__ void k_reset(DevState s) {
 __global__ __launch_bounds__(64) void k_active(DevTree t, DevState s,
                                                int start) {
   const int b = start + blockIdx.x;
+  if (box_clean(s, b)) return;
   const int lane = threadIdx.x;
   int* dst = s.act_list + (int64_t)b * s.ncap;
   int n = 0;
@@ -188,6 +190,7 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
                                              int start, int level, double tol) {
   __shared__ float hb[kHid];
   const int b = start + blockIdx.x;
+  if (box_clean(s, b)) return;
   const int n = s.act_cnt[b];
   int m = 0;
   if (n >= kMinCompress) {
@@ -278,6 +281,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
   int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
+    if (box_clean(s, b)) continue;
     const int kind = s.qkind[b];
     own[0] += s.id_sz[b];
     own[1] += s.row_sz[b];
@@ -329,6 +333,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
   int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
+    if (box_clean(s, b)) continue;
     const int kind = s.qkind[b];
     // Scratch of odd and even levels in separate halves: a level's ID runs
     // while the level below it is still being decoupled.
@@ -1877,7 +1882,7 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
 
   const int b = start + blockIdx.x;
   const int tid = threadIdx.x;
-  if (!pools_ok(s)) return;
+  if (!pools_ok(s) || box_clean(s, b)) return;
   const int n = s.act_cnt[b], k = s.rank[b];
   if (n <= NMIN || n > NMAX || k == 0) return;  // the other class's or k_decouple's box
   load_child_blocks<R>(t, s, b, n, chS, chN, pch, ploc, NT);
@@ -1956,7 +1961,7 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
 
   const int b = start + blockIdx.x;
   const int tid = threadIdx.x;
-  if (!pools_ok(s)) return;
+  if (!pools_ok(s) || box_clean(s, b)) return;
   const int n = s.act_cnt[b];
   if (n == 0) {
     if (tid == 0) s.upd_kind[b] = 0;

@@ -207,6 +207,8 @@ void HostTree::build(const double* points, int np, int sd, int cap) {
   side.resize(n_boxes);
   cx.resize(n_boxes);
   cy.resize(n_boxes);
+  mid_x.resize(n_boxes);
+  mid_y.resize(n_boxes);
   orig_off.resize(n_boxes);
   orig_cnt.resize(n_boxes);
   orig_dofs.clear();
@@ -225,6 +227,8 @@ void HostTree::build(const double* points, int np, int sd, int cap) {
     // box centre as in ie_solver/kernel/kernel.cpp:296-297
     cx[s] = b.c[0] + b.side / 2.0;
     cy[s] = b.c[1] + b.side / 2.0;
+    mid_x[s] = ((b.c[6] - b.c[0]) / 2.0) + b.c[0];
+    mid_y[s] = ((b.c[3] - b.c[1]) / 2.0) + b.c[1];
     orig_off[s] = (int)orig_dofs.size();
     orig_cnt[s] = (int)b.orig.size();
     orig_dofs.insert(orig_dofs.end(), b.orig.begin(), b.orig.end());
@@ -300,6 +304,8 @@ void HostTree::build_forest(const double* points, const int* poff, int np_, int 
   side.assign(ns, 0);
   cx.assign(ns, 0);
   cy.assign(ns, 0);
+  mid_x.assign(ns, 0);
+  mid_y.assign(ns, 0);
   orig_off.assign(ns, 0);
   orig_cnt.assign(ns, 0);
   nbr_off.assign(ns, 0);
@@ -326,6 +332,8 @@ void HostTree::build_forest(const double* points, const int* poff, int np_, int 
       side[s] = S.side[t];
       cx[s] = S.cx[t];
       cy[s] = S.cy[t];
+      mid_x[s] = S.mid_x[t];
+      mid_y[s] = S.mid_y[t];
     }
     id0 += S.n_boxes;
   }
@@ -352,6 +360,51 @@ void HostTree::build_forest(const double* points, const int* poff, int np_, int 
     src_cnt[s] = S.src_cnt[t];
     for (int u = 0; u < S.src_cnt[t]; u++) src.push_back(f[S.src[S.src_off[t] + u]]);
   }
+}
+
+bool HostTree::reinsert(const double* points, std::vector<int>* old_leaf,
+                        std::vector<int>* new_leaf) {
+  std::vector<int> was(n_points, -1), now(n_points, -1);
+  for (int s = 0; s < n_boxes; s++)
+    if (leaf[s])
+      for (int i = 0; i < orig_cnt[s]; i += sol_dim)
+        was[orig_dofs[orig_off[s] + i] / sol_dim] = s;
+  std::vector<int> fill(n_boxes, 0);
+  for (int r = 0; r < n_roots; r++)
+    for (int p = point_off[r]; p < point_off[r + 1]; p++) {
+      const double x = points[2 * p], y = points[2 * p + 1];
+      if (!(x >= bl_x[r] && x <= bl_x[r] + side[r] && y >= bl_y[r] && y <= bl_y[r] + side[r]))
+        return false;
+      int s = r;
+      while (!leaf[s]) {
+        const int q = (x < mid_x[s]) ? ((y < mid_y[s]) ? 0 : 1) : ((y < mid_y[s]) ? 3 : 2);
+        s = child[4 * (size_t)s + q];
+      }
+      now[p] = s;
+      fill[s] += sol_dim;
+      if (fill[s] > 2 * leaf_cap) return false;
+    }
+  // Every box lists the DOFs of the points below it, in point order (the
+  // order the build's insertion gives).
+  std::vector<int> cnt(n_boxes, 0);
+  for (int p = 0; p < n_points; p++)
+    for (int s = now[p]; s >= 0; s = parent[s]) cnt[s] += sol_dim;
+  std::vector<int> off(n_boxes, 0);
+  int tot = 0;
+  for (int s = 0; s < n_boxes; s++) {
+    off[s] = tot;
+    tot += cnt[s];
+  }
+  std::vector<int> dofs(tot), pos(off);
+  for (int p = 0; p < n_points; p++)
+    for (int s = now[p]; s >= 0; s = parent[s])
+      for (int d = 0; d < sol_dim; d++) dofs[pos[s]++] = sol_dim * p + d;
+  orig_off = off;
+  orig_cnt = cnt;
+  orig_dofs.swap(dofs);
+  if (old_leaf) old_leaf->swap(was);
+  if (new_leaf) new_leaf->swap(now);
+  return true;
 }
 
 }  // namespace iegpu

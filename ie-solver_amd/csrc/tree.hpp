@@ -37,6 +37,9 @@ struct HostTree {
   std::vector<int> level, parent, leaf;
   std::vector<int> child;  // 4 per slot: bl, tl, tr, br (-1 if none)
   std::vector<double> bl_x, bl_y, side, cx, cy;
+  // The split point of every box, computed as the build does (a point goes
+  // to the left / bottom child when below it).
+  std::vector<double> mid_x, mid_y;
 
   // Original DOFs of every box (CSR; only leaves' lists are read on device).
   std::vector<int> orig_off, orig_cnt, orig_dofs;
@@ -56,6 +59,16 @@ struct HostTree {
   // after problem.
   void build_forest(const double* points, const int* point_off, int n_problems,
                     int sol_dim, int leaf_cap);
+  // Puts moved points into the kept cells (QuadTree::perturb's descent of
+  // added points, ie_solver/quadtree/quadtree.cpp:680-697): every point
+  // descends from its problem's root to a leaf, and the original-DOF lists
+  // are rebuilt.  old_leaf / new_leaf get each point's leaf slot before and
+  // after.  A leaf may grow to 2 leaf_cap DOFs (the reference splits a
+  // leaf past leaf_cap).  Returns false, leaving the tree unchanged, if a
+  // point lies outside its root cell or a leaf would exceed that; the
+  // caller then rebuilds the tree.
+  bool reinsert(const double* points, std::vector<int>* old_leaf,
+                std::vector<int>* new_leaf);
 };
 
 }  // namespace iegpu

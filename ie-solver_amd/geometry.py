@@ -58,6 +58,26 @@ def star(n, center=(0.5, 0.5), r0=0.3, amp=0.2, k=7):
     return dict(points=p, normals=nm, curvatures=cv, weights=w)
 
 
+def bump_star(n, bump, t0, width=0.4, center=(0.5, 0.5), r0=0.3, amp=0.2, k=7):
+    """star() with r(t) raised by r0 * bump * (1 - u^2)^3 on the arc
+    |u| < 1, u = (t - t0) / width (t - t0 wrapped to [-pi, pi)): a local,
+    twice continuously differentiable deformation.  Nodes off the arc are
+    bit-identical to star()'s."""
+    t = np.arange(n, dtype=np.float64) * (2 * math.pi / n)
+    r = r0 * (1 + amp * np.cos(k * t))
+    dr = -r0 * amp * k * np.sin(k * t)
+    ddr = -r0 * amp * k * k * np.cos(k * t)
+    u = (np.mod(t - t0 + math.pi, 2 * math.pi) - math.pi) / width
+    on = np.abs(u) < 1
+    uo = u[on]
+    g = 1 - uo * uo
+    r[on] += r0 * bump * g ** 3
+    dr[on] += r0 * bump * (-6 * uo * g * g / width)
+    ddr[on] += r0 * bump * (-6 * g * (1 - 5 * uo * uo) / (width * width))
+    p, nm, cv, w = _radial_curve(n, center, r, dr, ddr, t)
+    return dict(points=p, normals=nm, curvatures=cv, weights=w)
+
+
 def wavy_coefficients(seed, kmin=2, kmax=12):
     """a_k ~ N(0, 0.01/k) (variance 0.01/k), k = kmin..kmax."""
     rng = np.random.default_rng(seed)

@@ -45,6 +45,7 @@ EXPORTS = [
     "iegpu_rank_model_train", "iegpu_rank_model_grad", "iegpu_set_rank_model",
     "iegpu_set_geometry_batch", "iegpu_set_launch_caps", "iegpu_get_launch_caps",
     "iegpu_autotune", "iegpu_set_overlap", "iegpu_set_keep_operator", "iegpu_sparse_matvec",
+    "iegpu_factor_solve_device", "iegpu_update_geometry", "iegpu_update_info",
 ]
 NUM_LAUNCH_CAPS = 7
 RANK_FEATURES = 9
@@ -192,6 +193,32 @@ class RSSolver:
         self.ndof = int(off[-1]) * sd
         self.hole_dim = 0
         self.dof_offsets = off.astype(np.int64) * sd
+
+    def update_geometry(self, g):
+        """New node data for the same points (QuadTree::perturb): a geometry
+        dict, or a list of them for a batch.  Returns True when the tree is
+        unchanged; the next factor then re-factors only the marked boxes
+        (update_info)."""
+        geoms = g if isinstance(g, (list, tuple)) else [g]
+        arrs = [_d(np.concatenate([q["points"] for q in geoms])).reshape(-1),
+                _d(np.concatenate([q["normals"] for q in geoms])).reshape(-1),
+                _d(np.concatenate([q["curvatures"] for q in geoms])),
+                _d(np.concatenate([q["weights"] for q in geoms]))]
+        kept = ctypes.c_int(0)
+        self._check(self.L.iegpu_update_geometry(self.h, arrs[2].shape[0],
+                                                 *[_pd(a) for a in arrs], ctypes.byref(kept)))
+        self._arrays = arrs
+        return bool(kept.value)
+
+    def update_info(self):
+        """(boxes marked by the last update, whether the last factorization
+        re-factored only them, per-box-id marks)."""
+        nb = self.tree_info()[1]
+        n, inc = ctypes.c_int(0), ctypes.c_int(0)
+        marks = np.zeros(nb, np.int32)
+        self._check(self.L.iegpu_update_info(self.h, ctypes.byref(n), ctypes.byref(inc),
+                                             _pi(marks)))
+        return n.value, bool(inc.value), marks
 
     def build_tree(self, leaf_cap=64):
         self._check(self.L.iegpu_build_tree(self.h, leaf_cap))

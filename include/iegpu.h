@@ -74,6 +74,34 @@ int iegpu_set_geometry_batch(iegpu_ctx* ctx, int pde, int n_problems,
                              const double* normals, const double* curvatures,
                              const double* weights);
 
+/* Replaces QuadTree::perturb (ie_solver/quadtree/quadtree.cpp:520-801)
+ * followed by skeletonize's skip of compressed boxes
+ * (ie_solver/skel_factorization/skel_factorization.cpp:157): new node
+ * data for the same n_points (same order, same problems).  Every node
+ * descends through the kept cells to a leaf, as the reference inserts
+ * added points.  A node whose point, normal, curvature or weight changed
+ * marks its old and its new leaf; each marked leaf marks itself, its
+ * neighbours, and the same for every ancestor (mark_neighbors_and_parents,
+ * quadtree.cpp:435-461); parents of marked boxes and the roots are marked
+ * too.  If every node stays inside its root cell and no leaf exceeds twice
+ * the leaf capacity (*tree_kept = 1), the next iegpu_factor /
+ * iegpu_factor_solve_device with the tolerance and precision of the stored
+ * factorization re-factors only the marked boxes and keeps every other
+ * box's skeleton and factors.  Otherwise the tree is rebuilt for the new
+ * nodes (*tree_kept = 0; the reference splits and merges leaves instead)
+ * and the next factorization covers every box.  The stored factorization
+ * is invalid until then.  Holes are kept and their border blocks rebuilt. */
+int iegpu_update_geometry(iegpu_ctx* ctx, int n_points, const double* points,
+                          const double* normals, const double* curvatures,
+                          const double* weights, int* tree_kept);
+
+/* After iegpu_update_geometry: the number of marked boxes, whether the last
+ * factorization re-factored only them (1) or every box (0), and per box id
+ * the marks (dirty: n_boxes ints, may be null).  No reference counterpart
+ * (the reference reads QuadTreeNode::compressed). */
+int iegpu_update_info(const iegpu_ctx* ctx, int* n_dirty, int* last_incremental,
+                      int* dirty);
+
 /* Build the quadtree (leaf boxes hold at most leaf_cap DOFs) and upload it.
  * Replaces QuadTree::initialize_tree, ie_solver/quadtree/quadtree.h:114-117
  * (quadtree.cpp:39-180; cap = MAX_LEAF_DOFS, quadtree.h:10). */

@@ -398,3 +398,23 @@ def mlp_loss_grad(model, X, y):
     g = np.concatenate([(x.T @ d1).ravel(), d1.sum(0), (h1.T @ d2).ravel(), d2.sum(0),
                         h2.T @ dout, [dout.sum()]])
     return float(np.mean(e ** 2)), g
+
+
+def update_marks(parent, neighbors, leaves):
+    """Boxes marked for re-factorization after a geometry update, given the
+    leaves that gain, lose or hold a changed point (QuadTree::perturb,
+    quadtree.cpp:680-697 for additions, the deletion walk after it), each
+    passed to mark_neighbors_and_parents (quadtree.cpp:435-461): the box
+    and its neighbours are marked, then the same for its parent, up to the
+    root.  parent[b] is -1 at a root; neighbors[b] lists box ids.  Returns
+    0/1 per box id."""
+    marks = np.zeros(len(parent), np.int32)
+    for leaf in leaves:
+        node = leaf
+        while node >= 0:
+            marks[node] = 1
+            for nb in neighbors[node]:
+                marks[nb] = 1
+            node = parent[node]
+    return marks
+

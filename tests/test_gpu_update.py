@@ -1,0 +1,168 @@
+"""Incremental re-factorization after a geometry update on the GPU
+(QuadTree::perturb, quadtree.cpp:520-801, then skeletonize skipping the
+boxes still compressed, skel_factorization.cpp:157).  With every box
+marked, or none, the result is bit-identical to a full factorization of
+the new nodes; after a local deformation the interior field stays within
+the reference's bound of the field from a fresh tree
+(ie_solver_test.cpp:577-729: 100 x tol, Stokes 300 x tol)."""
+import numpy as np
+import pytest
+
+import geometry as G
+import iesolver
+
+pytestmark = pytest.mark.gpu
+
+FP64 = iesolver.FP64
+
+
+def _solver(g, pde=iesolver.PDE_LAPLACE, method=None):
+    s = iesolver.RSSolver(0)
+    if isinstance(g, list):
+        s.set_geometry_batch(g, pde)
+    else:
+        s.set_geometry(g, pde)
+        if "holes" in g:
+            s.set_holes(g["holes"]["centers"], g["holes"]["radii"])
+    s.build_tree(64)
+    if method is not None:
+        s.set_id_method(*method)
+    return s
+
+
+@pytest.mark.parametrize("method", [None, (iesolver.ID_RANDOMIZED, 32)])
+def test_update_marking_every_box_is_bit_exact(method):
+    tol = 1e-10
+    g0 = G.wavy(32768, seed=3)
+    g1 = dict(g0, weights=g0["weights"] * (1 + 1e-6))  # every node changes
+    f = G.harmonic_data(g0["points"], *G.exterior_sources(3))
+    s = _solver(g0, method=method)
+    s.factor(tol, FP64)
+    assert s.update_geometry(g1)
+    n_dirty, _, marks = s.update_info()
+    assert marks.all() and n_dirty == marks.size
+    s.factor(tol, FP64)
+    assert s.update_info()[1]  # the incremental path ran
+    mu, _ = s.solve(f)
+    fresh = _solver(g1, method=method)
+    fresh.factor(tol, FP64)
+    mf, _ = fresh.solve(f)
+    assert np.array_equal(mu, mf, equal_nan=True)
+
+
+def test_update_without_changes_is_bit_exact():
+    tol = 1e-10
+    g = G.wavy(32768, seed=4)
+    f = G.harmonic_data(g["points"], *G.exterior_sources(4))
+    s = _solver(g)
+    s.factor(tol, FP64)
+    m0, _ = s.solve(f)
+    assert s.update_geometry(g)
+    s.factor(tol, FP64)
+    n_dirty, inc, _ = s.update_info()
+    assert inc and n_dirty == 1
+    m1, _ = s.solve(f)
+    assert np.array_equal(m0, m1)
+
+
+CASES = [
+    ("laplace", iesolver.PDE_LAPLACE, 1e-10, 100),
+    ("laplace", iesolver.PDE_LAPLACE, 1e-6, 100),
+    ("stokes", iesolver.PDE_STOKES, 1e-6, 300),
+    ("neumann", iesolver.PDE_LAPLACE_NEUMANN, 1e-6, 100),
+]
+
+
+@pytest.mark.parametrize("name,pde,tol,factor", CASES)
+def test_update_local_bump_matches_fresh_tree(name, pde, tol, factor):
+    n = 8192
+    g0 = G.star(n)
+    tg = G.interior_points(5, 100, radius=0.2)
+    s = _solver(g0, pde)
+    s.factor(tol, FP64)
+    for frame, bump in enumerate([0.01, 0.03, -0.02]):
+        g1 = G.bump_star(n, bump, 1.0 + 0.7 * frame)
+        if pde == iesolver.PDE_STOKES:
+            f = G.tangent_data(g1["normals"]).reshape(-1)
+        elif pde == iesolver.PDE_LAPLACE_NEUMANN:
+            f = np.ones(n)
+        else:
+            f = G.harmonic_data(g1["points"], *G.exterior_sources(5))
+        assert s.update_geometry(g1)
+        s.factor(tol, FP64)
+        n_dirty, inc, marks = s.update_info()
+        assert inc and n_dirty < marks.size
+        mu, _ = s.solve(f)
+        u = s.evaluate(tg, mu)
+        fresh = _solver(g1, pde)
+        fresh.factor(tol, FP64)
+        mf, _ = fresh.solve(f)
+        uf = fresh.evaluate(tg, mf)
+        d = np.abs(u - uf).max()
+        print(f"{name} tol {tol:g} frame {frame}: {n_dirty} of {marks.size} boxes "
+              f"re-factored, max field difference {d:.2e}", flush=True)
+        assert d <= factor * tol
+
+
+def test_update_one_problem_of_a_batch():
+    tol = 1e-10
+    geoms = [G.star(4096), G.wavy(8192, seed=1), G.star(2048, amp=0.1, k=5), G.wavy(4096, seed=2)]
+    f = G.batch_data(geoms, seed=7)
+    s = _solver(geoms)
+    s.factor(tol, FP64)
+    m0, _ = s.solve(f)
+    g1 = list(geoms)
+    g1[0] = G.bump_star(4096, 0.03, 1.0)
+    assert s.update_geometry(g1)
+    s.factor(tol, FP64)
+    assert s.update_info()[1]
+    m1, _ = s.solve(f)
+    off = s.dof_offsets
+    # the other problems' boxes and roots are re-used or re-formed from
+    # unchanged data: their solutions do not move
+    assert np.array_equal(m1[off[1]:], m0[off[1]:])
+    one = _solver(g1[0])
+    one.factor(tol, FP64)
+    m_one, _ = one.solve(f[:off[1]])
+    tg = G.interior_points(5, 100, radius=0.2)
+    d = np.abs(one.evaluate(tg, m1[:off[1]]) - one.evaluate(tg, m_one)).max()
+    assert d <= 100 * tol
+
+
+def test_update_with_holes_rebuilds_border():
+    tol = 1e-8
+    g0 = G.circle_with_holes(8192, n_holes=4, seed=2)
+    sel = np.zeros(g0["points"].shape[0], bool)
+    sel[100:300] = True  # part of the outer circle
+    g1 = dict(g0, weights=np.where(sel, g0["weights"] * 1.01, g0["weights"]))
+    f = G.tangent_data(g0["normals"]).reshape(-1)
+    s = _solver(g0, iesolver.PDE_STOKES)
+    s.factor(tol, FP64)
+    assert s.update_geometry(g1)
+    s.factor(tol, FP64)
+    n_dirty, inc, marks = s.update_info()
+    assert inc and n_dirty < marks.size
+    mu, alpha = s.solve(f)
+    fresh = _solver(g1, iesolver.PDE_STOKES)
+    fresh.factor(tol, FP64)
+    mf, af = fresh.solve(f)
+    assert np.abs(mu - mf).max() <= 300 * tol * np.abs(mf).max()
+    assert np.abs(alpha - af).max() <= 300 * tol * max(np.abs(af).max(), 1.0)
+
+
+def test_update_that_rebuilds_the_tree_factors_every_box():
+    tol = 1e-10
+    g0 = G.star(4096)
+    g1 = {k: v.copy() for k, v in g0.items()}
+    g1["points"][:200] = 0.8 + 1e-4 * np.stack([np.arange(200) % 14, np.arange(200) // 14], 1)
+    f = np.ones(4096)
+    s = _solver(g0)
+    s.factor(tol, FP64)
+    assert not s.update_geometry(g1)
+    s.factor(tol, FP64)
+    assert not s.update_info()[1]
+    mu, _ = s.solve(f)
+    fresh = _solver(g1)
+    fresh.factor(tol, FP64)
+    mf, _ = fresh.solve(f)
+    assert np.array_equal(mu, mf, equal_nan=True)
