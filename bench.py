@@ -234,6 +234,43 @@ def cpu_baseline(g, f, cfg, threads, reps):
     return statistics.median(times), mu
 
 
+def time_update(s, cfg, args):
+    """Incremental re-factorization after local deformations of the wavy
+    boundary (untimed in the headline): per frame, a bump of 1 % of r0 over
+    an arc of 0.4 rad moves to a new place; iegpu_update_geometry re-inserts
+    the nodes and marks boxes, the next factor re-factors only the marked
+    ones.  A full factorization of the same frame on the same tree gives
+    the comparison time and solution."""
+    import iesolver
+    frames = []
+    f_of = lambda g: G.harmonic_data(g["points"], *G.exterior_sources(cfg["seed"]))
+    for i in range(max(args.steps, 2)):
+        g1 = G.bump_wavy(cfg["n"], 0.01 * (1 if i % 2 == 0 else -1), 1.0 + 0.5 * i,
+                         seed=cfg["seed"])
+        t0 = time.perf_counter()
+        kept = s.update_geometry(g1)
+        t_host = (time.perf_counter() - t0) * 1e3
+        s.factor(cfg["tol"], iesolver.FP64)
+        inc_ms = s.timings()["factor_ms"]
+        n_dirty, inc, marks = s.update_info()
+        f1 = f_of(g1)
+        x_inc, _ = s.solve(f1)
+        s.set_keep_operator(False)  # drops the marks: the next factor covers every box
+        s.factor(cfg["tol"], iesolver.FP64)
+        full_ms = s.timings()["factor_ms"]
+        x_full, _ = s.solve(f1)
+        frames.append(dict(tree_kept=kept, incremental=inc, boxes_refactored=int(n_dirty),
+                           boxes=int(marks.size), update_host_ms=round(t_host, 2),
+                           incremental_factor_ms=round(inc_ms, 3), full_factor_ms=round(full_ms, 3),
+                           rel_diff_vs_full=float(np.linalg.norm(x_inc - x_full) /
+                                                  np.linalg.norm(x_full))))
+    inc = [fr["incremental_factor_ms"] for fr in frames[1:]]
+    full = [fr["full_factor_ms"] for fr in frames[1:]]
+    return dict(frames=frames, median_incremental_factor_ms=float(np.median(inc)),
+                median_full_factor_ms=float(np.median(full)),
+                speedup=float(np.median(full) / np.median(inc)))
+
+
 def run_batch(args, cfg):
     """The batch workload: every problem's boxes share each stage launch."""
     import torch
@@ -405,6 +442,9 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-profile", action="store_true",
                     help="skip per-stage HIP events (roofline fields become null)")
+    ap.add_argument("--update", action="store_true",
+                    help="also time incremental re-factorizations after local boundary "
+                         "deformations (iegpu_update_geometry), untimed in the headline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the RS factorization here runs on a single "
@@ -528,6 +568,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if args.update and cfg["shape"] == "wavy":
+        out["geometry_update"] = time_update(s, cfg, args)
     print(json.dumps(out), flush=True)
     s.close()
     return 0

@@ -196,12 +196,12 @@ struct iegpu_ctx {
       s_red_list, s_perm, s_rank, s_upd_kind, s_m_rows, s_row_pool, s_err,
       s_root_piv, s_qkind, s_jobs, s_njobs, s_lists, s_lcnt, s_sk_w, s_rid_done;
   DevBuf<int64_t> s_id_sz, s_row_sz, s_fac_sz, s_id_off, s_row_off, s_fac_off,
-      s_fac_top;
-  DevBuf<unsigned char> s_id_pool, s_fac_pool, s_root_mat;
+      s_fac_top, s_dec_sz, s_dec_off;
+  DevBuf<unsigned char> s_id_pool, s_fac_pool, s_root_mat, s_dec_pool;
   // X_rr and root blocks kept for the compressed forward apply
   bool keep_operator = false;
   DevBuf<unsigned char> s_fwd_pool, s_root_fwd;
-  int64_t id_cap = 0, row_cap = 0, fac_cap = 0;
+  int64_t id_cap = 0, row_cap = 0, fac_cap = 0, dec_cap = 0;
   // randomized ID
   int rid_mode = 0, rid_width = 0;
   DevBuf<double> d_omega;
@@ -298,6 +298,10 @@ struct iegpu_ctx {
     s.id_off = s_id_off.p;
     s.row_off = s_row_off.p;
     s.fac_off = s_fac_off.p;
+    s.dec_sz = s_dec_sz.p;
+    s.dec_off = s_dec_off.p;
+    s.dec_pool = s_dec_pool.p;
+    s.dec_cap = dec_cap;
     s.id_pool = s_id_pool.p;
     s.row_pool = s_row_pool.p;
     s.fac_pool = s_fac_pool.p;
@@ -452,6 +456,8 @@ void alloc_state(iegpu_ctx* c) {
   c->s_id_off.alloc(nb);
   c->s_row_off.alloc(nb);
   c->s_fac_off.alloc(nb);
+  c->s_dec_sz.alloc(nb);
+  c->s_dec_off.alloc(nb);
   c->s_fac_top.alloc(1);
   c->s_err.alloc(4);
   const size_t nr = c->tree.n_roots;
@@ -460,6 +466,7 @@ void alloc_state(iegpu_ctx* c) {
   c->s_id_pool.alloc((size_t)2 * c->id_cap * c->elem());  // odd and even levels
   c->s_row_pool.alloc((size_t)c->row_cap);
   c->s_fac_pool.alloc((size_t)c->fac_cap * c->elem());
+  c->s_dec_pool.alloc((size_t)c->dec_cap * c->elem());
   if (c->keep_operator) {
     c->s_fwd_pool.alloc((size_t)c->fac_cap * c->elem());
     c->s_root_fwd.alloc(nr * c->ncap * c->ncap * 8);
@@ -487,8 +494,10 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
   const bool split = c->overlap && !c->prof.on && c->stream2 != nullptr;
   hipStream_t st2 = split ? c->stream2 : st;
   // evs[0]: start; evs[1 + lv]: level lv's ID done; evs[n_levels + 1]: the
-  // decouple chain done; evs[n_levels + 2 + lv]: level lv's decouple done.
-  const size_t n_ev = 2 * (size_t)T.n_levels + 2;
+  // decouple chain done.  The two chains share no scratch: the ID pool
+  // (one half per level parity) belongs to the ID chain, the decouple pool
+  // to the decouple chain.
+  const size_t n_ev = (size_t)T.n_levels + 2;
   if (split && c->evs.size() < n_ev) {
     for (size_t i = c->evs.size(); i < n_ev; i++) {
       hipEvent_t e;
@@ -504,11 +513,6 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
   for (int lv = T.n_levels - 1; lv >= 1; lv--) {
     const int start = T.level_start[lv];
     const int count = T.level_start[lv + 1] - start;
-    // The ID scratch pool has one half per level parity, and the decouple
-    // of a box above kSmallBox DOFs may keep its operands in its ID slot:
-    // level lv's ID waits until level lv + 2 (same half) is decoupled.
-    if (split && lv + 2 < T.n_levels)
-      HIP_TRY(hipStreamWaitEvent(st, c->evs[T.n_levels + 2 + lv + 2], 0));
     for (int stage = 0; stage < kStageDecouple; stage++)
       c->prof.run(stage, lv, st, [&] {
         launch_factor_stage<R>(stage, g, t, s, lv, start, count, tol, c->caps, st);
@@ -520,7 +524,6 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
     c->prof.run(kStageDecouple, lv, st2, [&] {
       launch_factor_stage<R>(kStageDecouple, g, t, s, lv, start, count, tol, c->caps, st2);
     });
-    if (split) HIP_TRY(hipEventRecord(c->evs[T.n_levels + 2 + lv], st2));
     if (fx)
       c->prof.run(kProfSweepUp, lv, st2,
                   [&] { launch_sweep_up<R>(s, start, count, fv, false, true, st2); });
@@ -1259,6 +1262,7 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
     if (c->precision != precision) {
       c->s_id_pool.release();
       c->s_fac_pool.release();
+      c->s_dec_pool.release();
     }
     c->precision = precision;
     if (c->ncap == 0) c->ncap = 256;
@@ -1268,6 +1272,7 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
     if (c->id_cap == 0) c->id_cap = std::max<int64_t>(nd * 16, 1 << 22);
     if (c->row_cap == 0) c->row_cap = std::max<int64_t>(nd * 8, 1 << 16);
     if (c->fac_cap == 0) c->fac_cap = std::max<int64_t>(nd * 160, 1 << 22);
+    if (c->dec_cap == 0) c->dec_cap = std::max<int64_t>(nd * 8, 1 << 20);
     if (c->rid_mode != 0 && !c->d_omega.p) {
       const int64_t no = (int64_t)kRidTableRows * kRidRows;
       c->d_omega.alloc(no);
@@ -1330,7 +1335,7 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
       HIP_TRY(hipMemcpyAsync(h_err, c->s_err.p, sizeof(h_err), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
       errbits = h_err[0];
-      const int grow = errbits & (kErrBoxCap | kErrIdPool | kErrRowPool | kErrFacPool);
+      const int grow = errbits & (kErrBoxCap | kErrIdPool | kErrRowPool | kErrFacPool | kErrDecPool);
       if (!grow) {
         if (c->hole_dim > 0) {
           if (precision == IEGPU_FP32) enqueue_border_factor<float>(c);
@@ -1350,7 +1355,8 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
       // capacity as well: in an incremental run that bit counts only when
       // it comes alone.
       bool box_cap = (errbits & kErrBoxCap) != 0;
-      if (c->inc_run && (errbits & (kErrIdPool | kErrRowPool | kErrFacPool))) box_cap = false;
+      if (c->inc_run && (errbits & (kErrIdPool | kErrRowPool | kErrFacPool | kErrDecPool)))
+        box_cap = false;
       if (box_cap) c->inc_run = false;
       if (box_cap) {
         if (c->ncap >= kMaxBoxDofs) {
@@ -1365,6 +1371,8 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
         c->id_cap = std::max<int64_t>(2 * c->id_cap, ((int64_t)h_err[2] << 10) * 5 / 4);
       if (errbits & kErrRowPool)
         c->row_cap = std::max<int64_t>(2 * c->row_cap, ((int64_t)h_err[3] << 10) * 5 / 4);
+      if (errbits & kErrDecPool)
+        c->dec_cap = std::max<int64_t>(2 * c->dec_cap, ((int64_t)h_err[1] << 10) * 5 / 4);
       if (errbits & kErrFacPool) {
         c->fac_cap *= 2;
         if (c->inc_run) {

@@ -23,6 +23,7 @@ enum : int {
   kErrFacPool = 4,     // factor pool exhausted
   kErrSingular = 8,    // exact zero pivot in an LU
   kErrRowPool = 16,    // row-index pool exhausted
+  kErrDecPool = 32,    // decouple operand pool exhausted
 };
 
 // How a box's interpolative decomposition runs.
@@ -148,6 +149,12 @@ struct DevState {
   int64_t* id_off;
   int64_t* row_off;
   int64_t* fac_off;
+  // Operands of a general decouple (boxes above kSmallBox DOFs whose blocks
+  // do not fit LDS): level-local, on the decouple chain only.
+  int64_t* dec_sz;
+  int64_t* dec_off;
+  void* dec_pool;
+  int64_t dec_cap;
   void* id_pool;
   int* row_pool;
   void* fac_pool;
@@ -155,7 +162,7 @@ struct DevState {
   int64_t row_cap;
   int64_t fac_cap;
   int64_t* fac_top;    // running factor-pool offset across levels
-  int* err;            // [0] error bits, [1] spare, [2] / [3] ID / row
+  int* err;            // [0] error bits, [1] / [2] / [3] decouple / ID / row
                        // pool size a level needed on overflow (1024 elements)
   void* root_mat;      // per root: 2 ncap^2 elements (matrix + inverse)
   int* root_piv;       // per root: ncap + 64

@@ -219,11 +219,11 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
     int64_t scratch = kind == kQrcpLds ? ld * n : 0;
     if (kind == kQrcpStack || kind == kQrcpStack8)
       scratch = (int64_t)id_chunks(m, kind == kQrcpStack ? kRegMaxRows : kRegMaxRows8) * n * n;
-    // k_decouple keeps the operands of a box with more than kSmallBox
-    // active DOFs (n^2 + r^2 + 2 r k <= 2 n^2 entries) in the same slot
-    // when they do not fit its LDS buffer.
-    if (n > kSmallBox) scratch = max(scratch, 2 * (int64_t)n * n);
     s.id_sz[b] = scratch;
+    // k_decouple keeps the operands of a box with more than kSmallBox
+    // active DOFs (n^2 + r^2 + 2 r k <= 2 n^2 entries) in the decouple pool
+    // when they do not fit its LDS buffer.
+    s.dec_sz[b] = n > kSmallBox ? 2 * (int64_t)n * n : 0;
     s.row_sz[b] = kind == kQrcpLds ? (m - proxy_rows(g, level)) / 2 : 0;
     s.fac_sz[b] = (n > 0) ? fac_slot_size(n) : 0;
   }
@@ -272,13 +272,13 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
                                                int count, int level) {
   // Scanned per box: ID scratch, row-list and factor sizes, register-QRCP
   // and general-QRCP list entries, row-compression jobs.
-  constexpr int NQ = 10;
+  constexpr int NQ = 11;
   __shared__ int64_t wtot[NQ][16];  // per-wave totals, then their exclusive scan
   __shared__ int64_t tot[NQ];
   const int tid = threadIdx.x;
   const int chunk = (count + 1023) / 1024;
   const int lo = min(count, tid * chunk), hi = min(count, lo + chunk);
-  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
     if (box_clean(s, b)) continue;
@@ -293,6 +293,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     own[6] += (kind == kQrcpReg8 || kind == kQrcpStack8) ? 1 : 0;
     own[8] += (s.sk_w[b] > 0) ? 1 : 0;
     own[9] += rid_full_qrcp(s, b, kind) ? 1 : 0;
+    own[10] += s.dec_sz[b];
   }
   // Exclusive scan over the 1024 threads: inclusive scans within each wave
   // (shuffles), then a scan of the 16 wave totals.
@@ -340,6 +341,8 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     s.id_off[b] = (level & 1) * s.id_cap + off[0];
     s.row_off[b] = off[1];
     s.fac_off[b] = fbase + off[2];
+    s.dec_off[b] = off[10];
+    off[10] += s.dec_sz[b];
     off[0] += s.id_sz[b];
     off[1] += s.row_sz[b];
     off[2] += s.fac_sz[b];
@@ -380,13 +383,17 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
       atomicMax(s.err + 3, (int)((tb + 1023) >> 10));
     }
     if (fbase + tc > s.fac_cap) e |= kErrFacPool;
+    if (tot[10] > s.dec_cap) {
+      e |= kErrDecPool;
+      atomicMax(s.err + 1, (int)((tot[10] + 1023) >> 10));
+    }
     if (e) atomicOr(s.err, e);
     *s.fac_top = fbase + tc;
   }
 }
 
 __device__ inline bool pools_ok(const DevState& s) {
-  return (s.err[0] & (kErrIdPool | kErrRowPool | kErrFacPool | kErrBoxCap)) == 0;
+  return (s.err[0] & (kErrIdPool | kErrRowPool | kErrFacPool | kErrBoxCap | kErrDecPool)) == 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1995,7 +2002,7 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   // n^2 + r^2 + 2 r k <= 2 n^2 entries.
   R* P = (2 * (int64_t)n * n <= kLdsElems)
              ? sbuf
-             : reinterpret_cast<R*>(s.id_pool) + s.id_off[b];
+             : reinterpret_cast<R*>(s.dec_pool) + s.dec_off[b];
   R* X = P + (int64_t)n * n;
   R* Krs = X + (int64_t)r * r;
   R* Ksr = Krs + (int64_t)r * k;

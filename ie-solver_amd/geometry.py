@@ -67,13 +67,33 @@ def bump_star(n, bump, t0, width=0.4, center=(0.5, 0.5), r0=0.3, amp=0.2, k=7):
     r = r0 * (1 + amp * np.cos(k * t))
     dr = -r0 * amp * k * np.sin(k * t)
     ddr = -r0 * amp * k * k * np.cos(k * t)
+    _add_bump(t, r, dr, ddr, r0 * bump, t0, width)
+    p, nm, cv, w = _radial_curve(n, center, r, dr, ddr, t)
+    return dict(points=p, normals=nm, curvatures=cv, weights=w)
+
+
+def _add_bump(t, r, dr, ddr, h, t0, width):
+    """r += h (1 - u^2)^3 on |u| < 1, u = (t - t0) / width wrapped, with its
+    first and second derivatives; r, dr, ddr are updated in place."""
     u = (np.mod(t - t0 + math.pi, 2 * math.pi) - math.pi) / width
     on = np.abs(u) < 1
     uo = u[on]
     g = 1 - uo * uo
-    r[on] += r0 * bump * g ** 3
-    dr[on] += r0 * bump * (-6 * uo * g * g / width)
-    ddr[on] += r0 * bump * (-6 * g * (1 - 5 * uo * uo) / (width * width))
+    r[on] += h * g ** 3
+    dr[on] += h * (-6 * uo * g * g / width)
+    ddr[on] += h * (-6 * g * (1 - 5 * uo * uo) / (width * width))
+
+
+def bump_wavy(n, bump, t0, width=0.2, seed=0, center=(0.5, 0.5), r0=0.3):
+    """wavy() with the local bump of bump_star(); nodes off the arc are
+    bit-identical to wavy()'s."""
+    ks, a = wavy_coefficients(seed)
+    t = np.arange(n, dtype=np.float64) * (2 * math.pi / n)
+    kt = np.outer(t, ks)
+    r = r0 * (1 + np.cos(kt) @ a)
+    dr = -r0 * (np.sin(kt) @ (a * ks))
+    ddr = -r0 * (np.cos(kt) @ (a * ks * ks))
+    _add_bump(t, r, dr, ddr, r0 * bump, t0, width)
     p, nm, cv, w = _radial_curve(n, center, r, dr, ddr, t)
     return dict(points=p, normals=nm, curvatures=cv, weights=w)
 
