@@ -236,16 +236,18 @@ def cpu_baseline(g, f, cfg, threads, reps):
 
 def time_update(s, cfg, args):
     """Incremental re-factorization after local deformations of the wavy
-    boundary (untimed in the headline): per frame, a bump of 1 % of r0 over
-    an arc of 0.4 rad moves to a new place; iegpu_update_geometry re-inserts
-    the nodes and marks boxes, the next factor re-factors only the marked
-    ones.  A full factorization of the same frame on the same tree gives
-    the comparison time and solution."""
+    boundary (untimed in the headline): per frame, a bump of 3e-4 r0 over
+    an arc of 0.4 rad moves to a new place (larger bumps at this N sweep
+    nodes into large empty leaves past twice the leaf capacity, and the
+    tree is rebuilt); iegpu_update_geometry re-inserts the nodes and marks
+    boxes, the next factor re-factors only the marked ones.  A full
+    factorization of the same frame on the same tree gives the comparison
+    time and solution."""
     import iesolver
     frames = []
     f_of = lambda g: G.harmonic_data(g["points"], *G.exterior_sources(cfg["seed"]))
     for i in range(max(args.steps, 2)):
-        g1 = G.bump_wavy(cfg["n"], 0.01 * (1 if i % 2 == 0 else -1), 1.0 + 0.5 * i,
+        g1 = G.bump_wavy(cfg["n"], 3e-4 * (1 if i % 2 == 0 else -1), 1.0 + 0.5 * i,
                          seed=cfg["seed"])
         t0 = time.perf_counter()
         kept = s.update_geometry(g1)

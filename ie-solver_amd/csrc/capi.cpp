@@ -924,8 +924,9 @@ int iegpu_set_holes(iegpu_ctx* c, int n_holes, const double* centers,
   try {
     HIP_TRY(hipSetDevice(c->device));
     c->n_holes = n_holes;
+    // The holes enter only the border blocks, which every factorization
+    // rebuilds: a pending geometry update stays incremental.
     c->factored = false;
-    c->update_pending = false;
     if (n_holes == 0) {
       c->hole_dim = 0;
       return IEGPU_OK;
