@@ -7,6 +7,9 @@ the reference's bound of the field from a fresh tree
 (ie_solver_test.cpp:577-729: 100 x tol, Stokes 300 x tol)."""
 import numpy as np
 import pytest
+# torch's HIP runtime has to initialize before this library loads the
+# system one, so it is imported when the module is.
+import torch
 
 import geometry as G
 import iesolver
@@ -169,3 +172,27 @@ def test_update_that_rebuilds_the_tree_factors_every_box():
     fresh.factor(tol, FP64)
     mf, _ = fresh.solve(f)
     assert np.array_equal(mu, mf, equal_nan=True)
+
+
+def test_update_then_fused_factor_solve():
+    # iegpu_factor_solve_device after an update re-factors the marked boxes
+    # and gives the same x as iegpu_factor + iegpu_solve_device after the
+    # same update.
+    tol = 1e-10
+    n = 8192
+    g0, g1 = G.star(n), G.bump_star(n, 0.03, 1.0)
+    f = G.harmonic_data(g1["points"], *G.exterior_sources(5))
+    dev = torch.device("cuda:0")
+    fd = torch.from_numpy(f).to(dev)
+    xa, xb = torch.zeros_like(fd), torch.zeros_like(fd)
+    torch.cuda.synchronize()  # the library works on its own streams
+    a, b = _solver(g0), _solver(g0)
+    a.factor(tol, FP64)
+    b.factor(tol, FP64)
+    assert a.update_geometry(g1) and b.update_geometry(g1)
+    a.factor_solve_device(tol, FP64, fd.data_ptr(), xa.data_ptr())
+    b.factor(tol, FP64)
+    b.solve_device(fd.data_ptr(), xb.data_ptr())
+    torch.cuda.synchronize()
+    assert a.update_info()[1] and b.update_info()[1]
+    assert np.array_equal(xa.cpu().numpy(), xb.cpu().numpy())
