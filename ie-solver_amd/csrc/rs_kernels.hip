@@ -2104,12 +2104,14 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
 }
 
 // Gauss-Jordan inverse of the root block with the matrix in registers:
-// 512 threads on a 32 x 16 grid, thread (tr, tc) holding rows tr + 32 u and
-// columns tc + 16 v (n <= 192).  Per step the pivot column (kept in LDS by
+// 1024 threads on a 32 x 32 grid, thread (tr, tc) holding rows tr + 32 u and
+// columns tc + 32 v (n <= 192; 6 x 6 entries each: measured faster than
+// 32 x 16 threads with 6 x 12, 64 x 16 with 3 x 12 and 16 x 64 with
+// 12 x 3 at a 190-DOF root).  Per step the pivot column (kept in LDS by
 // its owners) picks the pivot row as in gj_invert, the pivot row's owners
 // publish it, and every thread updates its entries.  The inverse,
 // unpermuted, goes after the assembled matrix.
-constexpr int kRootTR = 32, kRootTC = 16, kRootRU = 6, kRootCV = 12;
+constexpr int kRootTR = 32, kRootTC = 32, kRootRU = 6, kRootCV = 6;
 
 template <typename R>
 __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int factor_lu) {
