@@ -397,6 +397,27 @@ class QuadTree {
   }
   void reset() { pde_ = -1; }
 
+  // quadtree.cpp:520-801: the boundary moved.  With the same node count
+  // the library puts the nodes into the kept cells and the next
+  // skeletonize re-factors only the boxes the reference marks
+  // (iegpu_update_geometry; a tree it cannot keep is rebuilt).  With
+  // another count the next skeletonize builds a new tree.  min / max keep
+  // the first tree's bounds, as the reference's do.
+  void perturb(const Boundary& b) {
+    const bool same_n = owned_ && b.weights.size() == owned_->weights.size();
+    delete owned_;
+    owned_ = new Copy(b);
+    boundary = owned_;
+    if (pde_ >= 0 && same_n) {
+      int kept = 0;
+      detail::check(ctx, iegpu_update_geometry(ctx, (int)b.weights.size(), b.points.data(),
+                                               b.normals.data(), b.curvatures.data(),
+                                               b.weights.data(), &kept));
+    } else {
+      pde_ = -1;
+    }
+  }
+
   // Uploads geometry and builds the device tree for `pde` (done once).
   void prepare(int pde) {
     if (pde_ == pde) return;

@@ -136,6 +136,45 @@ static void backward_case(const std::string& name, ie_solver_config config) {
   expect_le(name, solve_err(config, &boundary), 50 * config.id_tol);
 }
 
+// A circle of radius 0.25 with r raised by 0.25 h (1 - u^2)^3 on the arc
+// |u| < 1, u = (t - t0) / 0.5: a local, smooth deformation for the update
+// tests (the reference moves parts of its Ex1-Ex3 boundaries instead).
+class BumpCircle : public Boundary {
+ public:
+  double h = 0, t0 = 0;
+  void initialize(int N, BoundaryCondition bc) override {
+    clear_nodes();
+    for (int i = 0; i < N; i++) {
+      const double t = i * 2.0 * M_PI / N;
+      const double u = std::remainder(t - t0, 2 * M_PI) / 0.5;
+      double r = 0.25, dr = 0, ddr = 0;
+      if (std::fabs(u) < 1) {
+        const double g = 1 - u * u;
+        r += 0.25 * h * g * g * g;
+        dr += 0.25 * h * (-6 * u * g * g / 0.5);
+        ddr += 0.25 * h * (-6 * g * (1 - 5 * u * u) / 0.25);
+      }
+      const double c = std::cos(t), sn = std::sin(t);
+      const double x1 = dr * c - r * sn, y1 = dr * sn + r * c;  // d(x, y)/dt
+      const double x2 = ddr * c - 2 * dr * sn - r * c, y2 = ddr * sn + 2 * dr * c - r * sn;
+      const double sp = std::sqrt(x1 * x1 + y1 * y1);
+      push_node(0.5 + r * c, 0.5 + r * sn, y1 / sp, -x1 / sp,
+                (x1 * y2 - y1 * x2) / (sp * sp * sp), sp * 2 * M_PI / N);
+    }
+    if (bc == DEFAULT) {
+      boundary_values = ie_Mat(N, 1);
+      apply_boundary_condition(0, N, SINGLE_ELECTRON);
+    } else {
+      set_boundary_values_size(bc);
+      apply_boundary_condition(0, N, bc);
+    }
+  }
+  bool is_in_domain(const Vec2& a) override {
+    const double x = a.a[0] - 0.5, y = a.a[1] - 0.5;
+    return std::sqrt(x * x + y * y) + 2e-2 < 0.25 * (1 - std::fabs(h));
+  }
+};
+
 int main() {
   const char* tol_env = std::getenv("IE_TEST_TOL");
   const double tol = tol_env ? std::atof(tol_env) : DEFAULT_ID_TOL;
@@ -179,6 +218,34 @@ int main() {
     backward_case<Annulus>(p + "StokesAnnulusBackwardError", sa);
   }
 
+  {  // ie_solver_test.cpp:577-617 (Ex1UpdateLosesNoAcc): perturb the tree,
+     // solve, and compare with a fresh tree of the same boundary
+    ie_solver_config c = lap;
+    c.num_boundary_points = 4096;
+    BumpCircle b0;
+    b0.initialize(c.num_boundary_points, c.boundary_condition);
+    QuadTree qt;
+    qt.initialize_tree(&b0, std::vector<double>(), 1, 2);
+    std::vector<double> dom;
+    get_domain_points(c.domain_size, &dom, qt.min, qt.max, qt.min, qt.max);
+    boundary_integral_solve(c, &qt, dom);
+    for (int frame = 0; frame < 3; frame++) {
+      BumpCircle b1;
+      b1.h = 0.004 * (frame + 1);
+      b1.t0 = 1.0 + 2.0 * frame;
+      b1.initialize(c.num_boundary_points, c.boundary_condition);
+      qt.perturb(b1);
+      const ie_Mat sol = boundary_integral_solve(c, &qt, dom);
+      int n_dirty = 0, incremental = 0;
+      iegpu_update_info(qt.ctx, &n_dirty, &incremental, nullptr);
+      expect_le("UpdateIsIncremental frame " + std::to_string(frame), incremental ? 0 : 1, 0);
+      QuadTree fresh;
+      fresh.initialize_tree(&b1, std::vector<double>(), 1, 2);
+      const ie_Mat new_sol = boundary_integral_solve(c, &fresh, dom);
+      expect_le("UpdateLosesNoAcc frame " + std::to_string(frame),
+                (new_sol - sol).max_entry_magnitude(), 100 * c.id_tol);
+    }
+  }
   {  // ie_solver_test.cpp:404-416
     Circle b;
     b.initialize(lap.num_boundary_points, lap.boundary_condition);

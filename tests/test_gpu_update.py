@@ -97,6 +97,13 @@ def test_update_local_bump_matches_fresh_tree(name, pde, tol, factor):
         assert inc and n_dirty < marks.size
         mu, _ = s.solve(f)
         u = s.evaluate(tg, mu)
+        # every box whose ID or decouple inputs changed was re-factored, so
+        # a full factorization on the same tree gives the same bits
+        s.set_keep_operator(False)  # drops nothing but the marks
+        s.factor(tol, FP64)
+        assert not s.update_info()[1]
+        m_full, _ = s.solve(f)
+        assert np.array_equal(mu, m_full)
         fresh = _solver(g1, pde)
         fresh.factor(tol, FP64)
         mf, _ = fresh.solve(f)
