@@ -887,12 +887,13 @@ void upload_tree(iegpu_ctx* c) {
 // (quadtree.cpp:680-697: the leaf of every added point, and for deleted
 // points the leaf they leave, is passed to mark_neighbors_and_parents) and
 // mark_neighbors_and_parents (quadtree.cpp:435-461: the box, its
-// neighbours, then the same for its parent, up to the root).  A box on a
-// walk holds changed nodes, and at levels 1 and 2 a box's ID rows are all
-// the other boxes of its level (no proxy), not only its neighbours: such a
-// box is marked too when its rows include a box on a walk (the reference
-// keeps its old ID, which then misses the moved nodes: residuals of 2e-5
-// at tol 1e-10 on a bumped circle).  Then every marked box's parent is
+// neighbours, then the same for its parent, up to the root).  A box's ID
+// rows come from its row-source boxes' active lists, which change when a
+// source holds changed nodes or re-factors a child, and the sources are
+// not always among the neighbours (all boxes of the level at levels 1 and
+// 2): such a box is marked too (the reference keeps its old ID, which then
+// misses the moved nodes: residuals of 2e-5 at tol 1e-10 on a bumped
+// circle).  Then every marked box's parent is
 // marked (a box re-factored under a kept parent would leave the parent's
 // lists stale; the reference's rule gives this by itself on the tested
 // boundaries), and every root is re-factored.
@@ -909,10 +910,23 @@ std::vector<int> mark_update(const HostTree& T, const std::vector<int>& leaves) 
       for (int t = 0; t < T.nbr_cnt[s]; t++) dirty[T.nbr[T.nbr_off[s] + t]] = 1;
     }
   }
-  for (int lv = 1; lv <= 2 && lv < T.n_levels; lv++)
+  // Bottom up: a box's active list changes when it is a leaf holding
+  // changed nodes or when a child is re-factored (new skeleton); a box
+  // whose ID rows come from such a box is re-factored.
+  std::vector<char> chg(T.n_boxes, 0);
+  for (int s = 0; s < T.n_boxes; s++) chg[s] = T.leaf[s] && walk[s];
+  for (int lv = T.n_levels - 1; lv >= 1; lv--) {
     for (int s = T.level_start[lv]; s < T.level_start[lv + 1]; s++)
+      for (int q = 0; q < 4; q++) {
+        const int ch = T.child[4 * (size_t)s + q];
+        if (ch >= 0 && dirty[ch]) chg[s] = 1;
+      }
+    for (int s = T.level_start[lv]; s < T.level_start[lv + 1]; s++) {
+      if (chg[s]) dirty[s] = 1;
       for (int u = 0; u < T.src_cnt[s]; u++)
-        if (walk[T.src[T.src_off[s] + u]]) dirty[s] = 1;
+        if (chg[T.src[T.src_off[s] + u]]) dirty[s] = 1;
+    }
+  }
   for (int lv = T.n_levels - 1; lv >= 1; lv--)
     for (int s = T.level_start[lv]; s < T.level_start[lv + 1]; s++)
       if (dirty[s]) dirty[T.parent[s]] = 1;
