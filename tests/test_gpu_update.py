@@ -203,3 +203,34 @@ def test_update_then_fused_factor_solve():
     torch.cuda.synchronize()
     assert a.update_info()[1] and b.update_info()[1]
     assert np.array_equal(xa.cpu().numpy(), xb.cpu().numpy())
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_update_randomized_forms_equal_full_refactor(learned):
+    # The marks cover every box whose ID inputs changed, so in the
+    # randomized forms too an incremental factorization equals a full one
+    # on the same tree bit for bit.
+    tol = 1e-10
+    n = 65536
+    s = _solver(G.wavy(n, seed=0))
+    if learned:
+        import os
+        model = np.load(os.path.join(os.path.dirname(iesolver.__file__),
+                                     "rank_model_default.npy")).astype(np.float32)
+        s.set_rank_model(model, margin=3)
+        s.set_id_method(iesolver.ID_RANDOMIZED, 0)
+    else:
+        s.set_id_method(iesolver.ID_RANDOMIZED, 32)
+    s.factor(tol, FP64)
+    for frame in range(2):
+        g1 = G.bump_wavy(n, 3e-4 * (1 - 2 * frame), 1.0 + 0.5 * frame, seed=0)
+        f = G.harmonic_data(g1["points"], *G.exterior_sources(0))
+        assert s.update_geometry(g1)
+        s.factor(tol, FP64)
+        n_dirty, inc, marks = s.update_info()
+        assert inc and n_dirty < marks.size
+        mu, _ = s.solve(f)
+        s.set_keep_operator(False)
+        s.factor(tol, FP64)
+        m_full, _ = s.solve(f)
+        assert np.array_equal(mu, m_full), (learned, frame)
