@@ -444,9 +444,10 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-profile", action="store_true",
                     help="skip per-stage HIP events (roofline fields become null)")
-    ap.add_argument("--update", action="store_true",
-                    help="also time incremental re-factorizations after local boundary "
-                         "deformations (iegpu_update_geometry), untimed in the headline")
+    ap.add_argument("--no-update", action="store_true",
+                    help="skip the timing of incremental re-factorizations after local "
+                         "boundary deformations (iegpu_update_geometry; untimed in the "
+                         "headline)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the RS factorization here runs on a single "
@@ -570,7 +571,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    if args.update and cfg["shape"] == "wavy":
+    if not args.no_update and cfg["shape"] == "wavy":
         out["geometry_update"] = time_update(s, cfg, args)
     print(json.dumps(out), flush=True)
     s.close()
