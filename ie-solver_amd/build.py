@@ -19,7 +19,7 @@ HIP_FLAGS = [
     # keep hipcc from emitting packed FP32 arithmetic
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-HIP_SOURCES = ["rs_kernels.hip", "rs_holes.hip", "rs_eval.hip", "rs_matvec.hip", "rs_mlp.hip", "capi.cpp"]
+HIP_SOURCES = ["rs_kernels.hip", "rs_sweep.hip", "rs_holes.hip", "rs_eval.hip", "rs_matvec.hip", "rs_mlp.hip", "capi.cpp"]
 HOST_SOURCES = ["tree.cpp"]
 
 

@@ -1767,6 +1767,7 @@ int iegpu_set_launch_caps(iegpu_ctx* c, const int* caps, int n) {
   L.qrcp_id8 = caps[4];
   L.qrcp = caps[5];
   L.dsplit = caps[6];
+  L.dsweep = caps[7];
   return IEGPU_OK;
 }
 
@@ -1774,7 +1775,7 @@ int iegpu_get_launch_caps(const iegpu_ctx* c, int* caps, int n) {
   if (!c || !caps || n != kNumCaps) return IEGPU_ERR_INVALID;
   const LaunchCaps& L = c->caps;
   const int v[kNumCaps] = {L.assemble, L.qrcp_id, L.rowcomp4, L.rowcomp8, L.qrcp_id8, L.qrcp,
-                           L.dsplit};
+                           L.dsplit, L.dsweep};
   for (int i = 0; i < n; i++) caps[i] = v[i];
   return IEGPU_OK;
 }
@@ -1790,7 +1791,7 @@ int iegpu_autotune(iegpu_ctx* c, double id_tol, int precision, int reps, int* ca
   static const int kCand[kNumCaps][4] = {{64, 256, 1024, 4096}, {64, 256, 1024, 4096},
                                          {64, 256, 1024, 4096}, {64, 256, 1024, 4096},
                                          {64, 256, 1024, 4096}, {64, 256, 1024, 4096},
-                                         {1, 2048, 8192, 1 << 20}};
+                                         {1, 2048, 8192, 1 << 20}, {1, 2048, 4096, 1 << 20}};
   auto measure = [&](double* best) -> int {
     *best = 1e300;
     for (int r = 0; r <= reps; r++) {

@@ -188,6 +188,12 @@ struct DevState {
   const int* dirty;
 };
 
+// False once a pool overflow was reported: the level's remaining kernels
+// stop and the host repeats the factorization with larger pools.
+__device__ inline bool pools_ok(const DevState& s) {
+  return (s.err[0] & (kErrIdPool | kErrRowPool | kErrFacPool | kErrBoxCap | kErrDecPool)) == 0;
+}
+
 __device__ inline bool box_clean(const DevState& s, int b) {
   return s.dirty != nullptr && s.dirty[b] == 0;
 }
@@ -683,6 +689,16 @@ __device__ inline DofList source_dofs(const DevTree& t, const DevState& s,
     d.n = s.act_cnt[box];
   }
   return d;
+}
+
+// Carried Schur block of box c in its factor slot (null: none).
+template <typename R>
+__device__ const R* carried_block(const DevState& s, int c) {
+  if (!s.upd_kind[c]) return nullptr;
+  const R* F = reinterpret_cast<const R*>(s.fac_pool) + s.fac_off[c];
+  const int k = s.rank[c];
+  if (k == 0) return F;
+  return F + FacLayout::make(k, s.act_cnt[c] - k).S;
 }
 
 // Dense block (matrix, then its inverse) and pivots of root r (slot r).

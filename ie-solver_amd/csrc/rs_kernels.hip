@@ -392,9 +392,6 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
   }
 }
 
-__device__ inline bool pools_ok(const DevState& s) {
-  return (s.err[0] & (kErrIdPool | kErrRowPool | kErrFacPool | kErrBoxCap | kErrDecPool)) == 0;
-}
 
 // ---------------------------------------------------------------------------
 // ID matrix assembly: [K(ic, box); K(box, ic)^T; proxy] (m x n,
@@ -1724,14 +1721,6 @@ struct ChildUpdates {
   }
 };
 
-template <typename R>
-__device__ const R* carried_block(const DevState& s, int c) {
-  if (!s.upd_kind[c]) return nullptr;
-  const R* F = reinterpret_cast<const R*>(s.fac_pool) + s.fac_off[c];
-  const int k = s.rank[c];
-  if (k == 0) return F;
-  return F + FacLayout::make(k, s.act_cnt[c] - k).S;
-}
 
 
 // Children's carried blocks and the position map of a box's active list.
@@ -2702,7 +2691,9 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       break;
     }
     case kStageDecouple:
-      if (count >= caps.dsplit) {
+      if (count >= caps.dsweep) {  // one wave per box with |active| <= 64
+        launch_decouple_sweep<R>(g, t, s, start, count, st);
+      } else if (count >= caps.dsplit) {
         k_decouple_small<R, 0, kSmallBox1><<<count, 256, 0, st>>>(g, t, s, start);
         k_decouple_small<R, kSmallBox1, kSmallBox><<<count, 256, 0, st>>>(g, t, s, start);
       } else {

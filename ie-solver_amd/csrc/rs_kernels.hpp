@@ -45,13 +45,25 @@ struct LaunchCaps {
   // three), in two launches; smaller levels use one launch of the larger
   // class, whose single box latency is all they pay.
   int dsplit = 8192;
+  // Levels with at least this many boxes decouple their compressed boxes
+  // with |active| <= 64 by the register sweep, one wave per box
+  // (k_decouple_sweep: more boxes per CU); smaller levels by Gauss-Jordan
+  // in 256-thread workgroups (k_decouple_small, LDS classes by dsplit:
+  // four waves shorten each box's latency, which is all a small level pays).
+  int dsweep = 4096;
 };
-constexpr int kNumCaps = 7;
+constexpr int kNumCaps = 8;
 
 template <typename R>
 void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
                          const DevState& s, int level, int start, int count,
                          double tol, const LaunchCaps& caps, hipStream_t st);
+
+// Decouple of one level's compressed boxes with |active| <= 64, one wave
+// per box (rs_sweep.hip); the other boxes exit at once.
+template <typename R>
+void launch_decouple_sweep(const DevGeom& g, const DevTree& t, const DevState& s, int start,
+                           int count, hipStream_t st);
 
 // Roots (slots 0 .. n_roots-1, one per problem): active list, dense
 // skeleton block, inverse.

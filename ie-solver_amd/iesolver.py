@@ -47,7 +47,7 @@ EXPORTS = [
     "iegpu_autotune", "iegpu_set_overlap", "iegpu_set_keep_operator", "iegpu_sparse_matvec",
     "iegpu_factor_solve_device", "iegpu_update_geometry", "iegpu_update_info",
 ]
-NUM_LAUNCH_CAPS = 7
+NUM_LAUNCH_CAPS = 8
 RANK_FEATURES = 9
 RANK_MODEL_LEN = 1429
 RANK_PARAMS = 1409

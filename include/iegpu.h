@@ -244,10 +244,14 @@ int iegpu_set_overlap(iegpu_ctx* ctx, int on);
  * the 4- and 8-wave row compressions, the 8-wave register QRCP and the
  * general QRCP (default 1024 each); [6] the box count from which a level
  * decouples its small boxes in two LDS size classes (two launches, higher
- * occupancy for n <= 44) instead of one (default 8192).  Each in
- * [1, 2^20].  Launch shapes only: every box's arithmetic is the same for
- * any values.  No reference counterpart. */
-#define IEGPU_NUM_LAUNCH_CAPS 7
+ * occupancy for n <= 44) instead of one (default 8192); [7] the box count
+ * from which a level decouples its compressed boxes with |active| <= 64 by
+ * a register Gauss-Jordan sweep, one wave per box, instead of a
+ * Gauss-Jordan inverse by a 256-thread workgroup per box followed by the
+ * products (default 4096).  Each in [1, 2^20].  Launch shapes [0..6]
+ * change no box's arithmetic; the two decouple forms agree to rounding.
+ * No reference counterpart. */
+#define IEGPU_NUM_LAUNCH_CAPS 8
 int iegpu_set_launch_caps(iegpu_ctx* ctx, const int* caps, int n);
 int iegpu_get_launch_caps(const iegpu_ctx* ctx, int* caps, int n);
 

@@ -194,10 +194,13 @@ def test_batch_arguments():
 
 def test_launch_caps_arguments():
     s = iesolver.RSSolver(device=-1)
-    assert list(s.launch_caps()) == [1024] * 6 + [8192]
-    s.set_launch_caps([64, 256, 1024, 4096, 1, 2, 3])
-    assert list(s.launch_caps()) == [64, 256, 1024, 4096, 1, 2, 3]
+    assert list(s.launch_caps()) == [1024] * 6 + [8192, 4096]
+    s.set_launch_caps([64, 256, 1024, 4096, 1, 2, 3, 2])
+    assert list(s.launch_caps()) == [64, 256, 1024, 4096, 1, 2, 3, 2]
     with pytest.raises(iesolver.IeGpuError):
         s.set_launch_caps([0] * iesolver.NUM_LAUNCH_CAPS)
+    with pytest.raises(iesolver.IeGpuError):
+        s.set_launch_caps([64] * 7 + [(1 << 20) + 1])
+    assert list(s.launch_caps()) == [64, 256, 1024, 4096, 1, 2, 3, 2]
     with pytest.raises(iesolver.IeGpuError):
         s.set_launch_caps([64] * 3)
