@@ -218,6 +218,29 @@ def set_form(s, args):
     return model_info
 
 
+def host_cpu():
+    """The host's CPU model and core counts (the CPU baseline's cores)."""
+    model, phys = None, set()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cur = {}
+            for line in fh:
+                if ":" in line:
+                    k, v = [x.strip() for x in line.split(":", 1)]
+                    cur[k] = v
+                    if k == "model name" and model is None:
+                        model = v
+                elif cur:
+                    phys.add((cur.get("physical id"), cur.get("core id")))
+                    cur = {}
+            if cur:
+                phys.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return dict(model=model, logical=os.cpu_count(), physical=len(phys) or None, usable=usable)
+
+
 def cpu_baseline(g, f, cfg, threads, reps):
     import oracle
     o = oracle.OracleProblem(g, leaf_cap=cfg["leaf"])
@@ -257,7 +280,7 @@ def time_update(s, cfg, args):
         n_dirty, inc, marks = s.update_info()
         f1 = f_of(g1)
         x_inc, _ = s.solve(f1)
-        s.set_keep_operator(False)  # drops the marks: the next factor covers every box
+        s.discard_update()  # the next factor covers every box
         s.factor(cfg["tol"], iesolver.FP64)
         full_ms = s.timings()["factor_ms"]
         x_full, _ = s.solve(f1)
@@ -271,6 +294,83 @@ def time_update(s, cfg, args):
     return dict(frames=frames, median_incremental_factor_ms=float(np.median(inc)),
                 median_full_factor_ms=float(np.median(full)),
                 speedup=float(np.median(full) / np.median(inc)))
+
+
+def mixed_subrecord(s, f, mu_exact, cfg, args):
+    """The FP32 factorization (learned-rank randomized ID at tol 1e-6) as
+    the preconditioner of FP64 iterative refinement to a relative residual
+    of 1e-10, timed over the same steps as the headline (untimed in it).
+    iegpu_solve_refined takes host vectors, so b and x cross PCIe in the
+    step."""
+    import torch
+    import iesolver
+    tol32 = 1e-6
+    for _ in range(args.warmup):
+        s.factor(tol32, iesolver.FP32)
+        s.solve_refined(f, 1e-10, 10)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.factor(tol32, iesolver.FP32)
+        x, _, its, res = s.solve_refined(f, 1e-10, 10)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    N = f.size
+    return dict(form="fp32_learned_factor+fp64_refinement", ms_per_step=round(ms, 4),
+                unknowns_per_s=round(N / (ms * 1e-3), 1), factor_id_tol=tol32,
+                corrections=its, rel_residual=res,
+                rel_err_vs_exact=float(np.linalg.norm(x - mu_exact) / np.linalg.norm(mu_exact)),
+                residual_operator="matrix-free FP64 K x, O(N^2) pair evaluations",
+                note="b and x cross PCIe inside the step")
+
+
+def batch_subrecord(args):
+    """The 256-problem batch (learned and plain forms), a few steps each."""
+    import torch
+    import iesolver
+    cfg = CONFIGS["batch256"]
+    geoms, meta = G.star_batch(cfg["n_problems"], seed=cfg["seed"], nmin=cfg["nmin"],
+                               nmax=cfg["nmax"])
+    f = G.batch_data(geoms, seed=cfg["seed"])
+    dev = torch.device("cuda:0")
+    s = iesolver.RSSolver(0)
+    s.set_geometry_batch(geoms)
+    s.build_tree(cfg["leaf"])
+    fd = torch.from_numpy(f).to(dev)
+    xd = torch.empty_like(fd)
+    N = s.ndof
+
+    def run(nsteps):
+        for _ in range(nsteps):
+            s.factor_solve_device(cfg["tol"], iesolver.FP64, fd.data_ptr(), xd.data_ptr())
+
+    out = dict(workload="laplace_batch256", n_problems=cfg["n_problems"], n_unknowns=N,
+               problem_sizes=[int(meta["sizes"].min()), int(meta["sizes"].max())])
+    steps = max(2, args.steps // 4)
+    for form in ("plain", "learned"):
+        if form == "learned":
+            s.set_rank_model(np.load(os.path.join(ROOT, "ie-solver_amd",
+                                                  "rank_model_default.npy")), margin=3)
+            s.set_id_method(iesolver.ID_RANDOMIZED, 0)
+        else:
+            s.set_id_method(iesolver.ID_QRCP)
+        run(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(steps)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        out[form] = dict(ms_per_step=round(ms, 4), unknowns_per_s=round(N / (ms * 1e-3), 1),
+                         steps=steps)
+    y, _ = s.apply(xd.cpu().numpy())
+    off = s.dof_offsets
+    out["learned"]["worst_backward_error"] = float(max(
+        np.linalg.norm(y[off[p]:off[p + 1]] - f[off[p]:off[p + 1]]) /
+        np.linalg.norm(f[off[p]:off[p + 1]]) for p in range(cfg["n_problems"])))
+    out["speedup_learned_vs_plain"] = round(out["plain"]["ms_per_step"] /
+                                            out["learned"]["ms_per_step"], 4)
+    s.close()
+    return out
 
 
 def run_batch(args, cfg):
@@ -354,7 +454,8 @@ def run_batch(args, cfg):
 
     cpu, parity = None, None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        info_cpu = host_cpu()
+        threads = info_cpu["usable"] or 1
         k = cfg["cpu_problems"]
         times, parts = [], []
         for p in range(k):
@@ -365,7 +466,8 @@ def run_batch(args, cfg):
         cpu = dict(value=round(n_s / sum(times), 1), unit="unknowns/s", cores=threads, kind="port",
                    sample=f"the batch's first {k} problems ({n_s} unknowns), each factored and "
                           f"solved once after 1 warm-up, OpenMP over boxes, BLAS threads=1",
-                   seconds=round(sum(times), 4))
+                   seconds=round(sum(times), 4), cpu_model=info_cpu["model"],
+                   cpu_logical=info_cpu["logical"], cpu_physical_cores=info_cpu["physical"])
         parity = float(max(parts))
 
     out = {
@@ -442,6 +544,10 @@ def main():
     ap.add_argument("--no-fp32", action="store_true",
                     help="batch workload: skip the FP32 + refinement form")
     ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the FP32 + FP64-refinement sub-record")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 256-problem batch sub-record")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip per-stage HIP events (roofline fields become null)")
     ap.add_argument("--no-update", action="store_true",
@@ -525,12 +631,18 @@ def main():
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        # Every core this process may run on (no cap), and the reference's
+        # default of 4 threads (ie_solver/io/io.h:14).
+        info_cpu = host_cpu()
+        threads = info_cpu["usable"] or 1
         t_cpu, mu_cpu = cpu_baseline(g, f, cfg, threads, args.cpu_reps)
+        t_cpu4, _ = cpu_baseline(g, f, cfg, 4, max(1, args.cpu_reps // 2))
         cpu = dict(value=round(N / t_cpu, 1), unit="unknowns/s", cores=threads, kind="port",
                    sample=f"full workload ({args.config}, N={N}), median of {args.cpu_reps} "
                           f"factor+solve runs after 1 warm-up, OpenMP over boxes, BLAS threads=1",
-                   seconds=round(t_cpu, 4))
+                   seconds=round(t_cpu, 4), cpu_model=info_cpu["model"],
+                   cpu_logical=info_cpu["logical"], cpu_physical_cores=info_cpu["physical"],
+                   value_4_threads=round(N / t_cpu4, 1), seconds_4_threads=round(t_cpu4, 4))
         parity = float(np.linalg.norm(mu - mu_cpu) / np.linalg.norm(mu_cpu))
 
     out = {
@@ -571,8 +683,13 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if not args.no_mixed and cfg["shape"] == "wavy":
+        # the headline form's ID method, in FP32 (the model predicts per tol)
+        out["mixed_fp32_refined"] = mixed_subrecord(s, f, mu_exact, cfg, args)
     if not args.no_update and cfg["shape"] == "wavy":
         out["geometry_update"] = time_update(s, cfg, args)
+    if not args.no_batch and cfg["shape"] == "wavy":
+        out["batch256"] = batch_subrecord(args)
     print(json.dumps(out), flush=True)
     s.close()
     return 0

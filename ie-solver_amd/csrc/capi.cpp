@@ -238,6 +238,7 @@ struct iegpu_ctx {
   double last_tol = 0;  // id_tol and precision of the stored factorization
   int last_prec = -1;
   int64_t kept_top = 0;  // factor-pool elements in use after the last factor
+  int64_t full_top = 0;  // the same after the last full factorization
 
   int ndof() const { return n_points * sol_dim; }
   size_t elem() const { return precision == IEGPU_FP32 ? 4 : 8; }
@@ -323,6 +324,7 @@ struct iegpu_ctx {
     s.rid_margin = rid_margin;
     s.feat = record_features ? d_feat.p : nullptr;
     s.dirty = inc_run ? s_dirty.p : nullptr;
+    s.kept_top = inc_run ? kept_top : 0;
     return s;
   }
 };
@@ -822,6 +824,29 @@ int iegpu_update_geometry(iegpu_ctx* c, int n_points, const double* points,
   return IEGPU_OK;
 }
 
+int iegpu_update_geometry_batch(iegpu_ctx* c, int n_problems, const int* point_offsets,
+                                const double* points, const double* normals,
+                                const double* curvatures, const double* weights, int* tree_kept) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (tree_kept) *tree_kept = 0;
+  if (!c->have_tree) return fail(c, IEGPU_ERR_INVALID, "build_tree first");
+  if (n_problems != c->n_prob || !point_offsets)
+    return fail(c, IEGPU_ERR_INVALID, "an update keeps the number of problems");
+  for (int p = 0; p <= n_problems; p++)
+    if (point_offsets[p] != c->h_poff[p])
+      return fail(c, IEGPU_ERR_INVALID,
+                  "an update keeps every problem's number of points (problem " +
+                      std::to_string(p > 0 ? p - 1 : 0) + " differs)");
+  return iegpu_update_geometry(c, point_offsets[n_problems], points, normals, curvatures,
+                               weights, tree_kept);
+}
+
+int iegpu_discard_update(iegpu_ctx* c) {
+  if (!c) return IEGPU_ERR_INVALID;
+  c->update_pending = false;
+  return IEGPU_OK;
+}
+
 int iegpu_update_info(const iegpu_ctx* c, int* n_dirty, int* last_incremental, int* dirty) {
   if (!c) return IEGPU_ERR_INVALID;
   if (n_dirty) *n_dirty = c->n_dirty;
@@ -1305,7 +1330,11 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
     }
     // After iegpu_update_geometry on the same tree, only the marked boxes
     // are re-factored (same tolerance and precision as the kept factors).
-    c->inc_run = c->update_pending && c->last_tol == id_tol && c->last_prec == precision;
+    // Boxes whose new factors outgrow their kept slot take new ones past
+    // the pool top; once the top passes twice a full factorization's, the
+    // next factorization is a full one, which packs the pool again.
+    c->inc_run = c->update_pending && c->last_tol == id_tol && c->last_prec == precision &&
+                 c->kept_top <= 2 * c->full_top;
     c->update_pending = false;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -1421,6 +1450,7 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
     HIP_TRY(hipMemcpy(&top, c->s_fac_top.p, sizeof(top), hipMemcpyDeviceToHost));
     c->fac_bytes = (double)top * c->elem();
     c->kept_top = top;
+    if (!c->last_incremental) c->full_top = top;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (errbits & kErrSingular) {

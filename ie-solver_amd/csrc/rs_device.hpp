@@ -186,6 +186,10 @@ struct DevState {
   // previous factorization; the per-level kernels skip it and k_scan gives
   // it no scratch, no list entry and no new factor slot.
   const int* dirty;
+  // Factor-pool elements in use before an incremental run (0 otherwise): a
+  // re-factored box keeps its slot when that slot lies below this mark and
+  // its new factors fit it.
+  int64_t kept_top;
 };
 
 // False once a pool overflow was reported: the level's remaining kernels

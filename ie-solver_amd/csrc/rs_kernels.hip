@@ -225,7 +225,13 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
     // when they do not fit its LDS buffer.
     s.dec_sz[b] = n > kSmallBox ? 2 * (int64_t)n * n : 0;
     s.row_sz[b] = kind == kQrcpLds ? (m - proxy_rows(g, level)) / 2 : 0;
-    s.fac_sz[b] = (n > 0) ? fac_slot_size(n) : 0;
+    // fac_sz holds a box's slot size; in an incremental run a re-factored
+    // box whose kept slot fits its new factors reuses it (marked by the
+    // negated size, which k_scan restores) instead of taking a new one.
+    const int64_t need = (n > 0) ? fac_slot_size(n) : 0;
+    const int64_t old = s.fac_sz[b];
+    const bool reuse = s.dirty != nullptr && need <= old && s.fac_off[b] + old <= s.kept_top;
+    s.fac_sz[b] = reuse ? -old : need;
   }
   // Sketch width: fixed, or from the rank model (0: straight to the full
   // QRCP when the predicted rank does not fit the widest sketch).
@@ -285,7 +291,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     const int kind = s.qkind[b];
     own[0] += s.id_sz[b];
     own[1] += s.row_sz[b];
-    own[2] += s.fac_sz[b];
+    own[2] += max(s.fac_sz[b], int64_t(0));
     own[3] += ((kind == kQrcpReg || kind == kQrcpStack) && s.act_cnt[b] > kHalfCols) ? 1 : 0;
     own[7] += ((kind == kQrcpReg || kind == kQrcpStack) && s.act_cnt[b] <= kHalfCols) ? 1 : 0;
     own[4] += (kind == kQrcpLds) ? 1 : 0;
@@ -340,12 +346,16 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     // while the level below it is still being decoupled.
     s.id_off[b] = (level & 1) * s.id_cap + off[0];
     s.row_off[b] = off[1];
-    s.fac_off[b] = fbase + off[2];
+    if (s.fac_sz[b] < 0) {  // keeps its slot
+      s.fac_sz[b] = -s.fac_sz[b];
+    } else {
+      s.fac_off[b] = fbase + off[2];
+      off[2] += s.fac_sz[b];
+    }
     s.dec_off[b] = off[10];
     off[10] += s.dec_sz[b];
     off[0] += s.id_sz[b];
     off[1] += s.row_sz[b];
-    off[2] += s.fac_sz[b];
     if (kind == kQrcpReg || kind == kQrcpStack) {
       if (s.act_cnt[b] > kHalfCols) lreg[off[3]++] = b;
       else lhalf[off[7]++] = b;

@@ -68,7 +68,7 @@ __device__ inline int problem_of(const DevGeom& g, int i) {
 __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
                                                  const double* wn, double* y) {
   __shared__ double sx[kTile], sy[kTile], sa[kTile], sb[kTile], sm0[kTile],
-      sm1[kTile];
+      sm1[kTile], sd0[kTile], sd1[kTile];
   const int i = blockIdx.x * kTile + threadIdx.x;
   const bool live = i < g.n_points;
   const double xi = live ? g.pts[2 * i] : 0, yi = live ? g.pts[2 * i + 1] : 0;
@@ -90,20 +90,31 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
   for (int base = blo; base < bhi; base += kTile) {
     const int j = base + threadIdx.x;
     if (j < bhi) {
-      const double w = g.wts[j];
+      const double w = g.wts[j], kap = g.curv[j];
+      const double n0 = g.nrm[2 * j], n1 = g.nrm[2 * j + 1];
       sx[threadIdx.x] = g.pts[2 * j];
       sy[threadIdx.x] = g.pts[2 * j + 1];
+      // sd: the source's term at a target on the same point (its own node
+      // or a coincident one), whose entry is the diagonal value of
+      // kernel.cpp:149-157,245-263 for this source (rank-one parts aside).
       if (stokes) {
-        sa[threadIdx.x] = w * (1.0 / M_PI) * g.nrm[2 * j];
-        sb[threadIdx.x] = w * (1.0 / M_PI) * g.nrm[2 * j + 1];
-        sm0[threadIdx.x] = x[2 * j];
-        sm1[threadIdx.x] = x[2 * j + 1];
+        sa[threadIdx.x] = w * (1.0 / M_PI) * n0;
+        sb[threadIdx.x] = w * (1.0 / M_PI) * n1;
+        const double x0 = x[2 * j], x1 = x[2 * j + 1];
+        sm0[threadIdx.x] = x0;
+        sm1[threadIdx.x] = x1;
+        const double pot = -0.5 * kap * w * (1.0 / M_PI);
+        const double tx = -n1 * x0 + n0 * x1;
+        sd0[threadIdx.x] = -0.5 * x0 - pot * n1 * tx;
+        sd1[threadIdx.x] = -0.5 * x1 + pot * n0 * tx;
       } else if (neumann) {
         sa[threadIdx.x] = w * (1.0 / (2 * M_PI)) * x[j];
+        sd0[threadIdx.x] = (-0.5 + 0.5 * kap * w * (1.0 / (2 * M_PI))) * x[j];
       } else {
         const double s = -w * (1.0 / (2 * M_PI)) * x[j];
-        sa[threadIdx.x] = s * g.nrm[2 * j];
-        sb[threadIdx.x] = s * g.nrm[2 * j + 1];
+        sa[threadIdx.x] = s * n0;
+        sb[threadIdx.x] = s * n1;
+        sd0[threadIdx.x] = (0.5 + 0.5 * kap * w * (1.0 / (2 * M_PI))) * x[j];
       }
     }
     __syncthreads();
@@ -112,7 +123,11 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
       for (int q = q0; q < q1; q++) {
         const double r0 = xi - sx[q], r1 = yi - sy[q];
         const double d2 = fma(r0, r0, r1 * r1);
-        if (d2 == 0) continue;  // self term added below
+        if (d2 == 0) {  // same point: the source's diagonal value
+          a0 += sd0[q];
+          a1 += sd1[q];
+          continue;
+        }
         const double inv = rcp_f64(d2);
         const double pot = fma(r0, sa[q], r1 * sb[q]) * inv * inv;
         const double proj = fma(r0, sm0[q], r1 * sm1[q]) * pot;
@@ -123,36 +138,35 @@ __global__ __launch_bounds__(kTile) void k_apply(DevGeom g, const double* x,
       for (int q = q0; q < q1; q++) {
         const double r0 = xi - sx[q], r1 = yi - sy[q];
         const double d2 = fma(r0, r0, r1 * r1);
-        if (d2 == 0) continue;
+        if (d2 == 0) {
+          a0 += sd0[q];
+          continue;
+        }
         a0 = fma(fma(r0, tn0, r1 * tn1) * sa[q], rcp_f64(d2), a0);
       }
     } else {
       for (int q = q0; q < q1; q++) {
         const double r0 = xi - sx[q], r1 = yi - sy[q];
         const double d2 = fma(r0, r0, r1 * r1);
-        if (d2 == 0) continue;
+        if (d2 == 0) {
+          a0 += sd0[q];
+          continue;
+        }
         a0 = fma(fma(r0, sa[q], r1 * sb[q]), rcp_f64(d2), a0);
       }
     }
     __syncthreads();
   }
   if (!live) return;
-  const double w = g.wts[i], kap = g.curv[i];
-  const double n0 = g.nrm[2 * i], n1 = g.nrm[2 * i + 1];
-  if (stokes) {
+  if (stokes) {  // plus the n n^T part of every entry (kernel.cpp:245-263)
     const double c = wn[prob];
-    // self 2x2 block without its n n^T part (kernel.cpp:245-263)
-    const double pot = -0.5 * kap * w * (1.0 / M_PI);
-    const double t0 = -n1, t1 = n0;
-    const double x0 = x[2 * i], x1 = x[2 * i + 1];
-    const double tx = t0 * x0 + t1 * x1;
-    y[2 * i] = a0 - 0.5 * x0 + pot * t0 * tx + n0 * c;
-    y[2 * i + 1] = a1 - 0.5 * x1 + pot * t1 * tx + n1 * c;
+    y[2 * i] = a0 + tn0 * c;
+    y[2 * i + 1] = a1 + tn1 * c;
   } else if (neumann) {
     // sum_j w_j x_j (every entry's rank-one part, the diagonal's too)
-    y[i] = a0 + wn[prob] + (-0.5 + 0.5 * kap * w * (1.0 / (2 * M_PI))) * x[i];
+    y[i] = a0 + wn[prob];
   } else {
-    y[i] = a0 + (0.5 + 0.5 * kap * w * (1.0 / (2 * M_PI))) * x[i];
+    y[i] = a0;
   }
 }
 

@@ -46,6 +46,7 @@ EXPORTS = [
     "iegpu_set_geometry_batch", "iegpu_set_launch_caps", "iegpu_get_launch_caps",
     "iegpu_autotune", "iegpu_set_overlap", "iegpu_set_keep_operator", "iegpu_sparse_matvec",
     "iegpu_factor_solve_device", "iegpu_update_geometry", "iegpu_update_info",
+    "iegpu_update_geometry_batch", "iegpu_discard_update",
 ]
 NUM_LAUNCH_CAPS = 8
 RANK_FEATURES = 9
@@ -114,6 +115,10 @@ def load():
     L.iegpu_set_launch_caps.argtypes = [vp, _ip, ctypes.c_int]
     L.iegpu_get_launch_caps.argtypes = [vp, _ip, ctypes.c_int]
     L.iegpu_autotune.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp]
+    L.iegpu_update_geometry.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]
+    L.iegpu_update_geometry_batch.argtypes = [vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _ip]
+    L.iegpu_update_info.argtypes = [vp, _ip, _ip, _ip]
+    L.iegpu_discard_update.argtypes = [vp]
     _lib = L
     return L
 
@@ -199,16 +204,30 @@ class RSSolver:
         dict, or a list of them for a batch.  Returns True when the tree is
         unchanged; the next factor then re-factors only the marked boxes
         (update_info)."""
-        geoms = g if isinstance(g, (list, tuple)) else [g]
+        batch = isinstance(g, (list, tuple))
+        geoms = g if batch else [g]
         arrs = [_d(np.concatenate([q["points"] for q in geoms])).reshape(-1),
                 _d(np.concatenate([q["normals"] for q in geoms])).reshape(-1),
                 _d(np.concatenate([q["curvatures"] for q in geoms])),
                 _d(np.concatenate([q["weights"] for q in geoms]))]
         kept = ctypes.c_int(0)
-        self._check(self.L.iegpu_update_geometry(self.h, arrs[2].shape[0],
-                                                 *[_pd(a) for a in arrs], ctypes.byref(kept)))
+        if batch:
+            # every problem keeps its number of points (checked by the library)
+            off = np.zeros(len(geoms) + 1, dtype=np.int32)
+            off[1:] = np.cumsum([q["points"].shape[0] for q in geoms])
+            off = np.ascontiguousarray(off)
+            self._check(self.L.iegpu_update_geometry_batch(
+                self.h, len(geoms), _pi(off), *[_pd(a) for a in arrs], ctypes.byref(kept)))
+        else:
+            self._check(self.L.iegpu_update_geometry(self.h, arrs[2].shape[0],
+                                                     *[_pd(a) for a in arrs],
+                                                     ctypes.byref(kept)))
         self._arrays = arrs
         return bool(kept.value)
+
+    def discard_update(self):
+        """Drop a pending update's marks: the next factor covers every box."""
+        self._check(self.L.iegpu_discard_update(self.h))
 
     def update_info(self):
         """(boxes marked by the last update, whether the last factorization

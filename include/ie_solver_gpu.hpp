@@ -167,6 +167,7 @@ class Boundary {
  public:
   enum BoundaryShape { CIRCLE, ROUNDED_SQUARE, ANNULUS, DONUT, CUBIC_SPLINE, EX1, EX2, EX3 };
   int num_outer_nodes = -1;
+  std::vector<double> perturbation_parameters;
   std::vector<double> points, normals, curvatures, weights;
   std::vector<Hole> holes;
   ie_Mat boundary_values;
@@ -382,7 +383,7 @@ class QuadTree {
   void initialize_tree(Boundary* b, const std::vector<double>& dom,
                        int sol_dim, int dom_dim) {
     delete owned_;
-    owned_ = new Copy(*b);
+    owned_ = clone_shape(*b);
     boundary = owned_;
     domain_points = dom;
     solution_dimension = sol_dim;
@@ -403,16 +404,18 @@ class QuadTree {
   // (iegpu_update_geometry; a tree it cannot keep is rebuilt).  With
   // another count the next skeletonize builds a new tree.  min / max keep
   // the first tree's bounds, as the reference's do.
+  // The new arrays are copied into the tree's own boundary, as the
+  // reference does (quadtree.cpp:718-725): the caller may free b after.
   void perturb(const Boundary& b) {
-    const bool same_n = owned_ && b.weights.size() == owned_->weights.size();
-    delete owned_;
-    owned_ = new Copy(b);
-    boundary = owned_;
+    if (!owned_) throw std::invalid_argument("perturb before initialize_tree");
+    const bool same_n = b.weights.size() == owned_->weights.size();
+    copy_arrays(b, owned_);
     if (pde_ >= 0 && same_n) {
       int kept = 0;
-      detail::check(ctx, iegpu_update_geometry(ctx, (int)b.weights.size(), b.points.data(),
-                                               b.normals.data(), b.curvatures.data(),
-                                               b.weights.data(), &kept));
+      detail::check(ctx, iegpu_update_geometry(ctx, (int)owned_->weights.size(),
+                                               owned_->points.data(), owned_->normals.data(),
+                                               owned_->curvatures.data(),
+                                               owned_->weights.data(), &kept));
     } else {
       pde_ = -1;
     }
@@ -436,18 +439,24 @@ class QuadTree {
   }
 
  private:
+  // quadtree.cpp:718-725 (and the copies of quadtree.cpp:78-86).
+  static void copy_arrays(const Boundary& b, Boundary* to) {
+    to->points = b.points;
+    to->normals = b.normals;
+    to->weights = b.weights;
+    to->curvatures = b.curvatures;
+    to->boundary_values = b.boundary_values;
+    to->perturbation_parameters = b.perturbation_parameters;
+    to->holes = b.holes;
+    to->num_outer_nodes = b.num_outer_nodes;
+    to->boundary_shape = b.boundary_shape;
+  }
+  // A boundary of b's shape holding b's arrays, which the tree owns
+  // (quadtree.cpp:47-86).  Shapes this header does not define keep b's
+  // domain test by reference, so b must outlive the tree for them.
   class Copy : public Boundary {
    public:
-    explicit Copy(const Boundary& b) : inner_(&b) {
-      points = b.points;
-      normals = b.normals;
-      curvatures = b.curvatures;
-      weights = b.weights;
-      holes = b.holes;
-      boundary_values = b.boundary_values;
-      boundary_shape = b.boundary_shape;
-      num_outer_nodes = b.num_outer_nodes;
-    }
+    explicit Copy(const Boundary& b) : inner_(&b) {}
     void initialize(int, BoundaryCondition) override {}
     bool is_in_domain(const Vec2& a) override {
       return const_cast<Boundary*>(inner_)->is_in_domain(a);
@@ -456,7 +465,18 @@ class QuadTree {
    private:
     const Boundary* inner_;
   };
-  Copy* owned_ = nullptr;
+  static Boundary* clone_shape(const Boundary& b) {
+    Boundary* c;
+    switch (b.boundary_shape) {
+      case Boundary::CIRCLE: c = new Circle(); break;
+      case Boundary::ANNULUS: c = new Annulus(); break;
+      case Boundary::DONUT: c = new Donut(); break;
+      default: c = new Copy(b); break;
+    }
+    copy_arrays(b, c);
+    return c;
+  }
+  Boundary* owned_ = nullptr;
   int pde_ = -1;
 };
 

@@ -95,6 +95,21 @@ int iegpu_update_geometry(iegpu_ctx* ctx, int n_points, const double* points,
                           const double* normals, const double* curvatures,
                           const double* weights, int* tree_kept);
 
+/* iegpu_update_geometry for a batch (iegpu_set_geometry_batch): the
+ * point_offsets (n_problems + 1) must equal the batch's, so that every
+ * problem keeps its number of points; IEGPU_ERR_INVALID otherwise.  No
+ * reference counterpart (one boundary per QuadTree). */
+int iegpu_update_geometry_batch(iegpu_ctx* ctx, int n_problems, const int* point_offsets,
+                                const double* points, const double* normals,
+                                const double* curvatures, const double* weights,
+                                int* tree_kept);
+
+/* Drops a pending geometry update's marks: the next factorization covers
+ * every box (a full re-factorization of the kept tree).  No reference
+ * counterpart (the reference clears QuadTreeNode::compressed,
+ * quadtree.cpp:841-961 via reset). */
+int iegpu_discard_update(iegpu_ctx* ctx);
+
 /* After iegpu_update_geometry: the number of marked boxes, whether the last
  * factorization re-factored only them (1) or every box (0), and per box id
  * the marks (dirty: n_boxes ints, may be null).  No reference counterpart

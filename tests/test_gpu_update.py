@@ -99,7 +99,7 @@ def test_update_local_bump_matches_fresh_tree(name, pde, tol, factor):
         u = s.evaluate(tg, mu)
         # every box whose ID or decouple inputs changed was re-factored, so
         # a full factorization on the same tree gives the same bits
-        s.set_keep_operator(False)  # drops nothing but the marks
+        s.discard_update()  # the next factorization covers every box
         s.factor(tol, FP64)
         assert not s.update_info()[1]
         m_full, _ = s.solve(f)
@@ -230,7 +230,59 @@ def test_update_randomized_forms_equal_full_refactor(learned):
         n_dirty, inc, marks = s.update_info()
         assert inc and n_dirty < marks.size
         mu, _ = s.solve(f)
-        s.set_keep_operator(False)
+        s.discard_update()
         s.factor(tol, FP64)
         m_full, _ = s.solve(f)
         assert np.array_equal(mu, m_full), (learned, frame)
+
+
+def test_many_incremental_updates_keep_the_pool_bounded():
+    # 24 consecutive incremental factorizations with no full one between
+    # them: re-factored boxes reuse their kept slots, and the pool top stays
+    # within twice a full factorization's (past it, the next factorization
+    # is a full one); the last solution equals a full factorization's.
+    tol = 1e-10
+    n = 16384
+    s = _solver(G.star(n))
+    s.factor(tol, FP64)
+    full_bytes = s.timings()["factor_bytes"]
+    incs = 0
+    for frame in range(24):
+        g1 = G.bump_star(n, 0.004 * (1 + frame % 3), 0.3 + 0.25 * frame)
+        assert s.update_geometry(g1)
+        s.factor(tol, FP64)
+        incs += s.update_info()[1]
+        fb = s.timings()["factor_bytes"]
+        assert fb <= 2 * full_bytes * 1.25, (frame, fb, full_bytes)
+    assert incs >= 20
+    f = G.harmonic_data(g1["points"], *G.exterior_sources(5))
+    mu, _ = s.solve(f)
+    s.discard_update()
+    s.factor(tol, FP64)
+    m_full, _ = s.solve(f)
+    assert np.array_equal(mu, m_full)
+
+
+def test_update_marks_cover_changed_box_inputs():
+    # Full factorizations of the kept tree before and after a bump: every
+    # box whose active list or skeleton differs between them (its ID or
+    # decouple inputs changed) carries a mark, and the marks stay a small
+    # part of the tree.
+    tol = 1e-10
+    n = 8192
+    g0 = G.star(n)
+    s = _solver(g0)
+    s.factor(tol, FP64)
+    nb = s.tree_info()[1]
+    before = [(s.box_list(b, 2).tolist(), s.box_list(b, 3).tolist()) for b in range(nb)]
+    g1 = G.bump_star(n, 0.02, 1.3)
+    assert s.update_geometry(g1)
+    _, _, marks = s.update_info()
+    s.discard_update()
+    s.factor(tol, FP64)
+    after = [(s.box_list(b, 2).tolist(), s.box_list(b, 3).tolist()) for b in range(nb)]
+    changed = np.array([before[b] != after[b] for b in range(nb)])
+    print(f"{changed.sum()} boxes changed, {marks.sum()} marked of {nb}", flush=True)
+    assert changed.any()
+    assert np.all(marks[changed] == 1)
+    assert marks.sum() <= max(4 * changed.sum(), nb // 4)

@@ -204,3 +204,19 @@ def test_launch_caps_arguments():
     assert list(s.launch_caps()) == [64, 256, 1024, 4096, 1, 2, 3, 2]
     with pytest.raises(iesolver.IeGpuError):
         s.set_launch_caps([64] * 3)
+
+
+def test_batch_update_keeps_each_problem_size():
+    # A batch update whose total matches but whose problems trade points is
+    # rejected (the kept tree's point offsets would assign nodes to the
+    # wrong problem).
+    geoms = [G.star(300), G.star(500, amp=0.1, k=5)]
+    s = iesolver.RSSolver(device=-1)
+    s.set_geometry_batch(geoms)
+    s.build_tree(64)
+    shifted = [G.star(301), G.star(499, amp=0.1, k=5)]
+    with pytest.raises(iesolver.IeGpuError, match="number of points"):
+        s.update_geometry(shifted)
+    with pytest.raises(iesolver.IeGpuError):
+        s.update_geometry(geoms[:1])
+    s.discard_update()
