@@ -187,6 +187,8 @@ struct iegpu_ctx {
   // interior evaluation buffers
   DevBuf<double> e_tg, e_mu, e_alpha, e_partial, e_out;
   DevBuf<int> e_mask;
+  // the device address of the allocation-free evaluation's static scratch
+  double* e_scratch = nullptr;
 
   // factor state
   int ncap = 0;
@@ -639,6 +641,12 @@ int iegpu_create(iegpu_ctx** out, int device) {
     return IEGPU_ERR_HIP;
   if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
     (void)hipStreamDestroy(c->stream);
+    return IEGPU_ERR_HIP;
+  }
+  // The static scratch of iegpu_evaluate_noalloc on this device.
+  if (eval_scratch_address(device, &c->e_scratch) != 0) {
+    (void)hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream2);
     return IEGPU_ERR_HIP;
   }
   *out = c.release();
@@ -1780,6 +1788,34 @@ int iegpu_evaluate(iegpu_ctx* c, int n_targets, const double* targets,
   } catch (const HipError& e) {
     return from_hip(c, e);
   }
+  return IEGPU_OK;
+}
+
+int iegpu_evaluate_noalloc(iegpu_ctx* c, int n_targets, const double* d_targets,
+                           const int* d_in_domain, const double* d_mu, const double* d_alpha,
+                           double* d_out, void* stream) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (c->n_points == 0) return fail(c, IEGPU_ERR_INVALID, "set_geometry first");
+  if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
+  if (n_targets < 0 || (n_targets > 0 && (!d_targets || !d_mu || !d_out)))
+    return fail(c, IEGPU_ERR_INVALID, "bad evaluation arrays");
+  if (c->n_prob > 1)
+    return fail(c, IEGPU_ERR_INVALID, "interior evaluation takes one problem, not a batch");
+  if (n_targets == 0) return IEGPU_OK;
+  const int64_t need = eval_scratch_need(c->n_points, c->sol_dim, n_targets);
+  if (need > kEvalScratchElems)
+    return fail(c, IEGPU_ERR_CAPACITY,
+                "evaluation needs " + std::to_string(need) + " partial sums; the static scratch "
+                "holds " + std::to_string(kEvalScratchElems) + " (fewer targets per call)");
+  if (!c->e_scratch) return fail(c, IEGPU_ERR_HIP, "no scratch address for this device");
+  const hipError_t e0 = hipSetDevice(c->device);
+  if (e0 != hipSuccess) return fail(c, IEGPU_ERR_HIP, hipGetErrorString(e0));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  const bool holes = c->hole_dim > 0 && d_alpha;
+  launch_eval(c->geom(), n_targets, d_targets, d_in_domain, d_mu, holes ? c->n_holes : 0,
+              c->d_hole_c.p, holes ? d_alpha : nullptr, c->e_scratch, d_out, st);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, IEGPU_ERR_HIP, hipGetErrorString(e));
   return IEGPU_OK;
 }
 

@@ -14,11 +14,18 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "rs_device.hpp"
 #include "rs_eval.hpp"
 
 namespace iegpu {
+
+// Partial sums of the allocation-free evaluation (iegpu_evaluate_noalloc):
+// one copy per GPU, held until the process ends.  Host code uses the
+// address hipGetSymbolAddress returns for the current device, never this
+// name (which is a host-side stand-in, not a device address).
+static __device__ __align__(256) double scratch[kEvalScratchElems];
 
 namespace {
 
@@ -115,6 +122,30 @@ __global__ void k_eval_finish(DevGeom g, int nt, const double* tg,
 }
 
 }  // namespace
+
+int eval_scratch_address(int device, double** ptr) {
+  static std::mutex mu;
+  static double* addr[kEvalMaxDevices] = {};
+  if (device < 0 || device >= kEvalMaxDevices || !ptr) return (int)hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    // the device must be current: each GPU holds its own copy
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return (int)e;
+    if (cur != device) return (int)hipErrorInvalidDevice;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    addr[device] = static_cast<double*>(p);
+  }
+  *ptr = addr[device];
+  return 0;
+}
+
+int64_t eval_scratch_need(int64_t n_points, int sol_dim, int64_t n_targets) {
+  return (int64_t)eval_chunks(n_points) * sol_dim * n_targets;
+}
 
 int eval_chunks(int64_t n_points) {
   int64_t c = (n_points + 8191) / 8192;

@@ -46,7 +46,7 @@ EXPORTS = [
     "iegpu_set_geometry_batch", "iegpu_set_launch_caps", "iegpu_get_launch_caps",
     "iegpu_autotune", "iegpu_set_overlap", "iegpu_set_keep_operator", "iegpu_sparse_matvec",
     "iegpu_factor_solve_device", "iegpu_update_geometry", "iegpu_update_info",
-    "iegpu_update_geometry_batch", "iegpu_discard_update",
+    "iegpu_update_geometry_batch", "iegpu_discard_update", "iegpu_evaluate_noalloc",
 ]
 NUM_LAUNCH_CAPS = 8
 RANK_FEATURES = 9
@@ -119,6 +119,7 @@ def load():
     L.iegpu_update_geometry_batch.argtypes = [vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _ip]
     L.iegpu_update_info.argtypes = [vp, _ip, _ip, _ip]
     L.iegpu_discard_update.argtypes = [vp]
+    L.iegpu_evaluate_noalloc.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 

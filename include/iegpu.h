@@ -187,6 +187,21 @@ int iegpu_evaluate(iegpu_ctx* ctx, int n_targets, const double* targets,
                    const int* in_domain, const double* mu, const double* alpha,
                    double* out);
 
+/* As iegpu_evaluate, allocating nothing: device buffers d_targets (2 per
+ * target), d_in_domain (nullable), d_mu (ndof), d_alpha (hole_dim, nullable),
+ * d_out (sol_dim per target), ordered on `stream` (NULL: the context's
+ * stream), no synchronization.  The partial sums go to a static device
+ * array of 2^19 doubles per GPU (4 MiB, held until the process ends):
+ * eval_chunks * sol_dim * n_targets must fit it (eval_chunks =
+ * ceil(n_points / 8192), at most 64; e.g. 16384 targets at 262144 Laplace
+ * nodes), else IEGPU_ERR_CAPACITY.  Every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU (two streams, two contexts)
+ * race.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (HIP graph capture, serving loops). */
+int iegpu_evaluate_noalloc(iegpu_ctx* ctx, int n_targets, const double* d_targets,
+                           const int* d_in_domain, const double* d_mu, const double* d_alpha,
+                           double* d_out, void* stream);
+
 /* Sizes: tree levels, boxes, DOFs, hole border dimension. */
 int iegpu_tree_info(const iegpu_ctx* ctx, int* n_levels, int* n_boxes,
                     int* ndof, int* hole_dim);
