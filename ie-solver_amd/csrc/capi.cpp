@@ -586,11 +586,11 @@ void enqueue_border_factor(iegpu_ctx* c) {
 // g (hole_dim, device, may be NULL = 0) is the border right-hand side.
 template <typename R>
 void enqueue_solve(iegpu_ctx* c, double* x, double* alpha, hipStream_t st,
-                   const double* g = nullptr) {
+                   const double* g = nullptr, int nrhs = 1) {
   DevState s = c->dstate();
   const HostTree& T = c->tree;
   const int64_t nd = c->ndof();
-  SolveVec v{x, nd, 1};
+  SolveVec v{x, nd, nrhs};
   const bool holes = c->hole_dim > 0;
   for (int lv = T.n_levels - 1; lv >= 1; lv--) {
     const int start = T.level_start[lv];
@@ -1536,6 +1536,125 @@ int iegpu_solve(iegpu_ctx* c, const double* b, double* mu, double* alpha) {
     c->t_solve_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_solve_nrhs(iegpu_ctx* c, int nrhs, const double* b, double* mu, double* alpha) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!c->factored) return fail(c, IEGPU_ERR_INVALID, "factor first");
+  if (nrhs < 0 || (nrhs > 0 && (!b || !mu)))
+    return fail(c, IEGPU_ERR_INVALID, "bad right-hand sides");
+  const int h = c->hole_dim;
+  if (h > 0 && nrhs > 0 && !alpha) return fail(c, IEGPU_ERR_INVALID, "alpha required with holes");
+  if (nrhs == 0) return IEGPU_OK;
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t nd = c->ndof();
+    hipStream_t st = c->stream;
+    if (h > 0) {
+      // the bordered solve takes one column at a time
+      c->d_work.alloc(nd);
+      for (int q = 0; q < nrhs; q++) {
+        HIP_TRY(hipMemcpyAsync(c->d_work.p, b + nd * q, nd * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+        if (c->precision == IEGPU_FP32) enqueue_solve<float>(c, c->d_work.p, c->d_alpha.p, st);
+        else enqueue_solve<double>(c, c->d_work.p, c->d_alpha.p, st);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(mu + nd * q, c->d_work.p, nd * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(alpha + (int64_t)h * q, c->d_alpha.p, h * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+      }
+    } else {
+      // every column through each sweep kernel in one pass
+      c->d_work.alloc(nd * nrhs);
+      HIP_TRY(hipMemcpyAsync(c->d_work.p, b, nd * nrhs * sizeof(double), hipMemcpyHostToDevice,
+                             st));
+      if (c->precision == IEGPU_FP32) enqueue_solve<float>(c, c->d_work.p, nullptr, st, nullptr, nrhs);
+      else enqueue_solve<double>(c, c->d_work.p, nullptr, st, nullptr, nrhs);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(mu, c->d_work.p, nd * nrhs * sizeof(double), hipMemcpyDeviceToHost,
+                             st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_set_border(iegpu_ctx* c, int hole_dim, const double* U, const double* Psi) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
+  if (c->n_holes == 0) return fail(c, IEGPU_ERR_INVALID, "set_holes first");
+  if (hole_dim != c->hole_dim || !U || !Psi)
+    return fail(c, IEGPU_ERR_INVALID,
+                "U must be ndof x " + std::to_string(c->hole_dim) + " and Psi its transpose's shape");
+  const int64_t nd = c->ndof();
+  const int h = hole_dim;
+  for (int64_t i = 0; i < nd * h; i++)
+    if (!std::isfinite(U[i]) || !std::isfinite(Psi[i]))
+      return fail(c, IEGPU_ERR_INVALID, "non-finite border entry");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<double> PsiT(nd * h);
+    for (int k = 0; k < h; k++)  // Psi is h x ndof column-major
+      for (int64_t i = 0; i < nd; i++) PsiT[i + nd * k] = Psi[k + (int64_t)h * i];
+    c->d_U.upload(U, nd * h, c->stream);
+    c->d_PsiT.upload(PsiT, c->stream);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->factored = false;
+  } catch (const HipError& e) {
+    return from_hip(c, e);
+  }
+  return IEGPU_OK;
+}
+
+int iegpu_box_factors(iegpu_ctx* c, int box, int* k_out, int* r_out, double* T, double* L,
+                      double* U, double* Xinv, double* S) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (!c->factored) return fail(c, IEGPU_ERR_INVALID, "factor first");
+  if (box < 0 || box >= c->tree.n_boxes) return fail(c, IEGPU_ERR_INVALID, "bad box");
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    const int slot = c->tree.slot_of_id[box];
+    int n = 0, k = 0, kind = 0;
+    int64_t off = 0;
+    HIP_TRY(hipMemcpy(&n, c->s_act_cnt.p + slot, sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&k, c->s_rank.p + slot, sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&kind, c->s_upd_kind.p + slot, sizeof(int), hipMemcpyDeviceToHost));
+    const bool root = slot < c->tree.n_roots;
+    const int r = (k > 0 && !root) ? n - k : 0;
+    if (k_out) *k_out = root ? 0 : k;
+    if (r_out) *r_out = r;
+    if (r == 0 || root) return IEGPU_OK;
+    HIP_TRY(hipMemcpy(&off, c->s_fac_off.p + slot, sizeof(int64_t), hipMemcpyDeviceToHost));
+    const FacLayout fl = FacLayout::make(k, r);
+    const int64_t len = fl.S + (int64_t)k * k;
+    std::vector<double> h(len);
+    if (c->precision == IEGPU_FP32) {
+      std::vector<float> f(len);
+      HIP_TRY(hipMemcpy(f.data(), reinterpret_cast<float*>(c->s_fac_pool.p) + off,
+                        len * sizeof(float), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < len; i++) h[i] = f[i];
+    } else {
+      HIP_TRY(hipMemcpy(h.data(), reinterpret_cast<double*>(c->s_fac_pool.p) + off,
+                        len * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    auto put = [&](double* dst, int64_t at, int64_t cnt) {
+      if (dst) std::copy(h.begin() + at, h.begin() + at + cnt, dst);
+    };
+    put(T, fl.T, (int64_t)k * r);
+    put(L, fl.L, (int64_t)k * r);
+    put(U, fl.U, (int64_t)r * k);
+    put(Xinv, fl.X, (int64_t)r * r);
+    if (S) {
+      if (kind) put(S, fl.S, (int64_t)k * k);
+      else std::fill(S, S + (int64_t)k * k, 0.0);
+    }
   } catch (const HipError& e) {
     return from_hip(c, e);
   }

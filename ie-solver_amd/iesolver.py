@@ -47,6 +47,7 @@ EXPORTS = [
     "iegpu_autotune", "iegpu_set_overlap", "iegpu_set_keep_operator", "iegpu_sparse_matvec",
     "iegpu_factor_solve_device", "iegpu_update_geometry", "iegpu_update_info",
     "iegpu_update_geometry_batch", "iegpu_discard_update", "iegpu_evaluate_noalloc",
+    "iegpu_solve_nrhs", "iegpu_set_border", "iegpu_box_factors",
 ]
 NUM_LAUNCH_CAPS = 8
 RANK_FEATURES = 9
@@ -120,6 +121,9 @@ def load():
     L.iegpu_update_info.argtypes = [vp, _ip, _ip, _ip]
     L.iegpu_discard_update.argtypes = [vp]
     L.iegpu_evaluate_noalloc.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp, vp]
+    L.iegpu_solve_nrhs.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp]
+    L.iegpu_set_border.argtypes = [vp, ctypes.c_int, _dp, _dp]
+    L.iegpu_box_factors.argtypes = [vp, ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -350,6 +354,41 @@ class RSSolver:
             return mu, alpha
         self._check(self.L.iegpu_solve(self.h, _pd(b), _pd(mu), None))
         return mu, None
+
+    def solve_many(self, B):
+        """Solve for the columns of B (ndof x nrhs); returns (MU, ALPHA)."""
+        B = np.asfortranarray(B, dtype=np.float64)
+        nrhs = B.shape[1]
+        MU = np.zeros((self.ndof, nrhs), order="F")
+        AL = np.zeros((self.hole_dim, nrhs), order="F") if self.hole_dim else None
+        self._check(self.L.iegpu_solve_nrhs(self.h, nrhs, B.ctypes.data_as(_dp),
+                                            MU.ctypes.data_as(_dp),
+                                            None if AL is None else AL.ctypes.data_as(_dp)))
+        return MU, AL
+
+    def set_border(self, U, Psi):
+        """Caller-supplied border blocks U (ndof x h) and Psi (h x ndof)."""
+        U = np.asfortranarray(U, dtype=np.float64)
+        P = np.asfortranarray(Psi, dtype=np.float64)
+        self._check(self.L.iegpu_set_border(self.h, U.shape[1], U.ctypes.data_as(_dp),
+                                            P.ctypes.data_as(_dp)))
+
+    def box_factors(self, box):
+        """dict(k, r, T, L, U, Xinv, S) of one box (column-major blocks)."""
+        k, r = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.L.iegpu_box_factors(self.h, box, ctypes.byref(k), ctypes.byref(r),
+                                             None, None, None, None, None))
+        k, r = k.value, r.value
+        out = dict(k=k, r=r)
+        if r == 0:
+            return out
+        T, Lm, U = (np.zeros((k, r), order="F"), np.zeros((k, r), order="F"),
+                    np.zeros((r, k), order="F"))
+        X, S = np.zeros((r, r), order="F"), np.zeros((k, k), order="F")
+        self._check(self.L.iegpu_box_factors(self.h, box, None, None, *[a.ctypes.data_as(_dp)
+                                                                        for a in (T, Lm, U, X, S)]))
+        out.update(T=T, L=Lm, U=U, Xinv=X, S=S)
+        return out
 
     def solve_refined(self, b, rtol=1e-10, max_iter=10):
         """FP64 iterative refinement preconditioned by the stored factor.

@@ -437,6 +437,23 @@ class QuadTree {
     if (ctx) iegpu_tree_info(ctx, &nl, nullptr, nullptr, nullptr);
     return nl;
   }
+  // The factors skeletonize left on node `id` (QuadTreeNode::T, L, U,
+  // X_rr (inverted here) and schur_update, quadtree.h:55-66), read back
+  // from the device; all empty for an uncompressed node.
+  void node_factors(int id, ie_Mat* T, ie_Mat* L, ie_Mat* U, ie_Mat* X_rr_inv,
+                    ie_Mat* schur) const {
+    int k = 0, r = 0;
+    detail::check(ctx, iegpu_box_factors(ctx, id, &k, &r, nullptr, nullptr, nullptr, nullptr,
+                                         nullptr));
+    *T = ie_Mat(k * (r > 0), r);
+    *L = ie_Mat(k * (r > 0), r);
+    *U = ie_Mat(r, k * (r > 0));
+    *X_rr_inv = ie_Mat(r, r);
+    *schur = ie_Mat(k * (r > 0), k * (r > 0));
+    if (r == 0) return;
+    detail::check(ctx, iegpu_box_factors(ctx, id, nullptr, nullptr, T->mat, L->mat, U->mat,
+                                         X_rr_inv->mat, schur->mat));
+  }
 
  private:
   // quadtree.cpp:718-725 (and the copies of quadtree.cpp:78-86).
@@ -552,7 +569,9 @@ class SkelFactorization {
     if (strong) throw GpuError(IEGPU_ERR_INVALID, "strong admissibility is not implemented");
   }
 
-  // skel_factorization.cpp:140-310
+  // skel_factorization.cpp:140-310.  The border uses the caller's tree->U
+  // and tree->Psi (linear_solve.cpp:256-257); the hole centres stay with
+  // the library for interior evaluation.
   void skeletonize(const Kernel& K, QuadTree* tree) const {
     tree->prepare(gpu_pde(K.pde));
     if (tree->U.width() > 0) {
@@ -563,24 +582,34 @@ class SkelFactorization {
         r.push_back(h.radius);
       }
       detail::check(tree->ctx, iegpu_set_holes(tree->ctx, (int)r.size(), c.data(), r.data()));
+      int hd = 0, nd = 0;
+      iegpu_tree_info(tree->ctx, nullptr, nullptr, &nd, &hd);
+      if (hd > 0) {
+        if (tree->U.height() != nd || tree->U.width() != hd || tree->Psi.height() != hd ||
+            tree->Psi.width() != nd)
+          throw GpuError(IEGPU_ERR_INVALID, "tree->U / tree->Psi do not match the holes' border");
+        detail::check(tree->ctx, iegpu_set_border(tree->ctx, hd, tree->U.mat, tree->Psi.mat));
+      }
     } else {
       detail::check(tree->ctx, iegpu_set_holes(tree->ctx, 0, nullptr, nullptr));
     }
     detail::check(tree->ctx, iegpu_factor(tree->ctx, id_tol, precision));
   }
-  // skel_factorization.cpp:507-592
+  // skel_factorization.cpp:507-592: every column of b (apply_sweep_matrix
+  // sweeps vec->width() columns, :388-406).
   void solve(const QuadTree& tree, ie_Mat* x, const ie_Mat& b) const {
-    if (x->height() != b.height() || x->width() != 1) *x = ie_Mat(b.height(), 1);
-    detail::check(tree.ctx, iegpu_solve(tree.ctx, b.mat, x->mat, nullptr));
+    if (x->height() != b.height() || x->width() != b.width()) *x = ie_Mat(b.height(), b.width());
+    detail::check(tree.ctx, iegpu_solve_nrhs(tree.ctx, b.width(), b.mat, x->mat, nullptr));
   }
-  // skel_factorization.cpp:596-777
+  // skel_factorization.cpp:596-777, column by column of b.
   void multiply_connected_solve(const QuadTree& tree, ie_Mat* mu, ie_Mat* alpha,
                                 const ie_Mat& b) const {
-    if (mu->height() != b.height() || mu->width() != 1) *mu = ie_Mat(b.height(), 1);
+    if (mu->height() != b.height() || mu->width() != b.width())
+      *mu = ie_Mat(b.height(), b.width());
     int hd = 0;
     iegpu_tree_info(tree.ctx, nullptr, nullptr, nullptr, &hd);
-    if (alpha->height() != hd) *alpha = ie_Mat(hd, 1);
-    detail::check(tree.ctx, iegpu_solve(tree.ctx, b.mat, mu->mat, alpha->mat));
+    if (alpha->height() != hd || alpha->width() != b.width()) *alpha = ie_Mat(hd, b.width());
+    detail::check(tree.ctx, iegpu_solve_nrhs(tree.ctx, b.width(), b.mat, mu->mat, alpha->mat));
   }
 };
 

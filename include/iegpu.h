@@ -153,6 +153,34 @@ int iegpu_factor_solve_device(iegpu_ctx* ctx, double id_tol, int precision,
  * multiply_connected_solve, skel_factorization.h:92-94). */
 int iegpu_solve(iegpu_ctx* ctx, const double* b, double* mu, double* alpha);
 
+/* As iegpu_solve for nrhs right-hand sides at once: b and mu are ndof x
+ * nrhs column-major, alpha (with holes) hole_dim x nrhs.  Without holes
+ * every column passes through each sweep kernel together; with holes the
+ * bordered solve runs column by column.  Each column equals iegpu_solve of
+ * that column.  Replaces solve / multiply_connected_solve on a
+ * multi-column b (apply_sweep_matrix sweeps every column,
+ * ie_solver/skel_factorization/skel_factorization.cpp:388-406,507-592). */
+int iegpu_solve_nrhs(iegpu_ctx* ctx, int nrhs, const double* b, double* mu, double* alpha);
+
+/* The border blocks of a multiply connected problem from the caller: U
+ * (ndof x hole_dim, column-major) and Psi (hole_dim x ndof), replacing the
+ * ones iegpu_set_holes built from the hole formulas (hole_dim must equal
+ * theirs; the hole centres stay for interior evaluation).  A later
+ * iegpu_set_holes or geometry update rebuilds them from the formulas.
+ * Replaces the assignment to quadtree->U / quadtree->Psi that skeletonize
+ * and multiply_connected_solve read, ie_solver/linear_solve.cpp:256-257. */
+int iegpu_set_border(iegpu_ctx* ctx, int hole_dim, const double* U, const double* Psi);
+
+/* One box's factors (creation id `box`) after a factorization: its rank k
+ * and redundant count r (0 and 0 for uncompressed boxes and roots), and,
+ * where the pointer is not NULL, T (k x r), L (k x r), U (r x k), X_rr^{-1}
+ * (r x r) and the carried Schur block (k x k), column-major, rows and
+ * columns in the order of iegpu_box_list's skeleton (3) and redundant (4)
+ * lists.  Mirrors QuadTreeNode::{T, L, U, X_rr, schur_update},
+ * ie_solver/quadtree/quadtree.h:55-66 (X_rr is kept inverted here). */
+int iegpu_box_factors(iegpu_ctx* ctx, int box, int* k, int* r, double* T, double* L,
+                      double* U, double* Xinv, double* S);
+
 /* As iegpu_solve on device buffers (b may alias mu), ordered on `stream`
  * (NULL: the context's stream). */
 int iegpu_solve_device(iegpu_ctx* ctx, const double* d_b, double* d_mu,

@@ -287,6 +287,114 @@ int main() {
     ie_Mat sol = get_dom_sol(c, &dom, &b);
     expect_le("StokesDonutAnalyticAgreementTangent", stokes_error(sol, dom, &b), atol);
   }
+  {  // skel_factorization.cpp:388-406,507-592: a multi-column b solves every
+     // column; each equals that column solved alone, bit for bit
+    ie_solver_config c = lap;
+    c.num_boundary_points = 4096;
+    Circle b;
+    b.initialize(c.num_boundary_points, c.boundary_condition);
+    QuadTree q;
+    q.initialize_tree(&b, std::vector<double>(), 1, 2);
+    SkelFactorization sf(c.id_tol, false, c.num_threads);
+    Kernel kernel(1, 2, c.pde, q.boundary, std::vector<double>());
+    sf.skeletonize(kernel, &q);
+    const int n = c.num_boundary_points;
+    ie_Mat B(n, 3), X;
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < n; i++) B.set(i, j, std::cos(0.37 * (i + 1) * (j + 1)));
+    sf.solve(q, &X, B);
+    double diff = X.width() == 3 ? 0.0 : 1.0;
+    for (int j = 0; j < 3; j++) {
+      ie_Mat bj(n, 1), xj;
+      for (int i = 0; i < n; i++) bj.set(i, 0, B.get(i, j));
+      sf.solve(q, &xj, bj);
+      for (int i = 0; i < n; i++) diff = std::fmax(diff, std::fabs(xj.get(i, 0) - X.get(i, j)));
+    }
+    expect_le("MultiColumnSolveEqualsColumns", diff, 0.0);
+
+    // QuadTreeNode::T of a compressed box: its redundant columns of the
+    // kernel against every DOF outside the proxy circle (both directions)
+    // are the skeleton columns times T (interpolative decomposition).
+    int nb = 0;
+    iegpu_tree_info(q.ctx, nullptr, &nb, nullptr, nullptr);
+    std::vector<int> lev(nb), leaf(nb), rank(nb);
+    iegpu_box_info(q.ctx, lev.data(), nullptr, leaf.data(), nullptr, nullptr, rank.data(),
+                   nullptr);
+    int box = -1;
+    for (int id = nb - 1; id >= 0 && box < 0; id--)
+      if (leaf[id] && rank[id] > 0) box = id;
+    ie_Mat T, L, U, Xi, S;
+    q.node_factors(box, &T, &L, &U, &Xi, &S);
+    std::vector<int> sk(T.height()), rd(T.width()), orig(4096);
+    iegpu_box_list(q.ctx, box, 3, sk.data(), (int)sk.size());
+    iegpu_box_list(q.ctx, box, 4, rd.data(), (int)rd.size());
+    const int no = iegpu_box_list(q.ctx, box, 0, orig.data(), (int)orig.size());
+    double cx = 0, cy = 0, rad = 0;
+    for (int t = 0; t < no; t++) {
+      cx += q.boundary->points[2 * orig[t]] / no;
+      cy += q.boundary->points[2 * orig[t] + 1] / no;
+    }
+    for (int t = 0; t < no; t++)
+      rad = std::fmax(rad, std::hypot(q.boundary->points[2 * orig[t]] - cx,
+                                      q.boundary->points[2 * orig[t] + 1] - cy));
+    std::vector<int> far;
+    for (int i = 0; i < n; i++)
+      if (std::hypot(q.boundary->points[2 * i] - cx, q.boundary->points[2 * i + 1] - cy) >
+          6 * rad)
+        far.push_back(i);
+    const ie_Mat As = kernel(far, sk), Ar = kernel(far, rd);
+    const ie_Mat Ast = kernel(sk, far), Art = kernel(rd, far);
+    double err = 0, nrm = 0;
+    for (int i = 0; i < (int)far.size(); i++)
+      for (int jr = 0; jr < (int)rd.size(); jr++) {
+        double a = Ar.get(i, jr), at = Art.get(jr, i);
+        for (int js = 0; js < (int)sk.size(); js++) {
+          a -= As.get(i, js) * T.get(js, jr);
+          at -= Ast.get(js, i) * T.get(js, jr);
+        }
+        err = std::fmax(err, std::fmax(std::fabs(a), std::fabs(at)));
+        nrm = std::fmax(nrm, std::fmax(std::fabs(Ar.get(i, jr)), std::fabs(Art.get(jr, i))));
+      }
+    expect_le("NodeFactorsTReconstructsFarField", err / nrm, 1e3 * c.id_tol);
+    expect_le("NodeFactorsShapes", (double)(L.height() != T.height() || U.width() != T.height() ||
+                                            Xi.height() != T.width() || S.height() != T.height()),
+              0.0);
+  }
+  {  // skeletonize reads the caller's tree->U and tree->Psi
+     // (linear_solve.cpp:256-257): a doubled U halves nothing else, and the
+     // solution satisfies the caller's bordered system
+    ie_solver_config c = lap;
+    c.boundary_shape = Boundary::ANNULUS;
+    Annulus b;
+    b.initialize(c.num_boundary_points, c.boundary_condition);
+    QuadTree q;
+    q.initialize_tree(&b, std::vector<double>(), 1, 2);
+    SkelFactorization sf(c.id_tol, false, c.num_threads);
+    Kernel kernel(1, 2, c.pde, q.boundary, std::vector<double>());
+    q.U = initialize_U_mat(c.pde, b.holes, b.points);
+    q.Psi = initialize_Psi_mat(c.pde, b.holes, &b);
+    for (int j = 0; j < q.U.width(); j++)
+      for (int i = 0; i < q.U.height(); i++) q.U.set(i, j, 2 * q.U.get(i, j) + 0.25);
+    sf.skeletonize(kernel, &q);
+    ie_Mat mu, alpha;
+    linear_solve(sf, q, b.boundary_values, &mu, &alpha);
+    std::vector<int> all;
+    for (int i = 0; i < (int)b.weights.size(); i++) all.push_back(i);
+    const ie_Mat K = kernel(all, all);
+    double err = 0;
+    for (int i = 0; i < (int)all.size(); i++) {
+      double acc = -b.boundary_values.get(i, 0);
+      for (int j = 0; j < (int)all.size(); j++) acc += K.get(i, j) * mu.get(j, 0);
+      for (int k = 0; k < alpha.height(); k++) acc += q.U.get(i, k) * alpha.get(k, 0);
+      err = std::fmax(err, std::fabs(acc));
+    }
+    for (int k = 0; k < alpha.height(); k++) {
+      double acc = -alpha.get(k, 0);
+      for (int j = 0; j < (int)all.size(); j++) acc += q.Psi.get(k, j) * mu.get(j, 0);
+      err = std::fmax(err, std::fabs(acc));
+    }
+    expect_le("CallersBorderIsUsed", err, 50 * c.id_tol);
+  }
   {  // bie_time_trial (linear_solve.cpp:156-185) runs and reports times
     ie_solver_config c = lap;
     c.num_boundary_points = 4096;
