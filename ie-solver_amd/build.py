@@ -19,8 +19,8 @@ HIP_FLAGS = [
     # keep hipcc from emitting packed FP32 arithmetic
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-HIP_SOURCES = ["rs_kernels.hip", "rs_sweep.hip", "rs_holes.hip", "rs_eval.hip", "rs_matvec.hip", "rs_mlp.hip", "capi.cpp"]
-HOST_SOURCES = ["tree.cpp"]
+HIP_SOURCES = ["rs_kernels.hip", "rs_sweep.hip", "rs_holes.hip", "rs_eval.hip", "rs_matvec.hip", "rs_fmm.hip", "rs_mlp.hip", "capi.cpp"]
+HOST_SOURCES = ["tree.cpp", "fmm_tree.cpp"]
 
 
 def _run(cmd):
@@ -63,6 +63,11 @@ def build_cpp_tests():
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"), src,
           "-o", out, "-L", HERE, "-liesolver_amd",
           "-Wl,-rpath,$ORIGIN/../../ie-solver_amd"])
+    # CPU checks of the FMM tree and translations (host code only)
+    fsrc = os.path.join(root, "tests", "cpp", "fmm_tree_test.cpp")
+    if os.path.exists(fsrc):
+        _run(["g++", "-O2", "-std=c++17", "-Wall", fsrc, os.path.join(CSRC, "fmm_tree.cpp"),
+              "-o", os.path.join(root, "tests", "cpp", "fmm_tree_test.bin")])
     return out
 
 

@@ -14,6 +14,7 @@
 #include "rs_device.hpp"
 #include "rs_holes.hpp"
 #include "rs_eval.hpp"
+#include "rs_fmm.hpp"
 #include "rs_kernels.hpp"
 #include "rs_matvec.hpp"
 #include "rs_mlp.hpp"
@@ -225,6 +226,15 @@ struct iegpu_ctx {
   // iterative refinement: b, x, residual / correction (ndof), border parts,
   // reduction partials
   DevBuf<double> r_b, r_x, r_r, r_a, r_ra, r_part;
+  // FP64 operator form (iegpu_set_matvec) and the FMM's tree, translation
+  // tables and expansions
+  int mv_method = IEGPU_MATVEC_AUTO, fmm_order = 48, fmm_leaf = 32;
+  int mv_last = 0;
+  bool fmm_ready = false;
+  FmmHostTree fmm;
+  DevBuf<int> f_int;
+  DevBuf<double> f_dbl, f_tab, f_exp;
+  FmmDev fdev{};
 
   double t_factor_ms = 0, t_solve_ms = 0, t_root_ms = 0;
   double fac_bytes = 0;
@@ -715,6 +725,7 @@ int set_geometry(iegpu_ctx* c, int pde, int n_problems, const int* point_off,
     c->have_tree = false;
     c->factored = false;
     c->update_pending = false;
+    c->fmm_ready = false;
     c->n_holes = 0;
     c->hole_dim = 0;
   } catch (const HipError& e) {
@@ -750,6 +761,8 @@ int iegpu_set_geometry_batch(iegpu_ctx* c, int pde, int n_problems,
 namespace {
 void upload_tree(iegpu_ctx* c);
 void upload_border(iegpu_ctx* c);
+bool use_fmm(const iegpu_ctx* c);
+void prepare_fmm(iegpu_ctx* c);
 std::vector<int> mark_update(const HostTree& T, const std::vector<int>& leaves);
 }  // namespace
 
@@ -794,6 +807,7 @@ int iegpu_update_geometry(iegpu_ctx* c, int n_points, const double* points,
     c->h_wts.assign(weights, weights + n_points);
     c->factored = false;
     c->update_pending = false;
+    c->fmm_ready = false;
     c->tree = std::move(nt);
     if (keep) {
       std::vector<int> leaves;
@@ -882,6 +896,7 @@ int iegpu_build_tree(iegpu_ctx* c, int leaf_cap) {
     c->update_pending = false;
     if (c->device < 0) return IEGPU_OK;
     upload_tree(c);
+    if (use_fmm(c)) prepare_fmm(c);
   } catch (const HipError& e) {
     return from_hip(c, e);
   } catch (const std::exception& e) {
@@ -1759,6 +1774,122 @@ int iegpu_last_timings(const iegpu_ctx* c, double* t, int nt) {
   return IEGPU_OK;
 }
 
+namespace {
+// The FMM applies the operator of a single boundary; batches are summed
+// directly (each problem's block is small).
+bool use_fmm(const iegpu_ctx* c) {
+  if (c->n_prob != 1) return false;
+  if (c->mv_method == IEGPU_MATVEC_FMM) return true;
+  if (c->mv_method == IEGPU_MATVEC_DIRECT) return false;
+  return c->n_points >= IEGPU_FMM_AUTO_POINTS;
+}
+
+// Builds the FMM tree of the current geometry, uploads it with the
+// translation tables and sorts the geometry on the device.
+void prepare_fmm(iegpu_ctx* c) {
+  if (c->fmm_ready) return;
+  hipStream_t st = c->stream;
+  FmmHostTree& T = c->fmm;
+  T = FmmHostTree();
+  T.build(c->h_pts.data(), c->n_points, c->fmm_leaf);
+  std::vector<int> iv;
+  auto put = [&](const std::vector<int>& v) {
+    const size_t o = iv.size();
+    iv.insert(iv.end(), v.begin(), v.end());
+    return o;
+  };
+  const size_t o_par = put(T.parent), o_coff = put(T.child_off), o_ccnt = put(T.child_cnt),
+               o_vlo = put(T.vl_off), o_vl = put(T.vl), o_pto = put(T.pt_off),
+               o_nbo = put(T.nb_off), o_nb = put(T.nb), o_perm = put(T.perm);
+  iv.push_back(0);  // keeps every offset inside the buffer
+  c->f_int.upload(iv, st);
+  const int64_t nb = T.n_boxes(), np = c->n_points;
+  c->f_dbl.alloc(2 * nb + 6 * np + 2 * np);
+  HIP_TRY(hipMemcpyAsync(c->f_dbl.p, T.center.data(), sizeof(double) * 2 * nb,
+                         hipMemcpyHostToDevice, st));
+  if (!c->f_tab.p) {
+    std::vector<double> a, b, d;
+    fmm_tables(a, b, d);
+    a.insert(a.end(), b.begin(), b.end());
+    a.insert(a.end(), d.begin(), d.end());
+    c->f_tab.upload(a, st);
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  const int ne = fmm_n_exp(c->pde);
+  const int64_t per = nb * ne * c->fmm_order * 2;
+  c->f_exp.alloc(2 * per);
+  FmmDev& f = c->fdev;
+  f.order = c->fmm_order;
+  f.depth = T.depth;
+  f.n_leaves = T.n_leaves();
+  f.first_leaf = T.first_leaf();
+  f.side = T.side;
+  f.center = c->f_dbl.p;
+  double* sgeo = c->f_dbl.p + 2 * nb;
+  f.sgeo = sgeo;
+  f.xs = sgeo + 6 * np;
+  const int* ib = c->f_int.p;
+  f.parent = ib + o_par;
+  f.child_off = ib + o_coff;
+  f.child_cnt = ib + o_ccnt;
+  f.vl_off = ib + o_vlo;
+  f.vl = ib + o_vl;
+  f.pt_off = ib + o_pto;
+  f.nb_off = ib + o_nbo;
+  f.nb = ib + o_nb;
+  f.perm = ib + o_perm;
+  f.mexp = c->f_exp.p;
+  f.lexp = c->f_exp.p + per;
+  const int64_t tab = int64_t(kFmmMaxOrder) * kFmmMaxOrder;
+  f.t_m2m = c->f_tab.p;
+  f.t_m2l = c->f_tab.p + tab;
+  f.t_l2l = c->f_tab.p + 2 * tab;
+  launch_fmm_gather(c->geom(), f, sgeo, st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  c->fmm_ready = true;
+}
+
+// y = K x on the context's stream, in the form iegpu_set_matvec selects;
+// r_part holds the reduction partials.
+void enqueue_matvec(iegpu_ctx* c, const double* x, double* y, hipStream_t st) {
+  if (use_fmm(c)) {
+    prepare_fmm(c);
+    launch_fmm_matvec(c->geom(), c->fdev, c->fmm.level_off, x, y, c->r_part.p, st);
+    c->mv_last = IEGPU_MATVEC_FMM;
+  } else {
+    launch_matvec(c->geom(), x, y, c->r_part.p, st);
+    c->mv_last = IEGPU_MATVEC_DIRECT;
+  }
+}
+}  // namespace
+
+int iegpu_set_matvec(iegpu_ctx* c, int method, int order, int leaf_points) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (method < IEGPU_MATVEC_AUTO || method > IEGPU_MATVEC_FMM)
+    return fail(c, IEGPU_ERR_INVALID, "method must be AUTO (0), DIRECT (1) or FMM (2)");
+  if (order == 0) order = 48;
+  if (!fmm_order_supported(order))
+    return fail(c, IEGPU_ERR_INVALID, "FMM order must be 32, 48 or 64");
+  if (leaf_points == 0) leaf_points = 32;
+  if (leaf_points < 1 || leaf_points > 4096)
+    return fail(c, IEGPU_ERR_INVALID, "leaf_points must be in 1..4096");
+  if (order != c->fmm_order || leaf_points != c->fmm_leaf) c->fmm_ready = false;
+  c->mv_method = method;
+  c->fmm_order = order;
+  c->fmm_leaf = leaf_points;
+  return IEGPU_OK;
+}
+
+int iegpu_matvec_info(const iegpu_ctx* c, int* info, int n) {
+  if (!c || (n > 0 && !info)) return IEGPU_ERR_INVALID;
+  const bool t = c->fmm_ready;
+  const int v[4] = {c->mv_last, t ? c->fmm.depth : 0, t ? c->fmm.n_boxes() : 0,
+                    t ? c->fmm.n_leaves() : 0};
+  for (int i = 0; i < n && i < 4; i++) info[i] = v[i];
+  return IEGPU_OK;
+}
+
 int iegpu_solve_refined(iegpu_ctx* c, const double* b, double* x, double* alpha,
                         double rtol, int max_iter, int* iters, double* relres) {
   if (!c) return IEGPU_ERR_INVALID;
@@ -1792,7 +1923,7 @@ int iegpu_solve_refined(iegpu_ctx* c, const double* b, double* x, double* alpha,
     double rel = 0;
     for (;; it++) {
       // r = b - A x  (A bordered by [U; Psi, -I] with holes)
-      launch_matvec(c->geom(), c->r_x.p, c->r_r.p, c->r_part.p, st);
+      enqueue_matvec(c, c->r_x.p, c->r_r.p, st);
       if (h > 0)
         launch_border_apply(c->d_U.p, c->d_PsiT.p, c->r_x.p, c->r_a.p, nd, h, c->r_r.p,
                             c->r_ra.p, st);
@@ -1850,7 +1981,7 @@ int iegpu_apply(iegpu_ctx* c, const double* x, const double* alpha, double* y,
     c->r_x.upload(x, nd, st);
     c->r_r.alloc(nd);
     c->r_part.alloc(std::max<int64_t>(2 * residual_blocks(), matvec_partials(c->geom())));
-    launch_matvec(c->geom(), c->r_x.p, c->r_r.p, c->r_part.p, st);
+    enqueue_matvec(c, c->r_x.p, c->r_r.p, st);
     if (h > 0 && alpha && z) {
       c->r_a.upload(alpha, h, st);
       c->r_ra.alloc(h);

@@ -250,6 +250,10 @@ int matvec_partials(const DevGeom& g) {
   return g.n_prob > 256 ? g.n_prob : 256;
 }
 
+void launch_wn_dot(const DevGeom& g, const double* x, double* partial, hipStream_t st) {
+  k_wn_dot<<<g.n_prob, 256, 0, st>>>(g, x, partial);
+}
+
 void launch_matvec(const DevGeom& g, const double* x, double* y, double* partial,
                    hipStream_t st) {
   if (g.pde == kPdeStokes || g.pde == kPdeLaplaceNeumann)

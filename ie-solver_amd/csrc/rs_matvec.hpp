@@ -12,6 +12,10 @@ namespace iegpu {
 // Size of the partial-sum buffer launch_matvec needs.
 int matvec_partials(const DevGeom& g);
 
+// partial[p] = problem p's rank-one sum: sum_j w_j (n_j . x_j) for Stokes,
+// sum_j w_j x_j for Laplace Neumann (one workgroup per problem).
+void launch_wn_dot(const DevGeom& g, const double* x, double* partial, hipStream_t st);
+
 // y = K x (ndof), FP64, matrix-free.
 void launch_matvec(const DevGeom& g, const double* x, double* y, double* partial,
                    hipStream_t st);

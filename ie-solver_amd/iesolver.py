@@ -29,6 +29,10 @@ FP64 = 0
 FP32 = 1
 ID_QRCP = 0
 ID_RANDOMIZED = 1
+MATVEC_AUTO = 0
+MATVEC_DIRECT = 1
+MATVEC_FMM = 2
+FMM_AUTO_POINTS = 16384
 
 STATUS = {
     0: "OK", 1: "INVALID", 2: "NO_COMPRESSION", 3: "SINGULAR", 4: "OOM",
@@ -47,7 +51,8 @@ EXPORTS = [
     "iegpu_autotune", "iegpu_set_overlap", "iegpu_set_keep_operator", "iegpu_sparse_matvec",
     "iegpu_factor_solve_device", "iegpu_update_geometry", "iegpu_update_info",
     "iegpu_update_geometry_batch", "iegpu_discard_update", "iegpu_evaluate_noalloc",
-    "iegpu_solve_nrhs", "iegpu_set_border", "iegpu_box_factors",
+    "iegpu_solve_nrhs", "iegpu_set_border", "iegpu_box_factors", "iegpu_set_matvec",
+    "iegpu_matvec_info",
 ]
 NUM_LAUNCH_CAPS = 8
 RANK_FEATURES = 9
@@ -97,6 +102,8 @@ def load():
     L.iegpu_solve_refined.argtypes = [vp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                       _ip, _dp]
     L.iegpu_apply.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.iegpu_set_matvec.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.iegpu_matvec_info.argtypes = [vp, _ip, ctypes.c_int]
     L.iegpu_set_profiling.argtypes = [vp, ctypes.c_int]
     L.iegpu_profile.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _ip]
     L.iegpu_set_id_method.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -402,6 +409,20 @@ class RSSolver:
                                                None if alpha is None else _pd(alpha),
                                                rtol, max_iter, ctypes.byref(it), ctypes.byref(rr)))
         return mu, alpha, it.value, rr.value
+
+    def set_matvec(self, method=MATVEC_AUTO, order=0, leaf_points=0):
+        """Operator form of apply and solve_refined: MATVEC_DIRECT (O(N^2)
+        sum), MATVEC_FMM (fast multipole, `order` 32/48/64 coefficients,
+        about `leaf_points` nodes per leaf; 0 = defaults 48 and 32) or
+        MATVEC_AUTO (FMM from FMM_AUTO_POINTS nodes of one boundary)."""
+        self._check(self.L.iegpu_set_matvec(self.h, method, order, leaf_points))
+
+    def matvec_info(self):
+        """(form used by the last apply / solve_refined, FMM leaf level,
+        FMM boxes, FMM leaves)."""
+        v = np.zeros(4, dtype=np.int32)
+        self._check(self.L.iegpu_matvec_info(self.h, _pi(v), 4))
+        return tuple(int(t) for t in v)
 
     def apply(self, x, alpha=None):
         """FP64 operator: K x (+ U alpha), and Psi x - alpha with holes."""

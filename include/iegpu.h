@@ -205,6 +205,31 @@ int iegpu_solve_refined(iegpu_ctx* ctx, const double* b, double* x,
 int iegpu_apply(iegpu_ctx* ctx, const double* x, const double* alpha,
                 double* y, double* z);
 
+/* How iegpu_apply and iegpu_solve_refined apply the FP64 operator K:
+ *   IEGPU_MATVEC_DIRECT  the O(N^2) sum over every source node;
+ *   IEGPU_MATVEC_FMM     a fast multipole method: complex expansions of
+ *                        `order` coefficients (32, 48 or 64; 0 = 48) on a
+ *                        uniform quadtree whose non-empty leaves hold about
+ *                        `leaf_points` nodes (0 = 32), the adjacent leaves
+ *                        summed directly; O(N), ~1e-15 relative at order 48;
+ *   IEGPU_MATVEC_AUTO    (default) the FMM for a single boundary of at
+ *                        least IEGPU_FMM_AUTO_POINTS nodes, else direct.
+ * A batch of problems always takes the direct sum.  The FMM's tree is built
+ * from the geometry on first use (or by iegpu_build_tree) and rebuilt after
+ * a geometry change.  No reference counterpart: the reference forms K
+ * densely in its tests (ie_solver_test.cpp:63-93). */
+#define IEGPU_MATVEC_AUTO 0
+#define IEGPU_MATVEC_DIRECT 1
+#define IEGPU_MATVEC_FMM 2
+#define IEGPU_FMM_AUTO_POINTS 16384
+int iegpu_set_matvec(iegpu_ctx* ctx, int method, int order, int leaf_points);
+
+/* info[0..3] = operator form used by the last iegpu_apply /
+ * iegpu_solve_refined (IEGPU_MATVEC_DIRECT or IEGPU_MATVEC_FMM, 0 before
+ * any), the FMM tree's leaf level, box count and leaf count (0 without a
+ * tree). */
+int iegpu_matvec_info(const iegpu_ctx* ctx, int* info, int n);
+
 /* Field at n_targets interior points: out (sol_dim per target) =
  * K_domain mu + U_forward alpha, matrix-free on the GPU; targets with
  * in_domain[t] == 0 give 0 (in_domain may be NULL: all inside); alpha may

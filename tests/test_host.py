@@ -220,3 +220,37 @@ def test_batch_update_keeps_each_problem_size():
     with pytest.raises(iesolver.IeGpuError):
         s.update_geometry(geoms[:1])
     s.discard_update()
+
+
+def test_matvec_arguments():
+    s = iesolver.RSSolver(device=-1)
+    assert s.matvec_info() == (0, 0, 0, 0)  # nothing applied, no FMM tree
+    s.set_matvec(iesolver.MATVEC_FMM, 64, 16)
+    s.set_matvec(iesolver.MATVEC_AUTO)
+    s.set_matvec(iesolver.MATVEC_DIRECT, 32, 4096)
+    for args in [(3, 0, 0), (-1, 0, 0), (2, 40, 0), (2, 16, 0), (2, 48, -1), (2, 48, 5000)]:
+        with pytest.raises(iesolver.IeGpuError) as e:
+            s.set_matvec(*args)
+        assert e.value.code == 1, args
+    g = G.star(256)
+    s.set_geometry(g)
+    with pytest.raises(iesolver.IeGpuError) as e:
+        s.apply(np.ones(256))
+    assert e.value.code == 7  # host-only context
+    L = iesolver.load()
+    assert L.iegpu_set_matvec(None, 0, 0, 0) == 1
+    assert L.iegpu_matvec_info(None, None, 0) == 1
+
+
+def test_fmm_tree_and_translations_cpu():
+    """tests/cpp/fmm_tree_test.cpp: the FMM tree's lists count every pair of
+    leaves once, and its translations (in the kernels' order) reproduce the
+    direct sum of the three kernels to 1e-13."""
+    exe = os.path.join(ROOT, "tests", "cpp", "fmm_tree_test.bin")
+    if not os.path.exists(exe):
+        import build
+        build.build_cpp_tests()
+    p = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    print(p.stdout)
+    assert p.returncode == 0, p.stdout
