@@ -228,7 +228,7 @@ struct iegpu_ctx {
   DevBuf<double> r_b, r_x, r_r, r_a, r_ra, r_part;
   // FP64 operator form (iegpu_set_matvec) and the FMM's tree, translation
   // tables and expansions
-  int mv_method = IEGPU_MATVEC_AUTO, fmm_order = 48, fmm_leaf = 32;
+  int mv_method = IEGPU_MATVEC_AUTO, fmm_order = 48, fmm_leaf = 64;
   int mv_last = 0;
   bool fmm_ready = false;
   FmmHostTree fmm;
@@ -1871,7 +1871,7 @@ int iegpu_set_matvec(iegpu_ctx* c, int method, int order, int leaf_points) {
   if (order == 0) order = 48;
   if (!fmm_order_supported(order))
     return fail(c, IEGPU_ERR_INVALID, "FMM order must be 32, 48 or 64");
-  if (leaf_points == 0) leaf_points = 32;
+  if (leaf_points == 0) leaf_points = 64;
   if (leaf_points < 1 || leaf_points > 4096)
     return fail(c, IEGPU_ERR_INVALID, "leaf_points must be in 1..4096");
   if (order != c->fmm_order || leaf_points != c->fmm_leaf) c->fmm_ready = false;

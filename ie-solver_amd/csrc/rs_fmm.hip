@@ -28,9 +28,10 @@
 //   M2M        M'_k = sum_{m<=k} C(k,m) 2^-m d^(k-m) M_m      (d = (c - c')/s')
 //   M2L        L_l  = (-1)^l w^(l+1) sum_k C(k+l,l) w^k M_k   (w = s/(c_t - c_s))
 //   L2L        L'_m = 2^-(m+1) sum_{l>=m} C(l,m) d^(l-m) L_l   (d = (c' - c)/s)
-// One wave per box, one lane per coefficient.  Interaction lists hold
-// boxes at least one box apart, so the truncation error of order P falls
-// like 0.55^P or faster (P = 48: about 1e-15 relative on smooth curves).
+// One lane per coefficient: one wave per leaf, four per box in the
+// translations.  Interaction lists hold boxes at least one box apart, so
+// the truncation error of order P falls like 0.55^P or faster (P = 48:
+// about 1e-15 relative on smooth curves).
 
 #include <hip/hip_runtime.h>
 
@@ -97,45 +98,62 @@ __global__ void k_fmm_gather(DevGeom g, const int* perm, int n, double* sgeo) {
   }
 }
 
-// Leaf multipoles from the nodes' charges; also the sorted density.
+// Leaf multipoles from the nodes' charges (staged in LDS, 64 nodes at a
+// time); also the sorted density.
 template <int P, int PDE>
 __global__ __launch_bounds__(64) void k_fmm_p2m(FmmDev f, const double* __restrict__ x) {
-  constexpr int NE = Pde<PDE>::NE, SD = Pde<PDE>::SD;
+  constexpr int NE = Pde<PDE>::NE;
+  __shared__ double su[64][2], sq[64][NE][2];
   const int leaf = blockIdx.x, lane = threadIdx.x;
   const int b = f.first_leaf + leaf;
   const int p0 = f.pt_off[leaf], p1 = f.pt_off[leaf + 1];
-  for (int j = p0 + lane; j < p1; j += 64) {
-    const int o = f.perm[j];
-    f.xs[SD * j] = x[SD * o];
-    if (SD == 2) f.xs[SD * j + 1] = x[SD * o + 1];
-  }
   const double inv_s = 1.0 / ldexp(f.side, -f.depth);
   const double cx = f.center[2 * b], cy = f.center[2 * b + 1];
   const int k = lane < P ? lane : 0;
   cplx acc[NE];
 #pragma unroll
   for (int e = 0; e < NE; e++) acc[e] = {0.0, 0.0};
-  for (int j = p0; j < p1; j++) {
-    const double* gj = f.sgeo + 6 * (size_t)j;
-    const double px = gj[0], py = gj[1], n0 = gj[2], n1 = gj[3], w = gj[4];
-    const int o = f.perm[j];
-    const cplx pk = cpow({(px - cx) * inv_s, (py - cy) * inv_s}, k);
-    if (PDE == kPdeStokes) {
-      const double x0 = x[2 * o], x1 = x[2 * o + 1];
-      const double a = w * (0.25 / M_PI);
-      const cplx c1 = {a * (n0 * x0 - n1 * x1), a * (n0 * x1 + n1 * x0)};
-      const cplx c4 = cmul(c1, {px - cx, -(py - cy)});
-      const double c2 = w * (0.5 / M_PI) * (n0 * x0 + n1 * x1);
-      acc[0] = cfma(c1, pk, acc[0]);
-      acc[1] = {fma(c2, pk.r, acc[1].r), fma(c2, pk.i, acc[1].i)};
-      acc[2] = cfma(c4, pk, acc[2]);
-    } else if (PDE == kPdeLaplaceNeumann) {
-      const double q = w * (0.5 / M_PI) * x[o];
-      acc[0] = {fma(q, pk.r, acc[0].r), fma(q, pk.i, acc[0].i)};
-    } else {
-      const double q = -w * (0.5 / M_PI) * x[o];
-      acc[0] = cfma({q * n0, q * n1}, pk, acc[0]);
+  for (int base = p0; base < p1; base += 64) {
+    const int j = base + lane;
+    if (j < p1) {
+      const double* gj = f.sgeo + 6 * (size_t)j;
+      const double px = gj[0], py = gj[1], n0 = gj[2], n1 = gj[3], w = gj[4];
+      const int o = f.perm[j];
+      su[lane][0] = (px - cx) * inv_s;
+      su[lane][1] = (py - cy) * inv_s;
+      cplx q[NE];
+      if (PDE == kPdeStokes) {
+        const double x0 = x[2 * o], x1 = x[2 * o + 1];
+        f.xs[2 * j] = x0;
+        f.xs[2 * j + 1] = x1;
+        const double a = w * (0.25 / M_PI);
+        q[0] = {a * (n0 * x0 - n1 * x1), a * (n0 * x1 + n1 * x0)};
+        q[1] = {w * (0.5 / M_PI) * (n0 * x0 + n1 * x1), 0.0};
+        q[NE - 1] = cmul(q[0], {px - cx, -(py - cy)});
+      } else if (PDE == kPdeLaplaceNeumann) {
+        const double xj = x[o];
+        f.xs[j] = xj;
+        q[0] = {w * (0.5 / M_PI) * xj, 0.0};
+      } else {
+        const double xj = x[o];
+        f.xs[j] = xj;
+        const double a = -w * (0.5 / M_PI) * xj;
+        q[0] = {a * n0, a * n1};
+      }
+#pragma unroll
+      for (int e = 0; e < NE; e++) {
+        sq[lane][e][0] = q[e].r;
+        sq[lane][e][1] = q[e].i;
+      }
     }
+    __syncthreads();
+    const int cnt = min(64, p1 - base);
+    for (int t = 0; t < cnt; t++) {
+      const cplx pk = cpow({su[t][0], su[t][1]}, k);
+#pragma unroll
+      for (int e = 0; e < NE; e++) acc[e] = cfma({sq[t][e][0], sq[t][e][1]}, pk, acc[e]);
+    }
+    __syncthreads();
   }
   if (lane < P) {
 #pragma unroll
@@ -147,128 +165,206 @@ __global__ __launch_bounds__(64) void k_fmm_p2m(FmmDev f, const double* __restri
   }
 }
 
-// Multipoles of level `level`'s boxes from their children.
+// Multipoles of level `level`'s boxes from their children: four waves per
+// box, wave w translating child w (its coefficients staged in LDS with the
+// Stokes c4 part already moved to the parent's centre); the four partial
+// sums are added in wave order.
 template <int P, int NE>
-__global__ __launch_bounds__(64) void k_fmm_m2m(FmmDev f, int first, int level) {
-  const int b = first + blockIdx.x, lane = threadIdx.x;
+__global__ __launch_bounds__(256) void k_fmm_m2m(FmmDev f, int first, int level) {
+  __shared__ double sm[4][NE][P][2];
+  const int b = first + blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int k = lane < P ? lane : P - 1;
   const double inv_s = 1.0 / ldexp(f.side, -level);
   const double cx = f.center[2 * b], cy = f.center[2 * b + 1];
+  const int c0 = f.child_off[b], nc = f.child_cnt[b];
+  const bool mine = wv < nc;
+  const int c = c0 + (mine ? wv : 0);
+  const double dx = f.center[2 * c] - cx, dy = f.center[2 * c + 1] - cy;
+  if (mine && lane < P) {
+    const double* mc = f.mexp + 2 * ((size_t)c * NE * P);
+    cplx m[NE];
+#pragma unroll
+    for (int e = 0; e < NE; e++) m[e] = ld(mc + 2 * (e * P + lane));
+    if (NE == 3) m[NE - 1] = cfma({dx, -dy}, m[0], m[NE - 1]);  // conj(c_child - c_parent)
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+      sm[wv][e][lane][0] = m[e].r;
+      sm[wv][e][lane][1] = m[e].i;
+    }
+  }
   cplx acc[NE];
 #pragma unroll
   for (int e = 0; e < NE; e++) acc[e] = {0.0, 0.0};
-  const int c0 = f.child_off[b], c1 = c0 + f.child_cnt[b];
-  for (int c = c0; c < c1; c++) {
-    const double dx = f.center[2 * c] - cx, dy = f.center[2 * c + 1] - cy;
+  __syncthreads();
+  if (mine) {
+    double tm[P];  // C(k, m) 2^-m, 0 for m > k
+#pragma unroll
+    for (int m = 0; m < P; m++) tm[m] = f.t_m2m[m * kFmmMaxOrder + k];
     const cplx d = {dx * inv_s, dy * inv_s};
-    const cplx shift = {dx, -dy};  // conj(c_child - c_parent)
-    const double* mc = f.mexp + 2 * ((size_t)c * NE * P);
     cplx pw = {1.0, 0.0};
+#pragma unroll
     for (int m = P - 1; m >= 0; m--) {
-      const double t = f.t_m2m[m * kFmmMaxOrder + k];  // 0 for m > k
-      const cplx tp = cscale(t, pw);
-      const cplx m0 = ld(mc + 2 * m);
-      acc[0] = cfma(tp, m0, acc[0]);
-      if (NE == 3) {
-        acc[1] = cfma(tp, ld(mc + 2 * (P + m)), acc[1]);
-        acc[2] = cfma(tp, cfma(shift, m0, ld(mc + 2 * (2 * P + m))), acc[2]);
-      }
+      const cplx tp = cscale(tm[m], pw);
+#pragma unroll
+      for (int e = 0; e < NE; e++) acc[e] = cfma(tp, {sm[wv][e][m][0], sm[wv][e][m][1]}, acc[e]);
       const cplx q = cmul(pw, d);
       pw.r = m <= k ? q.r : pw.r;
       pw.i = m <= k ? q.i : pw.i;
     }
   }
+  __syncthreads();
   if (lane < P) {
 #pragma unroll
     for (int e = 0; e < NE; e++) {
+      sm[wv][e][lane][0] = acc[e].r;
+      sm[wv][e][lane][1] = acc[e].i;
+    }
+  }
+  __syncthreads();
+  if (wv == 0 && lane < P) {
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+      double r = sm[0][e][lane][0], i = sm[0][e][lane][1];
+      for (int w = 1; w < nc; w++) {
+        r += sm[w][e][lane][0];
+        i += sm[w][e][lane][1];
+      }
       double* m = f.mexp + 2 * (((size_t)b * NE + e) * P + lane);
-      m[0] = acc[e].r;
-      m[1] = acc[e].i;
+      m[0] = r;
+      m[1] = i;
     }
   }
 }
 
-// Locals of level `level`'s boxes: the parent's local shifted in (L2L)
-// plus the interaction list's multipoles (M2L).
+// A source box's multipole with its Stokes c4 coefficients moved to the
+// target's centre (lane < P: coefficient `lane`).
 template <int P, int NE>
-__global__ __launch_bounds__(64) void k_fmm_down(FmmDev f, int first, int level) {
-  __shared__ double sa[NE][P][2];
-  const int b = first + blockIdx.x, lane = threadIdx.x;
+__device__ inline void load_source(const FmmDev& f, int src, int lane, double cx, double cy,
+                                   cplx (&m)[NE]) {
+  const double* ms = f.mexp + 2 * ((size_t)src * NE * P);
+#pragma unroll
+  for (int e = 0; e < NE; e++) m[e] = ld(ms + 2 * (e * P + lane));
+  if (NE == 3)  // conj(c_source - c_target)
+    m[NE - 1] = cfma({f.center[2 * src] - cx, -(f.center[2 * src + 1] - cy)}, m[0], m[NE - 1]);
+}
+
+// Locals of level `level`'s boxes: the parent's local shifted in (L2L,
+// wave 0) plus the interaction list's multipoles (M2L): W waves per box,
+// wave w taking sources w, w + W, ... (the next one loaded while the
+// current one is summed); the partial sums are added in wave order.  Four
+// waves cut the latency of a box on the upper levels, where there are too
+// few boxes to fill the chip; one wave per box wastes none on the lower.
+template <int P, int NE, int W>
+__global__ __launch_bounds__(64 * W) void k_fmm_down(FmmDev f, int first, int level) {
+  __shared__ double sa[W][NE][P][2];
+  const int b = first + blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int j = lane < P ? lane : P - 1;
   const double s = ldexp(f.side, -level);
   const double cx = f.center[2 * b], cy = f.center[2 * b + 1];
   cplx acc[NE];
 #pragma unroll
   for (int e = 0; e < NE; e++) acc[e] = {0.0, 0.0};
-  if (level > 2) {
+  if (level > 2 && wv == 0) {
     const int p = f.parent[b];
-    const double inv_sp = 0.5 / s;
     const double dx = cx - f.center[2 * p], dy = cy - f.center[2 * p + 1];
+    const double inv_sp = 0.5 / s;
     const cplx d = {dx * inv_sp, dy * inv_sp};
-    const cplx shift = {-dx, dy};  // conj(c_parent - c_child)
     const double* lp = f.lexp + 2 * ((size_t)p * NE * P);
-    cplx pw = {1.0, 0.0};
-    for (int l = 0; l < P; l++) {
-      const double t = f.t_l2l[l * kFmmMaxOrder + j];  // 0 for l < j
-      const cplx tp = cscale(t, pw);
-      const cplx l0 = ld(lp + 2 * l);
-      acc[0] = cfma(tp, l0, acc[0]);
-      if (NE == 3) {
-        acc[1] = cfma(tp, ld(lp + 2 * (P + l)), acc[1]);
-        acc[2] = cfma(tp, cfma(shift, l0, ld(lp + 2 * (2 * P + l))), acc[2]);
+    cplx l[NE];
+#pragma unroll
+    for (int e = 0; e < NE; e++) l[e] = ld(lp + 2 * (e * P + j));
+    if (NE == 3) l[NE - 1] = cfma({-dx, dy}, l[0], l[NE - 1]);  // conj(c_parent - c_child)
+    if (lane < P) {
+#pragma unroll
+      for (int e = 0; e < NE; e++) {
+        sa[0][e][lane][0] = l[e].r;
+        sa[0][e][lane][1] = l[e].i;
       }
-      const cplx q = cmul(pw, d);
-      pw.r = l >= j ? q.r : pw.r;
-      pw.i = l >= j ? q.i : pw.i;
+    }
+    // one wave reads what it wrote: a wave-level barrier orders the LDS access
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    cplx pw = {1.0, 0.0};
+    // the table is read in the loop (once per box) so that its registers
+    // stay free for the M2L table below
+#pragma unroll 4
+    for (int q = 0; q < P; q++) {
+      const cplx tp = cscale(f.t_l2l[q * kFmmMaxOrder + j], pw);  // C(q, j) 2^-(j+1), 0 for q < j
+#pragma unroll
+      for (int e = 0; e < NE; e++) acc[e] = cfma(tp, {sa[0][e][q][0], sa[0][e][q][1]}, acc[e]);
+      const cplx t = cmul(pw, d);
+      pw.r = q >= j ? t.r : pw.r;
+      pw.i = q >= j ? t.i : pw.i;
     }
   }
+  __syncthreads();
   double tc[P];  // C(k + j, j), k = 0..P-1
 #pragma unroll
   for (int k = 0; k < P; k++) tc[k] = f.t_m2l[k * kFmmMaxOrder + j];
   const double sign = (j & 1) ? -1.0 : 1.0;
   const int v0 = f.vl_off[b], v1 = f.vl_off[b + 1];
-  for (int v = v0; v < v1; v++) {
-    const int src = f.vl[v];
-    const double dx = cx - f.center[2 * src], dy = cy - f.center[2 * src + 1];
-    const cplx w = crecip({dx / s, dy / s});
-    const cplx wj = cpow(w, j);
-    const cplx shift = {-dx, dy};  // conj(c_source - c_target)
-    const double* ms = f.mexp + 2 * ((size_t)src * NE * P);
-    if (lane < P) {
-      cplx m[NE];
+  const int rounds = (v1 - v0 + W - 1) / W;
+  cplx nxt[NE];
 #pragma unroll
-      for (int e = 0; e < NE; e++) m[e] = ld(ms + 2 * (e * P + lane));
-      if (NE == 3) m[NE - 1] = cfma(shift, m[0], m[NE - 1]);
+  for (int e = 0; e < NE; e++) nxt[e] = {0.0, 0.0};
+  if (v0 + wv < v1 && lane < P) load_source<P, NE>(f, f.vl[v0 + wv], lane, cx, cy, nxt);
+  for (int it = 0; it < rounds; it++) {
+    const int v = v0 + W * it + wv;
+    const bool live = v < v1;
+    const int src = f.vl[live ? v : v0];
+    cplx cur[NE];
+#pragma unroll
+    for (int e = 0; e < NE; e++) cur[e] = nxt[e];
+    if (v + W < v1 && lane < P) load_source<P, NE>(f, f.vl[v + W], lane, cx, cy, nxt);
+    const cplx w = crecip({(cx - f.center[2 * src]) / s, (cy - f.center[2 * src + 1]) / s});
+    const cplx wj = cpow(w, j);
+    if (lane < P) {
 #pragma unroll
       for (int e = 0; e < NE; e++) {
-        const cplx a = cmul(m[e], wj);
-        sa[e][lane][0] = a.r;
-        sa[e][lane][1] = a.i;
+        const cplx a = cmul(cur[e], wj);
+        sa[wv][e][lane][0] = a.r;
+        sa[wv][e][lane][1] = a.i;
       }
     }
     __syncthreads();
-    cplx sum[NE];
+    if (live) {
+      cplx sum[NE];
 #pragma unroll
-    for (int e = 0; e < NE; e++) sum[e] = {0.0, 0.0};
+      for (int e = 0; e < NE; e++) sum[e] = {0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < P; k++) {
+      for (int k = 0; k < P; k++) {
 #pragma unroll
-      for (int e = 0; e < NE; e++) {
-        sum[e].r = fma(tc[k], sa[e][k][0], sum[e].r);
-        sum[e].i = fma(tc[k], sa[e][k][1], sum[e].i);
+        for (int e = 0; e < NE; e++) {
+          sum[e].r = fma(tc[k], sa[wv][e][k][0], sum[e].r);
+          sum[e].i = fma(tc[k], sa[wv][e][k][1], sum[e].i);
+        }
       }
-    }
-    const cplx g = cscale(sign, cmul(wj, w));
+      const cplx g = cscale(sign, cmul(wj, w));
 #pragma unroll
-    for (int e = 0; e < NE; e++) acc[e] = cfma(g, sum[e], acc[e]);
+      for (int e = 0; e < NE; e++) acc[e] = cfma(g, sum[e], acc[e]);
+    }
     __syncthreads();
   }
-  if (lane < P) {
+  if (W > 1 && lane < P) {
 #pragma unroll
     for (int e = 0; e < NE; e++) {
+      sa[wv][e][lane][0] = acc[e].r;
+      sa[wv][e][lane][1] = acc[e].i;
+    }
+  }
+  if (W > 1) __syncthreads();
+  if (wv == 0 && lane < P) {
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+      double r = W > 1 ? sa[0][e][lane][0] : acc[e].r, i = W > 1 ? sa[0][e][lane][1] : acc[e].i;
+      for (int w = 1; w < W; w++) {
+        r += sa[w][e][lane][0];
+        i += sa[w][e][lane][1];
+      }
       double* m = f.lexp + 2 * (((size_t)b * NE + e) * P + lane);
-      m[0] = acc[e].r;
-      m[1] = acc[e].i;
+      m[0] = r;
+      m[1] = i;
     }
   }
 }
@@ -279,13 +375,15 @@ template <int P, int PDE>
 __global__ __launch_bounds__(64) void k_fmm_eval(FmmDev f, const double* wn, double* y) {
   constexpr int NE = Pde<PDE>::NE;
   __shared__ double sx[64], sy[64], sa[64], sb[64], sm0[64], sm1[64], sd0[64], sd1[64];
+  __shared__ double lb[NE * P * 2];  // the leaf's local coefficients
   const bool stokes = PDE == kPdeStokes, neumann = PDE == kPdeLaplaceNeumann;
   const int leaf = blockIdx.x, lane = threadIdx.x;
   const int b = f.first_leaf + leaf;
   const int p0 = f.pt_off[leaf], p1 = f.pt_off[leaf + 1];
   const double s = ldexp(f.side, -f.depth), inv_s = 1.0 / s;
   const double cx = f.center[2 * b], cy = f.center[2 * b + 1];
-  const double* lb = f.lexp + 2 * ((size_t)b * NE * P);
+  for (int t = lane; t < NE * P * 2; t += 64) lb[t] = f.lexp[(size_t)b * NE * P * 2 + t];
+  __syncthreads();
   const int q0 = f.nb_off[leaf], q1 = f.nb_off[leaf + 1];
   for (int base = p0; base < p1; base += 64) {
     const int i = base + lane;
@@ -297,6 +395,7 @@ __global__ __launch_bounds__(64) void k_fmm_eval(FmmDev f, const double* wn, dou
     double a0, a1 = 0;
     {
       cplx v0 = {0.0, 0.0}, dv0 = {0.0, 0.0};
+#pragma unroll 8
       for (int l = P - 1; l >= 0; l--) {
         const cplx c = ld(lb + 2 * l);
         dv0 = l > 0 ? cfma(dv0, u, cscale(double(l), c)) : dv0;
@@ -305,6 +404,7 @@ __global__ __launch_bounds__(64) void k_fmm_eval(FmmDev f, const double* wn, dou
       const cplx phi = cscale(inv_s, v0);
       if (stokes) {
         cplx v1 = {0.0, 0.0}, dv2 = {0.0, 0.0};
+#pragma unroll 8
         for (int l = P - 1; l >= 0; l--) {
           v1 = cfma(v1, u, ld(lb + 2 * (P + l)));
           if (l > 0) dv2 = cfma(dv2, u, cscale(double(l), ld(lb + 2 * (2 * P + l))));
@@ -396,6 +496,9 @@ __global__ __launch_bounds__(64) void k_fmm_eval(FmmDev f, const double* wn, dou
   }
 }
 
+// Levels with fewer boxes than this take four waves per box downward.
+constexpr int kDownWideBoxes = 1536;
+
 template <int P, int PDE>
 void run_fmm(const DevGeom& g, const FmmDev& f, const std::vector<int>& level_off,
              const double* x, double* y, const double* wn, hipStream_t st) {
@@ -404,11 +507,12 @@ void run_fmm(const DevGeom& g, const FmmDev& f, const std::vector<int>& level_of
   k_fmm_p2m<P, PDE><<<f.n_leaves, 64, 0, st>>>(f, x);
   for (int l = L - 1; l >= 2; l--) {
     const int nbx = level_off[l - 1] - level_off[l - 2];
-    k_fmm_m2m<P, NE><<<nbx, 64, 0, st>>>(f, level_off[l - 2], l);
+    k_fmm_m2m<P, NE><<<nbx, 256, 0, st>>>(f, level_off[l - 2], l);
   }
   for (int l = 2; l <= L; l++) {
     const int nbx = level_off[l - 1] - level_off[l - 2];
-    k_fmm_down<P, NE><<<nbx, 64, 0, st>>>(f, level_off[l - 2], l);
+    if (nbx < kDownWideBoxes) k_fmm_down<P, NE, 4><<<nbx, 256, 0, st>>>(f, level_off[l - 2], l);
+    else k_fmm_down<P, NE, 1><<<nbx, 64, 0, st>>>(f, level_off[l - 2], l);
   }
   k_fmm_eval<P, PDE><<<f.n_leaves, 64, 0, st>>>(f, wn, y);
 }

@@ -33,16 +33,22 @@ __device__ inline double rcp_f64(double d) {
 }
 
 // c_p = sum_j w_j (n_j . x_j) over Stokes DOF pairs, or sum_j w_j x_j for
-// Laplace Neumann (its rank-one term), over problem p's points (one
-// workgroup per problem; all points when n_prob = 1).
+// Laplace Neumann (its rank-one term), over problem p's points: one
+// workgroup per problem of a batch; a single boundary is split over
+// kWnChunks workgroups (partial[1 + b]) summed by k_wn_final into partial[0].
+constexpr int kWnChunks = 256;
+
 __global__ __launch_bounds__(256) void k_wn_dot(DevGeom g, const double* x,
                                                 double* partial) {
   __shared__ double red[256];
   const bool stokes = g.pde == kPdeStokes;
-  const int lo = g.n_prob > 1 ? g.prob_off[blockIdx.x] : 0;
-  const int hi = g.n_prob > 1 ? g.prob_off[blockIdx.x + 1] : g.n_points;
+  const bool split = g.n_prob == 1;
+  const int lo = split ? 0 : g.prob_off[blockIdx.x];
+  const int hi = split ? g.n_points : g.prob_off[blockIdx.x + 1];
+  const int64_t first = lo + (split ? (int64_t)blockIdx.x * 256 : 0) + threadIdx.x;
+  const int64_t step = split ? (int64_t)gridDim.x * 256 : 256;
   double acc = 0;
-  for (int64_t j = lo + threadIdx.x; j < hi; j += 256)
+  for (int64_t j = first; j < hi; j += step)
     acc += stokes ? g.wts[j] * (g.nrm[2 * j] * x[2 * j] + g.nrm[2 * j + 1] * x[2 * j + 1])
                   : g.wts[j] * x[j];
   red[threadIdx.x] = acc;
@@ -51,7 +57,18 @@ __global__ __launch_bounds__(256) void k_wn_dot(DevGeom g, const double* x,
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) partial[split ? 1 + blockIdx.x : blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_wn_final(double* partial) {
+  __shared__ double red[256];
+  red[threadIdx.x] = partial[1 + threadIdx.x];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[0] = red[0];
 }
 
 // Problem owning point i: the largest p with prob_off[p] <= i.
@@ -247,17 +264,21 @@ void launch_axpy(double* y, const double* x, int64_t n, hipStream_t st) {
 }
 
 int matvec_partials(const DevGeom& g) {
-  return g.n_prob > 256 ? g.n_prob : 256;
+  return g.n_prob > 1 + kWnChunks ? g.n_prob : 1 + kWnChunks;
 }
 
 void launch_wn_dot(const DevGeom& g, const double* x, double* partial, hipStream_t st) {
-  k_wn_dot<<<g.n_prob, 256, 0, st>>>(g, x, partial);
+  if (g.n_prob == 1) {
+    k_wn_dot<<<kWnChunks, 256, 0, st>>>(g, x, partial);
+    k_wn_final<<<1, 256, 0, st>>>(partial);
+  } else {
+    k_wn_dot<<<g.n_prob, 256, 0, st>>>(g, x, partial);
+  }
 }
 
 void launch_matvec(const DevGeom& g, const double* x, double* y, double* partial,
                    hipStream_t st) {
-  if (g.pde == kPdeStokes || g.pde == kPdeLaplaceNeumann)
-    k_wn_dot<<<g.n_prob, 256, 0, st>>>(g, x, partial);
+  if (g.pde == kPdeStokes || g.pde == kPdeLaplaceNeumann) launch_wn_dot(g, x, partial, st);
   k_apply<<<(g.n_points + kTile - 1) / kTile, kTile, 0, st>>>(g, x, partial, y);
 }
 

@@ -413,7 +413,7 @@ class RSSolver:
     def set_matvec(self, method=MATVEC_AUTO, order=0, leaf_points=0):
         """Operator form of apply and solve_refined: MATVEC_DIRECT (O(N^2)
         sum), MATVEC_FMM (fast multipole, `order` 32/48/64 coefficients,
-        about `leaf_points` nodes per leaf; 0 = defaults 48 and 32) or
+        about `leaf_points` nodes per leaf; 0 = defaults 48 and 64) or
         MATVEC_AUTO (FMM from FMM_AUTO_POINTS nodes of one boundary)."""
         self._check(self.L.iegpu_set_matvec(self.h, method, order, leaf_points))
 

@@ -210,7 +210,7 @@ int iegpu_apply(iegpu_ctx* ctx, const double* x, const double* alpha,
  *   IEGPU_MATVEC_FMM     a fast multipole method: complex expansions of
  *                        `order` coefficients (32, 48 or 64; 0 = 48) on a
  *                        uniform quadtree whose non-empty leaves hold about
- *                        `leaf_points` nodes (0 = 32), the adjacent leaves
+ *                        `leaf_points` nodes (0 = 64), the adjacent leaves
  *                        summed directly; O(N), ~1e-15 relative at order 48;
  *   IEGPU_MATVEC_AUTO    (default) the FMM for a single boundary of at
  *                        least IEGPU_FMM_AUTO_POINTS nodes, else direct.
