@@ -207,7 +207,7 @@ struct iegpu_ctx {
   int64_t id_cap = 0, row_cap = 0, fac_cap = 0, dec_cap = 0;
   // randomized ID
   int rid_mode = 0, rid_width = 0;
-  DevBuf<double> d_omega;
+  DevBuf<double> d_omega, d_omega_w;
   // factorization replayed as a captured HIP graph (iegpu_set_graphs)
   bool use_graphs = false;
   hipGraphExec_t fgraph = nullptr;
@@ -332,6 +332,7 @@ struct iegpu_ctx {
     s.sk_w = s_sk_w.p;
     s.rid_done = s_rid_done.p;
     s.omega = d_omega.p;
+    s.omega_w = d_omega_w.p;
     s.rank_model = d_model.p;
     s.rid_margin = rid_margin;
     s.feat = record_features ? d_feat.p : nullptr;
@@ -1350,6 +1351,9 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
       const int64_t no = (int64_t)kRidTableRows * kRidRows;
       c->d_omega.alloc(no);
       launch_gaussian(c->d_omega.p, no, 0x5EEDull, c->stream);
+      const int64_t nw = (int64_t)kRidTableRowsW * kRidRowsW;
+      c->d_omega_w.alloc(nw);
+      launch_gaussian(c->d_omega_w.p, nw, 0x5EEDull, c->stream);
     }
     // After iegpu_update_geometry on the same tree, only the marked boxes
     // are re-factored (same tolerance and precision as the kept factors).

@@ -65,6 +65,14 @@ constexpr int kRidRows = 32;
 constexpr int kRidOver = 6;
 constexpr int kRidMaxNear = 256;
 constexpr int kRidTableRows = 2 * kRidMaxNear + 4 * kNumProxy;  // Omega rows
+// The wide form for the few large boxes of the top levels (level <=
+// kRidWideTop): a sketch of kRidRowsW rows, up to kRidWideCols columns and
+// kRidMaxNearW row DOFs, with its own Gaussian table.
+constexpr int kRidRowsW = 64;
+constexpr int kRidWideCols = 64;
+constexpr int kRidMaxNearW = 512;
+constexpr int kRidWideTop = 4;
+constexpr int kRidTableRowsW = 2 * kRidMaxNearW + 4 * kNumProxy;
 
 // Rank model: an MLP kFeat -> kHid (tanh) -> kHid (tanh) -> 1 on per-box
 // features, predicting the ID rank.  Parameter blob (floats): feature
@@ -178,6 +186,7 @@ struct DevState {
   int* sk_w;           // per box: sketch rows (0: full QRCP)
   int* rid_done;       // per box: 1 once the randomized ID succeeded
   const double* omega; // kRidTableRows x kRidRows Gaussian table (row-major)
+  const double* omega_w;  // kRidTableRowsW x kRidRowsW table of the wide form
   const float* rank_model;  // kMlpLen floats (rid_mode 2)
   int rid_margin;           // ranks added to the prediction before sizing
   float* feat;              // kFeat per box, written by k_rows when non-null

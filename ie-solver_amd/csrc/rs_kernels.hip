@@ -234,18 +234,28 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
     s.fac_sz[b] = reuse ? -old : need;
   }
   // Sketch width: fixed, or from the rank model (0: straight to the full
-  // QRCP when the predicted rank does not fit the widest sketch).
+  // QRCP when the predicted rank does not fit the widest sketch).  Boxes
+  // too large for the narrow sketch on the top levels take the wide one.
+  const int near = (m - proxy_rows(g, level)) / 2;
   const bool elig = s.rid_mode != 0 && (kind == kQrcpReg || kind == kQrcpStack) &&
-                    (m - proxy_rows(g, level)) / 2 <= kRidMaxNear;
-  int w = elig ? s.rid_width : 0;
-  if (m > 0 && (s.feat != nullptr || (elig && s.rid_mode == 2))) {
+                    near <= kRidMaxNear;
+  const bool eligw = s.rid_mode != 0 && m > 0 && level <= kRidWideTop && n <= kRidWideCols &&
+                     near <= kRidMaxNearW;
+  int w = elig ? s.rid_width : (eligw ? kRidRowsW : 0);
+  if (m > 0 && (s.feat != nullptr || ((elig || eligw) && s.rid_mode == 2))) {
     float f[kFeat];
-    box_features(g, t, s, b, level, n, m, (m - proxy_rows(g, level)) / 2, tol, f);
+    box_features(g, t, s, b, level, n, m, near, tol, f);
     if (s.feat != nullptr && threadIdx.x == 0)
       for (int q = 0; q < kFeat; q++) s.feat[(int64_t)b * kFeat + q] = f[q];
-    if (elig && s.rid_mode == 2) {
+    if ((elig || eligw) && s.rid_mode == 2) {
+      // The model sizes the narrow sketch; a box it does not fit tries the
+      // wide one when eligible (few boxes, whose ranks the model has seen
+      // least), else goes straight to the full QRCP.
       const float kp = mlp_rank(s.rank_model, f, hb) + (float)s.rid_margin;
-      w = kp <= 16 - kRidOver ? 16 : (kp <= kRidRows - kRidOver ? kRidRows : 0);
+      w = (elig && kp <= 16 - kRidOver) ? 16
+          : (elig && kp <= kRidRows - kRidOver) ? kRidRows
+          : eligw ? kRidRowsW
+                  : 0;
     }
   }
   if (threadIdx.x == 0) {
@@ -715,7 +725,7 @@ __global__ __launch_bounds__(kXThreads) void k_rowcomp(DevGeom g, DevTree t, Dev
     } else {
       b = jobs[2 * job];
       q = jobs[2 * job + 1];
-      if (s.qkind[b] != kind) continue;
+      if (s.qkind[b] != kind || s.rid_done[b]) continue;  // rid_done: sketched
     }
     const int n = s.act_cnt[b], m = s.m_rows[b];
     const int nch = id_chunks(m, MAXM);
@@ -793,6 +803,7 @@ __device__ __forceinline__ void qrcp_id_box(const DevGeom& g, const DevTree& t, 
   __shared__ int colmap[MAXN];
   __shared__ int rowdofs[MAXM / 2];
   __shared__ double pxy[4][kNumProxy];
+  if (s.rid_done[b]) return;  // the wide sketch took it
 
   const int kind = s.qkind[b];
   const int n = s.act_cnt[b];
@@ -1162,6 +1173,7 @@ __global__ __launch_bounds__(256, 3) void k_rid(DevGeom g, DevTree t, DevState s
   const int b = list_at(s, start, kLstRid, blockIdx.x);
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int n = s.act_cnt[b], m = s.m_rows[b], l = s.sk_w[b];
+  if (l > kRidRows) return;  // the wide form's box
   const int P = proxy_rows(g, level) / 2;  // rows per proxy block
   const int cnt = (m - 2 * P) / 2;
   const int* act = s.act_list + (int64_t)b * s.ncap;
@@ -1286,24 +1298,32 @@ __global__ __launch_bounds__(256, 3) void k_rid(DevGeom g, DevTree t, DevState s
     k = n;  // every column taken: nothing redundant
   }
 
-  // T = R11^{-1} R12, column sweep in place on the rows of R: R11(i, q) =
-  // rr[i][spiv[q]], the redundant lanes' columns become T.
+  // T = R11^{-1} R12 by a column sweep in registers: lane c holds column c
+  // of R (rows 0..k-1), R11(i, q) = rr[i][spiv[q]] is read from LDS; the
+  // redundant lanes' columns become T.
+  constexpr int KMAX = LM - kRidOver;
   const bool red = avail;
-  for (int q = k - 1; q >= 0; q--) {
-    const int pq = spiv[q];
-    const R dinv = recip_any(rr[q * YLD + pq]);
-    if (red) {
-      const R tq = rr[q * YLD + lane] * dinv;
-      rr[q * YLD + lane] = tq;
-      for (int i = 0; i < q; i++) rr[i * YLD + lane] -= rr[i * YLD + pq] * tq;
+  R x[KMAX];
+#pragma unroll
+  for (int i = 0; i < KMAX; i++) x[i] = i < k ? rr[i * YLD + lane] : R(0);
+#pragma unroll
+  for (int q = KMAX - 1; q >= 0; q--) {
+    if (q < k) {
+      const int pq = spiv[q];
+      const R tq = x[q] * recip_any(rr[q * YLD + pq]);
+      x[q] = tq;
+#pragma unroll
+      for (int i = 0; i < q; i++) x[i] -= rr[i * YLD + pq] * tq;
     }
-    wave_sync();
   }
   const uint64_t redmask = __ballot(red);
   const int cp = __popcll(redmask & ((1ull << lane) - 1));  // redundant index
   R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
-  if (red)
-    for (int i = 0; i < k; i++) F[i + (int64_t)k * cp] = rr[i * YLD + lane];
+  if (red) {
+#pragma unroll
+    for (int i = 0; i < KMAX; i++)
+      if (i < k) F[i + (int64_t)k * cp] = x[i];
+  }
   int* out = s.out_list + (int64_t)b * s.ncap;
   int* rl = s.red_list + (int64_t)b * s.ncap;
   int* pm = s.perm + (int64_t)b * s.ncap;
@@ -1315,6 +1335,296 @@ __global__ __launch_bounds__(256, 3) void k_rid(DevGeom g, DevTree t, DevState s
   if (red) {
     pm[k + cp] = lane;
     rl[cp] = act[lane];
+  }
+  if (lane == 0) {
+    s.rank[b] = k;
+    s.out_cnt[b] = k;
+    s.rid_done[b] = 1;
+  }
+}
+
+// Wide randomized ID: the boxes of the top levels too large for k_rid (up
+// to kRidMaxNearW row DOFs, ranks up to kRidRowsW - kRidOver; columns up
+// to kRidWideCols = 64).  Same sketch, rank rule and outputs as k_rid with
+// a sketch of kRidRowsW rows from its own table, on 8 waves: wave w
+// sketches the 16-column tile w % 4 over every other 16-row block (part
+// w / 4; the two parts added in a fixed order); wave 0 then runs the
+// pivoted QR, one column per lane.  A box whose sketch shows no cut keeps
+// rid_done = 0: a register-QRCP box joins the fallback list, any other is
+// still on its full-QRCP list (whose kernels skip the boxes the sketch
+// took).
+template <typename R, int PDE, bool TG, bool NEAR>
+__device__ __forceinline__ void rid_segment_w(const double (*tab)[kRidMaxNearW + kNumProxy],
+                                              const int* rowdofs, const double* omega, int s0,
+                                              int s1, int idx0, int part, int parts,
+                                              const DofGeo& cg, bool cv,
+                                              typename Mfma16<R>::V4 (&acc)[4]) {
+  constexpr int LM = kRidRowsW;
+  constexpr bool stokes = PDE == kPdeStokes;
+  const int lane = threadIdx.x & 63, cl = lane & 15, kq = lane >> 4;
+  // the next block's Omega operands are loaded before this block's entries
+  auto load_omega = [&](int k0, R (&w)[4][4]) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int r = k0 + 4 * u + kq;
+      const double* om = omega + (int64_t)min(r, s1 - 1) * LM + cl;
+#pragma unroll
+      for (int q = 0; q < 4; q++) w[q][u] = r < s1 ? static_cast<R>(om[16 * q]) : R(0);
+    }
+  };
+  const int step = 16 * parts;
+  R nw[4][4];
+  load_omega(s0 + 16 * part, nw);
+  for (int k0 = s0 + 16 * part; k0 < s1; k0 += step) {
+    R w[4][4], a[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) w[q][u] = nw[q][u];
+    if (k0 + step < s1) load_omega(k0 + step, nw);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int r = k0 + 4 * u + kq;
+      const int rel = min(r, s1 - 1) - s0;
+      const int idx = idx0 + ((stokes && !NEAR) ? rel >> 1 : rel);
+      DofGeo rg;
+      rg.x = tab[0][idx];
+      rg.y = tab[1][idx];
+      rg.n1 = tab[2][idx];
+      rg.n2 = tab[3][idx];
+      rg.w = tab[4][idx];
+      rg.k = tab[5][idx];
+      rg.c = stokes ? (NEAR ? rowdofs[rel] & 1 : rel & 1) : 0;
+      const double e = TG ? kernel_entry(PDE, rg, cg, NEAR) : kernel_entry(PDE, cg, rg, NEAR);
+      a[u] = (cv && r < s1) ? static_cast<R>(e) : R(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) acc[q] = mfma16x16x4(w[q][u], a[u], acc[q]);
+  }
+}
+
+constexpr int kRidWideWaves = 8;
+
+template <typename R, int PDE>
+__global__ __launch_bounds__(64 * kRidWideWaves, 1) void k_rid_wide(
+    DevGeom g, DevTree t, DevState s, int start, int level, double tol_d) {
+  constexpr int NC = 1;  // columns per lane in the QR
+  constexpr int kTiles = 4, kParts = kRidWideWaves / kTiles;
+  constexpr int NT = 64 * kRidWideWaves;
+  constexpr int LM = kRidRowsW, YLD = 64 * NC + 1, TR = kRidMaxNearW + kNumProxy;
+  constexpr int kBuf = (LM * YLD * (int)sizeof(R) + 7) / 8 > 6 * TR
+                           ? (LM * YLD * (int)sizeof(R) + 7) / 8
+                           : 6 * TR;
+  using V4 = typename Mfma16<R>::V4;
+  // Row geometry during the sketch, then Y (row-major, ld YLD), then the
+  // rows of R.
+  __shared__ double buf[kBuf];
+  __shared__ int rowdofs[kRidMaxNearW];
+  __shared__ R vbuf[LM];
+  __shared__ int spiv[64];
+  double (*tab)[TR] = reinterpret_cast<double (*)[TR]>(buf);
+  R* ybuf = reinterpret_cast<R*>(buf);
+  if (!pools_ok(s)) return;
+  if ((int)blockIdx.x >= list_count(s, level, kLstRid)) return;
+  const int b = list_at(s, start, kLstRid, blockIdx.x);
+  const int n = s.act_cnt[b], m = s.m_rows[b], l = s.sk_w[b];
+  if (l != kRidRowsW) return;  // a narrow sketch (k_rid)
+  const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
+  const int P = proxy_rows(g, level) / 2;
+  const int cnt = (m - 2 * P) / 2;
+  const int* act = s.act_list + (int64_t)b * s.ncap;
+  if (wv == 0)
+    for_each_row_dof(g, t, s, b, level >= 3, [&](int pos, int dof) { rowdofs[pos] = dof; });
+  if (level >= 3 && tid < kNumProxy) {
+    double px, py, n1, n2;
+    proxy_point(t.cx[b], t.cy[b], t.side[b] * kRadiusRatio, tid, &px, &py, &n1, &n2);
+    tab[0][cnt + tid] = px;
+    tab[1][cnt + tid] = py;
+    tab[2][cnt + tid] = n1;
+    tab[3][cnt + tid] = n2;
+    tab[4][cnt + tid] = proxy_weight(t, b);
+    tab[5][cnt + tid] = 0;
+  }
+  __syncthreads();
+  for (int i = tid; i < cnt; i += NT) {
+    const DofGeo d = dof_geo(g, rowdofs[i]);
+    tab[0][i] = d.x;
+    tab[1][i] = d.y;
+    tab[2][i] = d.n1;
+    tab[3][i] = d.n2;
+    tab[4][i] = d.w;
+    tab[5][i] = d.k;
+  }
+  __syncthreads();
+
+  // Sketch: wave wv takes tile wv % kTiles, row part wv / kTiles.
+  const int kq = lane >> 4;
+  const int tile = wv % kTiles, part = wv / kTiles;
+  V4 acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) acc[q] = V4{0, 0, 0, 0};
+  const int c = 16 * tile + (lane & 15);
+  if (16 * tile < n) {
+    const bool cv = c < n;
+    const DofGeo cg = dof_geo(g, act[cv ? c : n - 1]);
+    rid_segment_w<R, PDE, true, true>(tab, rowdofs, s.omega_w, 0, cnt, 0, part, kParts, cg, cv,
+                                      acc);
+    rid_segment_w<R, PDE, false, true>(tab, rowdofs, s.omega_w, cnt, 2 * cnt, 0, part, kParts,
+                                       cg, cv, acc);
+    if (P > 0) {
+      rid_segment_w<R, PDE, true, false>(tab, rowdofs, s.omega_w, 2 * cnt, 2 * cnt + P, cnt,
+                                         part, kParts, cg, cv, acc);
+      rid_segment_w<R, PDE, false, false>(tab, rowdofs, s.omega_w, 2 * cnt + P, m, cnt, part,
+                                          kParts, cg, cv, acc);
+    }
+  }
+  __syncthreads();  // the row table becomes Y
+  for (int p = 0; p < kParts; p++) {  // the row parts added in order
+    if (part == p) {
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          R* y = &ybuf[(16 * q + Mfma16<R>::row(kq, v)) * YLD + c];
+          *y = p == 0 ? acc[q][v] : *y + acc[q][v];
+        }
+    }
+    __syncthreads();
+  }
+  if (wv != 0) return;
+
+  // Pivoted QR of the sketch (wave 0): lane c holds columns c + 64 h.
+  R y[NC][LM];
+#pragma unroll
+  for (int h = 0; h < NC; h++)
+#pragma unroll
+    for (int r = 0; r < LM; r++) y[h][r] = ybuf[r * YLD + 64 * h + lane];
+  R* rr = ybuf;  // rows of R, rr[j * YLD + c]
+  const R tol = static_cast<R>(tol_d);
+  bool av[NC];
+  R tl[NC];
+#pragma unroll
+  for (int h = 0; h < NC; h++) {
+    av[h] = 64 * h + lane < n;
+    R p4[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 1; r < LM; r++) p4[r & 3] += y[h][r] * y[h][r];
+    tl[h] = (p4[0] + p4[1]) + (p4[2] + p4[3]);
+  }
+  int k = -1;
+  R thresh = 0;
+  const int jlim = min(n, l - kRidOver);
+  int j = 0;
+  for (; j < jlim; j++) {
+    double key = -1.0;
+#pragma unroll
+    for (int h = 0; h < NC; h++)
+      if (av[h]) key = fmax(key, pivot_key<R, 7>(y[h][0] * y[h][0] + tl[h], 64 * h + lane));
+    const double best = wave_max(key);
+    const int p = pivot_key_pos<7>(best);  // column index
+    const int ph = p >> 6, pl = p & 63;
+    R alpha = 0, xt = 0;
+#pragma unroll
+    for (int h = 0; h < NC; h++)
+      if (ph == h) {
+        alpha = lane_value(y[h][0], pl);
+        xt = lane_value(tl[h], pl);
+      }
+    R beta, tau, sc;
+    householder_scalars(alpha, xt, &beta, &tau, &sc);
+    if (j == 0) {
+      thresh = fabs(tol * beta);
+    } else if (2 * fabs(beta) * sqrt(static_cast<R>(l) / static_cast<R>(l - j)) < thresh) {
+      k = j;
+      break;
+    }
+    if (lane == pl) {
+      vbuf[0] = R(1);
+#pragma unroll
+      for (int h = 0; h < NC; h++)
+        if (ph == h) {
+#pragma unroll
+          for (int r = 1; r < LM; r++) vbuf[r] = y[h][r] * sc;
+          av[h] = false;
+        }
+      spiv[j] = p;
+    }
+    wave_sync();
+#pragma unroll
+    for (int h = 0; h < NC; h++) {
+      R d4[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < LM; r++) d4[r & 3] += vbuf[r] * y[h][r];
+      const R w = tau * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
+      rr[j * YLD + 64 * h + lane] = (p == 64 * h + lane) ? beta : y[h][0] - w;
+      // vbuf is read again below rather than kept in registers beside y
+      __asm__ volatile("" ::: "memory");
+      R t4[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < LM - 1; r++) {
+        y[h][r] = y[h][r + 1] - w * vbuf[r + 1];
+        if (r > 0) t4[r & 3] += y[h][r] * y[h][r];
+      }
+      y[h][LM - 1] = 0;
+      tl[h] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+    }
+    wave_sync();
+  }
+  const bool regq = s.qkind[b] == kQrcpReg || s.qkind[b] == kQrcpStack;
+  if (k < 0) {
+    if (j < n) {  // no cut within the sketch: the full QRCP takes the box
+      if (regq && lane == 0) {
+        const int pos = atomicAdd(list_slot(s, level, kLstFallback), 1);
+        s.lists[(int64_t)kLstFallback * s.nbox + start + pos] = b;
+      }
+      return;
+    }
+    k = n;
+  }
+
+  // T = R11^{-1} R12 by a column sweep in registers, one column of the lane
+  // (lane + 64 h) at a time, R11 read from LDS.
+  constexpr int KMAX = LM - kRidOver;
+  uint64_t mk[NC];
+#pragma unroll
+  for (int h = 0; h < NC; h++) mk[h] = __ballot(av[h]);
+  const uint64_t below = (1ull << lane) - 1;
+  R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
+  int* out = s.out_list + (int64_t)b * s.ncap;
+  int* rl = s.red_list + (int64_t)b * s.ncap;
+  int* pm = s.perm + (int64_t)b * s.ncap;
+  int before = 0;
+#pragma unroll
+  for (int h = 0; h < NC; h++) {
+    R x[KMAX];
+#pragma unroll
+    for (int i = 0; i < KMAX; i++) x[i] = i < k ? rr[i * YLD + 64 * h + lane] : R(0);
+#pragma unroll
+    for (int q = KMAX - 1; q >= 0; q--) {
+      if (q < k) {
+        const int pq = spiv[q];
+        const R tq = x[q] * recip_any(rr[q * YLD + pq]);
+        x[q] = tq;
+#pragma unroll
+        for (int i = 0; i < q; i++) x[i] -= rr[i * YLD + pq] * tq;
+      }
+    }
+    const int cp = before + __popcll(mk[h] & below);  // redundant index
+    if (av[h]) {
+#pragma unroll
+      for (int i = 0; i < KMAX; i++)
+        if (i < k) F[i + (int64_t)k * cp] = x[i];
+      pm[k + cp] = 64 * h + lane;
+      rl[cp] = act[64 * h + lane];
+    }
+    before += __popcll(mk[h]);
+  }
+  if (lane < k) {
+    const int cc = spiv[lane];
+    pm[lane] = cc;
+    out[lane] = act[cc];
   }
   if (lane == 0) {
     s.rank[b] = k;
@@ -1518,7 +1828,9 @@ __global__ __launch_bounds__(NT) void k_qrcp(DevState s, int start, int level, d
   if (!pools_ok(s)) return;
   const int nl = list_count(s, level, kLstLds);
   for (int li = blockIdx.x; li < nl; li += gridDim.x) {
-    qrcp_general_box<R, NT>(s, start, level, list_at(s, start, kLstLds, li), tol_d);
+    const int b = list_at(s, start, kLstLds, li);
+    if (s.rid_done[b]) continue;  // the wide sketch took it
+    qrcp_general_box<R, NT>(s, start, level, b, tol_d);
     __syncthreads();
   }
 }
@@ -2637,6 +2949,12 @@ void launch_gaussian(double* out, int64_t n, uint64_t seed, hipStream_t st) {
   k_gaussian<<<256, 256, 0, st>>>(out, n, seed);
 }
 
+template <typename R, int PDE>
+void launch_rid_wide(const DevGeom& g, const DevTree& t, const DevState& s, int start,
+                     int level, int count, double tol, hipStream_t st) {
+  k_rid_wide<R, PDE><<<count, 64 * kRidWideWaves, 0, st>>>(g, t, s, start, level, tol);
+}
+
 template <typename R>
 void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
                          const DevState& s, int level, int start, int count,
@@ -2663,6 +2981,14 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
           k_rid<R, kPdeLaplaceNeumann><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
         else
           k_rid<R, kPdeLaplace><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
+        if (level <= kRidWideTop) {
+          if (g.pde == kPdeStokes)
+            launch_rid_wide<R, kPdeStokes>(g, t, s, start, level, count, tol, st);
+          else if (g.pde == kPdeLaplaceNeumann)
+            launch_rid_wide<R, kPdeLaplaceNeumann>(g, t, s, start, level, count, tol, st);
+          else
+            launch_rid_wide<R, kPdeLaplace>(g, t, s, start, level, count, tol, st);
+        }
       }
       break;
     case kStageQrcp: {
