@@ -33,22 +33,20 @@ __device__ inline double rcp_f64(double d) {
 }
 
 // c_p = sum_j w_j (n_j . x_j) over Stokes DOF pairs, or sum_j w_j x_j for
-// Laplace Neumann (its rank-one term), over problem p's points: one
-// workgroup per problem of a batch; a single boundary is split over
-// kWnChunks workgroups (partial[1 + b]) summed by k_wn_final into partial[0].
-constexpr int kWnChunks = 256;
+// Laplace Neumann (its rank-one term), over problem p's points, in one
+// fixed order: chunks of kWnChunk points, each reduced by 256 threads
+// (thread t sums points t, t + 256, ... of the chunk, then a tree), the
+// chunk sums added in chunk order.  A single boundary spreads its chunks
+// over workgroups (partial[1 + c]) and k_wn_final adds them into
+// partial[0]; a batch takes one workgroup per problem, walking its chunks.
+// Both give the same bits for the same problem.
+constexpr int kWnChunk = 4096;
 
-__global__ __launch_bounds__(256) void k_wn_dot(DevGeom g, const double* x,
-                                                double* partial) {
-  __shared__ double red[256];
+__device__ inline double wn_chunk_sum(const DevGeom& g, const double* x, int64_t lo,
+                                      int64_t hi, double* red) {
   const bool stokes = g.pde == kPdeStokes;
-  const bool split = g.n_prob == 1;
-  const int lo = split ? 0 : g.prob_off[blockIdx.x];
-  const int hi = split ? g.n_points : g.prob_off[blockIdx.x + 1];
-  const int64_t first = lo + (split ? (int64_t)blockIdx.x * 256 : 0) + threadIdx.x;
-  const int64_t step = split ? (int64_t)gridDim.x * 256 : 256;
   double acc = 0;
-  for (int64_t j = first; j < hi; j += step)
+  for (int64_t j = lo + threadIdx.x; j < hi; j += 256)
     acc += stokes ? g.wts[j] * (g.nrm[2 * j] * x[2 * j] + g.nrm[2 * j + 1] * x[2 * j + 1])
                   : g.wts[j] * x[j];
   red[threadIdx.x] = acc;
@@ -57,19 +55,40 @@ __global__ __launch_bounds__(256) void k_wn_dot(DevGeom g, const double* x,
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[split ? 1 + blockIdx.x : blockIdx.x] = red[0];
+  const double v = red[0];
+  __syncthreads();
+  return v;
 }
 
-__global__ __launch_bounds__(256) void k_wn_final(double* partial) {
+// One chunk of a single boundary per workgroup.
+__global__ __launch_bounds__(256) void k_wn_chunk(DevGeom g, const double* x,
+                                                  double* partial) {
   __shared__ double red[256];
-  red[threadIdx.x] = partial[1 + threadIdx.x];
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[0] = red[0];
+  const int64_t lo = (int64_t)blockIdx.x * kWnChunk;
+  const int64_t hi = lo + kWnChunk < g.n_points ? lo + kWnChunk : g.n_points;
+  const double v = wn_chunk_sum(g, x, lo, hi, red);
+  if (threadIdx.x == 0) partial[1 + blockIdx.x] = v;
 }
+
+__global__ void k_wn_final(double* partial, int nchunks) {
+  if (threadIdx.x != 0) return;
+  double acc = 0;
+  for (int c = 0; c < nchunks; c++) acc += partial[1 + c];
+  partial[0] = acc;
+}
+
+// One problem of a batch per workgroup.
+__global__ __launch_bounds__(256) void k_wn_dot(DevGeom g, const double* x,
+                                                double* partial) {
+  __shared__ double red[256];
+  const int64_t lo = g.prob_off[blockIdx.x], hi = g.prob_off[blockIdx.x + 1];
+  double acc = 0;
+  for (int64_t c = lo; c < hi; c += kWnChunk)
+    acc += wn_chunk_sum(g, x, c, c + kWnChunk < hi ? c + kWnChunk : hi, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+int wn_chunks(int n) { return (n + kWnChunk - 1) / kWnChunk; }
 
 // Problem owning point i: the largest p with prob_off[p] <= i.
 __device__ inline int problem_of(const DevGeom& g, int i) {
@@ -264,13 +283,15 @@ void launch_axpy(double* y, const double* x, int64_t n, hipStream_t st) {
 }
 
 int matvec_partials(const DevGeom& g) {
-  return g.n_prob > 1 + kWnChunks ? g.n_prob : 1 + kWnChunks;
+  const int single = 1 + wn_chunks(g.n_points);
+  return g.n_prob > single ? g.n_prob : single;
 }
 
 void launch_wn_dot(const DevGeom& g, const double* x, double* partial, hipStream_t st) {
   if (g.n_prob == 1) {
-    k_wn_dot<<<kWnChunks, 256, 0, st>>>(g, x, partial);
-    k_wn_final<<<1, 256, 0, st>>>(partial);
+    const int nc = wn_chunks(g.n_points);
+    k_wn_chunk<<<nc, 256, 0, st>>>(g, x, partial);
+    k_wn_final<<<1, 64, 0, st>>>(partial, nc);
   } else {
     k_wn_dot<<<g.n_prob, 256, 0, st>>>(g, x, partial);
   }
