@@ -252,6 +252,31 @@ struct iegpu_ctx {
   int64_t kept_top = 0;  // factor-pool elements in use after the last factor
   int64_t full_top = 0;  // the same after the last full factorization
 
+  // Launch plan: per level, the kSkip* bits of the ID-stage kernels whose
+  // lists were empty in the last full factorization of the same inputs
+  // (plan_epoch counts every change of geometry, tree, holes, border, ID
+  // method and rank model).  k_root verifies each skipped list; the host
+  // repeats the factorization without the plan on a miss.
+  uint64_t plan_epoch = 0;
+  bool plan_valid = false, plan_on = false;
+  int plan_mode = 1;        // iegpu_set_launch_plan
+  bool plan_used = false;   // the last factorization ran with the plan
+  int plan_misses = 0;      // factorizations repeated because a skipped list had boxes
+  struct PlanKey {
+    uint64_t epoch = 0;
+    double tol = 0;
+    int prec = -1, rid_mode = 0, rid_width = 0, rid_margin = 0, hole_dim = 0, levels = 0;
+    LaunchCaps caps;
+    bool operator==(const PlanKey& o) const {
+      return epoch == o.epoch && tol == o.tol && prec == o.prec && rid_mode == o.rid_mode &&
+             rid_width == o.rid_width && rid_margin == o.rid_margin &&
+             hole_dim == o.hole_dim && levels == o.levels &&
+             std::memcmp(&caps, &o.caps, sizeof(LaunchCaps)) == 0;
+    }
+  } plan_key;
+  std::vector<int> plan_skip;
+  DevBuf<int> d_plan_skip;
+
   int ndof() const { return n_points * sol_dim; }
   size_t elem() const { return precision == IEGPU_FP32 ? 4 : 8; }
 
@@ -338,6 +363,7 @@ struct iegpu_ctx {
     s.feat = record_features ? d_feat.p : nullptr;
     s.dirty = inc_run ? s_dirty.p : nullptr;
     s.kept_top = inc_run ? kept_top : 0;
+    s.plan_skip = plan_on ? d_plan_skip.p : nullptr;
     return s;
   }
 };
@@ -528,9 +554,10 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
   for (int lv = T.n_levels - 1; lv >= 1; lv--) {
     const int start = T.level_start[lv];
     const int count = T.level_start[lv + 1] - start;
+    const int skip = c->plan_on ? c->plan_skip[lv] : 0;
     for (int stage = 0; stage < kStageDecouple; stage++)
       c->prof.run(stage, lv, st, [&] {
-        launch_factor_stage<R>(stage, g, t, s, lv, start, count, tol, c->caps, st);
+        launch_factor_stage<R>(stage, g, t, s, lv, start, count, tol, c->caps, st, skip);
       });
     if (split) {
       HIP_TRY(hipEventRecord(c->evs[1 + lv], st));
@@ -727,6 +754,7 @@ int set_geometry(iegpu_ctx* c, int pde, int n_problems, const int* point_off,
     c->factored = false;
     c->update_pending = false;
     c->fmm_ready = false;
+    c->plan_epoch++;
     c->n_holes = 0;
     c->hole_dim = 0;
   } catch (const HipError& e) {
@@ -809,6 +837,7 @@ int iegpu_update_geometry(iegpu_ctx* c, int n_points, const double* points,
     c->factored = false;
     c->update_pending = false;
     c->fmm_ready = false;
+    c->plan_epoch++;
     c->tree = std::move(nt);
     if (keep) {
       std::vector<int> leaves;
@@ -884,6 +913,7 @@ int iegpu_update_info(const iegpu_ctx* c, int* n_dirty, int* last_incremental, i
 
 int iegpu_build_tree(iegpu_ctx* c, int leaf_cap) {
   if (!c) return IEGPU_ERR_INVALID;
+  c->plan_epoch++;
   if (c->n_points == 0) return fail(c, IEGPU_ERR_INVALID, "set_geometry first");
   if (leaf_cap < c->sol_dim || leaf_cap > kMaxBoxDofs / 2)
     return fail(c, IEGPU_ERR_INVALID, "leaf_cap out of range");
@@ -986,6 +1016,7 @@ std::vector<int> mark_update(const HostTree& T, const std::vector<int>& leaves) 
 
 int iegpu_set_holes(iegpu_ctx* c, int n_holes, const double* centers,
                     const double* radii) {
+  if (c) c->plan_epoch++;
   if (!c) return IEGPU_ERR_INVALID;
   if (c->n_points == 0) return fail(c, IEGPU_ERR_INVALID, "set_geometry first");
   if (n_holes < 0 || (n_holes > 0 && (!centers || !radii)))
@@ -1042,6 +1073,7 @@ void upload_border(iegpu_ctx* c) {
 
 int iegpu_set_id_method(iegpu_ctx* c, int method, int sketch_rows) {
   if (!c) return IEGPU_ERR_INVALID;
+  c->plan_epoch++;
   c->update_pending = false;  // the kept factors would mix two ID methods
   if (method == IEGPU_ID_QRCP) {
     c->rid_mode = 0;
@@ -1253,6 +1285,7 @@ int iegpu_rank_model_grad(iegpu_ctx* c, const float* model, const float* X, cons
 
 int iegpu_set_rank_model(iegpu_ctx* c, const float* model, int margin) {
   if (!c) return IEGPU_ERR_INVALID;
+  c->plan_epoch++;
   c->update_pending = false;  // the kept factors would mix two ID methods
   if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
   if (!model || margin < 0 || margin > 16) return fail(c, IEGPU_ERR_INVALID, "bad rank model");
@@ -1363,6 +1396,25 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
     c->inc_run = c->update_pending && c->last_tol == id_tol && c->last_prec == precision &&
                  c->kept_top <= 2 * c->full_top;
     c->update_pending = false;
+    iegpu_ctx::PlanKey key;
+    key.epoch = c->plan_epoch;
+    key.tol = id_tol;
+    key.prec = precision;
+    key.rid_mode = c->rid_mode;
+    key.rid_width = c->rid_width;
+    key.rid_margin = c->rid_margin;
+    key.hole_dim = c->hole_dim;
+    key.levels = c->tree.n_levels;
+    key.caps = c->caps;
+    const bool plan_ok = !c->inc_run && !c->use_graphs && c->plan_mode != 0;
+    c->plan_on = plan_ok && c->plan_valid && key == c->plan_key;
+    if (plan_ok && c->plan_mode == 2) {  // every list claimed empty (a test of the check)
+      c->plan_skip.assign(kMaxLevels, kSkipAssemble | kSkipRegQrcp | kSkipReg8Qrcp);
+      c->d_plan_skip.upload(c->plan_skip, c->stream);
+      c->plan_on = true;
+      c->plan_mode = 1;
+    }
+    c->plan_used = c->plan_on;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1416,6 +1468,13 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
       HIP_TRY(hipMemcpyAsync(h_err, c->s_err.p, sizeof(h_err), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
       errbits = h_err[0];
+      if (errbits & kErrPlan) {  // a skipped list had boxes: again, every kernel
+        c->plan_valid = false;
+        c->plan_on = false;
+        c->plan_used = false;
+        c->plan_misses++;
+        continue;
+      }
       const int grow = errbits & (kErrBoxCap | kErrIdPool | kErrRowPool | kErrFacPool | kErrDecPool);
       if (!grow) {
         if (c->hole_dim > 0) {
@@ -1468,6 +1527,26 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
         return fail(c, IEGPU_ERR_CAPACITY, "workspace growth did not converge");
       }
     }
+    if (plan_ok && !c->plan_on && !(errbits & kErrSingular)) {
+      // Record the plan from this factorization's per-level list counts.
+      std::vector<int> lc((size_t)2 * kMaxLevels * kNumLists);
+      HIP_TRY(hipMemcpy(lc.data(), c->s_lcnt.p, sizeof(int) * lc.size(), hipMemcpyDeviceToHost));
+      c->plan_skip.assign(kMaxLevels, 0);
+      for (int lv = 1; lv < c->tree.n_levels; lv++) {
+        auto cnt = [&](int L) { return lc[(size_t)2 * (lv * kNumLists + L)]; };
+        const int reg = c->rid_mode != 0 ? cnt(kLstFallback) : cnt(kLstReg) + cnt(kLstRegHalf);
+        int m = 0;
+        if (cnt(kLstLds) == 0) m |= kSkipAssemble;
+        if (reg == 0) m |= kSkipRegQrcp;
+        if (cnt(kLstReg8) == 0) m |= kSkipReg8Qrcp;
+        c->plan_skip[lv] = m;
+      }
+      c->d_plan_skip.upload(c->plan_skip, c->stream);
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      c->plan_key = key;
+      c->plan_valid = true;
+    }
+    c->plan_on = false;
     c->last_incremental = c->inc_run;
     c->inc_run = false;
     float ms = 0;
@@ -1607,6 +1686,7 @@ int iegpu_solve_nrhs(iegpu_ctx* c, int nrhs, const double* b, double* mu, double
 
 int iegpu_set_border(iegpu_ctx* c, int hole_dim, const double* U, const double* Psi) {
   if (!c) return IEGPU_ERR_INVALID;
+  c->plan_epoch++;
   if (c->device < 0) return fail(c, IEGPU_ERR_NO_DEVICE, "host-only context");
   if (c->n_holes == 0) return fail(c, IEGPU_ERR_INVALID, "set_holes first");
   if (hole_dim != c->hole_dim || !U || !Psi)
@@ -1867,6 +1947,24 @@ void enqueue_matvec(iegpu_ctx* c, const double* x, double* y, hipStream_t st) {
   }
 }
 }  // namespace
+
+int iegpu_set_launch_plan(iegpu_ctx* c, int mode) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (mode < 0 || mode > 2) return fail(c, IEGPU_ERR_INVALID, "mode must be 0, 1 or 2");
+  c->plan_mode = mode;
+  if (mode == 0) c->plan_valid = false;
+  return IEGPU_OK;
+}
+
+int iegpu_launch_plan_info(const iegpu_ctx* c, int* info, int n) {
+  if (!c || (n > 0 && !info)) return IEGPU_ERR_INVALID;
+  int skipped = 0;
+  if (c->plan_valid)
+    for (int m : c->plan_skip) skipped += __builtin_popcount(m);
+  const int v[3] = {c->plan_used ? 1 : 0, c->plan_misses, skipped};
+  for (int i = 0; i < n && i < 3; i++) info[i] = v[i];
+  return IEGPU_OK;
+}
 
 int iegpu_set_matvec(iegpu_ctx* c, int method, int order, int leaf_points) {
   if (!c) return IEGPU_ERR_INVALID;

@@ -24,6 +24,16 @@ enum : int {
   kErrSingular = 8,    // exact zero pivot in an LU
   kErrRowPool = 16,    // row-index pool exhausted
   kErrDecPool = 32,    // decouple operand pool exhausted
+  kErrPlan = 64,       // a list whose kernels the launch plan skipped was not empty
+};
+
+// Launch plan (DevState::plan_skip, per level): the ID-stage kernels not
+// launched because their lists were empty in the last factorization of the
+// same inputs.  k_root checks every skipped list and raises kErrPlan.
+enum : int {
+  kSkipAssemble = 1,   // k_assemble, k_qrcp (general QRCP list)
+  kSkipRegQrcp = 2,    // k_rowcomp<4>, k_qrcp_id n <= 32 and n <= 64 shapes
+  kSkipReg8Qrcp = 4,   // k_rowcomp<8>, k_qrcp_id<8, 4, 16>
 };
 
 // How a box's interpolative decomposition runs.
@@ -199,6 +209,8 @@ struct DevState {
   // re-factored box keeps its slot when that slot lies below this mark and
   // its new factors fit it.
   int64_t kept_top;
+  // Per level: kSkip* bits of the launch plan in use (null: every kernel).
+  const int* plan_skip;
 };
 
 // False once a pool overflow was reported: the level's remaining kernels
@@ -224,6 +236,20 @@ __device__ inline int row_jobs(const DevState& s, int b) {
 
 __device__ inline int list_count(const DevState& s, int level, int L) {
   return list_slot(s, level, L)[0];
+}
+
+// The kSkip* bits a level could have taken: the kernels whose lists are
+// empty (in the randomized form the register shapes take only the fallback
+// list).
+__device__ inline int plan_skippable(const DevState& s, int level) {
+  const int reg = s.rid_mode != 0 ? list_count(s, level, kLstFallback)
+                                  : list_count(s, level, kLstReg) +
+                                        list_count(s, level, kLstRegHalf);
+  int m = 0;
+  if (list_count(s, level, kLstLds) == 0) m |= kSkipAssemble;
+  if (reg == 0) m |= kSkipRegQrcp;
+  if (list_count(s, level, kLstReg8) == 0) m |= kSkipReg8Qrcp;
+  return m;
 }
 __device__ inline int list_at(const DevState& s, int start, int L, int idx) {
   return s.lists[(int64_t)L * s.nbox + start + idx];

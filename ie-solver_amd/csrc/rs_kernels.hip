@@ -2358,6 +2358,15 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
     s.m_rows[b] = 0;
     s.out_cnt[b] = s.act_cnt[b];
   }
+  if (b == 0 && tid == 0 && s.plan_skip != nullptr) {
+    // The launch plan skipped kernels whose lists it expected empty: any
+    // that is not makes the host run the factorization again in full.
+    for (int lv = 1; lv < t.n_levels; lv++)
+      if (s.plan_skip[lv] & ~plan_skippable(s, lv)) {
+        atomicOr(s.err, kErrPlan);
+        break;
+      }
+  }
   if (!pools_ok(s)) return;
   const int n = s.act_cnt[b];
   const bool leaf = t.leaf[b] != 0;
@@ -2958,7 +2967,7 @@ void launch_rid_wide(const DevGeom& g, const DevTree& t, const DevState& s, int 
 template <typename R>
 void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
                          const DevState& s, int level, int start, int count,
-                         double tol, const LaunchCaps& caps, hipStream_t st) {
+                         double tol, const LaunchCaps& caps, hipStream_t st, int skip) {
   if (count <= 0) return;
   switch (stage) {
     case kStageActive:
@@ -2971,7 +2980,8 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       k_scan<<<1, 1024, 0, st>>>(s, start, count, level);
       break;
     case kStageAssemble:
-      k_assemble<R><<<min(count, caps.assemble), 256, 0, st>>>(g, t, s, start, level);
+      if (!(skip & kSkipAssemble))
+        k_assemble<R><<<min(count, caps.assemble), 256, 0, st>>>(g, t, s, start, level);
       break;
     case kStageRid:
       if (s.rid_mode != 0) {
@@ -2996,7 +3006,9 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       // the level's box count) runs one box per workgroup, without the
       // grid-stride loop and the registers it keeps live (no spills).
       const int jobs_max = count * kMaxChunks;
-      if (s.rid_mode != 0) {
+      if (skip & kSkipRegQrcp) {
+        // no register-QRCP box on this level (launch plan)
+      } else if (s.rid_mode != 0) {
         // Only the boxes without a successful sketch reach the full QRCP.
         k_rowcomp<R, 4><<<min(jobs_max, caps.rowcomp4), kXThreads, 0, st>>>(g, t, s, start,
                                                                             level);
@@ -3015,15 +3027,18 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
         k_qrcp_id<R, 4, 8, 8, false><<<count, 512, 0, st>>>(g, t, s, start, level, tol);
         k_qrcp_id<R, 4, 8, 16, false><<<count, kXThreads, 0, st>>>(g, t, s, start, level, tol);
       }
-      k_rowcomp<R, 8><<<min(2 * count, caps.rowcomp8), kXThreads, 0, st>>>(g, t, s, start,
-                                                                          level);
-      if (count <= caps.qrcp_id8)
-        k_qrcp_id<R, 8, 4, 16, true, false><<<count, kXThreads, 0, st>>>(g, t, s, start, level,
-                                                                         tol);
-      else
-        k_qrcp_id<R, 8, 4, 16, true><<<caps.qrcp_id8, kXThreads, 0, st>>>(g, t, s, start, level,
-                                                                          tol);
-      k_qrcp<R, 512><<<min(count, caps.qrcp), 512, 0, st>>>(s, start, level, tol);
+      if (!(skip & kSkipReg8Qrcp)) {
+        k_rowcomp<R, 8><<<min(2 * count, caps.rowcomp8), kXThreads, 0, st>>>(g, t, s, start,
+                                                                            level);
+        if (count <= caps.qrcp_id8)
+          k_qrcp_id<R, 8, 4, 16, true, false><<<count, kXThreads, 0, st>>>(g, t, s, start,
+                                                                           level, tol);
+        else
+          k_qrcp_id<R, 8, 4, 16, true><<<caps.qrcp_id8, kXThreads, 0, st>>>(g, t, s, start,
+                                                                            level, tol);
+      }
+      if (!(skip & kSkipAssemble))
+        k_qrcp<R, 512><<<min(count, caps.qrcp), 512, 0, st>>>(s, start, level, tol);
       break;
     }
     case kStageDecouple:
@@ -3093,7 +3108,7 @@ void launch_fwd_down(const DevState& s, int start, int count, double* x, hipStre
 #define IEGPU_INSTANTIATE(R)                                                   \
   template void launch_factor_stage<R>(int, const DevGeom&, const DevTree&,   \
                                        const DevState&, int, int, int, double, \
-                                       const LaunchCaps&, hipStream_t);        \
+                                       const LaunchCaps&, hipStream_t, int);   \
   template void launch_root<R>(const DevGeom&, const DevTree&,                \
                                const DevState&, int, bool, hipStream_t);       \
   template void launch_sweep_up<R>(const DevState&, int, int, SolveVec, bool, \

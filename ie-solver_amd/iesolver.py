@@ -52,7 +52,7 @@ EXPORTS = [
     "iegpu_factor_solve_device", "iegpu_update_geometry", "iegpu_update_info",
     "iegpu_update_geometry_batch", "iegpu_discard_update", "iegpu_evaluate_noalloc",
     "iegpu_solve_nrhs", "iegpu_set_border", "iegpu_box_factors", "iegpu_set_matvec",
-    "iegpu_matvec_info",
+    "iegpu_matvec_info", "iegpu_set_launch_plan", "iegpu_launch_plan_info",
 ]
 NUM_LAUNCH_CAPS = 8
 RANK_FEATURES = 9
@@ -104,6 +104,8 @@ def load():
     L.iegpu_apply.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.iegpu_set_matvec.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.iegpu_matvec_info.argtypes = [vp, _ip, ctypes.c_int]
+    L.iegpu_set_launch_plan.argtypes = [vp, ctypes.c_int]
+    L.iegpu_launch_plan_info.argtypes = [vp, _ip, ctypes.c_int]
     L.iegpu_set_profiling.argtypes = [vp, ctypes.c_int]
     L.iegpu_profile.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _ip]
     L.iegpu_set_id_method.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -416,6 +418,20 @@ class RSSolver:
         about `leaf_points` nodes per leaf; 0 = defaults 48 and 64) or
         MATVEC_AUTO (FMM from FMM_AUTO_POINTS nodes of one boundary)."""
         self._check(self.L.iegpu_set_matvec(self.h, method, order, leaf_points))
+
+    def set_launch_plan(self, mode=1):
+        """1: skip the ID-stage kernels whose lists were empty in the last
+        full factorization of the same inputs (checked on the device); 0:
+        launch every kernel; 2: take every list as empty once (the check
+        repeats the factorization in full)."""
+        self._check(self.L.iegpu_set_launch_plan(self.h, mode))
+
+    def launch_plan_info(self):
+        """(last factorization used the plan, repeats after a miss,
+        skipped kernel groups in the plan)."""
+        v = np.zeros(3, dtype=np.int32)
+        self._check(self.L.iegpu_launch_plan_info(self.h, _pi(v), 3))
+        return tuple(int(t) for t in v)
 
     def matvec_info(self):
         """(form used by the last apply / solve_refined, FMM leaf level,

@@ -320,6 +320,27 @@ int iegpu_sparse_matvec(iegpu_ctx* ctx, const double* x, double* b);
  * skel_factorization.cpp:150-168). */
 int iegpu_set_overlap(iegpu_ctx* ctx, int on);
 
+/* Launch plan of the factorization.  After a full factorization the
+ * context records, per tree level, which ID-stage kernels (general-QRCP
+ * assembly and QRCP, register QRCP with its row compression, the n <= 128
+ * register QRCP with its row compression) had empty work lists; the next
+ * factorization with the same inputs (geometry, tree, holes, border,
+ * tolerance, precision, ID method, rank model, launch caps) does not
+ * launch those.  The root kernel checks every skipped list on the device;
+ * if one is not empty the factorization is repeated with every kernel (and
+ * the plan recorded again), so the result never depends on the plan.
+ * mode 1 (default): on; 0: off (every kernel on every factorization); 2:
+ * the next factorization takes every list as empty (a test of the check;
+ * it repeats in full), then back to 1.  Not used with captured graphs or
+ * incremental re-factorizations.  No reference counterpart. */
+int iegpu_set_launch_plan(iegpu_ctx* ctx, int mode);
+
+/* info[0..2] = whether the last factorization ran with a launch plan (1)
+ * or launched every kernel (0), the number of factorizations repeated
+ * because a skipped list had work, and the number of skipped kernel
+ * groups in the current plan (0 without one). */
+int iegpu_launch_plan_info(const iegpu_ctx* ctx, int* info, int n);
+
 /* Launch shapes of the factorization, caps[IEGPU_NUM_LAUNCH_CAPS]:
  * [0..5] workgroup caps of the kernels that walk a device-side box list
  * (the host knows a level's box count, not the list's length): the
