@@ -1409,7 +1409,8 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
     const bool plan_ok = !c->inc_run && !c->use_graphs && c->plan_mode != 0;
     c->plan_on = plan_ok && c->plan_valid && key == c->plan_key;
     if (plan_ok && c->plan_mode == 2) {  // every list claimed empty (a test of the check)
-      c->plan_skip.assign(kMaxLevels, kSkipAssemble | kSkipRegQrcp | kSkipReg8Qrcp);
+      c->plan_skip.assign(kMaxLevels, kSkipAssemble | kSkipRegQrcp | kSkipReg8Qrcp | kSkipRid |
+                                          kSkipRidWide);
       c->d_plan_skip.upload(c->plan_skip, c->stream);
       c->plan_on = true;
       c->plan_mode = 1;
@@ -1535,10 +1536,13 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
       for (int lv = 1; lv < c->tree.n_levels; lv++) {
         auto cnt = [&](int L) { return lc[(size_t)2 * (lv * kNumLists + L)]; };
         const int reg = c->rid_mode != 0 ? cnt(kLstFallback) : cnt(kLstReg) + cnt(kLstRegHalf);
+        const int wide = lc[(size_t)2 * (lv * kNumLists + kLstRid) + 1];
         int m = 0;
         if (cnt(kLstLds) == 0) m |= kSkipAssemble;
         if (reg == 0) m |= kSkipRegQrcp;
         if (cnt(kLstReg8) == 0) m |= kSkipReg8Qrcp;
+        if (cnt(kLstRid) == wide) m |= kSkipRid;
+        if (wide == 0) m |= kSkipRidWide;
         c->plan_skip[lv] = m;
       }
       c->d_plan_skip.upload(c->plan_skip, c->stream);

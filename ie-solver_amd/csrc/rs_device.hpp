@@ -34,6 +34,8 @@ enum : int {
   kSkipAssemble = 1,   // k_assemble, k_qrcp (general QRCP list)
   kSkipRegQrcp = 2,    // k_rowcomp<4>, k_qrcp_id n <= 32 and n <= 64 shapes
   kSkipReg8Qrcp = 4,   // k_rowcomp<8>, k_qrcp_id<8, 4, 16>
+  kSkipRid = 8,        // k_rid (no 16- or 32-row sketch on the level)
+  kSkipRidWide = 16,   // k_rid_wide (no 64-row sketch on the level)
 };
 
 // How a box's interpolative decomposition runs.
@@ -245,10 +247,14 @@ __device__ inline int plan_skippable(const DevState& s, int level) {
   const int reg = s.rid_mode != 0 ? list_count(s, level, kLstFallback)
                                   : list_count(s, level, kLstReg) +
                                         list_count(s, level, kLstRegHalf);
+  // kLstRid's second slot int counts the level's 64-row sketches.
+  const int wide = list_slot(s, level, kLstRid)[1];
   int m = 0;
   if (list_count(s, level, kLstLds) == 0) m |= kSkipAssemble;
   if (reg == 0) m |= kSkipRegQrcp;
   if (list_count(s, level, kLstReg8) == 0) m |= kSkipReg8Qrcp;
+  if (list_count(s, level, kLstRid) == wide) m |= kSkipRid;
+  if (wide == 0) m |= kSkipRidWide;
   return m;
 }
 __device__ inline int list_at(const DevState& s, int start, int L, int idx) {

@@ -288,13 +288,13 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
                                                int count, int level) {
   // Scanned per box: ID scratch, row-list and factor sizes, register-QRCP
   // and general-QRCP list entries, row-compression jobs.
-  constexpr int NQ = 11;
+  constexpr int NQ = 12;
   __shared__ int64_t wtot[NQ][16];  // per-wave totals, then their exclusive scan
   __shared__ int64_t tot[NQ];
   const int tid = threadIdx.x;
   const int chunk = (count + 1023) / 1024;
   const int lo = min(count, tid * chunk), hi = min(count, lo + chunk);
-  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = lo; i < hi; i++) {
     const int b = start + i;
     if (box_clean(s, b)) continue;
@@ -310,6 +310,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     own[8] += (s.sk_w[b] > 0) ? 1 : 0;
     own[9] += rid_full_qrcp(s, b, kind) ? 1 : 0;
     own[10] += s.dec_sz[b];
+    own[11] += (s.sk_w[b] == kRidRowsW) ? 1 : 0;
   }
   // Exclusive scan over the 1024 threads: inclusive scans within each wave
   // (shuffles), then a scan of the 16 wave totals.
@@ -391,6 +392,7 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     list_slot(s, level, kLstReg8)[0] = (int)tot[6];
     list_slot(s, level, kLstRegHalf)[0] = (int)tot[7];
     list_slot(s, level, kLstRid)[0] = (int)tot[8];
+    list_slot(s, level, kLstRid)[1] = (int)tot[11];  // of which 64-row sketches
     list_slot(s, level, kLstFallback)[0] = (int)tot[9];  // k_rid appends
     s.njobs[level] = (int)tot[5];
     int e = 0;
@@ -2985,13 +2987,16 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       break;
     case kStageRid:
       if (s.rid_mode != 0) {
-        if (g.pde == kPdeStokes)
+        if (skip & kSkipRid) {
+          // no 16- or 32-row sketch on this level (launch plan)
+        } else if (g.pde == kPdeStokes) {
           k_rid<R, kPdeStokes><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
-        else if (g.pde == kPdeLaplaceNeumann)
+        } else if (g.pde == kPdeLaplaceNeumann) {
           k_rid<R, kPdeLaplaceNeumann><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
-        else
+        } else {
           k_rid<R, kPdeLaplace><<<count, 256, 0, st>>>(g, t, s, start, level, tol);
-        if (level <= kRidWideTop) {
+        }
+        if (level <= kRidWideTop && !(skip & kSkipRidWide)) {
           if (g.pde == kPdeStokes)
             launch_rid_wide<R, kPdeStokes>(g, t, s, start, level, count, tol, st);
           else if (g.pde == kPdeLaplaceNeumann)

@@ -323,7 +323,8 @@ int iegpu_set_overlap(iegpu_ctx* ctx, int on);
 /* Launch plan of the factorization.  After a full factorization the
  * context records, per tree level, which ID-stage kernels (general-QRCP
  * assembly and QRCP, register QRCP with its row compression, the n <= 128
- * register QRCP with its row compression) had empty work lists; the next
+ * register QRCP with its row compression, the 32-row and 64-row sketches)
+ * had empty work lists; the next
  * factorization with the same inputs (geometry, tree, holes, border,
  * tolerance, precision, ID method, rank model, launch caps) does not
  * launch those.  The root kernel checks every skipped list on the device;
