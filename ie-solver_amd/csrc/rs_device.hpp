@@ -81,7 +81,7 @@ constexpr int kRidTableRows = 2 * kRidMaxNear + 4 * kNumProxy;  // Omega rows
 // kRidWideTop): a sketch of kRidRowsW rows, up to kRidWideCols columns and
 // kRidMaxNearW row DOFs, with its own Gaussian table.
 constexpr int kRidRowsW = 64;
-constexpr int kRidWideCols = 64;
+constexpr int kRidWideCols = 128;
 constexpr int kRidMaxNearW = 512;
 constexpr int kRidWideTop = 4;
 constexpr int kRidTableRowsW = 2 * kRidMaxNearW + 4 * kNumProxy;

@@ -1346,12 +1346,15 @@ __global__ __launch_bounds__(256, 3) void k_rid(DevGeom g, DevTree t, DevState s
 }
 
 // Wide randomized ID: the boxes of the top levels too large for k_rid (up
-// to kRidMaxNearW row DOFs, ranks up to kRidRowsW - kRidOver; columns up
-// to kRidWideCols = 64).  Same sketch, rank rule and outputs as k_rid with
-// a sketch of kRidRowsW rows from its own table, on 8 waves: wave w
-// sketches the 16-column tile w % 4 over every other 16-row block (part
-// w / 4; the two parts added in a fixed order); wave 0 then runs the
-// pivoted QR, one column per lane.  A box whose sketch shows no cut keeps
+// to kRidMaxNearW row DOFs and kRidWideCols columns, ranks up to kRidRowsW
+// - kRidOver).  Same sketch, rank rule and outputs as k_rid with a sketch
+// of kRidRowsW rows from its own table, on 8 waves: wave w sketches the
+// 16-column tile w % (4 QW) over every (8 / 4 QW)-th 16-row block, the
+// row parts added in a fixed order.  QW waves then run the pivoted QR,
+// wave q holding columns 64 q + lane: QW = 1 for n <= 64 (one wave, no
+// barrier), QW = 2 for n <= 128 (each step's pivot found from both waves'
+// maxima through LDS, the pivot column's Householder vector published by
+// its wave; two barriers per step).  A box whose sketch shows no cut keeps
 // rid_done = 0: a register-QRCP box joins the fallback list, any other is
 // still on its full-QRCP list (whose kernels skip the boxes the sketch
 // took).
@@ -1409,30 +1412,40 @@ __device__ __forceinline__ void rid_segment_w(const double (*tab)[kRidMaxNearW +
 
 constexpr int kRidWideWaves = 8;
 
-template <typename R, int PDE>
-__global__ __launch_bounds__(64 * kRidWideWaves, 1) void k_rid_wide(
-    DevGeom g, DevTree t, DevState s, int start, int level, double tol_d) {
-  constexpr int NC = 1;  // columns per lane in the QR
-  constexpr int kTiles = 4, kParts = kRidWideWaves / kTiles;
+// LDS of the wide sketch: the row table, then Y, then the rows of R (a
+// raw buffer sized for two QR waves), and the QR's small arrays.
+template <typename R>
+struct RidWideLds {
+  static constexpr int TR = kRidMaxNearW + kNumProxy;
+  static constexpr int YMAX = kRidRowsW * (2 * 64 + 1);  // two QR waves' Y
+  static constexpr int kBuf = (YMAX * (int)sizeof(R) + 7) / 8 > 6 * TR
+                                  ? (YMAX * (int)sizeof(R) + 7) / 8
+                                  : 6 * TR;
+  double buf[kBuf];
+  int rowdofs[kRidMaxNearW];
+  R vbuf[kRidRowsW];
+  int spiv[64];
+  double kbest[2];  // two QR waves: each wave's pivot key
+  R hs[2];          // two QR waves: beta and tau of the step
+  int nred0;        // two QR waves: redundant columns of wave 0
+};
+
+template <typename R, int PDE, int QW>
+__device__ __forceinline__ void rid_wide_box(const DevGeom& g, const DevTree& t,
+                                             const DevState& s, int start, int level,
+                                             double tol_d, int b, RidWideLds<R>& sh) {
+  constexpr int kTiles = 4 * QW, kParts = kRidWideWaves / kTiles;
   constexpr int NT = 64 * kRidWideWaves;
-  constexpr int LM = kRidRowsW, YLD = 64 * NC + 1, TR = kRidMaxNearW + kNumProxy;
-  constexpr int kBuf = (LM * YLD * (int)sizeof(R) + 7) / 8 > 6 * TR
-                           ? (LM * YLD * (int)sizeof(R) + 7) / 8
-                           : 6 * TR;
+  constexpr int LM = kRidRowsW, YLD = 64 * QW + 1, TR = RidWideLds<R>::TR;
   using V4 = typename Mfma16<R>::V4;
-  // Row geometry during the sketch, then Y (row-major, ld YLD), then the
-  // rows of R.
-  __shared__ double buf[kBuf];
-  __shared__ int rowdofs[kRidMaxNearW];
-  __shared__ R vbuf[LM];
-  __shared__ int spiv[64];
-  double (*tab)[TR] = reinterpret_cast<double (*)[TR]>(buf);
-  R* ybuf = reinterpret_cast<R*>(buf);
-  if (!pools_ok(s)) return;
-  if ((int)blockIdx.x >= list_count(s, level, kLstRid)) return;
-  const int b = list_at(s, start, kLstRid, blockIdx.x);
+  double (*tab)[TR] = reinterpret_cast<double (*)[TR]>(sh.buf);
+  R* ybuf = reinterpret_cast<R*>(sh.buf);
+  int* rowdofs = sh.rowdofs;
+  R* vbuf = sh.vbuf;
+  int* spiv = sh.spiv;
+  double* kbest = sh.kbest;
+  R* hs = sh.hs;
   const int n = s.act_cnt[b], m = s.m_rows[b], l = s.sk_w[b];
-  if (l != kRidRowsW) return;  // a narrow sketch (k_rid)
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int P = proxy_rows(g, level) / 2;
   const int cnt = (m - 2 * P) / 2;
@@ -1495,89 +1508,94 @@ __global__ __launch_bounds__(64 * kRidWideWaves, 1) void k_rid_wide(
     }
     __syncthreads();
   }
-  if (wv != 0) return;
+  if (wv >= QW) return;
 
-  // Pivoted QR of the sketch (wave 0): lane c holds columns c + 64 h.
-  R y[NC][LM];
+  // Pivoted QR of the sketch: wave wv holds column col = 64 wv + lane.
+  const int col = 64 * wv + lane;
+  R y[LM];
 #pragma unroll
-  for (int h = 0; h < NC; h++)
-#pragma unroll
-    for (int r = 0; r < LM; r++) y[h][r] = ybuf[r * YLD + 64 * h + lane];
+  for (int r = 0; r < LM; r++) y[r] = ybuf[r * YLD + col];
   R* rr = ybuf;  // rows of R, rr[j * YLD + c]
   const R tol = static_cast<R>(tol_d);
-  bool av[NC];
-  R tl[NC];
-#pragma unroll
-  for (int h = 0; h < NC; h++) {
-    av[h] = 64 * h + lane < n;
+  bool av = col < n;
+  R tl;
+  {
     R p4[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int r = 1; r < LM; r++) p4[r & 3] += y[h][r] * y[h][r];
-    tl[h] = (p4[0] + p4[1]) + (p4[2] + p4[3]);
+    for (int r = 1; r < LM; r++) p4[r & 3] += y[r] * y[r];
+    tl = (p4[0] + p4[1]) + (p4[2] + p4[3]);
   }
   int k = -1;
   R thresh = 0;
   const int jlim = min(n, l - kRidOver);
   int j = 0;
   for (; j < jlim; j++) {
-    double key = -1.0;
-#pragma unroll
-    for (int h = 0; h < NC; h++)
-      if (av[h]) key = fmax(key, pivot_key<R, 7>(y[h][0] * y[h][0] + tl[h], 64 * h + lane));
-    const double best = wave_max(key);
+    double best = wave_max(av ? pivot_key<R, 7>(y[0] * y[0] + tl, col) : -1.0);
+    if (QW == 2) {
+      if (lane == 0) kbest[wv] = best;
+      __syncthreads();
+      best = fmax(kbest[0], kbest[QW - 1]);
+    }
     const int p = pivot_key_pos<7>(best);  // column index
-    const int ph = p >> 6, pl = p & 63;
-    R alpha = 0, xt = 0;
+    const int pw = p >> 6, pl = p & 63;
+    R beta = 0, tau = 0, sc = 0;
+    if (wv == pw) {
+      const R alpha = lane_value(y[0], pl), xt = lane_value(tl, pl);
+      householder_scalars(alpha, xt, &beta, &tau, &sc);
+    }
+    // The pivot column's Householder vector: with two waves published
+    // before the cut test (both waves test the owner's beta), with one
+    // after it.
+    auto publish = [&]() {
+      if (wv == pw && lane == pl) {
+        vbuf[0] = R(1);
 #pragma unroll
-    for (int h = 0; h < NC; h++)
-      if (ph == h) {
-        alpha = lane_value(y[h][0], pl);
-        xt = lane_value(tl[h], pl);
+        for (int r = 1; r < LM; r++) vbuf[r] = y[r] * sc;
+        spiv[j] = p;
+        if (QW == 2) {
+          hs[0] = beta;
+          hs[1] = tau;
+        }
       }
-    R beta, tau, sc;
-    householder_scalars(alpha, xt, &beta, &tau, &sc);
+    };
+    if (QW == 2) {
+      publish();
+      __syncthreads();
+      beta = hs[0];
+      tau = hs[1];
+    }
     if (j == 0) {
       thresh = fabs(tol * beta);
     } else if (2 * fabs(beta) * sqrt(static_cast<R>(l) / static_cast<R>(l - j)) < thresh) {
       k = j;
       break;
     }
-    if (lane == pl) {
-      vbuf[0] = R(1);
-#pragma unroll
-      for (int h = 0; h < NC; h++)
-        if (ph == h) {
-#pragma unroll
-          for (int r = 1; r < LM; r++) vbuf[r] = y[h][r] * sc;
-          av[h] = false;
-        }
-      spiv[j] = p;
+    if (QW == 1) {
+      publish();
+      wave_sync();
     }
-    wave_sync();
+    if (col == p) av = false;
+    R d4[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int h = 0; h < NC; h++) {
-      R d4[4] = {0, 0, 0, 0};
+    for (int r = 0; r < LM; r++) d4[r & 3] += vbuf[r] * y[r];
+    const R w = tau * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
+    rr[j * YLD + col] = (col == p) ? beta : y[0] - w;
+    // vbuf is read again below rather than kept in registers beside y
+    __asm__ volatile("" ::: "memory");
+    R t4[4] = {0, 0, 0, 0};
 #pragma unroll
-      for (int r = 0; r < LM; r++) d4[r & 3] += vbuf[r] * y[h][r];
-      const R w = tau * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
-      rr[j * YLD + 64 * h + lane] = (p == 64 * h + lane) ? beta : y[h][0] - w;
-      // vbuf is read again below rather than kept in registers beside y
-      __asm__ volatile("" ::: "memory");
-      R t4[4] = {0, 0, 0, 0};
-#pragma unroll
-      for (int r = 0; r < LM - 1; r++) {
-        y[h][r] = y[h][r + 1] - w * vbuf[r + 1];
-        if (r > 0) t4[r & 3] += y[h][r] * y[h][r];
-      }
-      y[h][LM - 1] = 0;
-      tl[h] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+    for (int r = 0; r < LM - 1; r++) {
+      y[r] = y[r + 1] - w * vbuf[r + 1];
+      if (r > 0) t4[r & 3] += y[r] * y[r];
     }
-    wave_sync();
+    y[LM - 1] = 0;
+    tl = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+    if (QW == 1) wave_sync();
   }
   const bool regq = s.qkind[b] == kQrcpReg || s.qkind[b] == kQrcpStack;
   if (k < 0) {
     if (j < n) {  // no cut within the sketch: the full QRCP takes the box
-      if (regq && lane == 0) {
+      if (regq && tid == 0) {
         const int pos = atomicAdd(list_slot(s, level, kLstFallback), 1);
         s.lists[(int64_t)kLstFallback * s.nbox + start + pos] = b;
       }
@@ -1585,54 +1603,64 @@ __global__ __launch_bounds__(64 * kRidWideWaves, 1) void k_rid_wide(
     }
     k = n;
   }
+  // Every wave's rows of R are in LDS; wave 0's redundant count for wave 1.
+  const uint64_t mk = __ballot(av);
+  if (QW == 2) {
+    if (wv == 0 && lane == 0) sh.nred0 = __popcll(mk);
+    __syncthreads();
+  }
 
-  // T = R11^{-1} R12 by a column sweep in registers, one column of the lane
-  // (lane + 64 h) at a time, R11 read from LDS.
+  // T = R11^{-1} R12 by a column sweep in registers, R11 read from LDS.
   constexpr int KMAX = LM - kRidOver;
-  uint64_t mk[NC];
+  R x[KMAX];
 #pragma unroll
-  for (int h = 0; h < NC; h++) mk[h] = __ballot(av[h]);
-  const uint64_t below = (1ull << lane) - 1;
+  for (int i = 0; i < KMAX; i++) x[i] = i < k ? rr[i * YLD + col] : R(0);
+#pragma unroll
+  for (int q = KMAX - 1; q >= 0; q--) {
+    if (q < k) {
+      const int pq = spiv[q];
+      const R tq = x[q] * recip_any(rr[q * YLD + pq]);
+      x[q] = tq;
+#pragma unroll
+      for (int i = 0; i < q; i++) x[i] -= rr[i * YLD + pq] * tq;
+    }
+  }
+  const int cp = (wv == 1 ? sh.nred0 : 0) + __popcll(mk & ((1ull << lane) - 1));  // redundant index
   R* F = reinterpret_cast<R*>(s.fac_pool) + s.fac_off[b];
   int* out = s.out_list + (int64_t)b * s.ncap;
   int* rl = s.red_list + (int64_t)b * s.ncap;
   int* pm = s.perm + (int64_t)b * s.ncap;
-  int before = 0;
+  if (av) {
 #pragma unroll
-  for (int h = 0; h < NC; h++) {
-    R x[KMAX];
-#pragma unroll
-    for (int i = 0; i < KMAX; i++) x[i] = i < k ? rr[i * YLD + 64 * h + lane] : R(0);
-#pragma unroll
-    for (int q = KMAX - 1; q >= 0; q--) {
-      if (q < k) {
-        const int pq = spiv[q];
-        const R tq = x[q] * recip_any(rr[q * YLD + pq]);
-        x[q] = tq;
-#pragma unroll
-        for (int i = 0; i < q; i++) x[i] -= rr[i * YLD + pq] * tq;
-      }
-    }
-    const int cp = before + __popcll(mk[h] & below);  // redundant index
-    if (av[h]) {
-#pragma unroll
-      for (int i = 0; i < KMAX; i++)
-        if (i < k) F[i + (int64_t)k * cp] = x[i];
-      pm[k + cp] = 64 * h + lane;
-      rl[cp] = act[64 * h + lane];
-    }
-    before += __popcll(mk[h]);
+    for (int i = 0; i < KMAX; i++)
+      if (i < k) F[i + (int64_t)k * cp] = x[i];
+    pm[k + cp] = col;
+    rl[cp] = act[col];
   }
-  if (lane < k) {
+  if (wv == 0 && lane < k) {
     const int cc = spiv[lane];
     pm[lane] = cc;
     out[lane] = act[cc];
   }
-  if (lane == 0) {
+  if (tid == 0) {
     s.rank[b] = k;
     s.out_cnt[b] = k;
     s.rid_done[b] = 1;
   }
+}
+
+// One launch for both forms: boxes with n <= 64 run the one-wave QR,
+// the others the two-wave QR.
+template <typename R, int PDE>
+__global__ __launch_bounds__(64 * kRidWideWaves, 1) void k_rid_wide(
+    DevGeom g, DevTree t, DevState s, int start, int level, double tol_d) {
+  __shared__ RidWideLds<R> sh;
+  if (!pools_ok(s)) return;
+  if ((int)blockIdx.x >= list_count(s, level, kLstRid)) return;
+  const int b = list_at(s, start, kLstRid, blockIdx.x);
+  if (s.sk_w[b] != kRidRowsW) return;  // a narrow sketch (k_rid)
+  if (s.act_cnt[b] > 64) rid_wide_box<R, PDE, 2>(g, t, s, start, level, tol_d, b, sh);
+  else rid_wide_box<R, PDE, 1>(g, t, s, start, level, tol_d, b, sh);
 }
 
 // Gaussian table for the sketches: z = sqrt(-2 ln u1) cos(2 pi u2) (Box and

@@ -185,10 +185,10 @@ def test_rid_ranks_match_oracle_restatement(pde):
 
 
 def _rid_wide_eligible(m, n, level, pde):
-    # The wide sketch: top levels (<= 4), n <= 64, at most 512 near-field
+    # The wide sketch: top levels (<= 4), n <= 128, at most 512 near-field
     # row DOFs, boxes the narrow sketch does not take.
     P = (512 if pde == iesolver.PDE_STOKES else 256) if level >= 3 else 0
-    return (m > 0 and level <= 4 and n <= 64 and (m - P) // 2 <= 512
+    return (m > 0 and level <= 4 and n <= 128 and (m - P) // 2 <= 512
             and not _rid_eligible(m, n, level, pde))
 
 
