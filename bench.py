@@ -438,6 +438,21 @@ def run_batch(args, cfg):
         id_stats = dict(sketched=ns, fallback=nf, sketch_gflop=round(sf / 1e9, 3))
     info = s.box_info()
     nl, nb, nd, _ = s.tree_info()
+    # Comparator of the learned form: fixed 32-row sketches (the randomized
+    # form without the rank model), same steps, untimed in the headline.
+    fixed32 = None
+    if args.form == "learned" and not args.no_fixed32:
+        s.set_id_method(iesolver.ID_RANDOMIZED, 32)
+        f32_ms, _ = time_steps(step, s, args, False)
+        mu32 = mu_dev.cpu().numpy()
+        ns32, nf32, sf32 = s.id_stats(with_flops=True)
+        fixed32 = dict(ms_per_step=round(f32_ms, 4), unknowns_per_s=round(N / (f32_ms * 1e-3), 1),
+                       learned_speedup=round(f32_ms / ms_per_step, 4),
+                       rel_err_vs_exact=float(np.linalg.norm(mu32 - mu_exact) /
+                                              np.linalg.norm(mu_exact)),
+                       id_stats=dict(sketched=ns32, fallback=nf32,
+                                     sketch_gflop=round(sf32 / 1e9, 3)))
+        s.set_id_method(iesolver.ID_RANDOMIZED, 0)  # the learned form again
     stages_ms, roofline = roofline_from(prof, info, id_stats, args.steps, {})
 
     # FP32 factorization as the preconditioner of FP64 refinement (host
@@ -503,6 +518,7 @@ def run_batch(args, cfg):
         "plain_fp64_ms_per_step": round(plain_ms, 4) if plain_ms else None,
         "speedup_vs_plain_fp64": round(plain_ms / ms_per_step, 4) if plain_ms else None,
         "rel_err_vs_exact": err(mu),
+        "randomized_fixed32": fixed32,
         "plain_fp64_rel_err_vs_exact": err(mu_plain) if mu_plain is not None else None,
         "exact": f"FP64 refinement to relative residual {ref_res:.1e} ({ref_its} corrections)",
         "fp32_refined": (dict(ms_per_step=round(ms32, 4), id_tol=cfg["fp32_tol"],
@@ -556,6 +572,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true",
                     help="batch workload: skip the FP32 + refinement form")
     ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--no-fixed32", action="store_true",
+                    help="skip timing the fixed 32-row randomized form beside the learned one")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the FP32 + FP64-refinement sub-record")
     ap.add_argument("--no-batch", action="store_true",
@@ -631,6 +649,21 @@ def main():
         id_stats = dict(sketched=ns, fallback=nf, sketch_gflop=round(sf / 1e9, 3))
     info = s.box_info()
     nl, nb, nd, _ = s.tree_info()
+    # Comparator of the learned form: fixed 32-row sketches (the randomized
+    # form without the rank model), same steps, untimed in the headline.
+    fixed32 = None
+    if args.form == "learned" and not args.no_fixed32:
+        s.set_id_method(iesolver.ID_RANDOMIZED, 32)
+        f32_ms, _ = timed(False)
+        mu32 = mu_dev.cpu().numpy()
+        ns32, nf32, sf32 = s.id_stats(with_flops=True)
+        fixed32 = dict(ms_per_step=round(f32_ms, 4), unknowns_per_s=round(N / (f32_ms * 1e-3), 1),
+                       learned_speedup=round(f32_ms / ms_per_step, 4),
+                       rel_err_vs_exact=float(np.linalg.norm(mu32 - mu_exact) /
+                                              np.linalg.norm(mu_exact)),
+                       id_stats=dict(sketched=ns32, fallback=nf32,
+                                     sketch_gflop=round(sf32 / 1e9, 3)))
+        s.set_id_method(iesolver.ID_RANDOMIZED, 0)  # the learned form again
 
     # Accuracy: interior field vs the analytic harmonic function.
     import oracle
@@ -679,6 +712,7 @@ def main():
         "plain_fp64_ms_per_step": round(plain_ms, 4) if plain_ms else None,
         "speedup_vs_plain_fp64": round(plain_ms / ms_per_step, 4) if plain_ms else None,
         "rel_err_vs_exact": err_exact,
+        "randomized_fixed32": fixed32,
         "plain_fp64_rel_err_vs_exact": err_exact_plain,
         "exact": f"FP64 refinement to relative residual {ref_res:.1e} ({ref_its} corrections)",
         "rel_err_vs_plain_fp64": (float(np.linalg.norm(mu - mu_plain) / np.linalg.norm(mu_plain))
