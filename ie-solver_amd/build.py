@@ -68,6 +68,11 @@ def build_cpp_tests():
     if os.path.exists(fsrc):
         _run(["g++", "-O2", "-std=c++17", "-Wall", fsrc, os.path.join(CSRC, "fmm_tree.cpp"),
               "-o", os.path.join(root, "tests", "cpp", "fmm_tree_test.bin")])
+    # CPU checks of the quadtree's local updates (host code only)
+    tsrc = os.path.join(root, "tests", "cpp", "tree_update_test.cpp")
+    if os.path.exists(tsrc):
+        _run(["g++", "-O2", "-std=c++17", "-Wall", tsrc, os.path.join(CSRC, "tree.cpp"),
+              "-o", os.path.join(root, "tests", "cpp", "tree_update_test.bin")])
     return out
 
 

@@ -40,10 +40,14 @@ struct HipError {
     if (_e != hipSuccess) throw HipError{_e, #expr};    \
   } while (0)
 
+// Device array of n elements in an allocation of cap >= n: shrinking keeps
+// the allocation, growing past it allocates anew (a quarter more than asked
+// when it grows again) and drops the contents.
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  size_t cap = 0;
   DevBuf() {}
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -52,24 +56,54 @@ struct DevBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    cap = 0;
   }
   void alloc(size_t count) {
-    if (count == n && p) return;
+    if (p && count <= cap) {
+      n = count;
+      return;
+    }
+    const size_t want = p ? std::max(count, cap + cap / 4) : count;
     release();
     if (count == 0) return;
-    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    hipError_t e = hipMalloc(&p, want * sizeof(T));
     if (e != hipSuccess) {
       p = nullptr;
       throw HipError{e, "hipMalloc"};
     }
     n = count;
+    cap = want;
   }
   void upload(const T* h, size_t count, hipStream_t st) {
     alloc(count);
     if (count) HIP_TRY(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
   }
   void upload(const std::vector<T>& v, hipStream_t st) { upload(v.data(), v.size(), st); }
+  // Elements [off, off + count) of an array of the same size.
+  void upload_range(const T* h, size_t off, size_t count, hipStream_t st) {
+    if (off + count > n) throw HipError{hipErrorInvalidValue, "upload_range"};
+    if (count)
+      HIP_TRY(hipMemcpyAsync(p + off, h + off, count * sizeof(T), hipMemcpyHostToDevice, st));
+  }
 };
+
+// Rows of b renumbered: new row r = old row map[r], zero for map[r] < 0
+// (rows of `row` elements; map on the device).  The gather goes through
+// tmp, then back into b, which keeps its allocation when it is large enough.
+template <typename T>
+void remap_rows(DevBuf<T>& b, DevBuf<unsigned char>& tmp, const int* d_map, int n_rows,
+                int old_rows, size_t row, hipStream_t st) {
+  static_assert(sizeof(T) % 4 == 0, "32-bit words");
+  if (!b.p) return;
+  if (b.n != (size_t)old_rows * row) throw HipError{hipErrorInvalidValue, "remap_rows"};
+  const size_t count = (size_t)n_rows * row, bytes = count * sizeof(T);
+  if (bytes > tmp.cap || count > b.cap) HIP_TRY(hipStreamSynchronize(st));  // before a free
+  tmp.alloc(std::max(tmp.n, bytes));
+  launch_gather_rows(tmp.p, b.p, d_map, n_rows, (int)(row * sizeof(T) / 4), st);
+  if (count > b.cap) HIP_TRY(hipStreamSynchronize(st));
+  b.alloc(count);
+  if (bytes) HIP_TRY(hipMemcpyAsync(b.p, tmp.p, bytes, hipMemcpyDeviceToDevice, st));
+}
 
 // Profile table rows: the factor stages of rs_kernels.hpp, then these.
 enum : int {
@@ -243,6 +277,14 @@ struct iegpu_ctx {
   // and whether the next factorization may keep the other boxes' factors
   std::vector<int> h_dirty;
   DevBuf<int> s_dirty;
+  // how the last update changed the tree: leaves split, boxes merged into
+  // leaves, and per slot the box's creation id before (-1: new; empty: ids
+  // unchanged)
+  int upd_split = 0, upd_merged = 0;
+  bool upd_rebuilt = false;
+  std::vector<int> upd_old_id;
+  DevBuf<int> remap_map;              // old slot per new slot (device)
+  DevBuf<unsigned char> remap_tmp;    // staging of the renumbered rows
   bool update_pending = false;  // dirty flags valid for the next factor
   bool inc_run = false;         // the factorization being enqueued is incremental
   bool last_incremental = false;
@@ -463,6 +505,7 @@ void grow_keep(DevBuf<unsigned char>& b, size_t bytes, hipStream_t st) {
   b.release();
   b.p = p;
   b.n = bytes;
+  b.cap = bytes;
 }
 
 void alloc_state(iegpu_ctx* c) {
@@ -792,7 +835,8 @@ void upload_tree(iegpu_ctx* c);
 void upload_border(iegpu_ctx* c);
 bool use_fmm(const iegpu_ctx* c);
 void prepare_fmm(iegpu_ctx* c);
-std::vector<int> mark_update(const HostTree& T, const std::vector<int>& leaves);
+std::vector<int> mark_update(const HostTree& T, const std::vector<int>& seeds);
+void remap_state(iegpu_ctx* c, const std::vector<int>& old_slot, int old_nb);
 }  // namespace
 
 int iegpu_update_geometry(iegpu_ctx* c, int n_points, const double* points,
@@ -805,65 +849,75 @@ int iegpu_update_geometry(iegpu_ctx* c, int n_points, const double* points,
     return fail(c, IEGPU_ERR_INVALID, "an update keeps the number of points");
   if (!points || !normals || !curvatures || !weights)
     return fail(c, IEGPU_ERR_INVALID, "null array");
-  for (int i = 0; i < 2 * n_points; i++)
-    if (!std::isfinite(points[i]) || !std::isfinite(normals[i]))
-      return fail(c, IEGPU_ERR_INVALID, "non-finite point or normal");
-  for (int i = 0; i < n_points; i++)
-    if (!std::isfinite(curvatures[i]) || !std::isfinite(weights[i]))
-      return fail(c, IEGPU_ERR_INVALID, "non-finite curvature or weight");
   try {
+    // Changed nodes; only they can be non-finite (the stored data is finite).
     std::vector<int> changed;
     for (int i = 0; i < n_points; i++)
       if (points[2 * i] != c->h_pts[2 * i] || points[2 * i + 1] != c->h_pts[2 * i + 1] ||
           normals[2 * i] != c->h_nrm[2 * i] || normals[2 * i + 1] != c->h_nrm[2 * i + 1] ||
           curvatures[i] != c->h_curv[i] || weights[i] != c->h_wts[i])
         changed.push_back(i);
-    // The moved points go into the kept cells; if that fails the tree is
-    // rebuilt for the new nodes.
-    HostTree nt = c->tree;
-    std::vector<int> was, now;
-    const bool keep = nt.reinsert(points, &was, &now);
+    for (int i : changed) {
+      if (!std::isfinite(points[2 * i]) || !std::isfinite(points[2 * i + 1]) ||
+          !std::isfinite(normals[2 * i]) || !std::isfinite(normals[2 * i + 1]))
+        return fail(c, IEGPU_ERR_INVALID, "non-finite point or normal");
+      if (!std::isfinite(curvatures[i]) || !std::isfinite(weights[i]))
+        return fail(c, IEGPU_ERR_INVALID, "non-finite curvature or weight");
+    }
+    // The moved points go into the kept cells, boxes split and merged as
+    // QuadTree::perturb does; a point outside its root cell rebuilds the
+    // tree for the new nodes.
+    const int old_nb = c->tree.n_boxes;
+    TreeUpdate tu;
+    const bool keep = c->tree.update(c->h_pts.data(), points, changed, &tu);
     if (!keep) {
-      nt = HostTree();
+      HostTree nt;
       nt.build_forest(points, c->h_poff.data(), c->n_prob, c->sol_dim, c->tree.leaf_cap);
       if (nt.n_levels > kMaxLevels)
         return fail(c, IEGPU_ERR_INVALID, "quadtree deeper than 64 levels");
+      c->tree = std::move(nt);
     }
     const bool had_factor = c->factored;
-    c->h_pts.assign(points, points + 2 * n_points);
-    c->h_nrm.assign(normals, normals + 2 * n_points);
-    c->h_curv.assign(curvatures, curvatures + n_points);
-    c->h_wts.assign(weights, weights + n_points);
+    // Host copies: only the span of changed nodes
+    const int lo = changed.empty() ? 0 : changed.front();
+    const int cnt = changed.empty() ? 0 : changed.back() + 1 - lo;
+    std::copy(points + 2 * (size_t)lo, points + 2 * (size_t)(lo + cnt), c->h_pts.begin() + 2 * (size_t)lo);
+    std::copy(normals + 2 * (size_t)lo, normals + 2 * (size_t)(lo + cnt), c->h_nrm.begin() + 2 * (size_t)lo);
+    std::copy(curvatures + lo, curvatures + lo + cnt, c->h_curv.begin() + lo);
+    std::copy(weights + lo, weights + lo + cnt, c->h_wts.begin() + lo);
     c->factored = false;
     c->update_pending = false;
     c->fmm_ready = false;
     c->plan_epoch++;
-    c->tree = std::move(nt);
-    if (keep) {
-      std::vector<int> leaves;
-      for (int p : changed) {
-        leaves.push_back(was[p]);
-        leaves.push_back(now[p]);
-      }
-      c->h_dirty = mark_update(c->tree, leaves);
-    } else {
-      c->h_dirty.assign(c->tree.n_boxes, 1);
-    }
+    c->h_dirty = keep ? mark_update(c->tree, tu.seeds) : std::vector<int>(c->tree.n_boxes, 1);
     c->n_dirty = 0;
     for (int v : c->h_dirty) c->n_dirty += v;
+    c->upd_split = tu.n_split;
+    c->upd_merged = tu.n_merged;
+    c->upd_rebuilt = !keep;
+    c->upd_old_id.swap(tu.old_id);
     if (c->device >= 0) {
       HIP_TRY(hipSetDevice(c->device));
+      hipStream_t st = c->stream;
       if (c->fgraph) {
         HIP_TRY(hipGraphExecDestroy(c->fgraph));
         c->fgraph = nullptr;
       }
-      c->d_pts.upload(c->h_pts, c->stream);
-      c->d_nrm.upload(c->h_nrm, c->stream);
-      c->d_curv.upload(c->h_curv, c->stream);
-      c->d_wts.upload(c->h_wts, c->stream);
-      if (keep) c->s_dirty.upload(c->h_dirty, c->stream);
-      HIP_TRY(hipStreamSynchronize(c->stream));
-      upload_tree(c);
+      c->d_pts.upload_range(c->h_pts.data(), 2 * (size_t)lo, 2 * (size_t)cnt, st);
+      c->d_nrm.upload_range(c->h_nrm.data(), 2 * (size_t)lo, 2 * (size_t)cnt, st);
+      c->d_curv.upload_range(c->h_curv.data(), lo, cnt, st);
+      c->d_wts.upload_range(c->h_wts.data(), lo, cnt, st);
+      if (keep) c->s_dirty.upload(c->h_dirty, st);
+      if (!keep || tu.restructured) {
+        upload_tree(c);
+      } else if (tu.lists_changed) {
+        const HostTree& T = c->tree;
+        c->t_orig_off.upload(T.orig_off, st);
+        c->t_orig_cnt.upload(T.orig_cnt, st);
+        c->t_orig_dofs.upload(T.orig_dofs, st);
+      }
+      if (keep && tu.restructured && had_factor) remap_state(c, tu.old_slot, old_nb);
+      HIP_TRY(hipStreamSynchronize(st));
       if (c->n_holes > 0) upload_border(c);
       c->update_pending = keep && had_factor;
     }
@@ -911,6 +965,22 @@ int iegpu_update_info(const iegpu_ctx* c, int* n_dirty, int* last_incremental, i
   return IEGPU_OK;
 }
 
+int iegpu_update_tree_info(const iegpu_ctx* c, int* n_split, int* n_merged, int* old_id) {
+  if (!c) return IEGPU_ERR_INVALID;
+  if (n_split) *n_split = c->upd_split;
+  if (n_merged) *n_merged = c->upd_merged;
+  if (old_id && c->have_tree) {
+    const HostTree& T = c->tree;
+    for (int id = 0; id < T.n_boxes; id++) {
+      const int s = T.slot_of_id[id];
+      old_id[id] = c->upd_rebuilt ? -1
+                   : c->upd_old_id.empty() ? id
+                   : c->upd_old_id[s];
+    }
+  }
+  return IEGPU_OK;
+}
+
 int iegpu_build_tree(iegpu_ctx* c, int leaf_cap) {
   if (!c) return IEGPU_ERR_INVALID;
   c->plan_epoch++;
@@ -920,6 +990,9 @@ int iegpu_build_tree(iegpu_ctx* c, int leaf_cap) {
   try {
     c->tree = HostTree();
     c->tree.build_forest(c->h_pts.data(), c->h_poff.data(), c->n_prob, c->sol_dim, leaf_cap);
+    c->upd_split = c->upd_merged = 0;
+    c->upd_rebuilt = false;
+    c->upd_old_id.clear();
     if (c->tree.n_levels > kMaxLevels)
       return fail(c, IEGPU_ERR_INVALID, "quadtree deeper than 64 levels");
     c->have_tree = true;
@@ -961,10 +1034,12 @@ void upload_tree(iegpu_ctx* c) {
     HIP_TRY(hipStreamSynchronize(st));
 }
 
-// Boxes to re-factor after a geometry update, given the leaves that lost,
-// gained or hold a changed point, following QuadTree::perturb
-// (quadtree.cpp:680-697: the leaf of every added point, and for deleted
-// points the leaf they leave, is passed to mark_neighbors_and_parents) and
+// Boxes to re-factor after a geometry update.  The walks start from the
+// seeds: the leaves that lost, gained or hold a changed point, following
+// QuadTree::perturb (quadtree.cpp:680-697: the leaf of every added point,
+// and for deleted points the leaf they leave, is passed to
+// mark_neighbors_and_parents), and after a split or merge every new box and
+// every box whose leaf flag or row sources changed.  A walk follows
 // mark_neighbors_and_parents (quadtree.cpp:435-461: the box, its
 // neighbours, then the same for its parent, up to the root).  A box's ID
 // rows come from its row-source boxes' active lists, which change when a
@@ -976,11 +1051,11 @@ void upload_tree(iegpu_ctx* c) {
 // marked (a box re-factored under a kept parent would leave the parent's
 // lists stale; the reference's rule gives this by itself on the tested
 // boundaries), and every root is re-factored.
-std::vector<int> mark_update(const HostTree& T, const std::vector<int>& leaves) {
+std::vector<int> mark_update(const HostTree& T, const std::vector<int>& seeds) {
   std::vector<int> dirty(T.n_boxes, 0);
   std::vector<char> seen(T.n_boxes, 0), walk(T.n_boxes, 0);
-  for (int s : leaves) {
-    // one walk per leaf: the walk from a leaf depends on the leaf alone
+  for (int s : seeds) {
+    // one walk per seed: the walk from a box depends on the box alone
     if (s < 0 || seen[s]) continue;
     seen[s] = 1;
     for (; s >= 0; s = T.parent[s]) {
@@ -1011,6 +1086,28 @@ std::vector<int> mark_update(const HostTree& T, const std::vector<int>& leaves) 
       if (dirty[s]) dirty[T.parent[s]] = 1;
   for (int r = 0; r < T.n_roots; r++) dirty[r] = 1;
   return dirty;
+}
+
+// After boxes were split or merged: the kept boxes' factor state moves to
+// their new slots (new boxes get zeroed rows; every one of them is marked
+// for re-factoring).  Level-local scratch (lists, jobs) is not kept.
+void remap_state(iegpu_ctx* c, const std::vector<int>& old_slot, int old_nb) {
+  hipStream_t st = c->stream;
+  const int nb = (int)old_slot.size();
+  DevBuf<unsigned char>& tmp = c->remap_tmp;
+  c->remap_map.upload(old_slot, st);
+  const int* map = c->remap_map.p;
+  for (DevBuf<int>* b : {&c->s_act_cnt, &c->s_out_cnt, &c->s_rank, &c->s_upd_kind, &c->s_m_rows,
+                         &c->s_qkind, &c->s_sk_w, &c->s_rid_done})
+    remap_rows(*b, tmp, map, nb, old_nb, 1, st);
+  remap_rows(c->s_chpos, tmp, map, nb, old_nb, 4, st);
+  for (DevBuf<int>* b : {&c->s_act_list, &c->s_out_list, &c->s_red_list, &c->s_perm})
+    remap_rows(*b, tmp, map, nb, old_nb, (size_t)c->ncap, st);
+  for (DevBuf<int64_t>* b : {&c->s_id_sz, &c->s_row_sz, &c->s_fac_sz, &c->s_id_off,
+                             &c->s_row_off, &c->s_fac_off, &c->s_dec_sz, &c->s_dec_off})
+    remap_rows(*b, tmp, map, nb, old_nb, 1, st);
+  if (c->d_feat.p && c->d_feat.n == (size_t)old_nb * kFeat)
+    remap_rows(c->d_feat, tmp, map, nb, old_nb, kFeat, st);
 }
 }  // namespace
 
@@ -1820,8 +1917,7 @@ int iegpu_box_list(iegpu_ctx* c, int box, int which, int* out, int cap) {
     if (box < 0 || box >= T.n_boxes) return -fail(c, IEGPU_ERR_INVALID, "bad box");
     const int s = T.slot_of_id[box];
     if (which == 0) {
-      v.assign(T.orig_dofs.begin() + T.orig_off[s],
-               T.orig_dofs.begin() + T.orig_off[s] + T.orig_cnt[s]);
+      v = T.dofs_below(s);
     } else if (which == 1) {
       for (int t = 0; t < T.nbr_cnt[s]; t++) v.push_back(T.box_id[T.nbr[T.nbr_off[s] + t]]);
     } else if (which >= 2 && which <= 4) {

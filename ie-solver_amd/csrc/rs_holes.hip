@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "rs_device.hpp"
@@ -240,6 +241,29 @@ void launch_mark_skel(const DevState& s, int* mask, int64_t ndof,
 
 void launch_copy(double* dst, const double* src, int64_t n, hipStream_t st) {
   k_copy<<<512, 256, 0, st>>>(dst, src, n);
+}
+
+// dst row r = src row map[r] (rows of row_words 32-bit words); rows with
+// map[r] < 0 are zeroed.
+__global__ void k_gather_rows(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                              const int* __restrict__ map, int n_rows, int row_words) {
+  const int64_t total = (int64_t)n_rows * row_words;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(i / row_words);
+    const int o = map[r];
+    dst[i] = o >= 0 ? src[(int64_t)o * row_words + (i - (int64_t)r * row_words)] : 0u;
+  }
+}
+
+void launch_gather_rows(void* dst, const void* src, const int* map, int n_rows, int row_words,
+                        hipStream_t st) {
+  if (n_rows <= 0 || row_words <= 0) return;
+  const int64_t total = (int64_t)n_rows * row_words;
+  const int blocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
+  k_gather_rows<<<blocks, 256, 0, st>>>(static_cast<uint32_t*>(dst),
+                                        static_cast<const uint32_t*>(src), map, n_rows,
+                                        row_words);
 }
 
 int64_t border_partial_blocks(int64_t ndof) { return (ndof + kChunk - 1) / kChunk; }

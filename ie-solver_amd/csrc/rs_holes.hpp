@@ -12,6 +12,10 @@ namespace iegpu {
 void launch_mark_skel(const DevState& s, int* mask, int64_t ndof,
                       hipStream_t st);
 void launch_copy(double* dst, const double* src, int64_t n, hipStream_t st);
+// Row gather: dst row r = src row map[r] (rows of row_words 32-bit words;
+// map[r] < 0 zeroes the row).  Every map entry indexes a row of src.
+void launch_gather_rows(void* dst, const void* src, const int* map, int n_rows, int row_words,
+                        hipStream_t st);
 int64_t border_partial_blocks(int64_t ndof);
 void launch_border_gram(const double* P, const double* C, const int* mask,
                         int64_t ndof, int h, double* partial, double* out,

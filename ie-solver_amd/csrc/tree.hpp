@@ -17,6 +17,21 @@
 
 namespace iegpu {
 
+// What HostTree::update changed.
+struct TreeUpdate {
+  bool restructured = false;   // boxes were split or merged: slots renumbered
+  bool lists_changed = false;  // some leaf's DOF list changed
+  int n_split = 0;             // leaves subdivided
+  int n_merged = 0;            // boxes that became leaves, their subtrees removed
+  // Restructured: per slot after the update, the box's slot and creation
+  // id before (-1: a new box).
+  std::vector<int> old_slot, old_id;
+  // Slots (after the update) from which the re-factorization marks start:
+  // the old and new leaf of every changed point, every new box, every box
+  // whose leaf flag or row-source list changed.
+  std::vector<int> seeds;
+};
+
 struct HostTree {
   int n_points = 0;
   int sol_dim = 1;
@@ -41,8 +56,15 @@ struct HostTree {
   // to the left / bottom child when below it).
   std::vector<double> mid_x, mid_y;
 
-  // Original DOFs of every box (CSR; only leaves' lists are read on device).
+  // Original DOFs of every leaf (CSR, in DOF order; an internal box's list
+  // is empty here, the DOFs below it are its leaves' lists: dofs_below).
+  // Only leaves' lists are read on the device.
   std::vector<int> orig_off, orig_cnt, orig_dofs;
+  // Right and top edges of every box (the corners as the build computes
+  // them: a child's edges are its parent's edges or the parent's split
+  // point), and the leaf slot of every point.
+  std::vector<double> tr_x, tr_y;
+  std::vector<int> point_leaf;
   // Neighbour slots per box (reference order, all levels).
   std::vector<int> nbr_off, nbr_cnt, nbr;
   // ID row-source boxes per box:
@@ -59,16 +81,30 @@ struct HostTree {
   // after problem.
   void build_forest(const double* points, const int* point_off, int n_problems,
                     int sol_dim, int leaf_cap);
-  // Puts moved points into the kept cells (QuadTree::perturb's descent of
-  // added points, ie_solver/quadtree/quadtree.cpp:680-697): every point
-  // descends from its problem's root to a leaf, and the original-DOF lists
-  // are rebuilt.  old_leaf / new_leaf get each point's leaf slot before and
-  // after.  A leaf may grow to 2 leaf_cap DOFs (the reference splits a
-  // leaf past leaf_cap).  Returns false, leaving the tree unchanged, if a
-  // point lies outside its root cell or a leaf would exceed that; the
-  // caller then rebuilds the tree.
-  bool reinsert(const double* points, std::vector<int>* old_leaf,
-                std::vector<int>* new_leaf);
+  // Neighbour and row-source lists formed again over the whole tree from
+  // the boxes (quadtree.cpp:752-801); update() forms them only where they
+  // can change and must give the same lists.
+  void relink_all();
+  // Every DOF below box `slot`, in DOF order.
+  std::vector<int> dofs_below(int slot) const;
+  // New node positions for the same points (QuadTree::perturb,
+  // ie_solver/quadtree/quadtree.cpp:520-801), `changed` listing every point
+  // whose data changed.  A point whose coordinates changed leaves its leaf
+  // and descends from its problem's root (:648-668); then every leaf that
+  // received a point and holds more than leaf_cap DOFs is subdivided
+  // (:726-733, node_subdivide :275-432), and on the old path of every moved
+  // point the first box from the root that is not a leaf and holds fewer
+  // than leaf_cap DOFs becomes a leaf, its descendants removed (:689-712,
+  // :735-741, consolidate_node :464-517).  Neighbour and row-source lists
+  // are recomputed over the whole tree when boxes were split or merged
+  // (:743-801).  Surviving boxes keep their order within a level; new
+  // boxes follow them in creation order (level l of a forest is still
+  // level l of problem 0, then of problem 1, ...).  Leaf DOF lists stay in
+  // DOF order (the reference appends moved DOFs at the end).  Returns
+  // false, leaving the tree unchanged, if a point lies outside its root
+  // cell; the caller then rebuilds the tree.
+  bool update(const double* old_points, const double* points, const std::vector<int>& changed,
+              TreeUpdate* out);
 };
 
 }  // namespace iegpu

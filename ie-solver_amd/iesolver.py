@@ -43,6 +43,7 @@ EXPORTS = [
     "iegpu_create", "iegpu_destroy", "iegpu_last_error", "iegpu_set_geometry",
     "iegpu_build_tree", "iegpu_set_holes", "iegpu_factor", "iegpu_solve",
     "iegpu_solve_device", "iegpu_tree_info", "iegpu_box_info", "iegpu_box_list",
+    "iegpu_update_tree_info",
     "iegpu_last_timings", "iegpu_set_profiling", "iegpu_profile", "iegpu_evaluate",
     "iegpu_solve_refined", "iegpu_apply", "iegpu_set_id_method", "iegpu_id_stats",
     "iegpu_set_graphs", "iegpu_record_features", "iegpu_box_features", "iegpu_rank_model_init",
@@ -128,6 +129,7 @@ def load():
     L.iegpu_update_geometry.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]
     L.iegpu_update_geometry_batch.argtypes = [vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _ip]
     L.iegpu_update_info.argtypes = [vp, _ip, _ip, _ip]
+    L.iegpu_update_tree_info.argtypes = [vp, _ip, _ip, _ip]
     L.iegpu_discard_update.argtypes = [vp]
     L.iegpu_evaluate_noalloc.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp, vp]
     L.iegpu_solve_nrhs.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp]
@@ -220,10 +222,11 @@ class RSSolver:
         (update_info)."""
         batch = isinstance(g, (list, tuple))
         geoms = g if batch else [g]
-        arrs = [_d(np.concatenate([q["points"] for q in geoms])).reshape(-1),
-                _d(np.concatenate([q["normals"] for q in geoms])).reshape(-1),
-                _d(np.concatenate([q["curvatures"] for q in geoms])),
-                _d(np.concatenate([q["weights"] for q in geoms]))]
+        keys = ("points", "normals", "curvatures", "weights")
+        if batch:
+            arrs = [_d(np.concatenate([q[k] for q in geoms])).reshape(-1) for k in keys]
+        else:
+            arrs = [_d(g[k]).reshape(-1) for k in keys]
         kept = ctypes.c_int(0)
         if batch:
             # every problem keeps its number of points (checked by the library)
@@ -252,6 +255,16 @@ class RSSolver:
         self._check(self.L.iegpu_update_info(self.h, ctypes.byref(n), ctypes.byref(inc),
                                              _pi(marks)))
         return n.value, bool(inc.value), marks
+
+    def update_tree_info(self):
+        """(leaves the last update subdivided, boxes it merged into leaves,
+        per box id the id before the update, -1 for new boxes)."""
+        nb = self.tree_info()[1]
+        ns, nm = ctypes.c_int(0), ctypes.c_int(0)
+        old = np.zeros(nb, np.int32)
+        self._check(self.L.iegpu_update_tree_info(self.h, ctypes.byref(ns), ctypes.byref(nm),
+                                                  _pi(old)))
+        return ns.value, nm.value, old
 
     def build_tree(self, leaf_cap=64):
         self._check(self.L.iegpu_build_tree(self.h, leaf_cap))

@@ -77,20 +77,28 @@ int iegpu_set_geometry_batch(iegpu_ctx* ctx, int pde, int n_problems,
 /* Replaces QuadTree::perturb (ie_solver/quadtree/quadtree.cpp:520-801)
  * followed by skeletonize's skip of compressed boxes
  * (ie_solver/skel_factorization/skel_factorization.cpp:157): new node
- * data for the same n_points (same order, same problems).  Every node
- * descends through the kept cells to a leaf, as the reference inserts
- * added points.  A node whose point, normal, curvature or weight changed
- * marks its old and its new leaf; each marked leaf marks itself, its
- * neighbours, and the same for every ancestor (mark_neighbors_and_parents,
- * quadtree.cpp:435-461); parents of marked boxes and the roots are marked
- * too.  If every node stays inside its root cell and no leaf exceeds twice
- * the leaf capacity (*tree_kept = 1), the next iegpu_factor /
- * iegpu_factor_solve_device with the tolerance and precision of the stored
- * factorization re-factors only the marked boxes and keeps every other
- * box's skeleton and factors.  Otherwise the tree is rebuilt for the new
- * nodes (*tree_kept = 0; the reference splits and merges leaves instead)
- * and the next factorization covers every box.  The stored factorization
- * is invalid until then.  Holes are kept and their border blocks rebuilt. */
+ * data for the same n_points (same order, same problems).  A node whose
+ * coordinates changed leaves its leaf and descends through the kept cells
+ * to a new one, as the reference inserts added points; then, as there,
+ * every leaf that received a node and holds more than the leaf capacity is
+ * subdivided (:726-733), and on the old path of every moved node the first
+ * box from the root that is not a leaf and holds fewer DOFs than the leaf
+ * capacity becomes a leaf, its descendants removed (:689-712, :735-741).
+ * Neighbour lists are re-formed where they can change (the reference
+ * re-forms all of them, :743-801, with the same result).  Surviving boxes
+ * keep their order within a level, new boxes follow; box ids are
+ * renumbered (iegpu_update_tree_info maps them).  A node whose point,
+ * normal, curvature or weight changed marks its old and its new leaf, and
+ * every new box and every box whose leaf flag or ID row sources changed is
+ * marked; each marked box marks itself, its neighbours, and the same for
+ * every ancestor (mark_neighbors_and_parents, quadtree.cpp:435-461);
+ * parents of marked boxes and the roots are marked too.  The next
+ * iegpu_factor / iegpu_factor_solve_device with the tolerance and precision
+ * of the stored factorization re-factors only the marked boxes and keeps
+ * every other box's skeleton and factors (*tree_kept = 1).  A node outside
+ * its root cell rebuilds the tree for the new nodes (*tree_kept = 0) and
+ * the next factorization covers every box.  The stored factorization is
+ * invalid until then.  Holes are kept and their border blocks rebuilt. */
 int iegpu_update_geometry(iegpu_ctx* ctx, int n_points, const double* points,
                           const double* normals, const double* curvatures,
                           const double* weights, int* tree_kept);
@@ -116,6 +124,13 @@ int iegpu_discard_update(iegpu_ctx* ctx);
  * (the reference reads QuadTreeNode::compressed). */
 int iegpu_update_info(const iegpu_ctx* ctx, int* n_dirty, int* last_incremental,
                       int* dirty);
+
+/* After iegpu_update_geometry: the leaves it subdivided (n_split), the
+ * boxes it turned into leaves (n_merged), and per box id the id the box
+ * had before (old_id: n_boxes ints, -1 for a new box or after a rebuilt
+ * tree; may be null).  No reference counterpart (the reference's
+ * QuadTreeNode objects persist across perturb). */
+int iegpu_update_tree_info(const iegpu_ctx* ctx, int* n_split, int* n_merged, int* old_id);
 
 /* Build the quadtree (leaf boxes hold at most leaf_cap DOFs) and upload it.
  * Replaces QuadTree::initialize_tree, ie_solver/quadtree/quadtree.h:114-117

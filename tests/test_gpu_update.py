@@ -167,7 +167,7 @@ def test_update_that_rebuilds_the_tree_factors_every_box():
     tol = 1e-10
     g0 = G.star(4096)
     g1 = {k: v.copy() for k, v in g0.items()}
-    g1["points"][:200] = 0.8 + 1e-4 * np.stack([np.arange(200) % 14, np.arange(200) // 14], 1)
+    g1["points"][7] = [1.5, 1.5]  # outside the root cell: the tree is rebuilt
     f = np.ones(4096)
     s = _solver(g0)
     s.factor(tol, FP64)
@@ -179,6 +179,48 @@ def test_update_that_rebuilds_the_tree_factors_every_box():
     fresh.factor(tol, FP64)
     mf, _ = fresh.solve(f)
     assert np.array_equal(mu, mf, equal_nan=True)
+
+
+@pytest.mark.parametrize("method", [None, (iesolver.ID_RANDOMIZED, 32)])
+def test_update_split_and_merge_equal_full_refactor(method):
+    # Nodes crowded into a small square subdivide a leaf several levels
+    # deep and empty the leaves they left (merged into their parents); then
+    # they go back.  The kept boxes' factors move to their new slots, the
+    # new and changed boxes are re-factored, and each incremental
+    # factorization equals a full one on the same tree bit for bit.
+    tol = 1e-10
+    n = 16384
+    g0 = G.wavy(n, seed=6)
+    g1 = {k: v.copy() for k, v in g0.items()}
+    c = g0["points"][4000]
+    g1["points"][3900:4100] = c + 2e-4 * np.stack([np.arange(200) % 15, np.arange(200) // 15], 1)
+    f0 = G.harmonic_data(g0["points"], *G.exterior_sources(6))
+    s = _solver(g0, method=method)
+    s.factor(tol, FP64)
+    for frame, g in enumerate([g1, g0]):
+        levels0, boxes0 = s.tree_info()[:2]
+        assert s.update_geometry(g)
+        n_split, n_merged, old_id = s.update_tree_info()
+        levels1, boxes1 = s.tree_info()[:2]
+        s.factor(tol, FP64)
+        n_dirty, inc, marks = s.update_info()
+        print(f"frame {frame}: {n_split} split, {n_merged} merged, boxes {boxes0} -> {boxes1}, "
+              f"levels {levels0} -> {levels1}, {n_dirty} re-factored, "
+              f"{int((old_id < 0).sum())} new", flush=True)
+        assert n_split + n_merged > 0 and boxes1 != boxes0
+        assert inc and n_dirty < marks.size and np.all(marks[old_id < 0] == 1)
+        mu, _ = s.solve(f0)
+        s.discard_update()
+        s.factor(tol, FP64)
+        m_full, _ = s.solve(f0)
+        assert np.array_equal(mu, m_full), frame
+    # back on the first nodes: within the reference's bound of a fresh tree
+    fresh = _solver(g0, method=method)
+    fresh.factor(tol, FP64)
+    mf, _ = fresh.solve(f0)
+    tg = G.interior_points(6, 100, radius=0.2)
+    d = np.abs(s.evaluate(tg, mu) - fresh.evaluate(tg, mf)).max()
+    assert d <= 100 * tol
 
 
 def test_update_then_fused_factor_solve():
@@ -264,25 +306,29 @@ def test_many_incremental_updates_keep_the_pool_bounded():
 
 
 def test_update_marks_cover_changed_box_inputs():
-    # Full factorizations of the kept tree before and after a bump: every
-    # box whose active list or skeleton differs between them (its ID or
-    # decouple inputs changed) carries a mark, and the marks stay a small
-    # part of the tree.
+    # Full factorizations of the tree before and after a bump: every box
+    # whose active list or skeleton differs between them (its ID or decouple
+    # inputs changed), and every new box, carries a mark, and the marks stay
+    # a small part of the tree.  Boxes are matched through the ids the
+    # update reports.
     tol = 1e-10
     n = 8192
     g0 = G.star(n)
     s = _solver(g0)
     s.factor(tol, FP64)
-    nb = s.tree_info()[1]
-    before = [(s.box_list(b, 2).tolist(), s.box_list(b, 3).tolist()) for b in range(nb)]
+    nb0 = s.tree_info()[1]
+    before = [(s.box_list(b, 2).tolist(), s.box_list(b, 3).tolist()) for b in range(nb0)]
     g1 = G.bump_star(n, 0.02, 1.3)
     assert s.update_geometry(g1)
     _, _, marks = s.update_info()
+    old_id = s.update_tree_info()[2]
     s.discard_update()
     s.factor(tol, FP64)
+    nb = s.tree_info()[1]
     after = [(s.box_list(b, 2).tolist(), s.box_list(b, 3).tolist()) for b in range(nb)]
-    changed = np.array([before[b] != after[b] for b in range(nb)])
-    print(f"{changed.sum()} boxes changed, {marks.sum()} marked of {nb}", flush=True)
+    changed = np.array([old_id[b] < 0 or before[old_id[b]] != after[b] for b in range(nb)])
+    print(f"{changed.sum()} boxes changed ({int((old_id < 0).sum())} new), "
+          f"{marks.sum()} marked of {nb}", flush=True)
     assert changed.any()
     assert np.all(marks[changed] == 1)
     assert marks.sum() <= max(4 * changed.sum(), nb // 4)

@@ -254,3 +254,19 @@ def test_fmm_tree_and_translations_cpu():
                        timeout=120)
     print(p.stdout)
     assert p.returncode == 0, p.stdout
+
+
+def test_tree_local_updates_cpu():
+    """tests/cpp/tree_update_test.cpp: after local deformations that split
+    and merge leaves, the lists formed only where they can change equal the
+    tree-wide pass, the cells follow the capacity rule (a fresh build's
+    cells), every point lies in its leaf and surviving boxes keep their
+    cells and report their old ids."""
+    exe = os.path.join(ROOT, "tests", "cpp", "tree_update_test.bin")
+    if not os.path.exists(exe):
+        import build
+        build.build_cpp_tests()
+    p = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    print(p.stdout)
+    assert p.returncode == 0, p.stdout

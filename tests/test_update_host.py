@@ -75,48 +75,90 @@ def test_update_marks_follow_reference_rule(pde, bump):
     bi0, nbrs, leaf0 = _tree(s)
     assert s.update_geometry(g1)
     bi1, nbrs1, leaf1 = _tree(s)
-    # the cells, levels and neighbours are kept
-    assert nbrs1 == nbrs
-    for k in ("level", "parent", "is_leaf", "children"):
-        assert np.array_equal(bi0[k], bi1[k])
-    # every point lies in its new leaf's cell
+    n_split, n_merged, old_id = s.update_tree_info()
+    nb1 = len(bi1["level"])
+    survivors = np.flatnonzero(old_id >= 0)
+    if n_split == 0 and n_merged == 0:
+        # the cells, levels and neighbours are kept
+        assert np.array_equal(old_id, np.arange(nb1))
+        assert nbrs1 == nbrs
+        for k in ("level", "parent", "is_leaf", "children"):
+            assert np.array_equal(bi0[k], bi1[k])
+    else:
+        assert survivors.size < nb1 or nb1 < len(bi0["level"])
+        assert np.array_equal(bi0["level"][old_id[survivors]], bi1["level"][survivors])
+    # every point lies in its new leaf's cell (the oracle's cells of the
+    # old tree: surviving boxes keep theirs)
     o = oracle.OracleProblem(g0, pde=pde)
     bl = o.box_info()["bl_side"].reshape(-1, 3)
     lo = _leaf_of(leaf1, n, sd)
+    assert np.all(lo >= 0)
     p = g1["points"]
-    x0, y0, side = bl[lo, 0], bl[lo, 1], bl[lo, 2]
-    assert np.all((p[:, 0] >= x0) & (p[:, 0] <= x0 + side) & (p[:, 1] >= y0) & (p[:, 1] <= y0 + side))
-    # marks = the reference's walk from the old and new leaf of every
-    # changed node
+    kept_leaf = old_id[lo] >= 0
+    x0, y0, side = (bl[old_id[lo][kept_leaf], k] for k in range(3))
+    pk = p[kept_leaf]
+    assert np.all((pk[:, 0] >= x0) & (pk[:, 0] <= x0 + side) & (pk[:, 1] >= y0) & (pk[:, 1] <= y0 + side))
+    # marks cover the reference's walk from the old and new leaf of every
+    # changed node (boxes of the old tree, by their ids now), and every new box
     changed = np.flatnonzero(np.any(g0["points"] != g1["points"], 1) |
                              np.any(g0["normals"] != g1["normals"], 1) |
                              (g0["weights"] != g1["weights"]) | (g0["curvatures"] != g1["curvatures"]))
     la = _leaf_of(leaf0, n, sd)
-    moved = int(np.sum(la[changed] != lo[changed]))
-    leaves = np.concatenate([la[changed], lo[changed]]).tolist()
+    moved = int(np.sum(old_id[lo[changed]] != la[changed]))
+    new_of_old = np.full(len(bi0["level"]), -1)
+    new_of_old[old_id[survivors]] = survivors
+    leaves = [int(v) for v in la[changed] if new_of_old[v] >= 0]
+    leaves += [int(old_id[v]) for v in lo[changed] if old_id[v] >= 0]
     want = oracle.update_marks(bi0["parent"].tolist(), nbrs, leaves)
     nd, _, marks = s.update_info()
-    print(f"bump {bump}: {changed.size} changed nodes, {moved} change leaf, "
-          f"{nd} of {marks.size} boxes marked ({want.sum()} by the reference's walk)", flush=True)
-    # every box the reference marks is marked; the library adds boxes whose
-    # ID row sources (not always neighbours) hold changed nodes
-    assert np.all(marks >= want)
+    print(f"bump {bump}: {changed.size} changed nodes, {moved} change leaf, {n_split} leaves "
+          f"split, {n_merged} merged, {nd} of {marks.size} boxes marked "
+          f"({want.sum()} by the reference's walk)", flush=True)
+    want_now = want[old_id[survivors]] == 1
+    assert np.all(marks[survivors[want_now]] == 1)
+    assert np.all(marks[old_id < 0] == 1)
     assert nd == marks.sum() < marks.size
     if bump > 0.01:
         assert moved > 0
 
 
-def test_update_rebuilds_tree_when_a_leaf_overflows():
+def test_update_splits_a_leaf_that_overflows():
     n = 4096
     g0 = G.star(n)
     g1 = {k: v.copy() for k, v in g0.items()}
-    # crowd 200 nodes of the arc into a tiny square: no leaf of the kept
-    # tree can take them (a kept leaf holds at most 2 leaf_cap DOFs)
+    # crowd 200 nodes of the arc into a tiny square: the leaf there is
+    # subdivided until its boxes hold at most leaf_cap DOFs, and the leaves
+    # they left merge where they fall under it (quadtree.cpp:726-741)
     g1["points"][:200] = 0.8 + 1e-4 * np.stack([np.arange(200) % 14, np.arange(200) // 14], 1)
+    s = _host(g0)
+    levels0, boxes0 = s.tree_info()[:2]
+    assert s.update_geometry(g1)
+    n_split, n_merged, old_id = s.update_tree_info()
+    levels1, boxes1 = s.tree_info()[:2]
+    print(f"{n_split} leaves split, {n_merged} merged: {boxes0} -> {boxes1} boxes, "
+          f"{levels0} -> {levels1} levels", flush=True)
+    assert n_split >= 1 and levels1 > levels0
+    bi = s.box_info()
+    leaves = np.flatnonzero(bi["is_leaf"])
+    sizes = np.array([s.box_list(b, 0).size for b in leaves])
+    assert sizes.max() <= 64 and sizes.sum() == n
+    # the leaves that hold the crowded nodes are new boxes, and marked
+    _, _, marks = s.update_info()
+    assert np.all(marks[old_id < 0] == 1)
+    crowded = [int(b) for b in leaves if np.any(s.box_list(b, 0) < 200)]
+    assert all(old_id[b] < 0 for b in crowded)
+
+
+def test_update_rebuilds_tree_when_a_node_leaves_the_root():
+    n = 4096
+    g0 = G.star(n)
+    g1 = {k: v.copy() for k, v in g0.items()}
+    g1["points"][7] = [3.0, 3.0]  # outside the root cell
     s = _host(g0)
     assert not s.update_geometry(g1)
     n_marked, _, marks = s.update_info()
     assert n_marked == marks.size and marks.all()
+    assert np.all(s.update_tree_info()[2] == -1)
     fresh = _host(g1)
     assert s.tree_info() == fresh.tree_info()
     for b in range(s.tree_info()[1]):
