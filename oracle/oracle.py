@@ -39,6 +39,9 @@ def lib():
         L.rso_create.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
                                  ctypes.c_int]
         L.rso_destroy.argtypes = [ctypes.c_void_p]
+        L.rso_perturb.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.rso_mark.argtypes = [ctypes.c_void_p, _ip]
+        L.rso_compressed.argtypes = [ctypes.c_void_p, _ip]
         L.rso_record_id_mats.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.rso_set_holes.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.rso_hole_dim.argtypes = [ctypes.c_void_p]
@@ -117,6 +120,28 @@ class OracleProblem:
     @property
     def ndof(self):
         return self.n_points * self.sol_dim
+
+    def perturb(self, geom):
+        """New data for the same nodes (QuadTree::perturb): the tree is
+        updated and the boxes the reference marks lose their factors."""
+        self.geom = geom
+        self._keep = [_d(geom["points"]).reshape(-1), _d(geom["normals"]).reshape(-1),
+                      _d(geom["curvatures"]), _d(geom["weights"])]
+        lib().rso_perturb(self.h, *[_pd(a) for a in self._keep])
+
+    def mark(self, marks=None):
+        """Boxes (by id) that lose their factors; None: every box."""
+        if marks is None:
+            lib().rso_mark(self.h, None)
+        else:
+            m = np.ascontiguousarray(marks, dtype=np.int32)
+            assert m.size == self.num_boxes()
+            lib().rso_mark(self.h, _pi(m))
+
+    def compressed(self):
+        out = np.zeros(self.num_boxes(), np.int32)
+        lib().rso_compressed(self.h, _pi(out))
+        return out
 
     def factor(self, id_tol, num_threads=1):
         return lib().rso_factor(self.h, id_tol, num_threads)

@@ -20,6 +20,7 @@
 //   ID-matrix rows        ie_solver/kernel/kernel.cpp:293-423
 //   interpolative decomp. ie_solver/ie_mat.cpp:548-588
 //   tree build            ie_solver/quadtree/quadtree.cpp:39-432
+//   perturb (updates)     ie_solver/quadtree/quadtree.cpp:435-517, :520-801
 //   active / near lists   ie_solver/quadtree/quadtree.cpp:989-1074
 //   skeletonize           ie_solver/skel_factorization/skel_factorization.cpp:140-310
 //   decouple              .../skel_factorization.cpp:77-128
@@ -388,6 +389,7 @@ struct Problem {
   int leaf_cap = 64;
   std::vector<std::vector<Node*>> levels;
   std::vector<Node*> nodes;  // by id (creation order)
+  std::vector<Node*> dead;   // removed by consolidate, freed after perturb
   Node* root = nullptr;
   bool record_id_mats = false;
   // factor state
@@ -527,6 +529,13 @@ struct Problem {
     levels.push_back({root});
     for (int i = 0; i < g.n_points; i++)
       add_point(root, g.pts[2 * i], g.pts[2 * i + 1], i);
+    link_all();
+  }
+
+  // quadtree.cpp:131-175 (neighbours of every node, levels in order).
+  void link_all() {
+    for (auto& L : levels)
+      for (Node* a : L) a->neighbors.clear();
     for (size_t lv = 0; lv < levels.size(); lv++) {
       for (Node* a : levels[lv]) {
         if (a->parent) {
@@ -553,6 +562,147 @@ struct Problem {
         }
       }
     }
+  }
+
+  // The factor state mark_neighbors_and_parents clears (quadtree.cpp:438-447).
+  static void clear_factor(Node* n) {
+    n->compressed = false;
+    n->active.clear();
+    n->skel.clear();
+    n->redundant.clear();
+    n->perm.clear();
+    n->T = Mat();
+    n->L = Mat();
+    n->U = Mat();
+  }
+
+  // quadtree.cpp:435-461 (mark_neighbors_and_parents).
+  static void mark_neighbors_and_parents(Node* node) {
+    for (; node; node = node->parent) {
+      clear_factor(node);
+      for (Node* nb : node->neighbors) clear_factor(nb);
+    }
+  }
+
+  static Node* descend(Node* cur, double x, double y) {
+    while (!cur->is_leaf) {
+      const double midx = ((cur->corners[6] - cur->corners[0]) / 2.0) + cur->corners[0];
+      const double midy = ((cur->corners[3] - cur->corners[1]) / 2.0) + cur->corners[1];
+      if (x < midx && y < midy) cur = cur->children[0];
+      else if (x < midx && y >= midy) cur = cur->children[1];
+      else if (x >= midx && y < midy) cur = cur->children[3];
+      else cur = cur->children[2];
+    }
+    return cur;
+  }
+
+  // quadtree.cpp:464-517 (consolidate_node): the leaves' lists below the
+  // node, breadth first, become its list; its descendants leave their
+  // levels.
+  void consolidate(Node* node) {
+    node->original.clear();
+    std::vector<Node*> queue{node}, gone;
+    for (size_t i = 0; i < queue.size(); i++) {
+      Node* cur = queue[i];
+      if (cur->is_leaf) {
+        node->original.insert(node->original.end(), cur->original.begin(), cur->original.end());
+      } else {
+        for (Node* ch : cur->children) queue.push_back(ch);
+      }
+      if (cur != node) gone.push_back(cur);
+    }
+    for (Node* d : gone) {
+      auto& L = levels[d->level];
+      L.erase(std::find(L.begin(), L.end(), d));
+      dead.push_back(d);
+    }
+    for (int q = 0; q < 4; q++) node->children[q] = nullptr;
+    node->is_leaf = true;
+  }
+
+  // quadtree.cpp:520-801 (perturb) for new data of the same nodes: a node
+  // whose coordinates changed is deleted and added again (the reference
+  // finds unchanged nodes by their coordinates; nodes keep their index
+  // here).  Lists drop the deleted DOFs (:571-636); added nodes go to the
+  // leaf they fall in, whose walk is marked (:638-668); DOFs below every
+  // box are counted (:670-684); each deleted node's old path marks the
+  // first box from the root that is not a leaf and holds fewer than
+  // leaf_cap DOFs, and its old leaf's walk (:686-716); the new data
+  // replaces the old (:718-725); leaves that received nodes and exceed the
+  // cap are subdivided (:727-733); the marked boxes are consolidated
+  // (:735-741); all neighbour lists are formed again (:743-801).  Nodes are
+  // then numbered in their old order, the new ones after.
+  void perturb(const double* pts, const double* nrm, const double* curv, const double* wts) {
+    const int n = g.n_points, sd = g.sol_dim;
+    std::vector<char> moved(n, 0);
+    std::vector<int> changed;
+    for (int i = 0; i < n; i++)
+      if (pts[2 * i] != g.pts[2 * i] || pts[2 * i + 1] != g.pts[2 * i + 1]) {
+        moved[i] = 1;
+        changed.push_back(i);
+      }
+    auto drop = [&](std::vector<int>& v) {
+      v.erase(std::remove_if(v.begin(), v.end(), [&](int d) { return moved[d / sd] != 0; }),
+              v.end());
+    };
+    for (auto& L : levels)
+      for (Node* nd : L) {
+        if (nd->is_leaf) drop(nd->original);
+        drop(nd->active);
+        drop(nd->skel);
+        drop(nd->redundant);
+      }
+    std::vector<Node*> maybe_bursting;
+    for (int i : changed) {
+      Node* leaf = descend(root, pts[2 * i], pts[2 * i + 1]);
+      for (int d = 0; d < sd; d++) leaf->original.push_back(sd * i + d);
+      maybe_bursting.push_back(leaf);
+      mark_neighbors_and_parents(leaf);
+    }
+    std::unordered_map<Node*, int> below;
+    for (auto& L : levels)
+      for (Node* nd : L) below[nd] = nd->is_leaf ? (int)nd->original.size() : 0;
+    for (int lv = (int)levels.size() - 1; lv >= 1; lv--)
+      for (Node* nd : levels[lv]) below[nd->parent] += below[nd];
+    std::vector<Node*> sparse;
+    for (int i : changed) {
+      Node* cur = root;
+      bool path_marked = false;
+      while (!cur->is_leaf) {
+        if (below[cur] < leaf_cap && !path_marked) {
+          path_marked = true;
+          if (std::find(sparse.begin(), sparse.end(), cur) == sparse.end()) sparse.push_back(cur);
+        }
+        const double x = g.pts[2 * i], y = g.pts[2 * i + 1];
+        const double midx = ((cur->corners[6] - cur->corners[0]) / 2.0) + cur->corners[0];
+        const double midy = ((cur->corners[3] - cur->corners[1]) / 2.0) + cur->corners[1];
+        if (x < midx && y < midy) cur = cur->children[0];
+        else if (x < midx && y >= midy) cur = cur->children[1];
+        else if (x >= midx && y < midy) cur = cur->children[3];
+        else cur = cur->children[2];
+      }
+      mark_neighbors_and_parents(cur);
+    }
+    g.pts.assign(pts, pts + 2 * n);
+    g.nrm.assign(nrm, nrm + 2 * n);
+    g.curv.assign(curv, curv + n);
+    g.wts.assign(wts, wts + n);
+    for (Node* nd : maybe_bursting)
+      if (nd->is_leaf && (int)nd->original.size() > leaf_cap) subdivide(nd);
+    for (Node* nd : sparse) consolidate(nd);
+    link_all();
+    // number the nodes: survivors in their order, then the new ones
+    std::vector<char> is_dead(nodes.size(), 0);
+    for (Node* d : dead) is_dead[d->id] = 1;
+    std::vector<Node*> keep;
+    for (Node* nd : nodes)
+      if (!is_dead[nd->id]) keep.push_back(nd);
+    for (Node* d : dead) delete d;
+    dead.clear();
+    nodes.swap(keep);
+    for (size_t i = 0; i < nodes.size(); i++) nodes[i]->id = (int)i;
+    while (levels.size() > 1 && levels.back().empty()) levels.pop_back();
+    factored = false;
   }
 
   // quadtree.cpp:1028-1074 (remove_inactive_dofs_at_box, source lists).
@@ -1016,6 +1166,32 @@ void* rso_create(int pde, int n_points, const double* points,
 }
 
 void rso_destroy(void* h) { delete static_cast<Problem*>(h); }
+
+// New data for the same nodes (QuadTree::perturb); the next rso_factor
+// re-factors the boxes perturb cleared and keeps the others.
+int rso_perturb(void* h, const double* points, const double* normals, const double* curvatures,
+                const double* weights) {
+  static_cast<Problem*>(h)->perturb(points, normals, curvatures, weights);
+  return 0;
+}
+
+// Clears the factor state of every box b with marks[b] != 0 (by id), as
+// mark_neighbors_and_parents does for its walk; marks == null clears all
+// boxes (the next rso_factor is a full one).
+int rso_mark(void* h, const int* marks) {
+  Problem* p = static_cast<Problem*>(h);
+  for (size_t b = 0; b < p->nodes.size(); b++)
+    if (!marks || marks[b]) Problem::clear_factor(p->nodes[b]);
+  p->factored = false;
+  return 0;
+}
+
+// Per box id: 1 if its factor state is kept for the next rso_factor.
+int rso_compressed(void* h, int* out) {
+  Problem* p = static_cast<Problem*>(h);
+  for (size_t b = 0; b < p->nodes.size(); b++) out[b] = p->nodes[b]->compressed ? 1 : 0;
+  return 0;
+}
 
 void rso_record_id_mats(void* h, int on) {
   static_cast<Problem*>(h)->record_id_mats = on != 0;

@@ -13,6 +13,7 @@ import torch
 
 import geometry as G
 import iesolver
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -332,3 +333,39 @@ def test_update_marks_cover_changed_box_inputs():
     assert changed.any()
     assert np.all(marks[changed] == 1)
     assert marks.sum() <= max(4 * changed.sum(), nb // 4)
+
+
+@pytest.mark.parametrize("pde", [iesolver.PDE_LAPLACE, iesolver.PDE_STOKES])
+def test_update_incremental_matches_oracle_incremental(pde):
+    # The oracle restates QuadTree::perturb (its tree equals the library's)
+    # and skeletonize's skip of compressed boxes; given the library's marks
+    # it re-factors the same boxes.  The two incremental solutions agree to
+    # the ID tolerance (leaf DOF order differs: the reference appends moved
+    # DOFs at the end of a leaf's list).
+    tol = 1e-10
+    n = 8192
+    g0 = G.star(n)
+    g1 = G.bump_star(n, 0.03, 1.0 if pde == iesolver.PDE_LAPLACE else 2.5)
+    if pde == iesolver.PDE_STOKES:
+        f = G.tangent_data(g1["normals"]).reshape(-1)
+    else:
+        f = G.harmonic_data(g1["points"], *G.exterior_sources(5))
+    s = _solver(g0, pde)
+    s.factor(tol, FP64)
+    assert s.update_geometry(g1)
+    n_split, n_merged, _ = s.update_tree_info()
+    marks = s.update_info()[2]
+    s.factor(tol, FP64)
+    assert s.update_info()[1]
+    mu, _ = s.solve(f)
+    o = oracle.OracleProblem(g0, pde=pde)
+    o.factor(tol)
+    o.perturb(g1)
+    o.mark(marks)
+    o.factor(tol)
+    mo, _ = o.solve(f)
+    d = np.linalg.norm(mu - mo) / np.linalg.norm(mo)
+    print(f"pde {pde}: {n_split} split, {n_merged} merged, {int(marks.sum())} of {marks.size} "
+          f"re-factored; incremental vs oracle incremental {d:.2e}", flush=True)
+    assert n_split + n_merged > 0
+    assert d <= 10 * tol
