@@ -271,6 +271,11 @@ struct iegpu_ctx {
   FmmDev fdev{};
 
   double t_factor_ms = 0, t_solve_ms = 0, t_root_ms = 0;
+  // Timing events of the factorization and the solve (created on first
+  // use) and the pinned read-back of the factorization's error words and
+  // factor-pool top.
+  hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
+  int* h_rb = nullptr;
   double fac_bytes = 0;
 
   // geometry update (iegpu_update_geometry): boxes to re-factor per slot,
@@ -580,12 +585,14 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
   // evs[0]: start; evs[1 + lv]: level lv's ID done; evs[n_levels + 1]: the
   // decouple chain done.  The two chains share no scratch: the ID pool
   // (one half per level parity) belongs to the ID chain, the decouple pool
-  // to the decouple chain.
+  // to the decouple chain.  The events order the two streams of one device
+  // only, so they release at device scope (no system-scope cache
+  // write-back: 0.05 ms per step at N = 262,144).
   const size_t n_ev = (size_t)T.n_levels + 2;
   if (split && c->evs.size() < n_ev) {
     for (size_t i = c->evs.size(); i < n_ev; i++) {
       hipEvent_t e;
-      HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
       c->evs.push_back(e);
     }
   }
@@ -748,6 +755,9 @@ int iegpu_destroy(iegpu_ctx* c) {
   c->prof.release();
   if (c->fgraph) (void)hipGraphExecDestroy(c->fgraph);
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {c->ev_f0, c->ev_f1, c->ev_s0, c->ev_s1})
+    if (e) (void)hipEventDestroy(e);
+  if (c->h_rb) (void)hipHostFree(c->h_rb);
   hipStream_t st = c->stream, st2 = c->stream2;
   delete c;
   if (st) (void)hipStreamDestroy(st);
@@ -1513,9 +1523,10 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
       c->plan_mode = 1;
     }
     c->plan_used = c->plan_on;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    if (!c->ev_f0) HIP_TRY(hipEventCreate(&c->ev_f0));
+    if (!c->ev_f1) HIP_TRY(hipEventCreate(&c->ev_f1));
+    if (!c->h_rb) HIP_TRY(hipHostMalloc((void**)&c->h_rb, 8 * sizeof(int), hipHostMallocDefault));
+    hipEvent_t e0 = c->ev_f0, e1 = c->ev_f1;
     int errbits = 0;
     for (int attempt = 0; attempt < 8; attempt++) {
       alloc_state(c);
@@ -1562,8 +1573,12 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
         enqueue_factor<double>(c, id_tol, fb, fx);
       }
       HIP_TRY(hipGetLastError());
-      int h_err[4] = {0, 0, 0, 0};
-      HIP_TRY(hipMemcpyAsync(h_err, c->s_err.p, sizeof(h_err), hipMemcpyDeviceToHost, c->stream));
+      // One read-back: the error words and the factor-pool top.
+      if (c->hole_dim == 0) HIP_TRY(hipEventRecord(e1, c->stream));
+      int* h_err = c->h_rb;
+      HIP_TRY(hipMemcpyAsync(h_err, c->s_err.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipMemcpyAsync(h_err + 4, c->s_fac_top.p, sizeof(int64_t), hipMemcpyDeviceToHost,
+                             c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
       errbits = h_err[0];
       if (errbits & kErrPlan) {  // a skipped list had boxes: again, every kernel
@@ -1579,11 +1594,12 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
           if (precision == IEGPU_FP32) enqueue_border_factor<float>(c);
           else enqueue_border_factor<double>(c);
           HIP_TRY(hipGetLastError());
-          HIP_TRY(hipMemcpyAsync(h_err, c->s_err.p, sizeof(h_err), hipMemcpyDeviceToHost, c->stream));
+          HIP_TRY(hipMemcpyAsync(h_err, c->s_err.p, 4 * sizeof(int), hipMemcpyDeviceToHost,
+                                 c->stream));
+          HIP_TRY(hipEventRecord(e1, c->stream));
+          HIP_TRY(hipStreamSynchronize(c->stream));
+          errbits = h_err[0];
         }
-        HIP_TRY(hipEventRecord(e1, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        errbits = h_err[0];
         break;
       }
       // A larger box capacity changes every per-box list's stride, so the
@@ -1598,8 +1614,6 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
       if (box_cap) c->inc_run = false;
       if (box_cap) {
         if (c->ncap >= kMaxBoxDofs) {
-          (void)hipEventDestroy(e0);
-          (void)hipEventDestroy(e1);
           return fail(c, IEGPU_ERR_CAPACITY,
                       "a box's active DOF count exceeds " + std::to_string(kMaxBoxDofs));
         }
@@ -1620,8 +1634,6 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
         }
       }
       if (attempt == 7) {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         return fail(c, IEGPU_ERR_CAPACITY, "workspace growth did not converge");
       }
     }
@@ -1654,12 +1666,10 @@ int factor_impl(iegpu_ctx* c, double id_tol, int precision, const double* fb, do
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     c->t_factor_ms = ms;
     int64_t top = 0;
-    HIP_TRY(hipMemcpy(&top, c->s_fac_top.p, sizeof(top), hipMemcpyDeviceToHost));
+    std::memcpy(&top, c->h_rb + 4, sizeof(top));
     c->fac_bytes = (double)top * c->elem();
     c->kept_top = top;
     if (!c->last_incremental) c->full_top = top;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (errbits & kErrSingular) {
       c->factored = false;
       return fail(c, IEGPU_ERR_SINGULAR, "exact zero pivot in an LU factorization");
@@ -1718,9 +1728,9 @@ int iegpu_solve(iegpu_ctx* c, const double* b, double* mu, double* alpha) {
     const int64_t nd = c->ndof();
     c->d_work.alloc(nd);
     HIP_TRY(hipMemcpyAsync(c->d_work.p, b, nd * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    if (!c->ev_s0) HIP_TRY(hipEventCreate(&c->ev_s0));
+    if (!c->ev_s1) HIP_TRY(hipEventCreate(&c->ev_s1));
+    hipEvent_t e0 = c->ev_s0, e1 = c->ev_s1;
     HIP_TRY(hipEventRecord(e0, c->stream));
     double* da = c->hole_dim > 0 ? c->d_alpha.p : nullptr;
     if (c->precision == IEGPU_FP32) enqueue_solve<float>(c, c->d_work.p, da, c->stream);
@@ -1733,8 +1743,6 @@ int iegpu_solve(iegpu_ctx* c, const double* b, double* mu, double* alpha) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     c->t_solve_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
   } catch (const HipError& e) {
     return from_hip(c, e);
   }
