@@ -1156,11 +1156,11 @@ __device__ __forceinline__ void rid_segment(const double (*tab)[kRidMaxNear + kN
 // wave 0 alone runs the pivoted QR of Y, the other waves ending there.
 // QR layout: lane c holds column c's active rows, shifted so that the
 // step's row is always register 0; row j of R leaves the registers for
-// LDS (rr[j][c]) at step j, where the T solve finds R11 and R12.  Four
-// workgroups per CU (128 registers, a few spilled) measured 3 % faster
-// per factorization than three without spills.
+// LDS (rr[j][c]) at step j, where the T solve finds R11 and R12.  The
+// pivot column reaches every lane through readlane (uniform registers),
+// not LDS: 93 registers, five workgroups per CU.
 template <typename R, int PDE>
-__global__ __launch_bounds__(256, 4) void k_rid(DevGeom g, DevTree t, DevState s, int start,
+__global__ __launch_bounds__(256, 5) void k_rid(DevGeom g, DevTree t, DevState s, int start,
                                              int level, double tol_d) {
   constexpr int LM = kRidRows, YLD = 65, TR = kRidMaxNear + kNumProxy;
   using V4 = typename Mfma16<R>::V4;
@@ -1168,7 +1168,6 @@ __global__ __launch_bounds__(256, 4) void k_rid(DevGeom g, DevTree t, DevState s
   // YLD), then the rows of R, afterwards.
   __shared__ double tab[6][TR];
   __shared__ int rowdofs[kRidMaxNear];
-  __shared__ R vbuf[LM];    // Householder vector of the step (shifted rows)
   __shared__ int spiv[64];  // pivot column (lane) of each step
   static_assert(LM * YLD * sizeof(R) <= sizeof(tab), "Y fits the row table");
   R* ybuf = reinterpret_cast<R*>(&tab[0][0]);
@@ -1268,31 +1267,31 @@ __global__ __launch_bounds__(256, 4) void k_rid(DevGeom g, DevTree t, DevState s
       break;
     }
     if (lane == p) {
-      vbuf[0] = R(1);
-#pragma unroll
-      for (int r = 1; r < LM; r++) vbuf[r] = y[r] * sc;
       spiv[j] = p;
       avail = false;
     }
-    wave_sync();
+    // The pivot column's entries below row j, uniform across the wave:
+    // the Householder vector is (1, sc yp[1], ..., sc yp[LM-1]).
+    R yp[LM];
+#pragma unroll
+    for (int r = 1; r < LM; r++) yp[r] = lane_value(y[r], p);
     R d4[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int r = 0; r < LM; r++) d4[r & 3] += vbuf[r] * y[r];
-    const R w = tau * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
+    for (int r = 1; r < LM; r++) d4[r & 3] += yp[r] * y[r];
+    const R w = tau * (y[0] + sc * ((d4[0] + d4[1]) + (d4[2] + d4[3])));
+    const R ws = w * sc;
     // Row j of R leaves; the rest shift up one register.
     rr[j * YLD + lane] = (lane == p) ? beta : y[0] - w;
-    // vbuf is read again below rather than kept in registers beside y
-    __asm__ volatile("" ::: "memory");
     R t4[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int r = 0; r < LM - 1; r++) {
-      y[r] = y[r + 1] - w * vbuf[r + 1];
+      y[r] = y[r + 1] - ws * yp[r + 1];
       if (r > 0) t4[r & 3] += y[r] * y[r];
     }
     y[LM - 1] = 0;
     tl = (t4[0] + t4[1]) + (t4[2] + t4[3]);
-    wave_sync();
   }
+  wave_sync();
   if (k < 0) {
     if (j < n) {  // the sketch ran out before a cut: the full QRCP takes it
       if (lane == 0) {
