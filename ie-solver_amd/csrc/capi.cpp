@@ -582,8 +582,10 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
   if (fx && fb != fx) launch_copy(fx, fb, c->ndof(), st);
   const bool split = c->overlap && !c->prof.on && c->stream2 != nullptr;
   hipStream_t st2 = split ? c->stream2 : st;
-  // evs[0]: start; evs[1 + lv]: level lv's ID done; evs[n_levels + 1]: the
-  // decouple chain done.  The two chains share no scratch: the ID pool
+  // evs[1 + lv]: level lv's ID done (the decouple chain's first work waits
+  // for the deepest level's, which follows everything enqueued before it);
+  // evs[n_levels + 1]: the decouple chain done.  The two chains share no
+  // scratch: the ID pool
   // (one half per level parity) belongs to the ID chain, the decouple pool
   // to the decouple chain.  The events order the two streams of one device
   // only, so they release at device scope (no system-scope cache
@@ -597,10 +599,6 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
     }
   }
   launch_reset(s, st);
-  if (split) {
-    HIP_TRY(hipEventRecord(c->evs[0], st));
-    HIP_TRY(hipStreamWaitEvent(st2, c->evs[0], 0));
-  }
   for (int lv = T.n_levels - 1; lv >= 1; lv--) {
     const int start = T.level_start[lv];
     const int count = T.level_start[lv + 1] - start;

@@ -2297,10 +2297,12 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
 
 // Every other box: uncompressed boxes pass their children's blocks on;
 // compressed boxes with |active| > 64 keep P and X_rr in LDS when they fit
-// and in the box's scratch slot otherwise.
+// and in the box's scratch slot otherwise.  Few boxes of a level are
+// these, so a small grid walks the level (the others are skipped after
+// two loads) instead of one workgroup per box.
 template <typename R, int NT>
 __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
-                                                 DevState s, int start) {
+                                                 DevState s, int start, int count) {
   constexpr int kLdsElems = 65536 / sizeof(R);
   __shared__ R sbuf[kLdsElems];
   __shared__ signed char pch[kMaxBoxDofs];
@@ -2310,15 +2312,18 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   __shared__ const R* chS[4];
   __shared__ int chN[4];
 
-  const int b = start + blockIdx.x;
   const int tid = threadIdx.x;
-  if (!pools_ok(s) || box_clean(s, b)) return;
+  if (!pools_ok(s)) return;
+  for (int bi = blockIdx.x; bi < count; bi += gridDim.x) {
+  const int b = start + bi;
+  if (box_clean(s, b)) continue;
   const int n = s.act_cnt[b];
   if (n == 0) {
     if (tid == 0) s.upd_kind[b] = 0;
-    return;
+    continue;
   }
-  if (n <= kSmallBox && s.rank[b] > 0) return;  // k_decouple_small's box
+  if (n <= kSmallBox && s.rank[b] > 0) continue;  // k_decouple_small's box
+  __syncthreads();  // the previous box's LDS
   load_child_blocks<R>(t, s, b, n, chS, chN, pch, ploc, NT);
   __syncthreads();
   ChildUpdates<R> upd;
@@ -2336,7 +2341,7 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
         F[idx] = upd((int)(idx % n), (int)(idx / n));
     }
     if (tid == 0) s.upd_kind[b] = any ? 1 : 0;
-    return;
+    continue;
   }
   const int r = n - k;
   const FacLayout fl = FacLayout::make(k, r);
@@ -2371,6 +2376,7 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
   decouple_tail<R, NT>(X, prow, Krs, Ksr, P, r, k, F, fl, upd, sperm);
   if (tid == 0) s.upd_kind[b] = 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -3086,7 +3092,7 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       } else {
         k_decouple_small<R, 0, kSmallBox><<<count, 256, 0, st>>>(g, t, s, start);
       }
-      k_decouple<R, 256><<<count, 256, 0, st>>>(g, t, s, start);
+      k_decouple<R, 256><<<min(count, 512), 256, 0, st>>>(g, t, s, start, count);
       break;
     default:
       break;
