@@ -2303,8 +2303,11 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
 template <typename R, int NT>
 __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
                                                  DevState s, int start, int count) {
-  constexpr int kLdsElems = 65536 / sizeof(R);
+  // 128 KB: P, X_rr, K_rs and K_sr of a box up to 90 DOFs (FP64) stay in
+  // LDS; the Schur products are then LDS-bound, not bound by L2 latency.
+  constexpr int kLdsElems = 131072 / sizeof(R);
   __shared__ R sbuf[kLdsElems];
+  __shared__ R gjcol[2 * 16 * 8];  // the register Gauss-Jordan's column buffers
   __shared__ signed char pch[kMaxBoxDofs];
   __shared__ short ploc[kMaxBoxDofs];
   __shared__ short sperm[kMaxBoxDofs];
@@ -2366,12 +2369,10 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
     for (int64_t idx = tid; idx < (int64_t)r * r; idx += NT) Xf[idx] = X[idx];
     __syncthreads();
   }
-  // X_rr^{-1}: in registers for r <= 128 (the LDS buffer is free: the
-  // blocks of these boxes live in the scratch slot), else step by step.
+  // X_rr^{-1}: in registers for r <= 128, else step by step.
   int info;
-  if (NT == 256 && r <= 64 && P != sbuf) info = gj_invert_regs<R, 4>(X, r, prow, sbuf);
-  else if (NT == 256 && r <= 128 && P != sbuf)
-    info = gj_invert_regs<R, 8>(X, r, prow, sbuf);
+  if (NT == 256 && r <= 64) info = gj_invert_regs<R, 4>(X, r, prow, gjcol);
+  else if (NT == 256 && r <= 128) info = gj_invert_regs<R, 8>(X, r, prow, gjcol);
   else info = gj_invert_into<R, NT>(X, P, r, prow);
   if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
   decouple_tail<R, NT>(X, prow, Krs, Ksr, P, r, k, F, fl, upd, sperm);
