@@ -2384,21 +2384,25 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
 // Root: K(allskel, allskel) - carried blocks of the level-1 boxes, inverted.
 // ---------------------------------------------------------------------------
 
+// Assembly of root block blockIdx.x: gridDim.y workgroups share its n^2
+// entries (one workgroup was latency-bound on the kernel's gathers).
+constexpr int kRootSplit = 64;
+
 template <typename R, int NT>
-__global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
-                                             int factor_lu) {
+__global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s) {
   __shared__ signed char pch[kMaxBoxDofs];
   __shared__ short ploc[kMaxBoxDofs];
   __shared__ const R* chS[4];
   __shared__ int chN[4];
   const int b = blockIdx.x;  // root of problem b
   const int tid = threadIdx.x;
-  if (tid == 0) {  // the root is never compressed
+  const bool lead = blockIdx.y == 0 && tid == 0;
+  if (lead) {  // the root is never compressed
     s.rank[b] = 0;
     s.m_rows[b] = 0;
     s.out_cnt[b] = s.act_cnt[b];
   }
-  if (b == 0 && tid == 0 && s.plan_skip != nullptr) {
+  if (b == 0 && lead && s.plan_skip != nullptr) {
     // The launch plan skipped kernels whose lists it expected empty: any
     // that is not makes the host run the factorization again in full.
     for (int lv = 1; lv < t.n_levels; lv++)
@@ -2438,20 +2442,27 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
   upd.ploc = ploc;
   const int* act = s.act_list + (int64_t)b * s.ncap;
   R* A = root_block<R>(s, b);
-  for (int64_t idx = tid; idx < (int64_t)n * n; idx += NT) {
+  R* Af = s.root_fwd ? reinterpret_cast<R*>(s.root_fwd) + (int64_t)s.ncap * s.ncap * b : nullptr;
+  const int64_t nn = (int64_t)n * n, per = (nn + gridDim.y - 1) / gridDim.y;
+  const int64_t e0 = per * blockIdx.y, e1 = min(nn, e0 + per);
+  for (int64_t idx = e0 + tid; idx < e1; idx += NT) {
     const int i = (int)(idx % n), j = (int)(idx / n);
-    A[idx] = static_cast<R>(kernel_entry(g, act[i], act[j])) - upd(i, j);
+    const R v = static_cast<R>(kernel_entry(g, act[i], act[j])) - upd(i, j);
+    A[idx] = v;
+    if (Af) Af[idx] = v;
   }
-  __syncthreads();
-  if (s.root_fwd) {
-    R* Af = reinterpret_cast<R*>(s.root_fwd) + (int64_t)s.ncap * s.ncap * b;
-    for (int64_t idx = tid; idx < (int64_t)n * n; idx += NT) Af[idx] = A[idx];
-    __syncthreads();
-  }
-  if (factor_lu && n > kRootBlkMax) {
-    // Gauss-Jordan inverse (the reference LU-factors the root block) for
-    // blocks too large for k_root_inv and k_root_inv_blk; the inverse,
-    // unpermuted, goes after the assembled matrix.
+}
+
+// Gauss-Jordan inverse (the reference LU-factors the root block) of root
+// blocks too large for k_root_inv and k_root_inv_blk; the inverse,
+// unpermuted, goes after the assembled matrix.
+template <typename R, int NT>
+__global__ __launch_bounds__(NT) void k_root_big(DevState s, int factor_lu) {
+  const int b = blockIdx.x, tid = threadIdx.x;
+  if (!factor_lu || !pools_ok(s)) return;
+  const int n = s.act_cnt[b];
+  if (n > kRootBlkMax) {
+    R* A = root_block<R>(s, b);
     R* B = A + (int64_t)n * n;
     int* prow = root_pivots(s, b);
     const int info = gj_invert_into<R, NT>(A, B, n, prow);
@@ -2463,27 +2474,53 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s,
   }
 }
 
-// Gauss-Jordan inverse of the root block with the matrix in registers:
-// 1024 threads on a 32 x 32 grid, thread (tr, tc) holding rows tr + 32 u and
-// columns tc + 32 v (n <= 192; 6 x 6 entries each: measured faster than
-// 32 x 16 threads with 6 x 12, 64 x 16 with 3 x 12 and 16 x 64 with
-// 12 x 3 at a 190-DOF root).  Per step the pivot column (kept in LDS by
-// its owners) picks the pivot row as in gj_invert, the pivot row's owners
-// publish it, and every thread updates its entries.  The inverse,
-// unpermuted, goes after the assembled matrix.
+// Gauss-Jordan inverse of the root block with the matrix in registers,
+// blocked over panels of kRootPB columns: 1024 threads on a 32 x 32 grid,
+// thread (tr, tc) holding rows tr + 32 u and columns tc + 32 v (n <= 192;
+// 6 x 6 entries each), so that one wave holds every row of its 12 columns.
+// The block is bordered by the identity up to n8, a multiple of kRootPB
+// (its pivots stay in the block: border rows hold zeros in its columns and
+// lose every tie to a block row), and rows and columns past n8 are zero, so
+// no step or entry needs a bound.  Per panel:
+//  1. the panel's owners copy its columns to LDS;
+//  2. wave 0 runs the panel's pivot steps on those columns alone (they
+//     depend on nothing else): pivot search as in gj_invert, scaling,
+//     elimination; it records each step's pivot row p_k, 1 / pivot and the
+//     column as it was before the step (C[m][i]);
+//  3. in every wave, the owners of the pivot rows copy them to LDS and one
+//     lane per column forms the scaled pivot rows the steps would have
+//     made, R_k = (row p_k - sum_{m<k} C[m][p_k] R_m) / pivot_k;
+//  4. every thread applies the steps to its entries in step order,
+//     a -= C[m][i] R_m, a pivot row restarting from R_k at its own step;
+//     panel entries take wave 0's result.
+// Per entry these are the operations of the unblocked elimination in its
+// order, so the inverse is the same; two barriers per panel instead of two
+// per column, and one wave (not all sixteen) searches each pivot.  The
+// inverse, unpermuted, goes after the assembled matrix.
 constexpr int kRootTR = 32, kRootTC = 32, kRootRU = 6, kRootCV = 6;
+constexpr int kRootPB = 8;
 
 template <typename R>
 __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int factor_lu) {
-  constexpr int NMAX = kRootRegMax;
-  __shared__ R colbuf[2][NMAX];
-  __shared__ R rowbuf[NMAX];
+  constexpr int NMAX = kRootRegMax, PB = kRootPB, NS = NMAX / 64;
+  static_assert(kRootTC % PB == 0 && NMAX == kRootTR * kRootRU && NMAX == kRootTC * kRootCV &&
+                NMAX % 64 == 0 && NMAX % PB == 0 && NMAX <= 256, "root tiling");
+  __shared__ R pan[2][PB][NMAX];   // panel columns, in and out of step 2
+  __shared__ R cpre[PB][NMAX];     // column m before step m (zero where
+                                   // row i is pivot at a step >= m)
+  __shared__ R cpiv[PB][PB];       // C[m][p_k], m < k
+  __shared__ R rowk[PB][NMAX];     // pivot rows, then R_k
+  __shared__ R pinv[PB];
+  __shared__ int pstep[NMAX];      // elimination step that pivots on row i
   __shared__ int prow[NMAX];
   __shared__ int invrow[NMAX];
+  __shared__ R park[kRootRU * kRootCV][64];
   if (!factor_lu || !pools_ok(s)) return;
   const int n = s.act_cnt[blockIdx.x];
   if (n == 0 || n > NMAX) return;
+  const int n8 = (n + PB - 1) / PB * PB;
   const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int tr = tid % kRootTR, tc = tid / kRootTR;
   R* A = root_block<R>(s, blockIdx.x);
   R a[kRootRU][kRootCV];
@@ -2492,109 +2529,176 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
 #pragma unroll
     for (int v = 0; v < kRootCV; v++) {
       const int i = tr + kRootTR * u, c = tc + kRootTC * v;
-      a[u][v] = (i < n && c < n) ? A[i + (int64_t)n * c] : R(0);
+      a[u][v] = (i < n && c < n) ? A[i + (int64_t)n * c] : (i == c && i < n8) ? R(1) : R(0);
     }
-  if (tc == 0) {
-#pragma unroll
-    for (int u = 0; u < kRootRU; u++) {
-      const int i = tr + kRootTR * u;
-      if (i < n) colbuf[0][i] = a[u][0];
-    }
-  }
-  __syncthreads();
+  if (tid < NMAX) pstep[tid] = NMAX;
+  // wave 0: bit q set once row lane + 64 q is a pivot row (rows past n8
+  // never are)
   uint32_t used = 0;
+#pragma unroll
+  for (int q = 0; q < NS; q++)
+    if (lane + 64 * q >= n8) used |= 1u << q;
   int info = 0;
-  for (int j = 0; j < n; j++) {
-    const R* col = colbuf[j & 1];
-    R* coln = colbuf[(j + 1) & 1];
-    double key = -1.0;
-    uint32_t cand = 0;
-    for (int uu = 0; lane + 64 * uu < n; uu++) {
-      if ((used >> uu) & 1) continue;
-      const int i = lane + 64 * uu;
-      cand |= 1u << uu;
-      const R v = fabs(col[i]);
-      if (v == v) key = fmax(key, pivot_key<R, 8>(v, i));
-    }
-    const double best = wave_max(key);
-    int p;
-    if (best >= 0) {
-      p = pivot_key_pos<8>(best);
-    } else {  // NaN column: the first row not yet pivoted
-      p = 0x7fffffff;
-      for (int uu = 0; uu < 4; uu++) {
-        const uint64_t m = __ballot((cand >> uu) & 1);
-        if (m && p == 0x7fffffff) p = 64 * uu + __ffsll((unsigned long long)m) - 1;
-      }
-    }
-    if ((p & 63) == lane) used |= 1u << (p >> 6);
-    if (tid == 0) prow[j] = p;
-    const R piv = col[p];
-    if (piv == R(0) && info == 0) info = j + 1;
-    const R inv = R(1) / piv;
-    if (tr == p % kRootTR) {
-      const int up = p / kRootTR;
-#pragma unroll
-      for (int u = 0; u < kRootRU; u++) {
-        if (u != up) continue;
-#pragma unroll
-        for (int v = 0; v < kRootCV; v++) {
-          const int c = tc + kRootTC * v;
-          if (c < n) rowbuf[c] = (c == j) ? R(1) : a[u][v];
-        }
-      }
-    }
-    __syncthreads();
-    R cv[kRootRU], rv[kRootCV];
-#pragma unroll
-    for (int u = 0; u < kRootRU; u++) {
-      const int i = tr + kRootTR * u;
-      cv[u] = (i < n) ? col[i] : R(0);
-    }
-#pragma unroll
-    for (int v = 0; v < kRootCV; v++) {
-      const int c = tc + kRootTC * v;
-      rv[v] = (c < n) ? rowbuf[c] * inv : R(0);
-    }
-    // Column j restarts from zero, then one FMA per entry; the pivot row
-    // is overwritten with the scaled pivot row.
-    if (tc == j % kRootTC) {
-      const int vj = j / kRootTC;
+  for (int j0 = 0, pb = 0; j0 < n8; j0 += PB, pb ^= 1) {
+    const int tc0 = j0 % kRootTC, vj = j0 / kRootTC;
+    // 1. the panel's columns to LDS
+    if (tc >= tc0 && tc < tc0 + PB) {
 #pragma unroll
       for (int v = 0; v < kRootCV; v++) {
         if (v != vj) continue;
 #pragma unroll
-        for (int u = 0; u < kRootRU; u++) a[u][v] = R(0);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kRootRU; u++)
-#pragma unroll
-      for (int v = 0; v < kRootCV; v++) a[u][v] -= cv[u] * rv[v];
-    if (tr == p % kRootTR) {
-      const int up = p / kRootTR;
-#pragma unroll
-      for (int u = 0; u < kRootRU; u++) {
-        if (u != up) continue;
-#pragma unroll
-        for (int v = 0; v < kRootCV; v++) a[u][v] = rv[v];
-      }
-    }
-    if (j + 1 < n && tc == (j + 1) % kRootTC) {
-      const int vn = (j + 1) / kRootTC;
-#pragma unroll
-      for (int v = 0; v < kRootCV; v++) {
-        if (v != vn) continue;
-#pragma unroll
-        for (int u = 0; u < kRootRU; u++) {
-          const int i = tr + kRootTR * u;
-          if (i < n) coln[i] = a[u][v];
-        }
+        for (int u = 0; u < kRootRU; u++) pan[pb][tc - tc0][tr + kRootTR * u] = a[u][v];
       }
     }
     __syncthreads();
+    // 2. wave 0 eliminates within the panel, its own entries parked in
+    // LDS meanwhile (the panel and the matrix do not fit its registers)
+    if (wave == 0) {
+#pragma unroll
+      for (int u = 0; u < kRootRU; u++)
+#pragma unroll
+        for (int v = 0; v < kRootCV; v++) park[u * kRootCV + v][lane] = a[u][v];
+      R x[NS][PB];
+#pragma unroll
+      for (int q = 0; q < NS; q++)
+#pragma unroll
+        for (int k = 0; k < PB; k++) x[q][k] = pan[pb][k][lane + 64 * q];
+#pragma unroll
+      for (int k = 0; k < PB; k++) {
+        double key = -1.0;
+        uint32_t cand = 0;
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+          if ((used >> q) & 1) continue;
+          cand |= 1u << q;
+          const R v = fabs(x[q][k]);
+          if (v == v) key = fmax(key, pivot_key<R, 8>(v, lane + 64 * q));
+        }
+        const double best = wave_max(key);
+        int p;
+        if (best >= 0) {
+          p = pivot_key_pos<8>(best);
+        } else {  // NaN column: the first row not yet pivoted
+          p = 0x7fffffff;
+#pragma unroll
+          for (int q = 0; q < NS; q++) {
+            const uint64_t m = __ballot((cand >> q) & 1);
+            if (m && p == 0x7fffffff) p = 64 * q + __ffsll((unsigned long long)m) - 1;
+          }
+        }
+        const int pq = p >> 6, pl = p & 63;
+        // the pivot row's panel entries, in every lane
+        R xp[PB];
+#pragma unroll
+        for (int k2 = 0; k2 < PB; k2++) {
+          R v = x[0][k2];
+#pragma unroll
+          for (int q = 1; q < NS; q++)
+            if (pq == q) v = x[q][k2];
+          xp[k2] = lane_value(v, pl);
+        }
+        const R piv = xp[k];
+        if (piv == R(0) && info == 0) info = j0 + k + 1;
+        const R inv = R(1) / piv;
+        if (lane == 0) {
+          pinv[k] = inv;
+          prow[j0 + k] = p;
+          pstep[p] = j0 + k;
+        }
+#pragma unroll
+        for (int q = 0; q < NS; q++) cpre[k][lane + 64 * q] = x[q][k];
+        // the pivot row's C: the steps before this one go to cpiv (they
+        // form R_k), cpre keeps only the steps after it
+        if (lane <= k) {
+          if (lane < k) cpiv[k][lane] = cpre[lane][p];
+          cpre[lane][p] = R(0);
+        }
+        R rv[PB];
+#pragma unroll
+        for (int k2 = 0; k2 < PB; k2++) rv[k2] = (k2 == k ? R(1) : xp[k2]) * inv;
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+          const R c = x[q][k];
+          x[q][k] = R(0);
+#pragma unroll
+          for (int k2 = 0; k2 < PB; k2++) x[q][k2] -= c * rv[k2];
+        }
+        if (lane == pl) {
+          used |= 1u << pq;
+#pragma unroll
+          for (int q = 0; q < NS; q++)
+            if (q == pq) {
+#pragma unroll
+              for (int k2 = 0; k2 < PB; k2++) x[q][k2] = rv[k2];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);  // one step's registers at a time
+      }
+#pragma unroll
+      for (int q = 0; q < NS; q++)
+#pragma unroll
+        for (int k = 0; k < PB; k++) pan[pb][k][lane + 64 * q] = x[q][k];
+#pragma unroll
+      for (int u = 0; u < kRootRU; u++)
+#pragma unroll
+        for (int v = 0; v < kRootCV; v++) a[u][v] = park[u * kRootCV + v][lane];
+    }
+    __syncthreads();
+    // 3. the wave's pivot rows, then R_k for the wave's columns
+    int kk[kRootRU];
+#pragma unroll
+    for (int u = 0; u < kRootRU; u++) {
+      const int q = pstep[tr + kRootTR * u] - j0;
+      kk[u] = (q >= 0 && q < PB) ? q : -1;
+      if (kk[u] >= 0) {
+#pragma unroll
+        for (int v = 0; v < kRootCV; v++) rowk[kk[u]][tc + kRootTC * v] = a[u][v];
+      }
+    }
+    wave_sync();
+    if (lane < 2 * kRootCV) {
+      const int c = (tc & ~1) + (lane & 1) + kRootTC * (lane >> 1);
+      R r[PB];
+#pragma unroll
+      for (int k = 0; k < PB; k++) {
+        R t = rowk[k][c];
+#pragma unroll
+        for (int m = 0; m < k; m++) t -= cpiv[k][m] * r[m];
+        r[k] = t * pinv[k];
+        rowk[k][c] = r[k];
+      }
+    }
+    wave_sync();
+    // 4. the panel's steps on the thread's entries
+#pragma unroll
+    for (int u = 0; u < kRootRU; u++) {
+      if (kk[u] < 0) continue;
+#pragma unroll
+      for (int v = 0; v < kRootCV; v++) a[u][v] = rowk[kk[u]][tc + kRootTC * v];
+    }
+#pragma unroll
+    for (int m = 0; m < PB; m++) {
+      R cv[kRootRU], rv[kRootCV];
+#pragma unroll
+      for (int u = 0; u < kRootRU; u++) cv[u] = cpre[m][tr + kRootTR * u];
+#pragma unroll
+      for (int v = 0; v < kRootCV; v++) rv[v] = rowk[m][tc + kRootTC * v];
+#pragma unroll
+      for (int u = 0; u < kRootRU; u++)
+#pragma unroll
+        for (int v = 0; v < kRootCV; v++) a[u][v] -= cv[u] * rv[v];
+    }
+    if (tc >= tc0 && tc < tc0 + PB) {
+#pragma unroll
+      for (int v = 0; v < kRootCV; v++) {
+        if (v != vj) continue;
+#pragma unroll
+        for (int u = 0; u < kRootRU; u++) a[u][v] = pan[pb][tc - tc0][tr + kRootTR * u];
+      }
+    }
   }
-  if (tid < n) invrow[prow[tid]] = tid;
+  __syncthreads();
+  if (tid < n8) invrow[prow[tid]] = tid;
   __syncthreads();
   R* B = A + (int64_t)n * n;
 #pragma unroll
@@ -2606,7 +2710,6 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
     }
   if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
 }
-
 // ---------------------------------------------------------------------------
 // Solve sweeps (skel_factorization.cpp:507-592).  One wave per box; the
 // box's skeleton and redundant entries are gathered to LDS, updated and
@@ -3104,7 +3207,8 @@ template <typename R>
 void launch_root(const DevGeom& g, const DevTree& t, const DevState& s,
                  int n_roots, bool factor_lu, hipStream_t st) {
   k_active<<<n_roots, 64, 0, st>>>(t, s, 0);
-  k_root<R, 1024><<<n_roots, 1024, 0, st>>>(g, t, s, factor_lu ? 1 : 0);
+  k_root<R, 256><<<dim3(n_roots, kRootSplit), 256, 0, st>>>(g, t, s);
+  k_root_big<R, 1024><<<n_roots, 1024, 0, st>>>(s, factor_lu ? 1 : 0);
   k_root_inv<R><<<n_roots, kRootTR * kRootTC, 0, st>>>(s, factor_lu ? 1 : 0);
   k_root_inv_blk<R><<<n_roots, kRootBlkMax, 0, st>>>(s, factor_lu ? 1 : 0);
 }
