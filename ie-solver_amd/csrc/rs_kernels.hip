@@ -2504,7 +2504,8 @@ template <typename R>
 __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int factor_lu) {
   constexpr int NMAX = kRootRegMax, PB = kRootPB, NS = NMAX / 64;
   static_assert(kRootTC % PB == 0 && NMAX == kRootTR * kRootRU && NMAX == kRootTC * kRootCV &&
-                NMAX % 64 == 0 && NMAX % PB == 0 && NMAX <= 256, "root tiling");
+                NMAX % 64 == 0 && NMAX % PB == 0 && NMAX <= 256 && PB * PB <= 64,
+                "root tiling");
   __shared__ R pan[2][PB][NMAX];   // panel columns, in and out of step 2
   __shared__ R cpre[PB][NMAX];     // column m before step m (zero where
                                    // row i is pivot at a step >= m)
@@ -2563,18 +2564,29 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
       for (int q = 0; q < NS; q++)
 #pragma unroll
         for (int k = 0; k < PB; k++) x[q][k] = pan[pb][k][lane + 64 * q];
-#pragma unroll
-      for (int k = 0; k < PB; k++) {
+      // pivot keys of column k over the rows not yet pivots (the next
+      // column's are formed at the end of each step, beside its updates)
+      uint32_t cand = 0;
+      auto column_best = [&](int k) {
+        // the lane index is formed here each time: kept across the steps,
+        // the rows' key bits were spilled
+        int l = lane;
+        __asm__ volatile("" : "+v"(l));
         double key = -1.0;
-        uint32_t cand = 0;
+        cand = 0;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
-          if ((used >> q) & 1) continue;
-          cand |= 1u << q;
+          const bool ok = !((used >> q) & 1);
+          cand |= (ok ? 1u : 0u) << q;
           const R v = fabs(x[q][k]);
-          if (v == v) key = fmax(key, pivot_key<R, 8>(v, lane + 64 * q));
+          const double kq = pivot_key<R, 8>(v, l + 64 * q);
+          key = fmax(key, (ok && v == v) ? kq : -1.0);
         }
-        const double best = wave_max(key);
+        return wave_max(key);
+      };
+      double best = column_best(0);
+#pragma unroll
+      for (int k = 0; k < PB; k++) {
         int p;
         if (best >= 0) {
           p = pivot_key_pos<8>(best);
@@ -2607,12 +2619,6 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
         }
 #pragma unroll
         for (int q = 0; q < NS; q++) cpre[k][lane + 64 * q] = x[q][k];
-        // the pivot row's C: the steps before this one go to cpiv (they
-        // form R_k), cpre keeps only the steps after it
-        if (lane <= k) {
-          if (lane < k) cpiv[k][lane] = cpre[lane][p];
-          cpre[lane][p] = R(0);
-        }
         R rv[PB];
 #pragma unroll
         for (int k2 = 0; k2 < PB; k2++) rv[k2] = (k2 == k ? R(1) : xp[k2]) * inv;
@@ -2632,12 +2638,24 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
               for (int k2 = 0; k2 < PB; k2++) x[q][k2] = rv[k2];
             }
         }
+        if (k + 1 < PB) best = column_best(k + 1);
         __builtin_amdgcn_sched_barrier(0);  // one step's registers at a time
       }
 #pragma unroll
       for (int q = 0; q < NS; q++)
 #pragma unroll
         for (int k = 0; k < PB; k++) pan[pb][k][lane + 64 * q] = x[q][k];
+      // C at the pivot rows, one (step k, step m <= k) pair per lane: the
+      // steps before a row's own step go to cpiv (they form R_k), cpre
+      // keeps only the steps after it
+      if (lane < PB * PB) {
+        const int k = lane / PB, m = lane % PB;
+        if (m <= k) {
+          const int pk = prow[j0 + k];
+          if (m < k) cpiv[k][m] = cpre[m][pk];
+          cpre[m][pk] = R(0);
+        }
+      }
 #pragma unroll
       for (int u = 0; u < kRootRU; u++)
 #pragma unroll
