@@ -2161,11 +2161,12 @@ __device__ void decouple_blocks(const R* P, const R* T, int n, int k, R* Krs, R*
 }
 
 // From M / prow of gj_invert on X_rr: U, L, X_rr^{-1} and the carried block
-// into the factor slot.  W has room for r k entries.
+// into the factor slot.  W has room for r k entries; uss, when given, holds
+// the children's updates on the skeleton block (k x k), else upd forms them.
 template <typename R, int NT, typename Upd>
 __device__ void decouple_tail(const R* M, const int* prow, const R* Krs, const R* Ksr,
                               R* W, int r, int k, R* F, const FacLayout& fl,
-                              const Upd& upd, const short* sperm) {
+                              const Upd& upd, const short* sperm, const R* uss = nullptr) {
   const int tid = threadIdx.x;
   for (int idx = tid; idx < r * k; idx += NT) {  // U = X^{-1} K_rs
     const int j = idx % r, a = idx / r;
@@ -2205,7 +2206,8 @@ __device__ void decouple_tail(const R* M, const int* prow, const R* Krs, const R
   __syncthreads();
   for (int idx = tid; idx < k * k; idx += NT) {  // carried block
     const int a = idx % k, c = idx / k;
-    F[fl.S + idx] = upd(sperm[a], sperm[c]) + dot2(Ksr + a, k, W + (int64_t)r * c, 1, r);
+    const R u = uss ? uss[idx] : upd(sperm[a], sperm[c]);
+    F[fl.S + idx] = u + dot2(Ksr + a, k, W + (int64_t)r * c, 1, r);
   }
 }
 
@@ -2300,6 +2302,8 @@ __global__ __launch_bounds__(256) void k_decouple_small(DevGeom g, DevTree t,
 // and in the box's scratch slot otherwise.  Few boxes of a level are
 // these, so a small grid walks the level (the others are skipped after
 // two loads) instead of one workgroup per box.
+constexpr int kDecGeoMax = 96;
+
 template <typename R, int NT>
 __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
                                                  DevState s, int start, int count) {
@@ -2308,6 +2312,7 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   constexpr int kLdsElems = 131072 / sizeof(R);
   __shared__ R sbuf[kLdsElems];
   __shared__ R gjcol[2 * 16 * 8];  // the register Gauss-Jordan's column buffers
+  __shared__ DofGeo geo[kDecGeoMax];  // the box's DOFs, permuted, when n fits
   __shared__ signed char pch[kMaxBoxDofs];
   __shared__ short ploc[kMaxBoxDofs];
   __shared__ short sperm[kMaxBoxDofs];
@@ -2349,7 +2354,12 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   const int r = n - k;
   const FacLayout fl = FacLayout::make(k, r);
   const int* act = s.act_list + (int64_t)b * s.ncap;
-  for (int i = tid; i < n; i += NT) sperm[i] = (short)s.perm[(int64_t)b * s.ncap + i];
+  const bool lgeo = n <= kDecGeoMax;
+  for (int i = tid; i < n; i += NT) {
+    const int q = s.perm[(int64_t)b * s.ncap + i];
+    sperm[i] = (short)q;
+    if (lgeo) geo[i] = dof_geo(g, act[q]);
+  }
   __syncthreads();
   // n^2 + r^2 + 2 r k <= 2 n^2 entries.
   R* P = (2 * (int64_t)n * n <= kLdsElems)
@@ -2358,12 +2368,25 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   R* X = P + (int64_t)n * n;
   R* Krs = X + (int64_t)r * r;
   R* Ksr = Krs + (int64_t)r * k;
+  // T (k x r) and the children's updates on the skeleton block (k x k)
+  // join them in LDS when they fit: the Schur products then read no
+  // global memory.
+  const bool stage = P == sbuf && (int64_t)n * n + (int64_t)r * r + 3 * (int64_t)r * k +
+                                          (int64_t)k * k <= kLdsElems;
+  R* Tl = Ksr + (int64_t)r * k;
+  R* Uss = stage ? Tl + (int64_t)r * k : nullptr;
+  if (stage)
+    for (int idx = tid; idx < r * k; idx += NT) Tl[idx] = F[fl.T + idx];
   for (int64_t idx = tid; idx < (int64_t)n * n; idx += NT) {
-    const int ia = sperm[idx % n], ic = sperm[idx / n];
-    P[idx] = static_cast<R>(kernel_entry(g, act[ia], act[ic])) - upd(ia, ic);
+    const int i = (int)(idx % n), c = (int)(idx / n);
+    const int ia = sperm[i], ic = sperm[c];
+    const R u = upd(ia, ic);
+    const double kv = lgeo ? kernel_entry(g.pde, geo[i], geo[c]) : kernel_entry(g, act[ia], act[ic]);
+    P[idx] = static_cast<R>(kv) - u;
+    if (Uss && i < k && c < k) Uss[i + k * c] = u;
   }
   __syncthreads();
-  decouple_blocks<R, NT>(P, F + fl.T, n, k, Krs, Ksr, X);
+  decouple_blocks<R, NT>(P, stage ? Tl : F + fl.T, n, k, Krs, Ksr, X);
   if (s.fwd_pool) {
     R* Xf = reinterpret_cast<R*>(s.fwd_pool) + s.fac_off[b];
     for (int64_t idx = tid; idx < (int64_t)r * r; idx += NT) Xf[idx] = X[idx];
@@ -2375,7 +2398,7 @@ __global__ __launch_bounds__(NT) void k_decouple(DevGeom g, DevTree t,
   else if (NT == 256 && r <= 128) info = gj_invert_regs<R, 8>(X, r, prow, gjcol);
   else info = gj_invert_into<R, NT>(X, P, r, prow);
   if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
-  decouple_tail<R, NT>(X, prow, Krs, Ksr, P, r, k, F, fl, upd, sperm);
+  decouple_tail<R, NT>(X, prow, Krs, Ksr, P, r, k, F, fl, upd, sperm, Uss);
   if (tid == 0) s.upd_kind[b] = 1;
   }
 }
