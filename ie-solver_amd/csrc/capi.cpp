@@ -584,13 +584,16 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
   hipStream_t st2 = split ? c->stream2 : st;
   // evs[1 + lv]: level lv's ID done (the decouple chain's first work waits
   // for the deepest level's, which follows everything enqueued before it);
-  // evs[n_levels + 1]: the decouple chain done.  The two chains share no
+  // evs[n_levels + 1]: the decouple chain done; evs[n_levels + 2] and
+  // evs[n_levels + 3]: level 1's decouple forked to the ID stream (idle
+  // from there to the root) for its boxes above 64 DOFs, and joined back,
+  // so its two kernels run side by side.  The two chains share no
   // scratch: the ID pool
   // (one half per level parity) belongs to the ID chain, the decouple pool
   // to the decouple chain.  The events order the two streams of one device
   // only, so they release at device scope (no system-scope cache
   // write-back: 0.05 ms per step at N = 262,144).
-  const size_t n_ev = (size_t)T.n_levels + 2;
+  const size_t n_ev = (size_t)T.n_levels + 4;
   if (split && c->evs.size() < n_ev) {
     for (size_t i = c->evs.size(); i < n_ev; i++) {
       hipEvent_t e;
@@ -611,9 +614,19 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
       HIP_TRY(hipEventRecord(c->evs[1 + lv], st));
       HIP_TRY(hipStreamWaitEvent(st2, c->evs[1 + lv], 0));
     }
+    const bool fork = split && lv == 1;
+    if (fork) {
+      HIP_TRY(hipEventRecord(c->evs[T.n_levels + 2], st2));
+      HIP_TRY(hipStreamWaitEvent(st, c->evs[T.n_levels + 2], 0));
+    }
     c->prof.run(kStageDecouple, lv, st2, [&] {
-      launch_factor_stage<R>(kStageDecouple, g, t, s, lv, start, count, tol, c->caps, st2);
+      launch_factor_stage<R>(kStageDecouple, g, t, s, lv, start, count, tol, c->caps, st2, 0,
+                             fork ? st : nullptr);
     });
+    if (fork) {
+      HIP_TRY(hipEventRecord(c->evs[T.n_levels + 3], st));
+      HIP_TRY(hipStreamWaitEvent(st2, c->evs[T.n_levels + 3], 0));
+    }
     if (fx)
       c->prof.run(kProfSweepUp, lv, st2,
                   [&] { launch_sweep_up<R>(s, start, count, fv, false, true, st2); });

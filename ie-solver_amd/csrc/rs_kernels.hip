@@ -3151,7 +3151,8 @@ void launch_rid_wide(const DevGeom& g, const DevTree& t, const DevState& s, int 
 template <typename R>
 void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
                          const DevState& s, int level, int start, int count,
-                         double tol, const LaunchCaps& caps, hipStream_t st, int skip) {
+                         double tol, const LaunchCaps& caps, hipStream_t st, int skip,
+                         hipStream_t st_big) {
   if (count <= 0) return;
   switch (stage) {
     case kStageActive:
@@ -3237,7 +3238,9 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       } else {
         k_decouple_small<R, 0, kSmallBox><<<count, 256, 0, st>>>(g, t, s, start);
       }
-      k_decouple<R, 256><<<min(count, 512), 256, 0, st>>>(g, t, s, start, count);
+      // the boxes above the small-box bound on st_big when given (their
+      // own stream: the two kernels touch different boxes)
+      k_decouple<R, 256><<<min(count, 512), 256, 0, st_big ? st_big : st>>>(g, t, s, start, count);
       break;
     default:
       break;
@@ -3296,7 +3299,8 @@ void launch_fwd_down(const DevState& s, int start, int count, double* x, hipStre
 #define IEGPU_INSTANTIATE(R)                                                   \
   template void launch_factor_stage<R>(int, const DevGeom&, const DevTree&,   \
                                        const DevState&, int, int, int, double, \
-                                       const LaunchCaps&, hipStream_t, int);   \
+                                       const LaunchCaps&, hipStream_t, int,    \
+                                       hipStream_t);                           \
   template void launch_root<R>(const DevGeom&, const DevTree&,                \
                                const DevState&, int, bool, hipStream_t);       \
   template void launch_sweep_up<R>(const DevState&, int, int, SolveVec, bool, \

@@ -55,12 +55,13 @@ struct LaunchCaps {
 constexpr int kNumCaps = 8;
 
 // `skip`: kSkip* bits of kernels not to launch (their lists are empty by
-// the launch plan, which k_root verifies).
+// the launch plan, which k_root verifies).  `st_big`: the decouple stage's
+// boxes above 64 DOFs go on this stream when given (the caller orders it).
 template <typename R>
 void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
                          const DevState& s, int level, int start, int count,
                          double tol, const LaunchCaps& caps, hipStream_t st,
-                         int skip = 0);
+                         int skip = 0, hipStream_t st_big = nullptr);
 
 // Decouple of one level's compressed boxes with |active| <= 64, one wave
 // per box (rs_sweep.hip); the other boxes exit at once.
