@@ -2504,22 +2504,24 @@ __global__ __launch_bounds__(NT) void k_root_big(DevState s, int factor_lu) {
 // The block is bordered by the identity up to n8, a multiple of kRootPB
 // (its pivots stay in the block: border rows hold zeros in its columns and
 // lose every tie to a block row), and rows and columns past n8 are zero, so
-// no step or entry needs a bound.  Per panel:
-//  1. the panel's owners copy its columns to LDS;
-//  2. wave 0 runs the panel's pivot steps on those columns alone (they
-//     depend on nothing else): pivot search as in gj_invert, scaling,
-//     elimination; it records each step's pivot row p_k, 1 / pivot and the
-//     column as it was before the step (C[m][i]);
-//  3. in every wave, the owners of the pivot rows copy them to LDS and one
+// no step or entry needs a bound.
+//
+// Wave 0 runs each panel's pivot steps on the panel's columns alone (they
+// depend on nothing else): pivot search as in gj_invert, scaling,
+// elimination; it records each step's pivot row p_k, 1 / pivot and the
+// column as it was before the step (C[m][i]).  Applying panel P:
+//  a. in every wave, the owners of its pivot rows copy them to LDS and one
 //     lane per column forms the scaled pivot rows the steps would have
 //     made, R_k = (row p_k - sum_{m<k} C[m][p_k] R_m) / pivot_k;
-//  4. every thread applies the steps to its entries in step order,
+//  b. every thread applies the steps to its entries in step order,
 //     a -= C[m][i] R_m, a pivot row restarting from R_k at its own step;
 //     panel entries take wave 0's result.
-// Per entry these are the operations of the unblocked elimination in its
-// order, so the inverse is the same; two barriers per panel instead of two
-// per column, and one wave (not all sixteen) searches each pivot.  The
-// inverse, unpermuted, goes after the assembled matrix.
+// With one step of lookahead: after (a), the owners of panel P+1's columns
+// apply P to those columns alone and publish them; after a barrier wave 0
+// eliminates panel P+1 while every wave (wave 0 after it) applies P to the
+// rest of its entries; a barrier ends the panel.  Per entry these are the
+// operations of the unblocked elimination in its order, so the inverse is
+// the same.  The inverse, unpermuted, goes after the assembled matrix.
 constexpr int kRootTR = 32, kRootTC = 32, kRootRU = 6, kRootCV = 6;
 constexpr int kRootPB = 8;
 
@@ -2529,12 +2531,12 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
   static_assert(kRootTC % PB == 0 && NMAX == kRootTR * kRootRU && NMAX == kRootTC * kRootCV &&
                 NMAX % 64 == 0 && NMAX % PB == 0 && NMAX <= 256 && PB * PB <= 64,
                 "root tiling");
-  __shared__ R pan[2][PB][NMAX];   // panel columns, in and out of step 2
-  __shared__ R cpre[PB][NMAX];     // column m before step m (zero where
-                                   // row i is pivot at a step >= m)
-  __shared__ R cpiv[PB][PB];       // C[m][p_k], m < k
-  __shared__ R rowk[PB][NMAX];     // pivot rows, then R_k
-  __shared__ R pinv[PB];
+  __shared__ R pan[2][PB][NMAX];   // panel columns, in and out of the elimination
+  __shared__ R cpre[2][PB][NMAX];  // column m before step m (zero where row i
+                                   // is pivot at a step >= m); by panel parity
+  __shared__ R cpiv[2][PB][PB];    // C[m][p_k], m < k
+  __shared__ R pinv[2][PB];
+  __shared__ R rowk[PB][NMAX];     // pivot rows, then R_k (each wave its columns)
   __shared__ int pstep[NMAX];      // elimination step that pivots on row i
   __shared__ int prow[NMAX];
   __shared__ int invrow[NMAX];
@@ -2563,129 +2565,138 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
   for (int q = 0; q < NS; q++)
     if (lane + 64 * q >= n8) used |= 1u << q;
   int info = 0;
-  for (int j0 = 0, pb = 0; j0 < n8; j0 += PB, pb ^= 1) {
-    const int tc0 = j0 % kRootTC, vj = j0 / kRootTC;
-    // 1. the panel's columns to LDS
-    if (tc >= tc0 && tc < tc0 + PB) {
+
+  // The thread owns panel j0's column in its entries a[.][vj] when
+  // tc in [tc0, tc0 + PB) (the same for both halves of a wave).
+  auto owns = [&](int j0) { return tc >= j0 % kRootTC && tc < j0 % kRootTC + PB; };
+  auto publish = [&](int j0, int pb) {
+    const int vj = j0 / kRootTC;
 #pragma unroll
-      for (int v = 0; v < kRootCV; v++) {
-        if (v != vj) continue;
+    for (int v = 0; v < kRootCV; v++) {
+      if (v != vj) continue;
 #pragma unroll
-        for (int u = 0; u < kRootRU; u++) pan[pb][tc - tc0][tr + kRootTR * u] = a[u][v];
-      }
+      for (int u = 0; u < kRootRU; u++) pan[pb][tc - j0 % kRootTC][tr + kRootTR * u] = a[u][v];
     }
-    __syncthreads();
-    // 2. wave 0 eliminates within the panel, its own entries parked in
-    // LDS meanwhile (the panel and the matrix do not fit its registers)
-    if (wave == 0) {
+  };
+  // wave 0, its own entries parked in LDS meanwhile (the panel and the
+  // matrix do not fit its registers)
+  auto eliminate = [&](int j0, int pb) {
 #pragma unroll
-      for (int u = 0; u < kRootRU; u++)
+    for (int u = 0; u < kRootRU; u++)
 #pragma unroll
-        for (int v = 0; v < kRootCV; v++) park[u * kRootCV + v][lane] = a[u][v];
-      R x[NS][PB];
+      for (int v = 0; v < kRootCV; v++) park[u * kRootCV + v][lane] = a[u][v];
+    R x[NS][PB];
 #pragma unroll
-      for (int q = 0; q < NS; q++)
+    for (int q = 0; q < NS; q++)
 #pragma unroll
-        for (int k = 0; k < PB; k++) x[q][k] = pan[pb][k][lane + 64 * q];
-      // pivot keys of column k over the rows not yet pivots (the next
-      // column's are formed at the end of each step, beside its updates)
-      uint32_t cand = 0;
-      auto column_best = [&](int k) {
-        // the lane index is formed here each time: kept across the steps,
-        // the rows' key bits were spilled
-        int l = lane;
-        __asm__ volatile("" : "+v"(l));
-        double key = -1.0;
-        cand = 0;
+      for (int k = 0; k < PB; k++) x[q][k] = pan[pb][k][lane + 64 * q];
+    // pivot keys of column k over the rows not yet pivots (the next
+    // column's are formed at the end of each step, beside its updates)
+    uint32_t cand = 0;
+    auto column_best = [&](int k) {
+      // the lane index is formed here each time: kept across the steps,
+      // the rows' key bits were spilled
+      int l = lane;
+      __asm__ volatile("" : "+v"(l));
+      double key = -1.0;
+      cand = 0;
+#pragma unroll
+      for (int q = 0; q < NS; q++) {
+        const bool ok = !((used >> q) & 1);
+        cand |= (ok ? 1u : 0u) << q;
+        const R v = fabs(x[q][k]);
+        const double kq = pivot_key<R, 8>(v, l + 64 * q);
+        key = fmax(key, (ok && v == v) ? kq : -1.0);
+      }
+      return wave_max(key);
+    };
+    double best = column_best(0);
+#pragma unroll
+    for (int k = 0; k < PB; k++) {
+      int p;
+      if (best >= 0) {
+        p = pivot_key_pos<8>(best);
+      } else {  // NaN column: the first row not yet pivoted
+        p = 0x7fffffff;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
-          const bool ok = !((used >> q) & 1);
-          cand |= (ok ? 1u : 0u) << q;
-          const R v = fabs(x[q][k]);
-          const double kq = pivot_key<R, 8>(v, l + 64 * q);
-          key = fmax(key, (ok && v == v) ? kq : -1.0);
+          const uint64_t m = __ballot((cand >> q) & 1);
+          if (m && p == 0x7fffffff) p = 64 * q + __ffsll((unsigned long long)m) - 1;
         }
-        return wave_max(key);
-      };
-      double best = column_best(0);
+      }
+      const int pq = p >> 6, pl = p & 63;
+      // the pivot row's panel entries, in every lane
+      R xp[PB];
 #pragma unroll
-      for (int k = 0; k < PB; k++) {
-        int p;
-        if (best >= 0) {
-          p = pivot_key_pos<8>(best);
-        } else {  // NaN column: the first row not yet pivoted
-          p = 0x7fffffff;
+      for (int k2 = 0; k2 < PB; k2++) {
+        R v = x[0][k2];
 #pragma unroll
-          for (int q = 0; q < NS; q++) {
-            const uint64_t m = __ballot((cand >> q) & 1);
-            if (m && p == 0x7fffffff) p = 64 * q + __ffsll((unsigned long long)m) - 1;
+        for (int q = 1; q < NS; q++)
+          if (pq == q) v = x[q][k2];
+        xp[k2] = lane_value(v, pl);
+      }
+      const R piv = xp[k];
+      if (piv == R(0) && info == 0) info = j0 + k + 1;
+      const R inv = R(1) / piv;
+      if (lane == 0) {
+        pinv[pb][k] = inv;
+        prow[j0 + k] = p;
+        pstep[p] = j0 + k;
+      }
+#pragma unroll
+      for (int q = 0; q < NS; q++) cpre[pb][k][lane + 64 * q] = x[q][k];
+      R rv[PB];
+#pragma unroll
+      for (int k2 = 0; k2 < PB; k2++) rv[k2] = (k2 == k ? R(1) : xp[k2]) * inv;
+#pragma unroll
+      for (int q = 0; q < NS; q++) {
+        const R c = x[q][k];
+        x[q][k] = R(0);
+#pragma unroll
+        for (int k2 = 0; k2 < PB; k2++) x[q][k2] -= c * rv[k2];
+      }
+      if (lane == pl) {
+        used |= 1u << pq;
+#pragma unroll
+        for (int q = 0; q < NS; q++)
+          if (q == pq) {
+#pragma unroll
+            for (int k2 = 0; k2 < PB; k2++) x[q][k2] = rv[k2];
           }
-        }
-        const int pq = p >> 6, pl = p & 63;
-        // the pivot row's panel entries, in every lane
-        R xp[PB];
-#pragma unroll
-        for (int k2 = 0; k2 < PB; k2++) {
-          R v = x[0][k2];
-#pragma unroll
-          for (int q = 1; q < NS; q++)
-            if (pq == q) v = x[q][k2];
-          xp[k2] = lane_value(v, pl);
-        }
-        const R piv = xp[k];
-        if (piv == R(0) && info == 0) info = j0 + k + 1;
-        const R inv = R(1) / piv;
-        if (lane == 0) {
-          pinv[k] = inv;
-          prow[j0 + k] = p;
-          pstep[p] = j0 + k;
-        }
-#pragma unroll
-        for (int q = 0; q < NS; q++) cpre[k][lane + 64 * q] = x[q][k];
-        R rv[PB];
-#pragma unroll
-        for (int k2 = 0; k2 < PB; k2++) rv[k2] = (k2 == k ? R(1) : xp[k2]) * inv;
-#pragma unroll
-        for (int q = 0; q < NS; q++) {
-          const R c = x[q][k];
-          x[q][k] = R(0);
-#pragma unroll
-          for (int k2 = 0; k2 < PB; k2++) x[q][k2] -= c * rv[k2];
-        }
-        if (lane == pl) {
-          used |= 1u << pq;
-#pragma unroll
-          for (int q = 0; q < NS; q++)
-            if (q == pq) {
-#pragma unroll
-              for (int k2 = 0; k2 < PB; k2++) x[q][k2] = rv[k2];
-            }
-        }
-        if (k + 1 < PB) best = column_best(k + 1);
-        __builtin_amdgcn_sched_barrier(0);  // one step's registers at a time
       }
-#pragma unroll
-      for (int q = 0; q < NS; q++)
-#pragma unroll
-        for (int k = 0; k < PB; k++) pan[pb][k][lane + 64 * q] = x[q][k];
-      // C at the pivot rows, one (step k, step m <= k) pair per lane: the
-      // steps before a row's own step go to cpiv (they form R_k), cpre
-      // keeps only the steps after it
-      if (lane < PB * PB) {
-        const int k = lane / PB, m = lane % PB;
-        if (m <= k) {
-          const int pk = prow[j0 + k];
-          if (m < k) cpiv[k][m] = cpre[m][pk];
-          cpre[m][pk] = R(0);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kRootRU; u++)
-#pragma unroll
-        for (int v = 0; v < kRootCV; v++) a[u][v] = park[u * kRootCV + v][lane];
+      if (k + 1 < PB) best = column_best(k + 1);
+      __builtin_amdgcn_sched_barrier(0);  // one step's registers at a time
     }
-    __syncthreads();
-    // 3. the wave's pivot rows, then R_k for the wave's columns
+#pragma unroll
+    for (int q = 0; q < NS; q++)
+#pragma unroll
+      for (int k = 0; k < PB; k++) pan[pb][k][lane + 64 * q] = x[q][k];
+    // C at the pivot rows, one (step k, step m <= k) pair per lane: the
+    // steps before a row's own step go to cpiv (they form R_k), cpre
+    // keeps only the steps after it
+    if (lane < PB * PB) {
+      const int k = lane / PB, m = lane % PB;
+      if (m <= k) {
+        const int pk = prow[j0 + k];
+        if (m < k) cpiv[pb][k][m] = cpre[pb][m][pk];
+        cpre[pb][m][pk] = R(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRootRU; u++)
+#pragma unroll
+      for (int v = 0; v < kRootCV; v++) a[u][v] = park[u * kRootCV + v][lane];
+  };
+
+  // prologue: panel 0 eliminated
+  if (owns(0)) publish(0, 0);
+  __syncthreads();
+  if (wave == 0) eliminate(0, 0);
+  __syncthreads();
+  for (int j0 = 0, pb = 0; j0 < n8; j0 += PB, pb ^= 1) {
+    const int j1 = j0 + PB, pn = pb ^ 1;
+    const bool nxt = j1 < n8;
+    // a. the wave's pivot rows of panel j0, then R_k for the wave's columns
     int kk[kRootRU];
 #pragma unroll
     for (int u = 0; u < kRootRU; u++) {
@@ -2704,13 +2715,42 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
       for (int k = 0; k < PB; k++) {
         R t = rowk[k][c];
 #pragma unroll
-        for (int m = 0; m < k; m++) t -= cpiv[k][m] * r[m];
-        r[k] = t * pinv[k];
+        for (int m = 0; m < k; m++) t -= cpiv[pb][k][m] * r[m];
+        r[k] = t * pinv[pb][k];
         rowk[k][c] = r[k];
       }
     }
     wave_sync();
-    // 4. the panel's steps on the thread's entries
+    // lookahead: panel j1's columns take panel j0's steps (as in b, which
+    // repeats them on the thread's entries) and are published
+    if (nxt && owns(j1)) {
+      const int v1 = j1 / kRootTC;
+#pragma unroll
+      for (int v = 0; v < kRootCV; v++) {
+        if (v != v1) continue;
+        const int c = tc + kRootTC * v;
+        R col[kRootRU];
+#pragma unroll
+        for (int u = 0; u < kRootRU; u++) col[u] = kk[u] >= 0 ? rowk[kk[u]][c] : a[u][v];
+#pragma unroll
+        for (int m = 0; m < PB; m++) {
+          const R rv = rowk[m][c];
+#pragma unroll
+          for (int u = 0; u < kRootRU; u++) col[u] -= cpre[pb][m][tr + kRootTR * u] * rv;
+        }
+#pragma unroll
+        for (int u = 0; u < kRootRU; u++) pan[pn][tc - j1 % kRootTC][tr + kRootTR * u] = col[u];
+      }
+    }
+    __syncthreads();
+    if (wave == 0 && nxt) eliminate(j1, pn);
+    // b. panel j0's steps on the thread's entries (the pivot steps of panel
+    // j1 are after j0's in pstep: kk formed again, not kept across)
+#pragma unroll
+    for (int u = 0; u < kRootRU; u++) {
+      const int q = pstep[tr + kRootTR * u] - j0;
+      kk[u] = (q >= 0 && q < PB) ? q : -1;
+    }
 #pragma unroll
     for (int u = 0; u < kRootRU; u++) {
       if (kk[u] < 0) continue;
@@ -2721,7 +2761,7 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
     for (int m = 0; m < PB; m++) {
       R cv[kRootRU], rv[kRootCV];
 #pragma unroll
-      for (int u = 0; u < kRootRU; u++) cv[u] = cpre[m][tr + kRootTR * u];
+      for (int u = 0; u < kRootRU; u++) cv[u] = cpre[pb][m][tr + kRootTR * u];
 #pragma unroll
       for (int v = 0; v < kRootCV; v++) rv[v] = rowk[m][tc + kRootTC * v];
 #pragma unroll
@@ -2729,16 +2769,17 @@ __global__ __launch_bounds__(kRootTR * kRootTC) void k_root_inv(DevState s, int 
 #pragma unroll
         for (int v = 0; v < kRootCV; v++) a[u][v] -= cv[u] * rv[v];
     }
-    if (tc >= tc0 && tc < tc0 + PB) {
+    if (owns(j0)) {
+      const int vj = j0 / kRootTC;
 #pragma unroll
       for (int v = 0; v < kRootCV; v++) {
         if (v != vj) continue;
 #pragma unroll
-        for (int u = 0; u < kRootRU; u++) a[u][v] = pan[pb][tc - tc0][tr + kRootTR * u];
+        for (int u = 0; u < kRootRU; u++) a[u][v] = pan[pb][tc - j0 % kRootTC][tr + kRootTR * u];
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
   if (tid < n8) invrow[prow[tid]] = tid;
   __syncthreads();
   R* B = A + (int64_t)n * n;
