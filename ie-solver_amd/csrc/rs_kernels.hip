@@ -2476,27 +2476,6 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s) {
   }
 }
 
-// Gauss-Jordan inverse (the reference LU-factors the root block) of root
-// blocks too large for k_root_inv and k_root_inv_blk; the inverse,
-// unpermuted, goes after the assembled matrix.
-template <typename R, int NT>
-__global__ __launch_bounds__(NT) void k_root_big(DevState s, int factor_lu) {
-  const int b = blockIdx.x, tid = threadIdx.x;
-  if (!factor_lu || !pools_ok(s)) return;
-  const int n = s.act_cnt[b];
-  if (n > kRootBlkMax) {
-    R* A = root_block<R>(s, b);
-    R* B = A + (int64_t)n * n;
-    int* prow = root_pivots(s, b);
-    const int info = gj_invert_into<R, NT>(A, B, n, prow);
-    if (info != 0 && tid == 0) atomicOr(s.err, kErrSingular);
-    for (int64_t idx = tid; idx < (int64_t)n * n; idx += NT) {
-      const int j = (int)(idx % n), jj = (int)(idx / n);
-      B[j + (int64_t)n * prow[jj]] = A[prow[j] + (int64_t)n * jj];
-    }
-  }
-}
-
 // Gauss-Jordan inverse of the root block with the matrix in registers,
 // blocked over panels of kRootPB columns: 1024 threads on a 32 x 32 grid,
 // thread (tr, tc) holding rows tr + 32 u and columns tc + 32 v (n <= 192;
@@ -2941,7 +2920,22 @@ __global__ __launch_bounds__(kRootBlkMax) void k_root_inv_blk(DevState s, int fa
   if (!factor_lu || !pools_ok(s)) return;
   const int root = blockIdx.x;
   const int n = s.act_cnt[root];
-  if (n <= kRootRegMax || n > kRootBlkMax) return;
+  if (n <= kRootRegMax) return;
+  if (n > kRootBlkMax) {
+    // Gauss-Jordan inverse (the reference LU-factors the root block) in
+    // global memory for blocks too large for the register forms; the
+    // inverse, unpermuted, goes after the assembled matrix
+    R* A = root_block<R>(s, root);
+    R* B = A + (int64_t)n * n;
+    int* gp = root_pivots(s, root);
+    const int info = gj_invert_into<R, kRootBlkMax>(A, B, n, gp);
+    if (info != 0 && threadIdx.x == 0) atomicOr(s.err, kErrSingular);
+    for (int64_t idx = threadIdx.x; idx < (int64_t)n * n; idx += kRootBlkMax) {
+      const int j = (int)(idx % n), jj = (int)(idx / n);
+      B[j + (int64_t)n * gp[jj]] = A[gp[j] + (int64_t)n * jj];
+    }
+    return;
+  }
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int i = tid;  // own row
   const bool live = i < n;
@@ -3293,7 +3287,6 @@ void launch_root(const DevGeom& g, const DevTree& t, const DevState& s,
                  int n_roots, bool factor_lu, hipStream_t st) {
   k_active<<<n_roots, 64, 0, st>>>(t, s, 0);
   k_root<R, 256><<<dim3(n_roots, kRootSplit), 256, 0, st>>>(g, t, s);
-  k_root_big<R, 1024><<<n_roots, 1024, 0, st>>>(s, factor_lu ? 1 : 0);
   k_root_inv<R><<<n_roots, kRootTR * kRootTC, 0, st>>>(s, factor_lu ? 1 : 0);
   k_root_inv_blk<R><<<n_roots, kRootBlkMax, 0, st>>>(s, factor_lu ? 1 : 0);
 }
