@@ -3274,8 +3274,13 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
         k_decouple_small<R, 0, kSmallBox><<<count, 256, 0, st>>>(g, t, s, start);
       }
       // the boxes above the small-box bound on st_big when given (their
-      // own stream: the two kernels touch different boxes)
-      k_decouple<R, 256><<<min(count, 512), 256, 0, st_big ? st_big : st>>>(g, t, s, start, count);
+      // own stream: the two kernels touch different boxes), there with
+      // 1024 threads (level 1's two 84-DOF boxes: the P fill 47k -> 16k
+      // cycles, 12 % less in all)
+      if (st_big)
+        k_decouple<R, 1024><<<min(count, 512), 1024, 0, st_big>>>(g, t, s, start, count);
+      else
+        k_decouple<R, 256><<<min(count, 512), 256, 0, st>>>(g, t, s, start, count);
       break;
     default:
       break;
