@@ -233,7 +233,7 @@ struct iegpu_ctx {
       s_red_list, s_perm, s_rank, s_upd_kind, s_m_rows, s_row_pool, s_err,
       s_root_piv, s_qkind, s_jobs, s_njobs, s_lists, s_lcnt, s_sk_w, s_rid_done;
   DevBuf<int64_t> s_id_sz, s_row_sz, s_fac_sz, s_id_off, s_row_off, s_fac_off,
-      s_fac_top, s_dec_sz, s_dec_off;
+      s_fac_top, s_dec_sz, s_dec_off, s_scan_blk;
   DevBuf<unsigned char> s_id_pool, s_fac_pool, s_root_mat, s_dec_pool;
   // X_rr and root blocks kept for the compressed forward apply
   bool keep_operator = false;
@@ -394,6 +394,7 @@ struct iegpu_ctx {
     s.row_cap = row_cap;
     s.fac_cap = fac_cap;
     s.fac_top = s_fac_top.p;
+    s.scan_blk = s_scan_blk.p;
     s.err = s_err.p;
     s.root_mat = s_root_mat.p;
     s.root_piv = s_root_piv.p;
@@ -539,6 +540,7 @@ void alloc_state(iegpu_ctx* c) {
   c->s_lcnt.alloc((size_t)2 * kMaxLevels * kNumLists);
   c->s_jobs.alloc((size_t)2 * kMaxChunks * nb);
   c->s_njobs.alloc(kMaxLevels);
+  c->s_scan_blk.alloc(((size_t)nb / 256 + 2) * 16);
   c->s_id_sz.alloc(nb);
   c->s_row_sz.alloc(nb);
   c->s_fac_sz.alloc(nb);

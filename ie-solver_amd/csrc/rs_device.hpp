@@ -182,6 +182,8 @@ struct DevState {
   int64_t row_cap;
   int64_t fac_cap;
   int64_t* fac_top;    // running factor-pool offset across levels
+  int64_t* scan_blk;   // a level's per-block scan totals (16 words per 256
+                       // boxes, then the factor-pool base)
   int* err;            // [0] error bits, [1] / [2] / [3] decouple / ID / row
                        // pool size a level needed on overflow (1024 elements)
   void* root_mat;      // per root: 2 ncap^2 elements (matrix + inverse)

@@ -274,50 +274,119 @@ __global__ __launch_bounds__(64) void k_rows(DevGeom g, DevTree t, DevState s,
   }
 }
 
-// Exclusive scans of the per-box sizes of one level (single workgroup).
-// ID scratch and row lists are level-local; factor slots append to the
-// running pool offset.
-// With the randomized ID, a register-QRCP box that gets no sketch (not
-// eligible, or sent straight to the full QRCP by the rank model) starts the
-// level's full-QRCP list; k_rid appends the boxes whose sketch fails.
+// A register-QRCP box the randomized ID does not sketch goes to the full
+// QRCP list at the scan.
 __device__ inline bool rid_full_qrcp(const DevState& s, int b, int kind) {
   return s.rid_mode != 0 && (kind == kQrcpReg || kind == kQrcpStack) && s.sk_w[b] == 0;
 }
 
-__global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
-                                               int count, int level) {
-  // Scanned per box: ID scratch, row-list and factor sizes, register-QRCP
-  // and general-QRCP list entries, row-compression jobs.
-  constexpr int NQ = 12;
-  __shared__ int64_t wtot[NQ][16];  // per-wave totals, then their exclusive scan
-  __shared__ int64_t tot[NQ];
-  const int tid = threadIdx.x;
-  const int chunk = (count + 1023) / 1024;
-  const int lo = min(count, tid * chunk), hi = min(count, lo + chunk);
-  int64_t own[NQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int i = lo; i < hi; i++) {
-    const int b = start + i;
-    if (box_clean(s, b)) continue;
-    const int kind = s.qkind[b];
-    own[0] += s.id_sz[b];
-    own[1] += s.row_sz[b];
-    own[2] += max(s.fac_sz[b], int64_t(0));
-    own[3] += ((kind == kQrcpReg || kind == kQrcpStack) && s.act_cnt[b] > kHalfCols) ? 1 : 0;
-    own[7] += ((kind == kQrcpReg || kind == kQrcpStack) && s.act_cnt[b] <= kHalfCols) ? 1 : 0;
-    own[4] += (kind == kQrcpLds) ? 1 : 0;
-    own[5] += row_jobs(s, b);
-    own[6] += (kind == kQrcpReg8 || kind == kQrcpStack8) ? 1 : 0;
-    own[8] += (s.sk_w[b] > 0) ? 1 : 0;
-    own[9] += rid_full_qrcp(s, b, kind) ? 1 : 0;
-    own[10] += s.dec_sz[b];
-    own[11] += (s.sk_w[b] == kRidRowsW) ? 1 : 0;
+// Scanned per box: ID scratch, row-list and factor sizes, register-QRCP
+// and general-QRCP list entries, row-compression jobs, sketches, decouple
+// operands.
+constexpr int kScanQ = 12;
+constexpr int kScanLd = 16;      // words per block in scan_blk
+constexpr int kScanMulti = 2048; // levels of at least this many boxes: three kernels
+
+__device__ inline void scan_counts(const DevState& s, int b, int64_t (&own)[kScanQ]) {
+  if (box_clean(s, b)) return;
+  const int kind = s.qkind[b];
+  own[0] += s.id_sz[b];
+  own[1] += s.row_sz[b];
+  own[2] += max(s.fac_sz[b], int64_t(0));
+  own[3] += ((kind == kQrcpReg || kind == kQrcpStack) && s.act_cnt[b] > kHalfCols) ? 1 : 0;
+  own[7] += ((kind == kQrcpReg || kind == kQrcpStack) && s.act_cnt[b] <= kHalfCols) ? 1 : 0;
+  own[4] += (kind == kQrcpLds) ? 1 : 0;
+  own[5] += row_jobs(s, b);
+  own[6] += (kind == kQrcpReg8 || kind == kQrcpStack8) ? 1 : 0;
+  own[8] += (s.sk_w[b] > 0) ? 1 : 0;
+  own[9] += rid_full_qrcp(s, b, kind) ? 1 : 0;
+  own[10] += s.dec_sz[b];
+  own[11] += (s.sk_w[b] == kRidRowsW) ? 1 : 0;
+}
+
+// Box b's slots and list entries from the exclusive offsets off (advanced
+// past the box).
+__device__ inline void scan_place(const DevState& s, int b, int start, int level,
+                                  int64_t fbase, int64_t (&off)[kScanQ]) {
+  if (box_clean(s, b)) return;
+  int* lreg = s.lists + (int64_t)kLstReg * s.nbox + start;
+  int* llds = s.lists + (int64_t)kLstLds * s.nbox + start;
+  int* lreg8 = s.lists + (int64_t)kLstReg8 * s.nbox + start;
+  int* lhalf = s.lists + (int64_t)kLstRegHalf * s.nbox + start;
+  int* lrid = s.lists + (int64_t)kLstRid * s.nbox + start;
+  int* lfb = s.lists + (int64_t)kLstFallback * s.nbox + start;
+  int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
+  const int kind = s.qkind[b];
+  // Scratch of odd and even levels in separate halves: a level's ID runs
+  // while the level below it is still being decoupled.
+  s.id_off[b] = (level & 1) * s.id_cap + off[0];
+  s.row_off[b] = off[1];
+  if (s.fac_sz[b] < 0) {  // keeps its slot
+    s.fac_sz[b] = -s.fac_sz[b];
+  } else {
+    s.fac_off[b] = fbase + off[2];
+    off[2] += s.fac_sz[b];
   }
-  // Exclusive scan over the 1024 threads: inclusive scans within each wave
-  // (shuffles), then a scan of the 16 wave totals.
-  const int lane = tid & 63, wv = tid >> 6;
-  int64_t inc[NQ];
+  s.dec_off[b] = off[10];
+  off[10] += s.dec_sz[b];
+  off[0] += s.id_sz[b];
+  off[1] += s.row_sz[b];
+  if (kind == kQrcpReg || kind == kQrcpStack) {
+    if (s.act_cnt[b] > kHalfCols) lreg[off[3]++] = b;
+    else lhalf[off[7]++] = b;
+  }
+  if (kind == kQrcpLds) llds[off[4]++] = b;
+  if (kind == kQrcpReg8 || kind == kQrcpStack8) lreg8[off[6]++] = b;
+  if (s.sk_w[b] > 0) lrid[off[8]++] = b;
+  if (rid_full_qrcp(s, b, kind)) lfb[off[9]++] = b;
+  const int nch = row_jobs(s, b);
+  for (int q = 0; q < nch; q++) {
+    jobs[2 * off[5]] = b;
+    jobs[2 * off[5] + 1] = q;
+    off[5]++;
+  }
+}
+
+// The level's list lengths, pool checks and the factor-pool top.
+__device__ inline void scan_finish(const DevState& s, int level, const int64_t* tot,
+                                   int64_t fbase) {
+  const int64_t ta = tot[0], tb = tot[1], tc = tot[2];
+  list_slot(s, level, kLstReg)[0] = (int)tot[3];
+  list_slot(s, level, kLstLds)[0] = (int)tot[4];
+  list_slot(s, level, kLstReg8)[0] = (int)tot[6];
+  list_slot(s, level, kLstRegHalf)[0] = (int)tot[7];
+  list_slot(s, level, kLstRid)[0] = (int)tot[8];
+  list_slot(s, level, kLstRid)[1] = (int)tot[11];  // of which 64-row sketches
+  list_slot(s, level, kLstFallback)[0] = (int)tot[9];  // k_rid appends
+  s.njobs[level] = (int)tot[5];
+  int e = 0;
+  if (ta > s.id_cap) {
+    e |= kErrIdPool;
+    atomicMax(s.err + 2, (int)((ta + 1023) >> 10));  // size needed, in 1024 elements
+  }
+  if (tb > s.row_cap) {
+    e |= kErrRowPool;
+    atomicMax(s.err + 3, (int)((tb + 1023) >> 10));
+  }
+  if (fbase + tc > s.fac_cap) e |= kErrFacPool;
+  if (tot[10] > s.dec_cap) {
+    e |= kErrDecPool;
+    atomicMax(s.err + 1, (int)((tot[10] + 1023) >> 10));
+  }
+  if (e) atomicOr(s.err, e);
+  *s.fac_top = fbase + tc;
+}
+
+// Exclusive scan of the kScanQ values of the block's threads (NT a
+// multiple of 64): off = sum over lower threads; tot (LDS, kScanQ) = the
+// block's totals.  wtot: kScanQ x NT/64 LDS words.
+template <int NT>
+__device__ inline void block_scan(const int64_t (&own)[kScanQ], int64_t (&off)[kScanQ],
+                                  int64_t (*wtot)[NT / 64], int64_t* tot) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int64_t inc[kScanQ];
 #pragma unroll
-  for (int q = 0; q < NQ; q++) {
+  for (int q = 0; q < kScanQ; q++) {
     int64_t v = own[q];
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -328,9 +397,9 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     if (lane == 63) wtot[q][wv] = v;
   }
   __syncthreads();
-  if (tid < NQ) {
+  if (tid < kScanQ) {
     int64_t run = 0;
-    for (int w = 0; w < 16; w++) {
+    for (int w = 0; w < NT / 64; w++) {
       const int64_t t = wtot[tid][w];
       wtot[tid][w] = run;
       run += t;
@@ -338,80 +407,92 @@ __global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
     tot[tid] = run;
   }
   __syncthreads();
-  const int64_t fbase = *s.fac_top;
-  int64_t off[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; q++) off[q] = wtot[q][wv] + inc[q] - own[q];
-  int* lreg = s.lists + (int64_t)kLstReg * s.nbox + start;
-  int* llds = s.lists + (int64_t)kLstLds * s.nbox + start;
-  int* lreg8 = s.lists + (int64_t)kLstReg8 * s.nbox + start;
-  int* lhalf = s.lists + (int64_t)kLstRegHalf * s.nbox + start;
-  int* lrid = s.lists + (int64_t)kLstRid * s.nbox + start;
-  int* lfb = s.lists + (int64_t)kLstFallback * s.nbox + start;
-  int* jobs = s.jobs + (int64_t)2 * kMaxChunks * start;
-  for (int i = lo; i < hi; i++) {
-    const int b = start + i;
-    if (box_clean(s, b)) continue;
-    const int kind = s.qkind[b];
-    // Scratch of odd and even levels in separate halves: a level's ID runs
-    // while the level below it is still being decoupled.
-    s.id_off[b] = (level & 1) * s.id_cap + off[0];
-    s.row_off[b] = off[1];
-    if (s.fac_sz[b] < 0) {  // keeps its slot
-      s.fac_sz[b] = -s.fac_sz[b];
-    } else {
-      s.fac_off[b] = fbase + off[2];
-      off[2] += s.fac_sz[b];
+  for (int q = 0; q < kScanQ; q++) off[q] = wtot[q][wv] + inc[q] - own[q];
+}
+
+// Exclusive scans of the per-box sizes of one level (single workgroup;
+// each thread a contiguous chunk of boxes).  ID scratch and row lists are
+// level-local; factor slots append to the running pool offset.
+// With the randomized ID, a register-QRCP box that gets no sketch (not
+// eligible, or sent straight to the full QRCP by the rank model) starts the
+// level's full-QRCP list; k_rid appends the boxes whose sketch fails.
+__global__ __launch_bounds__(1024) void k_scan(DevState s, int start,
+                                               int count, int level) {
+  __shared__ int64_t wtot[kScanQ][16];
+  __shared__ int64_t tot[kScanQ];
+  const int tid = threadIdx.x;
+  const int chunk = (count + 1023) / 1024;
+  const int lo = min(count, tid * chunk), hi = min(count, lo + chunk);
+  int64_t own[kScanQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = lo; i < hi; i++) scan_counts(s, start + i, own);
+  int64_t off[kScanQ];
+  block_scan<1024>(own, off, wtot, tot);
+  const int64_t fbase = *s.fac_top;
+  for (int i = lo; i < hi; i++) scan_place(s, start + i, start, level, fbase, off);
+  __syncthreads();
+  if (tid == 0) scan_finish(s, level, tot, fbase);
+}
+
+// The same for levels of kScanMulti boxes or more over many workgroups of
+// 256 boxes (one workgroup walked them at one chunk of boxes per thread
+// at 15-70 us per level): each block's totals (k_scan_blocks), their
+// exclusive scan and the level's totals (k_scan_totals; the factor-pool
+// base goes after the block offsets), then each box's offsets
+// (k_scan_place).  Same offsets and list order as k_scan.
+__global__ __launch_bounds__(256) void k_scan_blocks(DevState s, int start, int count) {
+  __shared__ int64_t wtot[kScanQ][4];
+  __shared__ int64_t tot[kScanQ];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  int64_t own[kScanQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < count) scan_counts(s, start + i, own);
+  int64_t off[kScanQ];
+  block_scan<256>(own, off, wtot, tot);
+  if (threadIdx.x < kScanQ) s.scan_blk[(int64_t)blockIdx.x * kScanLd + threadIdx.x] = tot[threadIdx.x];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_totals(DevState s, int nblk, int level) {
+  __shared__ int64_t wtot[kScanQ][16];
+  __shared__ int64_t tot[kScanQ];
+  const int tid = threadIdx.x;
+  const int chunk = (nblk + 1023) / 1024;
+  const int lo = min(nblk, tid * chunk), hi = min(nblk, lo + chunk);
+  int64_t own[kScanQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = lo; j < hi; j++)
+#pragma unroll
+    for (int q = 0; q < kScanQ; q++) own[q] += s.scan_blk[(int64_t)j * kScanLd + q];
+  int64_t off[kScanQ];
+  block_scan<1024>(own, off, wtot, tot);
+  for (int j = lo; j < hi; j++)
+#pragma unroll
+    for (int q = 0; q < kScanQ; q++) {
+      int64_t* w = s.scan_blk + (int64_t)j * kScanLd + q;
+      const int64_t t = *w;
+      *w = off[q];
+      off[q] += t;
     }
-    s.dec_off[b] = off[10];
-    off[10] += s.dec_sz[b];
-    off[0] += s.id_sz[b];
-    off[1] += s.row_sz[b];
-    if (kind == kQrcpReg || kind == kQrcpStack) {
-      if (s.act_cnt[b] > kHalfCols) lreg[off[3]++] = b;
-      else lhalf[off[7]++] = b;
-    }
-    if (kind == kQrcpLds) llds[off[4]++] = b;
-    if (kind == kQrcpReg8 || kind == kQrcpStack8) lreg8[off[6]++] = b;
-    if (s.sk_w[b] > 0) lrid[off[8]++] = b;
-    if (rid_full_qrcp(s, b, kind)) lfb[off[9]++] = b;
-    {
-      const int nch = row_jobs(s, b);
-      for (int q = 0; q < nch; q++) {
-        jobs[2 * off[5]] = b;
-        jobs[2 * off[5] + 1] = q;
-        off[5]++;
-      }
-    }
-  }
   __syncthreads();
   if (tid == 0) {
-    const int64_t ta = tot[0], tb = tot[1], tc = tot[2];
-    list_slot(s, level, kLstReg)[0] = (int)tot[3];
-    list_slot(s, level, kLstLds)[0] = (int)tot[4];
-    list_slot(s, level, kLstReg8)[0] = (int)tot[6];
-    list_slot(s, level, kLstRegHalf)[0] = (int)tot[7];
-    list_slot(s, level, kLstRid)[0] = (int)tot[8];
-    list_slot(s, level, kLstRid)[1] = (int)tot[11];  // of which 64-row sketches
-    list_slot(s, level, kLstFallback)[0] = (int)tot[9];  // k_rid appends
-    s.njobs[level] = (int)tot[5];
-    int e = 0;
-    if (ta > s.id_cap) {
-      e |= kErrIdPool;
-      atomicMax(s.err + 2, (int)((ta + 1023) >> 10));  // size needed, in 1024 elements
-    }
-    if (tb > s.row_cap) {
-      e |= kErrRowPool;
-      atomicMax(s.err + 3, (int)((tb + 1023) >> 10));
-    }
-    if (fbase + tc > s.fac_cap) e |= kErrFacPool;
-    if (tot[10] > s.dec_cap) {
-      e |= kErrDecPool;
-      atomicMax(s.err + 1, (int)((tot[10] + 1023) >> 10));
-    }
-    if (e) atomicOr(s.err, e);
-    *s.fac_top = fbase + tc;
+    const int64_t fbase = *s.fac_top;
+    s.scan_blk[(int64_t)nblk * kScanLd] = fbase;
+    scan_finish(s, level, tot, fbase);
   }
+}
+
+__global__ __launch_bounds__(256) void k_scan_place(DevState s, int start, int count,
+                                                    int level, int nblk) {
+  __shared__ int64_t wtot[kScanQ][4];
+  __shared__ int64_t tot[kScanQ];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  int64_t own[kScanQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < count) scan_counts(s, start + i, own);
+  int64_t off[kScanQ];
+  block_scan<256>(own, off, wtot, tot);
+  if (i >= count) return;
+  const int64_t* base = s.scan_blk + (int64_t)blockIdx.x * kScanLd;
+#pragma unroll
+  for (int q = 0; q < kScanQ; q++) off[q] += base[q];
+  scan_place(s, start + i, start, level, s.scan_blk[(int64_t)nblk * kScanLd], off);
 }
 
 
@@ -3197,7 +3278,14 @@ void launch_factor_stage(int stage, const DevGeom& g, const DevTree& t,
       k_rows<<<count, 64, 0, st>>>(g, t, s, start, level, tol);
       break;
     case kStageScan:
-      k_scan<<<1, 1024, 0, st>>>(s, start, count, level);
+      if (count >= kScanMulti) {
+        const int nblk = (count + 255) / 256;
+        k_scan_blocks<<<nblk, 256, 0, st>>>(s, start, count);
+        k_scan_totals<<<1, 1024, 0, st>>>(s, nblk, level);
+        k_scan_place<<<nblk, 256, 0, st>>>(s, start, count, level, nblk);
+      } else {
+        k_scan<<<1, 1024, 0, st>>>(s, start, count, level);
+      }
       break;
     case kStageAssemble:
       if (!(skip & kSkipAssemble))
