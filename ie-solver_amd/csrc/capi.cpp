@@ -586,10 +586,11 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
   hipStream_t st2 = split ? c->stream2 : st;
   // evs[1 + lv]: level lv's ID done (the decouple chain's first work waits
   // for the deepest level's, which follows everything enqueued before it);
-  // evs[n_levels + 1]: the decouple chain done; evs[n_levels + 2] and
-  // evs[n_levels + 3]: level 1's decouple forked to the ID stream (idle
-  // from there to the root) for its boxes above 64 DOFs, and joined back,
-  // so its two kernels run side by side.  The two chains share no
+  // evs[n_levels + 1]: the decouple chain done (without a level 1);
+  // evs[n_levels + 2] and evs[n_levels + 3]: level 1's decouple forked to
+  // the ID stream (idle from there to the root) for its boxes above 64
+  // DOFs, and the ID stream's join of the decouple chain's end, so level
+  // 1's two decouple kernels run side by side.  The two chains share no
   // scratch: the ID pool
   // (one half per level parity) belongs to the ID chain, the decouple pool
   // to the decouple chain.  The events order the two streams of one device
@@ -604,6 +605,7 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
     }
   }
   launch_reset(s, st);
+  bool joined = false;
   for (int lv = T.n_levels - 1; lv >= 1; lv--) {
     const int start = T.level_start[lv];
     const int count = T.level_start[lv + 1] - start;
@@ -625,15 +627,20 @@ void enqueue_factor(iegpu_ctx* c, double tol, const double* fb = nullptr,
       launch_factor_stage<R>(kStageDecouple, g, t, s, lv, start, count, tol, c->caps, st2, 0,
                              fork ? st : nullptr);
     });
+    // After the fork the ID stream joins the decouple chain's last work and
+    // runs level 1's upward sweep itself (no hop back and forth).
+    hipStream_t sw = st2;
     if (fork) {
-      HIP_TRY(hipEventRecord(c->evs[T.n_levels + 3], st));
-      HIP_TRY(hipStreamWaitEvent(st2, c->evs[T.n_levels + 3], 0));
+      HIP_TRY(hipEventRecord(c->evs[T.n_levels + 3], st2));
+      HIP_TRY(hipStreamWaitEvent(st, c->evs[T.n_levels + 3], 0));
+      sw = st;
+      joined = true;
     }
     if (fx)
-      c->prof.run(kProfSweepUp, lv, st2,
-                  [&] { launch_sweep_up<R>(s, start, count, fv, false, true, st2); });
+      c->prof.run(kProfSweepUp, lv, sw,
+                  [&] { launch_sweep_up<R>(s, start, count, fv, false, true, sw); });
   }
-  if (split) {
+  if (split && !joined) {
     HIP_TRY(hipEventRecord(c->evs[T.n_levels + 1], st2));
     HIP_TRY(hipStreamWaitEvent(st, c->evs[T.n_levels + 1], 0));
   }
