@@ -1654,23 +1654,39 @@ __device__ __forceinline__ void rid_wide_box(const DevGeom& g, const DevTree& t,
       k = j;
       break;
     }
-    if (QW == 1) {
-      publish();
-      wave_sync();
-    }
     if (col == p) av = false;
-    R d4[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int r = 0; r < LM; r++) d4[r & 3] += vbuf[r] * y[r];
-    const R w = tau * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
-    rr[j * YLD + col] = (col == p) ? beta : y[0] - w;
-    // vbuf is read again below rather than kept in registers beside y
-    __asm__ volatile("" ::: "memory");
     R t4[4] = {0, 0, 0, 0};
+    if (QW == 1) {
+      // One wave: the pivot column reaches every lane through readlane
+      // (twice: kept between the dot and the update it would not fit the
+      // registers beside y); the Householder vector is (1, sc yp[1..]).
+      if (lane == pl) spiv[j] = p;
+      R d4[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int r = 0; r < LM - 1; r++) {
-      y[r] = y[r + 1] - w * vbuf[r + 1];
-      if (r > 0) t4[r & 3] += y[r] * y[r];
+      for (int r = 1; r < LM; r++) d4[r & 3] += lane_value(y[r], pl) * y[r];
+      const R w = tau * (y[0] + sc * ((d4[0] + d4[1]) + (d4[2] + d4[3])));
+      const R ws = w * sc;
+      rr[j * YLD + col] = (col == p) ? beta : y[0] - w;
+      int pl2 = pl;
+      __asm__ volatile("" : "+s"(pl2));
+#pragma unroll
+      for (int r = 0; r < LM - 1; r++) {
+        y[r] = y[r + 1] - ws * lane_value(y[r + 1], pl2);
+        if (r > 0) t4[r & 3] += y[r] * y[r];
+      }
+    } else {
+      R d4[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < LM; r++) d4[r & 3] += vbuf[r] * y[r];
+      const R w = tau * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
+      rr[j * YLD + col] = (col == p) ? beta : y[0] - w;
+      // vbuf is read again below rather than kept in registers beside y
+      __asm__ volatile("" ::: "memory");
+#pragma unroll
+      for (int r = 0; r < LM - 1; r++) {
+        y[r] = y[r + 1] - w * vbuf[r + 1];
+        if (r > 0) t4[r & 3] += y[r] * y[r];
+      }
     }
     y[LM - 1] = 0;
     tl = (t4[0] + t4[1]) + (t4[2] + t4[3]);
