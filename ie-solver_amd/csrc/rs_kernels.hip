@@ -1507,10 +1507,11 @@ struct RidWideLds {
                                   : 6 * TR;
   double buf[kBuf];
   int rowdofs[kRidMaxNearW];
-  R vbuf[kRidRowsW];
+  R vbuf[4][kRidRowsW];  // two QR waves: each wave's candidate Householder
+                         // vector, by step parity
   int spiv[64];
-  double kbest[2];  // two QR waves: each wave's pivot key
-  R hs[2];          // two QR waves: beta and tau of the step
+  double kbest[4];  // two QR waves: each wave's pivot key, by step parity
+  R hs[4][2];       // two QR waves: each candidate's beta and tau
   int nred0;        // two QR waves: redundant columns of wave 0
 };
 
@@ -1525,10 +1526,8 @@ __device__ __forceinline__ void rid_wide_box(const DevGeom& g, const DevTree& t,
   double (*tab)[TR] = reinterpret_cast<double (*)[TR]>(sh.buf);
   R* ybuf = reinterpret_cast<R*>(sh.buf);
   int* rowdofs = sh.rowdofs;
-  R* vbuf = sh.vbuf;
   int* spiv = sh.spiv;
   double* kbest = sh.kbest;
-  R* hs = sh.hs;
   const int n = s.act_cnt[b], m = s.m_rows[b], l = s.sk_w[b];
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int P = proxy_rows(g, level) / 2;
@@ -1615,38 +1614,44 @@ __device__ __forceinline__ void rid_wide_box(const DevGeom& g, const DevTree& t,
   int j = 0;
   for (; j < jlim; j++) {
     double best = wave_max(av ? pivot_key<R, 7>(y[0] * y[0] + tl, col) : -1.0);
-    if (QW == 2) {
-      if (lane == 0) kbest[wv] = best;
-      __syncthreads();
-      best = fmax(kbest[0], kbest[QW - 1]);
-    }
-    const int p = pivot_key_pos<7>(best);  // column index
-    const int pw = p >> 6, pl = p & 63;
+    int p, pw, pl;
     R beta = 0, tau = 0, sc = 0;
-    if (wv == pw) {
-      const R alpha = lane_value(y[0], pl), xt = lane_value(tl, pl);
-      householder_scalars(alpha, xt, &beta, &tau, &sc);
-    }
-    // The pivot column's Householder vector: with two waves published
-    // before the cut test (both waves test the owner's beta), with one
-    // after it.
-    auto publish = [&]() {
-      if (wv == pw && lane == pl) {
-        vbuf[0] = R(1);
+    const R* vb = nullptr;
+    if (QW == 2) {
+      // Each wave publishes its own best column's Householder vector before
+      // the step's one barrier; after it both read the winner's (by step
+      // parity: a wave a step ahead writes the other slot).
+      const int par = j & 1;
+      if (best >= 0) {
+        const int ql = pivot_key_pos<7>(best) & 63;
+        const R alpha = lane_value(y[0], ql), xt = lane_value(tl, ql);
+        R b0, t0, s0;
+        householder_scalars(alpha, xt, &b0, &t0, &s0);
+        if (lane == ql) {
+          R* w = sh.vbuf[2 * par + wv];
+          w[0] = R(1);
 #pragma unroll
-        for (int r = 1; r < LM; r++) vbuf[r] = y[r] * sc;
-        spiv[j] = p;
-        if (QW == 2) {
-          hs[0] = beta;
-          hs[1] = tau;
+          for (int r = 1; r < LM; r++) w[r] = y[r] * s0;
+          sh.hs[2 * par + wv][0] = b0;
+          sh.hs[2 * par + wv][1] = t0;
         }
       }
-    };
-    if (QW == 2) {
-      publish();
+      if (lane == 0) kbest[2 * par + wv] = best;
       __syncthreads();
-      beta = hs[0];
-      tau = hs[1];
+      best = fmax(kbest[2 * par], kbest[2 * par + 1]);
+      p = pivot_key_pos<7>(best);  // column index
+      pw = p >> 6;
+      pl = p & 63;
+      beta = sh.hs[2 * par + pw][0];
+      tau = sh.hs[2 * par + pw][1];
+      vb = sh.vbuf[2 * par + pw];
+      if (wv == 0 && lane == 0) spiv[j] = p;
+    } else {
+      p = pivot_key_pos<7>(best);  // column index
+      pw = p >> 6;
+      pl = p & 63;
+      const R alpha = lane_value(y[0], pl), xt = lane_value(tl, pl);
+      householder_scalars(alpha, xt, &beta, &tau, &sc);
     }
     if (j == 0) {
       thresh = fabs(tol * beta);
@@ -1677,14 +1682,14 @@ __device__ __forceinline__ void rid_wide_box(const DevGeom& g, const DevTree& t,
     } else {
       R d4[4] = {0, 0, 0, 0};
 #pragma unroll
-      for (int r = 0; r < LM; r++) d4[r & 3] += vbuf[r] * y[r];
+      for (int r = 0; r < LM; r++) d4[r & 3] += vb[r] * y[r];
       const R w = tau * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
       rr[j * YLD + col] = (col == p) ? beta : y[0] - w;
-      // vbuf is read again below rather than kept in registers beside y
+      // the vector is read again below rather than kept in registers beside y
       __asm__ volatile("" ::: "memory");
 #pragma unroll
       for (int r = 0; r < LM - 1; r++) {
-        y[r] = y[r + 1] - w * vbuf[r + 1];
+        y[r] = y[r + 1] - w * vb[r + 1];
         if (r > 0) t4[r & 3] += y[r] * y[r];
       }
     }
