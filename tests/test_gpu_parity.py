@@ -149,6 +149,32 @@ def test_fp32_factor_residual():
     assert err < 1e-4
 
 
+def test_root_above_blocked_limit_matches_oracle():
+    # A star with k=24, amplitude 0.25 at tol 1e-10 has a root skeleton of
+    # 668 DOFs in the oracle (and no box above that): above kRootBlkMax =
+    # 512, so the root block is inverted by the Gauss-Jordan form in global
+    # memory (inside k_root_inv_blk's launch).  Checked against the
+    # oracle's factorization of the same tree and against the dense solve.
+    g = G.star(4096, amp=0.25, k=24)
+    ys, q = G.exterior_sources(3)
+    f = G.harmonic_data(g["points"], ys, q)
+    s = _gpu(g)
+    bi = s.box_info()
+    root_n = int(bi["n_active"][bi["level"] == 0][0])
+    o = oracle.OracleProblem(g)
+    o.factor(TOL)
+    oi = o.box_info()
+    root_o = int(oi["n_active"][oi["level"] == 0][0])
+    mu, _ = s.solve(f)
+    mo = o.solve(f)[0]
+    md = np.linalg.solve(oracle.dense_matrix(g), f)
+    err, err_d = _rel(mu, mo), _rel(mu, md)
+    print(f"root n={root_n} (oracle {root_o}), rel diff vs oracle {err:.3e}, "
+          f"vs dense {err_d:.3e} (oracle vs dense {_rel(mo, md):.3e})", flush=True)
+    assert 512 < root_n <= 1024 and abs(root_n - root_o) <= 2
+    assert err <= 10 * TOL
+
+
 @pytest.mark.parametrize("precision", [iesolver.FP64, iesolver.FP32])
 def test_large_root_blocked_inverse(precision):
     # A star with k=9, amplitude 0.25 has a root skeleton of 283 DOFs: the
