@@ -82,7 +82,7 @@ constexpr int kRidTableRows = 2 * kRidMaxNear + 4 * kNumProxy;  // Omega rows
 // kRidMaxNearW row DOFs, with its own Gaussian table.
 constexpr int kRidRowsW = 64;
 constexpr int kRidWideCols = 128;
-constexpr int kRidMaxNearW = 512;
+constexpr int kRidMaxNearW = 1280;  // the level-1/2 row sets of small problems in a batch
 constexpr int kRidWideTop = 4;
 constexpr int kRidTableRowsW = 2 * kRidMaxNearW + 4 * kNumProxy;
 
