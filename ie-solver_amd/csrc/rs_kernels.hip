@@ -2598,9 +2598,9 @@ __global__ __launch_bounds__(NT) void k_root(DevGeom g, DevTree t, DevState s) {
 //     a -= C[m][i] R_m, a pivot row restarting from R_k at its own step;
 //     panel entries take wave 0's result.
 // With one step of lookahead: after (a), the owners of panel P+1's columns
-// apply P to those columns alone and publish them; after a barrier wave 0
-// eliminates panel P+1 while every wave (wave 0 after it) applies P to the
-// rest of its entries; a barrier ends the panel.  Per entry these are the
+// apply P to a copy of those columns and publish it; after a barrier wave 0
+// eliminates panel P+1 from that copy while every wave (wave 0 after it)
+// applies P to all its entries; a barrier ends the panel.  Per entry these are the
 // operations of the unblocked elimination in its order, so the inverse is
 // the same.  The inverse, unpermuted, goes after the assembled matrix.
 constexpr int kRootTR = 32, kRootTC = 32, kRootRU = 6, kRootCV = 6;
